@@ -1,0 +1,317 @@
+"""MI355X (gfx950) EllipsesInFlows time-step path.
+
+Python view of the C-ABI in ``include/eif_gpu.h`` (``libeif_gpu.so``).  The
+functions keep the reference's names and argument meaning
+(NaokiHori/EllipsesInFlows include/fluid.h:27-38, include/suspensions.h:44-63,
+include/param.h:55-58, src/main.c:13-76); arrays are numpy float64 in the
+reference's halo-padded layouts.
+
+The compute path is the HIP library only: importing works without a GPU, but
+creating a :class:`Solver` raises if the library is missing or no device is
+visible.  There is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+__all__ = ["Solver", "EIFError", "load_library", "PARTICLE_FIELDS", "library_path"]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+library_path = os.path.join(_HERE, "libeif_gpu.so")
+
+# member order of particle_t (include/suspensions.h:8-31)
+PARTICLE_FIELDS = (
+    "den", "a", "b", "x", "y", "az", "dx", "dy", "daz", "ux", "uy", "vz",
+    "dux", "duy", "dvz", "fux", "fuy", "tvz", "iux0", "iux1", "iuy0", "iuy1",
+    "ivz0", "ivz1", "cfx0", "cfx1", "cfy0", "cfy1", "ctz0", "ctz1",
+)
+NREC = len(PARTICLE_FIELDS)
+
+POISSON_THOMAS = 0
+POISSON_PARTITION = 1
+
+FIELD_PSI, FIELD_SRCUXA, FIELD_SRCUYA, FIELD_DUX, FIELD_DUY = range(5)
+
+# every symbol declared in include/eif_gpu.h
+EXPORTED = (
+    "fluid_gpu_create", "fluid_gpu_destroy", "fluid_gpu_get_grid", "fluid_gpu_upload",
+    "fluid_gpu_download", "fluid_gpu_download_field", "fluid_gpu_upload_field",
+    "fluid_gpu_update_boundaries_ux", "fluid_gpu_update_boundaries_uy",
+    "fluid_gpu_update_boundaries_p", "fluid_gpu_decide_dt", "fluid_gpu_set_dt",
+    "fluid_gpu_get_dt", "fluid_gpu_update_velocity", "poisson_gpu_compute_potential",
+    "fluid_gpu_correct_velocity", "fluid_gpu_update_pressure", "fluid_gpu_divergence",
+    "fluid_gpu_momentum", "fluid_gpu_energy", "suspensions_gpu_create",
+    "suspensions_gpu_n_particles", "suspensions_gpu_download", "suspensions_gpu_upload",
+    "suspensions_gpu_reset_particle_increments", "suspensions_gpu_compute_inertia",
+    "suspensions_gpu_exchange_momentum", "suspensions_gpu_update_momentum_fleid",
+    "suspensions_gpu_compute_collision_force", "suspensions_gpu_increment_particles",
+    "suspensions_gpu_update_particles", "suspensions_gpu_colliding_pairs",
+    "eif_gpu_integrate", "eif_gpu_step", "eif_gpu_synchronize", "eif_gpu_last_error",
+    "eif_gpu_profile_enable", "eif_gpu_profile_reset", "eif_gpu_profile_query",
+    "eif_gpu_profile_names",
+)
+
+
+class EIFError(RuntimeError):
+    pass
+
+
+class _Config(ctypes.Structure):
+    _fields_ = [
+        ("itot", ctypes.c_int), ("jtot", ctypes.c_int), ("ly", ctypes.c_double),
+        ("Re", ctypes.c_double), ("Fr", ctypes.c_double), ("extfrcy", ctypes.c_double),
+        ("uy_wall_xm", ctypes.c_double), ("uy_wall_xp", ctypes.c_double),
+        ("safefactors", ctypes.c_double * 3), ("poisson_mode", ctypes.c_int),
+    ]
+
+
+_lib = None
+_D = ctypes.c_double
+_PD = ctypes.POINTER(ctypes.c_double)
+_PI = ctypes.POINTER(ctypes.c_int)
+_VP = ctypes.c_void_p
+
+
+def load_library(path=None):
+    """Load libeif_gpu.so (no device is touched)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or library_path
+    if not os.path.exists(p):
+        raise EIFError("libeif_gpu.so not built (%s); run __graft_entry__.build()" % p)
+    lib = ctypes.CDLL(p)
+    lib.eif_gpu_last_error.restype = ctypes.c_char_p
+    lib.eif_gpu_profile_names.restype = ctypes.c_char_p
+    lib.fluid_gpu_create.argtypes = [ctypes.POINTER(_Config), ctypes.POINTER(_VP)]
+    lib.suspensions_gpu_n_particles.argtypes = [_VP]
+    for name in EXPORTED:
+        f = getattr(lib, name)
+        if name not in ("eif_gpu_last_error", "eif_gpu_profile_names"):
+            f.restype = ctypes.c_int
+    lib.eif_gpu_profile_query.argtypes = [_VP, ctypes.c_char_p,
+                                          ctypes.POINTER(ctypes.c_long), _PD]
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_PD)
+
+
+class Solver:
+    """One simulation context on the GPU (fluid + suspensions)."""
+
+    def __init__(self, itot, jtot, ly=1.0, Re=2.334e3, Fr=np.finfo(float).max,
+                 extfrcy=2.337e-4, uy_wall_xm=0.0, uy_wall_xp=0.0,
+                 safefactors=(0.75, 0.75, 0.95), poisson_mode=POISSON_THOMAS):
+        self.lib = load_library()
+        cfg = _Config()
+        cfg.itot, cfg.jtot, cfg.ly = int(itot), int(jtot), float(ly)
+        cfg.Re, cfg.Fr, cfg.extfrcy = float(Re), float(Fr), float(extfrcy)
+        cfg.uy_wall_xm, cfg.uy_wall_xp = float(uy_wall_xm), float(uy_wall_xp)
+        for k in range(3):
+            cfg.safefactors[k] = float(safefactors[k])
+        cfg.poisson_mode = int(poisson_mode)
+        h = _VP()
+        self._check(self.lib.fluid_gpu_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self.h = h
+        self.itot, self.jtot, self.ly = int(itot), int(jtot), float(ly)
+        self.n_particles = 0
+
+    # -- helpers ----------------------------------------------------------
+    def _check(self, rc):
+        if rc != 0:
+            raise EIFError("eif_gpu error %d: %s" % (rc, self.lib.eif_gpu_last_error().decode()))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.fluid_gpu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def shapes(self):
+        nx, ny = self.itot, self.jtot
+        return {"ux": (ny + 2, nx + 1), "uy": (ny + 2, nx + 2), "p": (ny + 2, nx + 2),
+                "psi": (ny + 2, nx + 2), "srcuxa": (ny, nx - 1), "srcuya": (ny, nx),
+                "dux": (ny + 2, nx + 2), "duy": (ny + 2, nx + 2)}
+
+    # -- grid and fields --------------------------------------------------
+    def grid(self):
+        nx, ny = self.itot, self.jtot
+        arrs = [np.zeros(n) for n in (nx + 1, nx + 2, nx, nx + 1, ny + 1, ny)]
+        self._check(self.lib.fluid_gpu_get_grid(self.h, *[_ptr(a) for a in arrs]))
+        return dict(zip(("xf", "xc", "dxf", "dxc", "yf", "yc"), arrs))
+
+    def upload(self, ux=None, uy=None, p=None):
+        arrs = []
+        for name, a in (("ux", ux), ("uy", uy), ("p", p)):
+            if a is None:
+                arrs.append(None)
+                continue
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            if a.size != int(np.prod(self.shapes[name])):
+                raise ValueError("%s has %d values, expected shape %s" % (name, a.size, self.shapes[name]))
+            arrs.append(a)
+        self._check(self.lib.fluid_gpu_upload(self.h, *[None if a is None else _ptr(a) for a in arrs]))
+
+    def download(self):
+        out = {k: np.empty(self.shapes[k]) for k in ("ux", "uy", "p")}
+        self._check(self.lib.fluid_gpu_download(self.h, _ptr(out["ux"]), _ptr(out["uy"]), _ptr(out["p"])))
+        return out
+
+    def field(self, which):
+        sel = {"psi": FIELD_PSI, "srcuxa": FIELD_SRCUXA, "srcuya": FIELD_SRCUYA,
+               "dux": FIELD_DUX, "duy": FIELD_DUY}[which]
+        out = np.empty(self.shapes[which])
+        self._check(self.lib.fluid_gpu_download_field(self.h, sel, _ptr(out)))
+        return out
+
+    def set_field(self, which, arr):
+        sel = {"psi": FIELD_PSI, "srcuxa": FIELD_SRCUXA, "srcuya": FIELD_SRCUYA}[which]
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        if a.size != int(np.prod(self.shapes[which])):
+            raise ValueError("bad size for %s" % which)
+        self._check(self.lib.fluid_gpu_upload_field(self.h, sel, _ptr(a)))
+
+    # -- the reference's functions -----------------------------------------
+    def update_boundaries(self):
+        self._check(self.lib.fluid_gpu_update_boundaries_ux(self.h))
+        self._check(self.lib.fluid_gpu_update_boundaries_uy(self.h))
+        self._check(self.lib.fluid_gpu_update_boundaries_p(self.h))
+
+    def decide_dt(self):
+        dt = _D()
+        self._check(self.lib.fluid_gpu_decide_dt(self.h, ctypes.byref(dt)))
+        return dt.value
+
+    def set_dt(self, dt):
+        self._check(self.lib.fluid_gpu_set_dt(self.h, _D(dt)))
+
+    def get_dt(self):
+        dt = _D()
+        self._check(self.lib.fluid_gpu_get_dt(self.h, ctypes.byref(dt)))
+        return dt.value
+
+    def update_velocity(self, rk):
+        self._check(self.lib.fluid_gpu_update_velocity(self.h, int(rk)))
+
+    def compute_potential(self, rk):
+        self._check(self.lib.poisson_gpu_compute_potential(self.h, int(rk)))
+
+    def correct_velocity(self, rk):
+        self._check(self.lib.fluid_gpu_correct_velocity(self.h, int(rk)))
+
+    def update_pressure(self):
+        self._check(self.lib.fluid_gpu_update_pressure(self.h))
+
+    def divergence(self):
+        a, b = _D(), _D()
+        self._check(self.lib.fluid_gpu_divergence(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def momentum(self):
+        o = np.zeros(2)
+        self._check(self.lib.fluid_gpu_momentum(self.h, _ptr(o)))
+        return o
+
+    def energy(self):
+        o = np.zeros(2)
+        self._check(self.lib.fluid_gpu_energy(self.h, _ptr(o)))
+        return o
+
+    def set_particles(self, den, a, b, x, y, az, ux=None, uy=None, vz=None):
+        arrs = [np.ascontiguousarray(v, dtype=np.float64) for v in (den, a, b, x, y, az)]
+        n = arrs[0].size
+        for v in (ux, uy, vz):
+            arrs.append(np.zeros(n) if v is None else np.ascontiguousarray(v, dtype=np.float64))
+        if any(v.size != n for v in arrs):
+            raise ValueError("particle arrays differ in length")
+        self._check(self.lib.suspensions_gpu_create(self.h, int(n), *[_ptr(v) for v in arrs]))
+        self.n_particles = n
+
+    def particles(self):
+        out = np.zeros((self.n_particles, NREC))
+        if self.n_particles:
+            self._check(self.lib.suspensions_gpu_download(self.h, _ptr(out)))
+        return out
+
+    def set_particle_records(self, rec):
+        rec = np.ascontiguousarray(rec, dtype=np.float64)
+        if rec.shape != (self.n_particles, NREC):
+            raise ValueError("expected (%d, %d) records" % (self.n_particles, NREC))
+        self._check(self.lib.suspensions_gpu_upload(self.h, _ptr(rec)))
+
+    def reset_particle_increments(self):
+        self._check(self.lib.suspensions_gpu_reset_particle_increments(self.h))
+
+    def compute_inertia(self, cnstep):
+        self._check(self.lib.suspensions_gpu_compute_inertia(self.h, int(cnstep)))
+
+    def exchange_momentum(self):
+        self._check(self.lib.suspensions_gpu_exchange_momentum(self.h))
+
+    def update_momentum_field(self):
+        self._check(self.lib.suspensions_gpu_update_momentum_fleid(self.h))
+
+    def compute_collision_force(self, cnstep):
+        self._check(self.lib.suspensions_gpu_compute_collision_force(self.h, int(cnstep)))
+
+    def increment_particles(self, rk):
+        r = _D()
+        self._check(self.lib.suspensions_gpu_increment_particles(self.h, int(rk), ctypes.byref(r)))
+        return r.value
+
+    def update_particles(self):
+        self._check(self.lib.suspensions_gpu_update_particles(self.h))
+
+    def colliding_pairs(self):
+        n = ctypes.c_int()
+        self._check(self.lib.suspensions_gpu_colliding_pairs(self.h, 0, None, ctypes.byref(n)))
+        buf = np.zeros(2 * max(n.value, 1), dtype=np.int32)
+        self._check(self.lib.suspensions_gpu_colliding_pairs(
+            self.h, n.value, buf.ctypes.data_as(_PI), ctypes.byref(n)))
+        return [tuple(int(v) for v in buf[2 * k:2 * k + 2]) for k in range(n.value)]
+
+    def integrate(self):
+        sub = (ctypes.c_int * 3)()
+        self._check(self.lib.eif_gpu_integrate(self.h, sub))
+        return list(sub)
+
+    def step(self):
+        dt = _D()
+        self._check(self.lib.eif_gpu_step(self.h, ctypes.byref(dt)))
+        return dt.value
+
+    def synchronize(self):
+        self._check(self.lib.eif_gpu_synchronize(self.h))
+
+    # -- profiling ------------------------------------------------------------
+    def profile(self, enable=True):
+        self._check(self.lib.eif_gpu_profile_enable(self.h, int(bool(enable))))
+
+    def profile_reset(self):
+        self._check(self.lib.eif_gpu_profile_reset(self.h))
+
+    def profile_query(self, name):
+        cnt = ctypes.c_long()
+        tot = _D()
+        self._check(self.lib.eif_gpu_profile_query(self.h, name.encode(), ctypes.byref(cnt), ctypes.byref(tot)))
+        return cnt.value, tot.value
+
+    def profile_names(self):
+        return self.lib.eif_gpu_profile_names().decode().split(",")

@@ -1,0 +1,642 @@
+// eif_api.cpp -- the C-ABI of include/eif_gpu.h: context set-up, transfers,
+// the time-step orchestration of src/main.c:13-76, errors and profiling.
+#include "eif_internal.hpp"
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <algorithm>
+
+namespace eif {
+
+static thread_local std::string g_err;
+
+void set_error(const std::string &msg) { g_err = msg; }
+
+int hip_fail(hipError_t e, const char *what, const char *file, int line) {
+  char buf[512];
+  snprintf(buf, sizeof(buf), "%s failed: %s (%s:%d)", what, hipGetErrorString(e), file,
+           line);
+  g_err = buf;
+  return EIF_ERR_HIP;
+}
+
+// ---------------------------------------------------------------------
+// profiling: HIP events around each named launch on the context stream
+// ---------------------------------------------------------------------
+static const char *k_prof_names =
+    "bc,predictor,rhs_dct2,thomas,dct3_psi,correct_pressure,decide_dt,"
+    "ibm_inertia,ibm_exchange,ibm_apply,collision";
+
+struct Profiler {
+  bool on = false;
+  struct Rec {
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+    size_t used = 0;
+    long count = 0;
+    double total_ms = 0.;
+  };
+  std::map<std::string, Rec> recs;
+  std::vector<std::string> open;
+};
+
+static int prof_flush(Ctx *c) {
+  if (!c->prof) return EIF_OK;
+  for (auto &kv : c->prof->recs) {
+    auto &r = kv.second;
+    for (size_t k = 0; k < r.used; k++) {
+      float ms = 0.f;
+      EIF_HIP(hipEventSynchronize(r.ev[k].second));
+      EIF_HIP(hipEventElapsedTime(&ms, r.ev[k].first, r.ev[k].second));
+      r.total_ms += ms;
+      r.count++;
+    }
+    r.used = 0;
+  }
+  return EIF_OK;
+}
+
+void prof_begin(Ctx *c, const char *name) {
+  if (!c->prof || !c->prof->on) return;
+  auto &r = c->prof->recs[name];
+  if (r.used == r.ev.size()) {
+    hipEvent_t a, b;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return;
+    r.ev.push_back({a, b});
+  }
+  (void)hipEventRecord(r.ev[r.used].first, c->stream);
+}
+
+void prof_end(Ctx *c, const char *name) {
+  if (!c->prof || !c->prof->on) return;
+  auto &r = c->prof->recs[name];
+  if (r.used >= r.ev.size()) return;
+  (void)hipEventRecord(r.ev[r.used].second, c->stream);
+  r.used++;
+  if (r.used >= 4096) prof_flush(c);
+}
+
+int check_flags(Ctx *c) {
+  int h = 0;
+  EIF_HIP(hipMemcpyAsync(&h, c->flags, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  EIF_HIP(hipStreamSynchronize(c->stream));
+  if (h) {
+    std::string m = "device capacity exceeded:";
+    if (h & 1) m += " IBM box scratch;";
+    if (h & 2) m += " tile box list;";
+    if (h & 4) m += " colliding pair list;";
+    if (h & 8) m += " equivalent-circle iteration cap;";
+    if (h & 16) m += " contacts per particle;";
+    set_error(m);
+    EIF_HIP(hipMemsetAsync(c->flags, 0, sizeof(int), c->stream));
+    return EIF_ERR_CAPACITY;
+  }
+  return EIF_OK;
+}
+
+}  // namespace eif
+
+using namespace eif;
+
+#define CHECK_CTX(c)                    \
+  do {                                  \
+    if (!(c)) {                         \
+      set_error("null context");        \
+      return EIF_ERR_ARG;               \
+    }                                   \
+  } while (0)
+
+#define TRY(x)            \
+  do {                    \
+    int _r = (x);         \
+    if (_r) return _r;    \
+  } while (0)
+
+struct eif_ctx : public Ctx {};
+
+extern "C" {
+
+const char *eif_gpu_last_error(void) { return g_err.c_str(); }
+const char *eif_gpu_profile_names(void) { return k_prof_names; }
+
+static void free_ctx(Ctx *c) {
+  double **bufs[] = {&c->xf, &c->xc, &c->dxf, &c->dxc, &c->yf, &c->yc, &c->ux, &c->uy,
+                     &c->p, &c->psi, &c->ux_alt, &c->uy_alt, &c->srcuxa, &c->srcuya,
+                     &c->dux, &c->duy, &c->qhat, &c->ctab, &c->th_val, &c->th_u,
+                     &c->th_q1, &c->th_den, &c->th_last, &c->pt_fwd, &c->pt_u,
+                     &c->pt_v, &c->pt_w, &c->pt_bnd, &c->pt_red, &c->dscal, &c->dred,
+                     &c->prt, &c->pconst, &c->sc_fx, &c->sc_fy, &c->sc_dist, &c->sc_pos,
+                     &c->pair_val, &c->wall_val};
+  for (auto b : bufs)
+    if (*b) (void)hipFree(*b);
+  void *others[] = {c->th_flag, c->boxes, c->box_off, c->tile_cnt, c->tile_list,
+                    c->active, c->flags, c->pair_cnt, c->pair_ids, c->wall_hit};
+  for (auto o : others)
+    if (o) (void)hipFree(o);
+  if (c->hscal) (void)hipHostFree(c->hscal);
+  if (c->prof) {
+    for (auto &kv : c->prof->recs)
+      for (auto &e : kv.second.ev) {
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
+      }
+    delete c->prof;
+  }
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+}
+
+int fluid_gpu_create(const eif_config *cfg, eif_ctx **out) {
+  if (!cfg || !out) {
+    set_error("null argument");
+    return EIF_ERR_ARG;
+  }
+  *out = nullptr;
+  if (cfg->itot < 2 || cfg->jtot < 3 || !(cfg->ly > 0.) || !(cfg->Re > 0.)) {
+    set_error("invalid grid: need itot >= 2, jtot >= 3, ly > 0, Re > 0");
+    return EIF_ERR_ARG;
+  }
+  if ((long)cfg->itot * cfg->jtot > (1L << 31) / 4) {
+    set_error("grid too large for one context");
+    return EIF_ERR_ARG;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+    set_error("no HIP device available");
+    return EIF_ERR_NODEVICE;
+  }
+  eif_ctx *c = new eif_ctx();
+  c->cfg = *cfg;
+  const int nx = cfg->itot, ny = cfg->jtot;
+  c->nx = nx;
+  c->ny = ny;
+  c->L = Layout{nx, ny};
+  c->lx = 1.;
+  c->ly = cfg->ly;
+  int rc = EIF_OK;
+  auto fail = [&](int r) {
+    free_ctx(c);
+    delete c;
+    return r;
+  };
+  if (hipGetDevice(&c->device) != hipSuccess) return fail(hip_fail(hipErrorUnknown, "hipGetDevice", __FILE__, __LINE__));
+  {
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return fail(hip_fail(e, "hipStreamCreate", __FILE__, __LINE__));
+  }
+  // grid, src/param/init.c:105-167 (one process)
+  {
+    const double lx = c->lx, ly = c->ly;
+    const double dx = lx / nx;
+    c->dx = dx;
+    c->h_xf.assign(nx + 1, 0.);
+    c->h_xc.assign(nx + 2, 0.);
+    c->h_dxf.assign(nx, 0.);
+    c->h_dxc.assign(nx + 1, 0.);
+    c->h_yf.assign(ny + 1, 0.);
+    c->h_yc.assign(ny, 0.);
+    for (int i = 1; i <= nx + 1; i++) c->h_xf[i - 1] = 1. * (i - 1) * dx;
+    c->h_xf[0] = 0.;
+    c->h_xf[nx] = lx;
+    for (int i = 1; i <= nx; i++) c->h_dxf[i - 1] = c->h_xf[i] - c->h_xf[i - 1];
+    c->h_xc[0] = c->h_xf[0];
+    for (int i = 1; i <= nx; i++) c->h_xc[i] = 0.5 * (c->h_xf[i - 1] + c->h_xf[i]);
+    c->h_xc[nx + 1] = c->h_xf[nx];
+    for (int i = 1; i <= nx + 1; i++) c->h_dxc[i - 1] = c->h_xc[i] - c->h_xc[i - 1];
+    const double dy = ly / ny;
+    c->dy = dy;
+    const double yoffset = dy * 0;
+    for (int j = 1; j <= ny + 1; j++) c->h_yf[j - 1] = yoffset + 1. * (j - 1) * dy;
+    for (int j = 1; j <= ny; j++) c->h_yc[j - 1] = yoffset + 0.5 * (2 * j - 1) * dy;
+  }
+  // RK coefficients, src/param/init.c:169-191
+  c->alpha[0] = 32. / 60.;
+  c->beta[0] = 0. / 60.;
+  c->alpha[1] = 25. / 60.;
+  c->beta[1] = -17. / 60.;
+  c->alpha[2] = 45. / 60.;
+  c->beta[2] = -25. / 60.;
+  for (int k = 0; k < 3; k++) c->gamma[k] = c->alpha[k] + c->beta[k];
+  auto dalloc = [&](double **p, size_t n, const double *src) -> int {
+    hipError_t e = hipMalloc(p, n * sizeof(double));
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc", __FILE__, __LINE__);
+    if (src)
+      e = hipMemcpy(*p, src, n * sizeof(double), hipMemcpyHostToDevice);
+    else
+      e = hipMemset(*p, 0, n * sizeof(double));
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpy", __FILE__, __LINE__);
+    return EIF_OK;
+  };
+  const Layout &L = c->L;
+  if ((rc = dalloc(&c->xf, nx + 1, c->h_xf.data())) ||
+      (rc = dalloc(&c->xc, nx + 2, c->h_xc.data())) ||
+      (rc = dalloc(&c->dxf, nx, c->h_dxf.data())) ||
+      (rc = dalloc(&c->dxc, nx + 1, c->h_dxc.data())) ||
+      (rc = dalloc(&c->yf, ny + 1, c->h_yf.data())) ||
+      (rc = dalloc(&c->yc, ny, c->h_yc.data())) ||
+      (rc = dalloc(&c->ux, L.ux_len(), nullptr)) ||
+      (rc = dalloc(&c->ux_alt, L.ux_len(), nullptr)) ||
+      (rc = dalloc(&c->uy, L.p_len(), nullptr)) ||
+      (rc = dalloc(&c->uy_alt, L.p_len(), nullptr)) ||
+      (rc = dalloc(&c->p, L.p_len(), nullptr)) ||
+      (rc = dalloc(&c->psi, L.p_len(), nullptr)) ||
+      (rc = dalloc(&c->dux, L.p_len(), nullptr)) ||
+      (rc = dalloc(&c->duy, L.p_len(), nullptr)) ||
+      (rc = dalloc(&c->srcuxa, L.sx_len(), nullptr)) ||
+      (rc = dalloc(&c->srcuya, L.sy_len(), nullptr)) ||
+      (rc = dalloc(&c->dscal, S_NSCAL, nullptr)))
+    return fail(rc);
+  c->nred = 1024;
+  if ((rc = dalloc(&c->dred, 16 + 2 * (size_t)c->nred, nullptr))) return fail(rc);
+  {
+    hipError_t e = hipHostMalloc((void **)&c->hscal, S_NSCAL * sizeof(double), 0);
+    if (e != hipSuccess) return fail(hip_fail(e, "hipHostMalloc", __FILE__, __LINE__));
+    e = hipMalloc(&c->flags, 4 * sizeof(int));
+    if (e == hipSuccess) e = hipMemset(c->flags, 0, 4 * sizeof(int));
+    if (e != hipSuccess) return fail(hip_fail(e, "hipMalloc", __FILE__, __LINE__));
+  }
+  if ((rc = poisson_init(c))) return fail(rc);
+  c->prof = new Profiler();
+  // initial boundary values (src/fluid/init.c:33-85)
+  if ((rc = launch_bc_ux(c, c->ux)) || (rc = launch_bc_uy(c, c->uy)) ||
+      (rc = launch_bc_ux(c, c->ux_alt)) || (rc = launch_bc_uy(c, c->uy_alt)) ||
+      (rc = launch_bc_p(c, c->p)))
+    return fail(rc);
+  if (hipStreamSynchronize(c->stream) != hipSuccess)
+    return fail(hip_fail(hipErrorUnknown, "hipStreamSynchronize", __FILE__, __LINE__));
+  *out = c;
+  return EIF_OK;
+}
+
+int fluid_gpu_destroy(eif_ctx *c) {
+  if (!c) return EIF_OK;
+  (void)hipStreamSynchronize(c->stream);
+  free_ctx(c);
+  delete c;
+  return EIF_OK;
+}
+
+int fluid_gpu_get_grid(const eif_ctx *c, double *xf, double *xc, double *dxf,
+                       double *dxc, double *yf, double *yc) {
+  CHECK_CTX(c);
+  auto cp = [](double *d, const std::vector<double> &s) {
+    if (d) memcpy(d, s.data(), s.size() * sizeof(double));
+  };
+  cp(xf, c->h_xf);
+  cp(xc, c->h_xc);
+  cp(dxf, c->h_dxf);
+  cp(dxc, c->h_dxc);
+  cp(yf, c->h_yf);
+  cp(yc, c->h_yc);
+  return EIF_OK;
+}
+
+int fluid_gpu_upload(eif_ctx *c, const double *ux, const double *uy, const double *p) {
+  CHECK_CTX(c);
+  const Layout &L = c->L;
+  if (ux) EIF_HIP(hipMemcpyAsync(c->ux, ux, L.ux_len() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  if (uy) EIF_HIP(hipMemcpyAsync(c->uy, uy, L.p_len() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  if (p) EIF_HIP(hipMemcpyAsync(c->p, p, L.p_len() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  EIF_HIP(hipStreamSynchronize(c->stream));
+  return EIF_OK;
+}
+
+int fluid_gpu_download(eif_ctx *c, double *ux, double *uy, double *p) {
+  CHECK_CTX(c);
+  const Layout &L = c->L;
+  if (ux) EIF_HIP(hipMemcpyAsync(ux, c->ux, L.ux_len() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (uy) EIF_HIP(hipMemcpyAsync(uy, c->uy, L.p_len() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (p) EIF_HIP(hipMemcpyAsync(p, c->p, L.p_len() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  EIF_HIP(hipStreamSynchronize(c->stream));
+  return EIF_OK;
+}
+
+static int field_ptr(eif_ctx *c, int which, double **ptr, size_t *n) {
+  const Layout &L = c->L;
+  switch (which) {
+    case EIF_FIELD_PSI: *ptr = c->psi; *n = L.p_len(); return EIF_OK;
+    case EIF_FIELD_SRCUXA: *ptr = c->srcuxa; *n = L.sx_len(); return EIF_OK;
+    case EIF_FIELD_SRCUYA: *ptr = c->srcuya; *n = L.sy_len(); return EIF_OK;
+    case EIF_FIELD_DUX: *ptr = c->dux; *n = L.p_len(); return EIF_OK;
+    case EIF_FIELD_DUY: *ptr = c->duy; *n = L.p_len(); return EIF_OK;
+  }
+  set_error("unknown field selector");
+  return EIF_ERR_ARG;
+}
+
+int fluid_gpu_download_field(eif_ctx *c, int which, double *host) {
+  CHECK_CTX(c);
+  if (!host) {
+    set_error("null host buffer");
+    return EIF_ERR_ARG;
+  }
+  double *ptr;
+  size_t n;
+  TRY(field_ptr(c, which, &ptr, &n));
+  if (which == EIF_FIELD_DUX || which == EIF_FIELD_DUY) TRY(launch_materialise_du(c));
+  EIF_HIP(hipMemcpyAsync(host, ptr, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  EIF_HIP(hipStreamSynchronize(c->stream));
+  return EIF_OK;
+}
+
+int fluid_gpu_upload_field(eif_ctx *c, int which, const double *host) {
+  CHECK_CTX(c);
+  if (!host || which == EIF_FIELD_DUX || which == EIF_FIELD_DUY) {
+    set_error("field cannot be uploaded");
+    return EIF_ERR_ARG;
+  }
+  double *ptr;
+  size_t n;
+  TRY(field_ptr(c, which, &ptr, &n));
+  EIF_HIP(hipMemcpyAsync(ptr, host, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  EIF_HIP(hipStreamSynchronize(c->stream));
+  return EIF_OK;
+}
+
+int fluid_gpu_update_boundaries_ux(eif_ctx *c) { CHECK_CTX(c); return launch_bc_ux(c, c->ux); }
+int fluid_gpu_update_boundaries_uy(eif_ctx *c) { CHECK_CTX(c); return launch_bc_uy(c, c->uy); }
+int fluid_gpu_update_boundaries_p(eif_ctx *c) { CHECK_CTX(c); return launch_bc_p(c, c->p); }
+
+int fluid_gpu_get_dt(eif_ctx *c, double *dt_out) {
+  CHECK_CTX(c);
+  if (!dt_out) return EIF_OK;
+  EIF_HIP(hipMemcpyAsync(c->hscal, c->dscal + S_DT, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  EIF_HIP(hipStreamSynchronize(c->stream));
+  *dt_out = c->hscal[0];
+  return EIF_OK;
+}
+
+int fluid_gpu_decide_dt(eif_ctx *c, double *dt_out) {
+  CHECK_CTX(c);
+  TRY(launch_decide_dt(c));
+  if (dt_out) return fluid_gpu_get_dt(c, dt_out);
+  return EIF_OK;
+}
+
+int fluid_gpu_set_dt(eif_ctx *c, double dt) {
+  CHECK_CTX(c);
+  if (!(dt > 0.) || !std::isfinite(dt)) {
+    set_error("dt must be positive and finite");
+    return EIF_ERR_ARG;
+  }
+  c->hscal[S_DT] = dt;
+  EIF_HIP(hipMemcpyAsync(c->dscal + S_DT, c->hscal + S_DT, sizeof(double), hipMemcpyHostToDevice, c->stream));
+  EIF_HIP(hipStreamSynchronize(c->stream));
+  return EIF_OK;
+}
+
+static int check_rk(int rk) {
+  if (rk < 0 || rk > 2) {
+    set_error("rkstep must be 0, 1 or 2");
+    return EIF_ERR_ARG;
+  }
+  return EIF_OK;
+}
+
+int fluid_gpu_update_velocity(eif_ctx *c, int rk) {
+  CHECK_CTX(c);
+  TRY(check_rk(rk));
+  return launch_predictor(c, rk);
+}
+
+int poisson_gpu_compute_potential(eif_ctx *c, int rk) {
+  CHECK_CTX(c);
+  TRY(check_rk(rk));
+  return launch_poisson(c, rk);
+}
+
+int fluid_gpu_correct_velocity(eif_ctx *c, int rk) {
+  CHECK_CTX(c);
+  TRY(check_rk(rk));
+  return launch_correct_pressure(c, rk, true, false);
+}
+
+int fluid_gpu_update_pressure(eif_ctx *c) {
+  CHECK_CTX(c);
+  return launch_correct_pressure(c, 0, false, true);
+}
+
+int fluid_gpu_divergence(eif_ctx *c, double *divmax, double *divsum) {
+  CHECK_CTX(c);
+  double o[2];
+  TRY(launch_diagnostics(c, 0, o));
+  if (divmax) *divmax = o[0];
+  if (divsum) *divsum = o[1];
+  return EIF_OK;
+}
+
+int fluid_gpu_momentum(eif_ctx *c, double moms[2]) {
+  CHECK_CTX(c);
+  return launch_diagnostics(c, 1, moms);
+}
+
+int fluid_gpu_energy(eif_ctx *c, double q[2]) {
+  CHECK_CTX(c);
+  return launch_diagnostics(c, 2, q);
+}
+
+int suspensions_gpu_create(eif_ctx *c, int n, const double *den, const double *a,
+                           const double *b, const double *x, const double *y,
+                           const double *az, const double *ux, const double *uy,
+                           const double *vz) {
+  CHECK_CTX(c);
+  if (n < 0 || (n > 0 && !(den && a && b && x && y && az && ux && uy && vz))) {
+    set_error("invalid particle arrays");
+    return EIF_ERR_ARG;
+  }
+  if (c->np > 0 || c->prt) {
+    set_error("particles already created for this context");
+    return EIF_ERR_STATE;
+  }
+  for (int k = 0; k < n; k++)
+    if (!(a[k] > 0.) || !(b[k] > 0.) || !(den[k] > 0.)) {
+      set_error("particle semi-axes and densities must be positive");
+      return EIF_ERR_ARG;
+    }
+  std::vector<double> rec((size_t)(n > 0 ? n : 1) * R_N, 0.);
+  for (int k = 0; k < n; k++) {
+    double *r = rec.data() + (size_t)k * R_N;
+    r[R_DEN] = den[k]; r[R_A] = a[k]; r[R_B] = b[k];
+    r[R_X] = x[k]; r[R_Y] = y[k]; r[R_AZ] = az[k];
+    r[R_UX] = ux[k]; r[R_UY] = uy[k]; r[R_VZ] = vz[k];
+  }
+  EIF_HIP(hipMalloc(&c->prt, rec.size() * sizeof(double)));
+  EIF_HIP(hipMemcpy(c->prt, rec.data(), rec.size() * sizeof(double), hipMemcpyHostToDevice));
+  c->np = n;
+  return susp_init(c);
+}
+
+int suspensions_gpu_n_particles(const eif_ctx *c) { return c ? c->np : 0; }
+
+int suspensions_gpu_download(eif_ctx *c, double *records) {
+  CHECK_CTX(c);
+  if (c->np == 0) return EIF_OK;
+  EIF_HIP(hipMemcpyAsync(records, c->prt, (size_t)c->np * R_N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  EIF_HIP(hipStreamSynchronize(c->stream));
+  return EIF_OK;
+}
+
+int suspensions_gpu_upload(eif_ctx *c, const double *records) {
+  CHECK_CTX(c);
+  if (c->np == 0) return EIF_OK;
+  EIF_HIP(hipMemcpyAsync(c->prt, records, (size_t)c->np * R_N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  EIF_HIP(hipStreamSynchronize(c->stream));
+  return EIF_OK;
+}
+
+int suspensions_gpu_reset_particle_increments(eif_ctx *c) {
+  CHECK_CTX(c);
+  return launch_reset(c);
+}
+
+int suspensions_gpu_compute_inertia(eif_ctx *c, int cnstep) {
+  CHECK_CTX(c);
+  if (cnstep != 0 && cnstep != 1) {
+    set_error("cnstep must be 0 or 1");
+    return EIF_ERR_ARG;
+  }
+  return launch_inertia(c, cnstep);
+}
+
+int suspensions_gpu_exchange_momentum(eif_ctx *c) {
+  CHECK_CTX(c);
+  return launch_exchange(c);
+}
+
+int suspensions_gpu_update_momentum_fleid(eif_ctx *c) {
+  CHECK_CTX(c);
+  return launch_update_momentum(c);
+}
+
+int suspensions_gpu_compute_collision_force(eif_ctx *c, int cnstep) {
+  CHECK_CTX(c);
+  if (cnstep != 0 && cnstep != 1) {
+    set_error("cnstep must be 0 or 1");
+    return EIF_ERR_ARG;
+  }
+  return launch_collision(c, cnstep);
+}
+
+int suspensions_gpu_increment_particles(eif_ctx *c, int rk, double *residual) {
+  CHECK_CTX(c);
+  TRY(check_rk(rk));
+  TRY(launch_increment(c, rk));
+  if (residual) {
+    EIF_HIP(hipMemcpyAsync(c->hscal + S_RESIDUAL, c->dscal + S_RESIDUAL, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    EIF_HIP(hipStreamSynchronize(c->stream));
+    *residual = c->hscal[S_RESIDUAL];
+    TRY(check_flags(c));
+  }
+  return EIF_OK;
+}
+
+int suspensions_gpu_update_particles(eif_ctx *c) {
+  CHECK_CTX(c);
+  return launch_update_particles(c);
+}
+
+int suspensions_gpu_colliding_pairs(eif_ctx *c, int capacity, int *pairs, int *n_pairs) {
+  CHECK_CTX(c);
+  std::vector<std::pair<int, int>> out;
+  if (c->np > 0) {
+    int cnt = 0;
+    EIF_HIP(hipMemcpyAsync(&cnt, c->pair_cnt, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    EIF_HIP(hipStreamSynchronize(c->stream));
+    cnt = std::min(cnt, c->pair_cap);
+    std::vector<int> ids(2 * (size_t)(cnt > 0 ? cnt : 1));
+    std::vector<int> hits(2 * (size_t)c->np);
+    if (cnt > 0)
+      EIF_HIP(hipMemcpyAsync(ids.data(), c->pair_ids, 2 * (size_t)cnt * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    EIF_HIP(hipMemcpyAsync(hits.data(), c->wall_hit, 2 * (size_t)c->np * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    EIF_HIP(hipStreamSynchronize(c->stream));
+    std::vector<std::pair<int, int>> pp;
+    for (int s = 0; s < cnt; s++) pp.push_back({ids[2 * s], ids[2 * s + 1]});
+    std::sort(pp.begin(), pp.end());
+    out = pp;
+    for (int w = 0; w < 2; w++)
+      for (int n = 0; n < c->np; n++)
+        if (hits[2 * n + w]) out.push_back({n, -1 - w});
+  }
+  if (n_pairs) *n_pairs = (int)out.size();
+  if (pairs)
+    for (int s = 0; s < (int)out.size() && s < capacity; s++) {
+      pairs[2 * s] = out[s].first;
+      pairs[2 * s + 1] = out[s].second;
+    }
+  if ((int)out.size() > capacity && pairs) {
+    set_error("pair buffer too small");
+    return EIF_ERR_CAPACITY;
+  }
+  return EIF_OK;
+}
+
+// integrate(), src/main.c:13-55
+int eif_gpu_integrate(eif_ctx *c, int substeps[3]) {
+  CHECK_CTX(c);
+  for (int rk = 0; rk < 3; rk++) {
+    TRY(launch_reset(c));
+    TRY(launch_bc_ux(c, c->ux));
+    TRY(launch_bc_uy(c, c->uy));
+    TRY(launch_bc_p(c, c->p));
+    TRY(launch_inertia(c, 0));
+    TRY(launch_predictor(c, rk));
+    TRY(launch_exchange(c));
+    TRY(launch_update_momentum(c));
+    TRY(launch_poisson(c, rk));
+    TRY(launch_correct_pressure(c, rk, true, true));
+    TRY(launch_collision(c, 0));
+    int substep;
+    for (substep = 0;; substep++) {
+      TRY(launch_inertia(c, 1));
+      TRY(launch_collision(c, 1));
+      double residual = 0.;
+      TRY(suspensions_gpu_increment_particles(c, rk, &residual));
+      if (residual < 1.e-8) break;
+      if (substep > 100) break;
+    }
+    if (substeps) substeps[rk] = substep + 1;
+    TRY(launch_update_particles(c));
+  }
+  return EIF_OK;
+}
+
+// src/main.c:71-76 (decide_dt + integrate)
+int eif_gpu_step(eif_ctx *c, double *dt_out) {
+  CHECK_CTX(c);
+  TRY(launch_decide_dt(c));
+  TRY(eif_gpu_integrate(c, nullptr));
+  if (dt_out) return fluid_gpu_get_dt(c, dt_out);
+  return EIF_OK;
+}
+
+int eif_gpu_synchronize(eif_ctx *c) {
+  CHECK_CTX(c);
+  EIF_HIP(hipStreamSynchronize(c->stream));
+  return check_flags(c);
+}
+
+int eif_gpu_profile_enable(eif_ctx *c, int enable) {
+  CHECK_CTX(c);
+  c->prof->on = enable != 0;
+  return EIF_OK;
+}
+
+int eif_gpu_profile_reset(eif_ctx *c) {
+  CHECK_CTX(c);
+  TRY(prof_flush(c));
+  for (auto &kv : c->prof->recs) {
+    kv.second.count = 0;
+    kv.second.total_ms = 0.;
+  }
+  return EIF_OK;
+}
+
+int eif_gpu_profile_query(eif_ctx *c, const char *name, long *count, double *total_ms) {
+  CHECK_CTX(c);
+  TRY(prof_flush(c));
+  auto it = c->prof->recs.find(name ? name : "");
+  if (count) *count = it == c->prof->recs.end() ? 0 : it->second.count;
+  if (total_ms) *total_ms = it == c->prof->recs.end() ? 0. : it->second.total_ms;
+  return EIF_OK;
+}
+
+}  // extern "C"

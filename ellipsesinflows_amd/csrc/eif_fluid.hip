@@ -1,0 +1,471 @@
+// eif_fluid.hip -- boundary conditions, RK predictor, projection/pressure
+// update, time-step size and diagnostics of the EllipsesInFlows fluid
+// solver on gfx950.
+//
+// Arithmetic follows NaokiHori/EllipsesInFlows expression by expression
+// (operand order, no contraction), so these kernels reproduce the
+// reference's rounding:
+//   boundary conditions  src/fluid/boundary_conditions.c:7-62
+//   predictor            src/fluid/update_velocity.c:9-237
+//   projection           src/fluid/correct_velocity.c:6-42
+//   pressure             src/fluid/update_pressure.c:7-24
+//   time-step size       src/param/decide_dt.c:11-81
+//   diagnostics          src/logging.c:40-172
+#include "eif_internal.hpp"
+
+namespace eif {
+
+// ---------------------------------------------------------------------
+// boundary conditions (one process: the y-halo exchange is a periodic
+// copy of rows ny -> 0 and 1 -> ny+1, src/parallel/halo.c:7-31)
+// ---------------------------------------------------------------------
+__global__ void k_bc_ux(Layout L, double *ux) {
+  const int nx = L.nx, ny = L.ny;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 2 && t <= nx) {  // halo rows, i = 2..nx
+    ux[L.ux(t, 0)] = ux[L.ux(t, ny)];
+    ux[L.ux(t, ny + 1)] = ux[L.ux(t, 1)];
+  }
+  if (t <= ny + 1) {  // impermeable walls
+    ux[L.ux(1, t)] = 0.;
+    ux[L.ux(nx + 1, t)] = 0.;
+  }
+}
+
+__global__ void k_bc_uy(Layout L, double *uy, double wxm, double wxp) {
+  const int nx = L.nx, ny = L.ny;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 1 && t <= nx) {
+    uy[L.p(t, 0)] = uy[L.p(t, ny)];
+    uy[L.p(t, ny + 1)] = uy[L.p(t, 1)];
+  }
+  if (t <= ny + 1) {  // no-slip (possibly moving) walls
+    uy[L.p(0, t)] = wxm;
+    uy[L.p(nx + 1, t)] = wxp;
+  }
+}
+
+// Neumann walls after the halo copy; the wall values of the halo rows
+// read the interior rows they were copied from, so one pass suffices.
+__global__ void k_bc_p(Layout L, double *p) {
+  const int nx = L.nx, ny = L.ny;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 1 && t <= nx) {
+    p[L.p(t, 0)] = p[L.p(t, ny)];
+    p[L.p(t, ny + 1)] = p[L.p(t, 1)];
+  }
+  if (t <= ny + 1) {
+    const int src = t == 0 ? ny : (t == ny + 1 ? 1 : t);
+    p[L.p(0, t)] = p[L.p(1, src)];
+    p[L.p(nx + 1, t)] = p[L.p(nx, src)];
+  }
+}
+
+static int bc_grid(const Ctx *c) {
+  const int n = (c->nx + 2 > c->ny + 2 ? c->nx + 2 : c->ny + 2);
+  return (n + 255) / 256;
+}
+
+int launch_bc_ux(Ctx *c, double *ux) {
+  ProfScope ps(c, "bc");
+  hipLaunchKernelGGL(k_bc_ux, dim3(bc_grid(c)), dim3(256), 0, c->stream, c->L, ux);
+  EIF_HIP(hipGetLastError());
+  return EIF_OK;
+}
+int launch_bc_uy(Ctx *c, double *uy) {
+  ProfScope ps(c, "bc");
+  hipLaunchKernelGGL(k_bc_uy, dim3(bc_grid(c)), dim3(256), 0, c->stream, c->L, uy,
+                     c->cfg.uy_wall_xm, c->cfg.uy_wall_xp);
+  EIF_HIP(hipGetLastError());
+  return EIF_OK;
+}
+int launch_bc_p(Ctx *c, double *p) {
+  ProfScope ps(c, "bc");
+  hipLaunchKernelGGL(k_bc_p, dim3(bc_grid(c)), dim3(256), 0, c->stream, c->L, p);
+  EIF_HIP(hipGetLastError());
+  return EIF_OK;
+}
+
+// ---------------------------------------------------------------------
+// predictor: compute_src_ux/uy + update_ux/uy + boundary conditions,
+// fused; reads (ux, uy, p, srcA) and writes (ux_new, uy_new, srcA).
+// Each thread owns column i and marches over PRED_ROWS rows, keeping the
+// three-row window of ux/uy in registers.
+// ---------------------------------------------------------------------
+constexpr int PRED_BX = 256;
+constexpr int PRED_ROWS = 16;
+
+struct PredArgs {
+  Layout L;
+  const double *ux, *uy, *p;
+  double *uxn, *uyn, *sa_x, *sa_y;
+  const double *dxf, *dxc;
+  const double *dscal;
+  double dy, Re, ext, al, be, ga, wxm, wxp;
+  int use_b;
+};
+
+__global__ __launch_bounds__(PRED_BX) void k_predictor(PredArgs a) {
+  const Layout L = a.L;
+  const int nx = L.nx, ny = L.ny;
+  const int i = blockIdx.x * PRED_BX + threadIdx.x + 1;  // 1..nx
+  const int j0 = blockIdx.y * PRED_ROWS + 1;
+  if (i > nx || j0 > ny) return;
+  const int j1 = min(j0 + PRED_ROWS - 1, ny);
+  const double dt = a.dscal[S_DT];
+  const double aldt = a.al * dt, bedt = a.be * dt, gadt = a.ga * dt;
+  const double dy = a.dy;
+  const double rre = 1. / a.Re;
+  // grid factors of this column
+  const bool hx = i >= 2;  // ux(i) is an unknown for i = 2..nx
+  const double dxf_im = hx ? a.dxf[i - 2] : 0.;  // DXF(i-1)
+  const double dxf_i = a.dxf[i - 1];             // DXF(i)
+  const double dxc_i = a.dxc[i - 1];             // DXC(i)
+  const double dxc_ip = a.dxc[i];                // DXC(i+1)
+  // ux window: columns i-1, i, i+1 (ux index i-1 exists for i >= 2)
+  const double *ux = a.ux, *uy = a.uy, *p = a.p;
+  auto ldux = [&](int ii, int jj) { return ux[L.ux(ii, jj)]; };
+  double uxm_w = hx ? ldux(i - 1, j0 - 1) : 0., uxm_c = ldux(i, j0 - 1), uxm_e = ldux(i + 1, j0 - 1);
+  double uxc_w = hx ? ldux(i - 1, j0) : 0., uxc_c = ldux(i, j0), uxc_e = ldux(i + 1, j0);
+  double uym_w = uy[L.p(i - 1, j0 - 1)], uym_c = uy[L.p(i, j0 - 1)], uym_e = uy[L.p(i + 1, j0 - 1)];
+  double uyc_w = uy[L.p(i - 1, j0)], uyc_c = uy[L.p(i, j0)], uyc_e = uy[L.p(i + 1, j0)];
+  double pm_c = p[L.p(i, j0 - 1)];
+  for (int j = j0; j <= j1; j++) {
+    const double uxp_w = hx ? ldux(i - 1, j + 1) : 0.;
+    const double uxp_c = ldux(i, j + 1), uxp_e = ldux(i + 1, j + 1);
+    const double uyp_w = uy[L.p(i - 1, j + 1)], uyp_c = uy[L.p(i, j + 1)],
+                 uyp_e = uy[L.p(i + 1, j + 1)];
+    const double pc_w = p[L.p(i - 1, j)], pc_c = p[L.p(i, j)];
+    // ---- x-momentum at face (i, j), src/fluid/update_velocity.c:28-88
+    if (hx) {
+      const double g_w = (-uxc_w + uxc_c) / dxf_im;
+      const double g_e = (-uxc_c + uxc_e) / dxf_i;
+      const double g_s = (-uxm_c + uxc_c) / dy;
+      const double g_n = (-uxc_c + uxp_c) / dy;
+      const double cw = dxf_im / (2. * dxc_i);
+      const double ce = dxf_i / (2. * dxc_i);
+      const double aw = 0.5 * uxc_w + 0.5 * uxc_c;
+      const double ae = 0.5 * uxc_c + 0.5 * uxc_e;
+      const double adv1 = -cw * aw * g_w - ce * ae * g_e;
+      const double vs = cw * uyc_w + ce * uyc_c;
+      const double vn = cw * uyp_w + ce * uyp_c;
+      const double adv2 = -0.5 * vs * g_s - 0.5 * vn * g_n;
+      const double dif1 = rre * (g_e - g_w) / dxc_i;
+      const double dif2 = rre * (g_n - g_s) / dy;
+      const double pre = -(pc_c - pc_w) / dxc_i;
+      const double A = (adv1 + adv2) + (dif1 + dif2);
+      const long si = L.sx(i, j);
+      const double bterm = a.use_b ? bedt * a.sa_x[si] : 0.;
+      a.sa_x[si] = A;
+      const double v = uxc_c + (aldt * A + bterm + gadt * pre);
+      a.uxn[L.ux(i, j)] = v;
+      if (j == ny) a.uxn[L.ux(i, 0)] = v;
+      if (j == 1) a.uxn[L.ux(i, ny + 1)] = v;
+    }
+    // ---- y-momentum at face (i, j), src/fluid/update_velocity.c:112-171
+    {
+      const double g_w = (-uyc_w + uyc_c) / dxc_i;
+      const double g_e = (-uyc_c + uyc_e) / dxc_ip;
+      const double g_s = (-uym_c + uyc_c) / dy;
+      const double g_n = (-uyc_c + uyp_c) / dy;
+      const double cw = dxc_i / (2. * dxf_i);
+      const double ce = dxc_ip / (2. * dxf_i);
+      const double uw = 0.5 * uxm_c + 0.5 * uxc_c;
+      const double ue = 0.5 * uxm_e + 0.5 * uxc_e;
+      const double adv1 = -cw * uw * g_w - ce * ue * g_e;
+      const double as = 0.5 * uym_c + 0.5 * uyc_c;
+      const double an = 0.5 * uyc_c + 0.5 * uyp_c;
+      const double adv2 = -0.5 * as * g_s - 0.5 * an * g_n;
+      const double dif1 = rre * (g_e - g_w) / dxf_i;
+      const double dif2 = rre * (g_n - g_s) / dy;
+      const double pre = -(pc_c - pm_c) / dy;
+      const double A = (adv1 + adv2) + (dif1 + dif2) + a.ext;
+      const long si = L.sy(i, j);
+      const double bterm = a.use_b ? bedt * a.sa_y[si] : 0.;
+      a.sa_y[si] = A;
+      const double v = uyc_c + (aldt * A + bterm + gadt * pre);
+      a.uyn[L.p(i, j)] = v;
+      if (j == ny) a.uyn[L.p(i, 0)] = v;
+      if (j == 1) a.uyn[L.p(i, ny + 1)] = v;
+    }
+    // walls of the new fields (boundary_conditions.c:19-22, :38-41)
+    if (i == 1) {
+      a.uxn[L.ux(1, j)] = 0.;
+      a.uyn[L.p(0, j)] = a.wxm;
+      if (j == 1) {
+        a.uxn[L.ux(1, 0)] = 0.;
+        a.uxn[L.ux(1, ny + 1)] = 0.;
+        a.uyn[L.p(0, 0)] = a.wxm;
+        a.uyn[L.p(0, ny + 1)] = a.wxm;
+      }
+    }
+    if (i == nx) {
+      a.uxn[L.ux(nx + 1, j)] = 0.;
+      a.uyn[L.p(nx + 1, j)] = a.wxp;
+      if (j == 1) {
+        a.uxn[L.ux(nx + 1, 0)] = 0.;
+        a.uxn[L.ux(nx + 1, ny + 1)] = 0.;
+        a.uyn[L.p(nx + 1, 0)] = a.wxp;
+        a.uyn[L.p(nx + 1, ny + 1)] = a.wxp;
+      }
+    }
+    // slide the window
+    uxm_w = uxc_w; uxm_c = uxc_c; uxm_e = uxc_e;
+    uxc_w = uxp_w; uxc_c = uxp_c; uxc_e = uxp_e;
+    uym_w = uyc_w; uym_c = uyc_c; uym_e = uyc_e;
+    uyc_w = uyp_w; uyc_c = uyp_c; uyc_e = uyp_e;
+    pm_c = pc_c;
+  }
+}
+
+int launch_predictor(Ctx *c, int rk) {
+  ProfScope ps(c, "predictor");
+  PredArgs a;
+  a.L = c->L;
+  a.ux = c->ux; a.uy = c->uy; a.p = c->p;
+  a.uxn = c->ux_alt; a.uyn = c->uy_alt;
+  a.sa_x = c->srcuxa; a.sa_y = c->srcuya;
+  a.dxf = c->dxf; a.dxc = c->dxc;
+  a.dscal = c->dscal;
+  a.dy = c->dy; a.Re = c->cfg.Re; a.ext = c->cfg.extfrcy;
+  a.al = c->alpha[rk]; a.be = c->beta[rk]; a.ga = c->gamma[rk];
+  a.wxm = c->cfg.uy_wall_xm; a.wxp = c->cfg.uy_wall_xp;
+  a.use_b = c->beta[rk] != 0.;
+  dim3 grid((c->nx + PRED_BX - 1) / PRED_BX, (c->ny + PRED_ROWS - 1) / PRED_ROWS);
+  hipLaunchKernelGGL(k_predictor, grid, dim3(PRED_BX), 0, c->stream, a);
+  EIF_HIP(hipGetLastError());
+  std::swap(c->ux, c->ux_alt);
+  std::swap(c->uy, c->uy_alt);
+  return EIF_OK;
+}
+
+// ---------------------------------------------------------------------
+// projection + pressure update, fused, in place
+// (correct_velocity.c:16-41, update_pressure.c:14-23) with the boundary
+// values of ux, uy and p refreshed by the threads that own the source
+// interior values.
+// ---------------------------------------------------------------------
+__global__ void k_correct_pressure(Layout L, double *ux, double *uy, double *p,
+                                   const double *psi, const double *dxc,
+                                   const double *dscal, double ga, double dy,
+                                   int do_correct, int do_pressure) {
+  const int nx = L.nx, ny = L.ny;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int j = blockIdx.y + 1;
+  if (i > nx || j > ny) return;
+  const double dt = dscal[S_DT];
+  const double ps_c = psi[L.p(i, j)];
+  if (do_correct) {
+    if (i >= 2) {
+      const double dxi = 1. / dxc[i - 1];
+      const long k = L.ux(i, j);
+      const double v = ux[k] - ga * dt * (ps_c - psi[L.p(i - 1, j)]) * dxi;
+      ux[k] = v;
+      if (j == ny) ux[L.ux(i, 0)] = v;
+      if (j == 1) ux[L.ux(i, ny + 1)] = v;
+    }
+    {
+      const long k = L.p(i, j);
+      const double v = uy[k] - ga * dt * (ps_c - psi[L.p(i, j - 1)]) / dy;
+      uy[k] = v;
+      if (j == ny) uy[L.p(i, 0)] = v;
+      if (j == 1) uy[L.p(i, ny + 1)] = v;
+    }
+  }
+  if (do_pressure) {
+    const long k = L.p(i, j);
+    const double v = p[k] + ps_c;
+    p[k] = v;
+    if (j == ny) p[L.p(i, 0)] = v;
+    if (j == 1) p[L.p(i, ny + 1)] = v;
+    if (i == 1) {
+      p[L.p(0, j)] = v;
+      if (j == ny) p[L.p(0, 0)] = v;
+      if (j == 1) p[L.p(0, ny + 1)] = v;
+    }
+    if (i == nx) {
+      p[L.p(nx + 1, j)] = v;
+      if (j == ny) p[L.p(nx + 1, 0)] = v;
+      if (j == 1) p[L.p(nx + 1, ny + 1)] = v;
+    }
+  }
+}
+
+int launch_correct_pressure(Ctx *c, int rk, bool correct, bool pressure) {
+  ProfScope ps(c, "correct_pressure");
+  dim3 grid((c->nx + 255) / 256, c->ny);
+  hipLaunchKernelGGL(k_correct_pressure, grid, dim3(256), 0, c->stream, c->L, c->ux,
+                     c->uy, c->p, c->psi, c->dxc, c->dscal, c->gamma[rk], c->dy,
+                     (int)correct, (int)pressure);
+  EIF_HIP(hipGetLastError());
+  return EIF_OK;
+}
+
+// ---------------------------------------------------------------------
+// time-step size, src/param/decide_dt.c:11-81.  Every candidate is a
+// positive finite double, so the IEEE bit pattern orders like the value
+// and an integer atomicMin gives the exact minimum in any order.
+// ---------------------------------------------------------------------
+__device__ inline void atomic_min_pos(double *addr, double v) {
+  atomicMin(reinterpret_cast<unsigned long long *>(addr),
+            (unsigned long long)__double_as_longlong(v));
+}
+__device__ inline void atomic_max_pos(double *addr, double v) {
+  atomicMax(reinterpret_cast<unsigned long long *>(addr),
+            (unsigned long long)__double_as_longlong(v));
+}
+
+__global__ void k_dt_init(double *dred) { dred[0] = 5.e-2; }
+
+__global__ __launch_bounds__(256) void k_dt_adv(Layout L, const double *ux,
+                                                const double *uy,
+                                                const double *dxc, double dy,
+                                                double *dred) {
+  const int nx = L.nx, ny = L.ny;
+  const double small = 1.e-8;
+  double m = 5.e-2;
+  const long n = (long)nx * ny;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n;
+       t += (long)gridDim.x * blockDim.x) {
+    const int i = (int)(t % nx) + 1, j = (int)(t / nx) + 1;
+    if (i >= 2) {
+      const double vel = fmax(fabs(ux[L.ux(i, j)]), small);
+      m = fmin(m, dxc[i - 1] / vel);
+    }
+    const double vel = fmax(fabs(uy[L.p(i, j)]), small);
+    m = fmin(m, dy / vel);
+  }
+  for (int off = 32; off > 0; off >>= 1) m = fmin(m, __shfl_down(m, off, 64));
+  if ((threadIdx.x & 63) == 0) atomic_min_pos(dred, m);
+}
+
+__global__ void k_dt_final(const double *dred, const double *prt, int np,
+                           double sf0, double sf2, double dt_dif_x,
+                           double dt_dif_y, double delta, double *dscal) {
+  __shared__ double red[256];
+  const double small = 1.e-8;
+  double dt_par = 1.7976931348623157e308;
+  for (int n = threadIdx.x; n < np; n += blockDim.x) {
+    const double *q = prt + (long)n * R_N;
+    const double pux = fmax(fabs(q[R_UX]) + fabs(fmax(q[R_A], q[R_B]) * q[R_VZ]), small);
+    const double puy = fmax(fabs(q[R_UY]) + fabs(fmax(q[R_A], q[R_B]) * q[R_VZ]), small);
+    const double val = fmax(pux, puy);
+    dt_par = fmin(dt_par, delta / val);
+  }
+  red[threadIdx.x] = dt_par;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double dt_adv = dred[0];
+    dt_adv *= sf0;
+    double dtp = red[0];
+    dtp *= sf2;
+    dscal[S_DT] = fmin(fmin(dt_adv, fmin(dt_dif_x, dt_dif_y)), dtp);
+  }
+}
+
+int launch_decide_dt(Ctx *c) {
+  ProfScope ps(c, "decide_dt");
+  hipLaunchKernelGGL(k_dt_init, dim3(1), dim3(1), 0, c->stream, c->dred);
+  const long n = (long)c->nx * c->ny;
+  int blocks = (int)((n + 255) / 256);
+  if (blocks > 2048) blocks = 2048;
+  hipLaunchKernelGGL(k_dt_adv, dim3(blocks), dim3(256), 0, c->stream, c->L, c->ux,
+                     c->uy, c->dxc, c->dy, c->dred);
+  // diffusive limits are grid constants (decide_dt.c:50-62)
+  double dxmin = c->lx;
+  for (int i = 2; i <= c->nx; i++) dxmin = fmin(dxmin, c->h_dxc[i - 1]);
+  double dt_dif_x = 0.25 * c->cfg.Re * (dxmin * dxmin);
+  double dt_dif_y = 0.25 * c->cfg.Re * (c->dy * c->dy);
+  dt_dif_x *= c->cfg.safefactors[1];
+  dt_dif_y *= c->cfg.safefactors[1];
+  const double delta = fmin(c->dx, c->dy);
+  hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(256), 0, c->stream, c->dred, c->prt,
+                     c->np, c->cfg.safefactors[0], c->cfg.safefactors[2], dt_dif_x,
+                     dt_dif_y, delta, c->dscal);
+  EIF_HIP(hipGetLastError());
+  return EIF_OK;
+}
+
+// ---------------------------------------------------------------------
+// diagnostics of src/logging.c (block sums in a fixed tree, then one
+// ordered pass over the block partials)
+// ---------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_diag(Layout L, const double *ux,
+                                              const double *uy,
+                                              const double *dxf,
+                                              const double *dxc, double dy,
+                                              int which, double *part) {
+  __shared__ double s0[256], s1[256];
+  const int nx = L.nx, ny = L.ny;
+  double a0 = 0., a1 = 0.;
+  const long n = (long)(nx + 1) * ny;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n;
+       t += (long)gridDim.x * blockDim.x) {
+    const int i = (int)(t % (nx + 1)) + 1, j = (int)(t / (nx + 1)) + 1;
+    if (which == 0) {  // divergence max / sum over i = 1..nx
+      if (i <= nx) {
+        const double div = (ux[L.ux(i + 1, j)] - ux[L.ux(i, j)]) / dxf[i - 1]
+                           + (uy[L.p(i, j + 1)] - uy[L.p(i, j)]) / dy;
+        a0 = fmax(a0, fabs(div));
+        a1 += div;
+      }
+    } else if (which == 1) {  // momentum
+      a0 += ux[L.ux(i, j)] * (dxc[i - 1] * dy);
+      if (i <= nx) a1 += uy[L.p(i, j)] * (dxf[i - 1] * dy);
+    } else {  // energy
+      const double u = ux[L.ux(i, j)];
+      a0 += 0.5 * (u * u) * (dxc[i - 1] * dy);
+      if (i <= nx) {
+        const double v = uy[L.p(i, j)];
+        a1 += 0.5 * (v * v) * (dxf[i - 1] * dy);
+      }
+    }
+  }
+  s0[threadIdx.x] = a0;
+  s1[threadIdx.x] = a1;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      s0[threadIdx.x] = which == 0 ? fmax(s0[threadIdx.x], s0[threadIdx.x + s])
+                                   : s0[threadIdx.x] + s0[threadIdx.x + s];
+      s1[threadIdx.x] += s1[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = s0[0];
+    part[2 * blockIdx.x + 1] = s1[0];
+  }
+}
+
+__global__ void k_diag_final(const double *part, int nb, int which, double *out) {
+  if (threadIdx.x != 0) return;
+  double a0 = 0., a1 = 0.;
+  for (int b = 0; b < nb; b++) {
+    a0 = which == 0 ? fmax(a0, part[2 * b]) : a0 + part[2 * b];
+    a1 += part[2 * b + 1];
+  }
+  out[0] = a0;
+  out[1] = a1;
+}
+
+int launch_diagnostics(Ctx *c, int which, double *out) {
+  const int nb = c->nred;
+  hipLaunchKernelGGL(k_diag, dim3(nb), dim3(256), 0, c->stream, c->L, c->ux, c->uy,
+                     c->dxf, c->dxc, c->dy, which, c->dred + 16);
+  hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(64), 0, c->stream, c->dred + 16, nb,
+                     which, c->dscal + S_DIVMAX);
+  EIF_HIP(hipGetLastError());
+  EIF_HIP(hipMemcpyAsync(c->hscal, c->dscal + S_DIVMAX, 2 * sizeof(double),
+                         hipMemcpyDeviceToHost, c->stream));
+  EIF_HIP(hipStreamSynchronize(c->stream));
+  out[0] = c->hscal[0];
+  out[1] = c->hscal[1];
+  return EIF_OK;
+}
+
+}  // namespace eif

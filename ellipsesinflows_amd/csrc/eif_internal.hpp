@@ -1,0 +1,164 @@
+// eif_internal.hpp -- shared definitions of the gfx950 EllipsesInFlows path.
+//
+// Everything is FP64, compiled with -ffp-contract=off so that expressions
+// round exactly like the reference's C99 build (ISO mode, no contraction).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstddef>
+#include <string>
+#include <vector>
+#include <map>
+#include "eif_gpu.h"
+
+// the reference's value of pi (include/common.h:7)
+#define EIF_PI 3.1415926535897932
+
+namespace eif {
+
+// ---------------------------------------------------------------------
+// Layouts of the reference (include/arrays/fluid.h, param.h,
+// suspensions.h) for one process owning all jtot rows.
+// ---------------------------------------------------------------------
+struct Layout {
+  int nx, ny;
+  __host__ __device__ long ux(int i, int j) const { return (long)j * (nx + 1) + (i - 1); }
+  __host__ __device__ long p(int i, int j) const { return (long)j * (nx + 2) + i; }
+  __host__ __device__ long sx(int i, int j) const { return (long)(j - 1) * (nx - 1) + (i - 2); }
+  __host__ __device__ long sy(int i, int j) const { return (long)(j - 1) * nx + (i - 1); }
+  long ux_len() const { return (long)(nx + 1) * (ny + 2); }
+  long p_len() const { return (long)(nx + 2) * (ny + 2); }
+  long sx_len() const { return (long)(nx - 1) * ny; }
+  long sy_len() const { return (long)nx * ny; }
+};
+
+// particle record member offsets (include/suspensions.h:8-31)
+enum {
+  R_DEN = 0, R_A, R_B, R_X, R_Y, R_AZ, R_DX, R_DY, R_DAZ,
+  R_UX, R_UY, R_VZ, R_DUX, R_DUY, R_DVZ, R_FUX, R_FUY, R_TVZ,
+  R_IUX = 18, R_IUY = 20, R_IVZ = 22, R_CFX = 24, R_CFY = 26, R_CTZ = 28,
+  R_N = 30
+};
+
+// device scalars (one small array, so graph-captured work reads them)
+enum { S_DT = 0, S_RESIDUAL = 1, S_DIVMAX = 2, S_DIVSUM = 3, S_NSCAL = 8 };
+
+// IBM box of one (particle, periodic image)
+struct Box {
+  int imin, imax, jmin, jmax;
+  long off;  // first cell of this box in the per-cell scratch arrays
+};
+
+// tile of the box-local deposit/apply step
+constexpr int TILE_X = 64;
+constexpr int TILE_Y = 4;
+constexpr int TILE_CAP = 48;     // boxes registered per tile
+constexpr int CONTACT_CAP = 64;  // colliding partners per particle
+constexpr int EQC_ITER_CAP = 100000;  // guard of the uncapped fixed point
+
+struct Profiler;
+
+struct Ctx {
+  eif_config cfg;
+  int nx, ny;
+  double lx, ly, dx, dy;
+  Layout L;
+  hipStream_t stream = nullptr;
+  int device = 0;
+  // host copies of the grid (exact reference arithmetic)
+  std::vector<double> h_xf, h_xc, h_dxf, h_dxc, h_yf, h_yc;
+  double alpha[3], beta[3], gamma[3];
+  // device grid
+  double *xf = nullptr, *xc = nullptr, *dxf = nullptr, *dxc = nullptr,
+         *yf = nullptr, *yc = nullptr;
+  // fluid fields (reference layouts); *_alt are ping-pong partners
+  double *ux = nullptr, *uy = nullptr, *p = nullptr, *psi = nullptr;
+  double *ux_alt = nullptr, *uy_alt = nullptr;
+  double *srcuxa = nullptr, *srcuya = nullptr;
+  double *dux = nullptr, *duy = nullptr;  // materialised only on request
+  // Poisson
+  double *qhat = nullptr;      // spectral field, x-contiguous rows [ny][nx]
+  double *ctab = nullptr;      // C[m] = cos(pi m/(2 nx)), m=0..2nx
+  double *th_val = nullptr;    // Thomas tables [ny-1][nx] (forward factors)
+  double *th_u = nullptr;      // [ny-1][nx] modified upper diagonal
+  double *th_q1 = nullptr;     // [ny-1][nx] border solution
+  double *th_den = nullptr;    // [nx]
+  double *th_last = nullptr;   // [nx] q[n-1] of the current solve
+  unsigned char *th_flag = nullptr;  // [ny-1][nx] zero-pivot flags
+  // partition solver tables
+  int part_L = 0, part_P = 0;
+  double *pt_fwd = nullptr, *pt_u = nullptr, *pt_v = nullptr, *pt_w = nullptr;
+  double *pt_bnd = nullptr, *pt_red = nullptr;
+  // scalars
+  double *dscal = nullptr;
+  double *hscal = nullptr;  // pinned
+  double *dred = nullptr;   // reduction partials
+  int nred = 0;
+  // particles
+  int np = 0;
+  double *prt = nullptr;       // np x R_N records
+  double *pconst = nullptr;    // per particle: mass, moi
+  Box *boxes = nullptr;        // np x 3
+  long *box_off = nullptr;     // np+1 offsets into per-cell scratch
+  long scratch_cells = 0;
+  int box_cap_x = 0, box_cap_y = 0;
+  double *sc_fx = nullptr, *sc_fy = nullptr;  // per box cell forcing*dt
+  double *sc_dist = nullptr;                  // cached signed distance
+  double *sc_pos = nullptr;                   // np x 3: positions of cache
+  int *tile_cnt = nullptr, *tile_list = nullptr;
+  int ntx = 0, nty = 0;
+  int *active = nullptr;   // [0] count, then tile ids
+  int *flags = nullptr;    // device error flags
+  // collisions
+  int *pair_cnt = nullptr;
+  int pair_cap = 0;
+  int *pair_ids = nullptr;        // pair_cap x 2
+  double *pair_val = nullptr;     // pair_cap x 6 contributions
+  double *wall_val = nullptr;     // np x 2 walls x 3
+  int *wall_hit = nullptr;        // np x 2
+  int last_npairs = 0;
+  Profiler *prof = nullptr;
+};
+
+// error state
+void set_error(const std::string &msg);
+int hip_fail(hipError_t e, const char *what, const char *file, int line);
+
+#define EIF_HIP(call)                                                     \
+  do {                                                                    \
+    hipError_t _e = (call);                                               \
+    if (_e != hipSuccess) return ::eif::hip_fail(_e, #call, __FILE__, __LINE__); \
+  } while (0)
+
+// profiling helpers (host)
+void prof_begin(Ctx *c, const char *name);
+void prof_end(Ctx *c, const char *name);
+struct ProfScope {
+  Ctx *c;
+  const char *n;
+  ProfScope(Ctx *c_, const char *n_) : c(c_), n(n_) { prof_begin(c, n); }
+  ~ProfScope() { prof_end(c, n); }
+};
+
+// launchers implemented in the .hip files
+int launch_bc_ux(Ctx *c, double *ux);
+int launch_bc_uy(Ctx *c, double *uy);
+int launch_bc_p(Ctx *c, double *p);
+int launch_predictor(Ctx *c, int rk);
+int launch_correct_pressure(Ctx *c, int rk, bool correct, bool pressure);
+int launch_decide_dt(Ctx *c);
+int launch_diagnostics(Ctx *c, int which, double *out);
+int poisson_init(Ctx *c);
+int launch_poisson(Ctx *c, int rk);
+int susp_init(Ctx *c);
+int launch_inertia(Ctx *c, int cnstep);
+int launch_exchange(Ctx *c);
+int launch_update_momentum(Ctx *c);
+int launch_materialise_du(Ctx *c);
+int launch_collision(Ctx *c, int cnstep);
+int launch_increment(Ctx *c, int rk);
+int launch_update_particles(Ctx *c);
+int launch_reset(Ctx *c);
+int check_flags(Ctx *c);
+
+}  // namespace eif

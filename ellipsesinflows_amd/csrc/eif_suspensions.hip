@@ -1,0 +1,1020 @@
+// eif_suspensions.hip -- ellipses coupled to the fluid on gfx950:
+// Eulerian IBM weights, inertia integrals, momentum exchange and its
+// box-local deposit, soft-sphere collisions, Crank-Nicolson increments.
+//
+// Per-element arithmetic follows NaokiHori/EllipsesInFlows:
+//   ellipse geometry      src/ellipse.c:7-65
+//   IBM weights / boxes   src/suspensions/auxiliary.c:11-75
+//   inertia               src/suspensions/inertia.c:9-70
+//   momentum exchange     src/suspensions/exchange.c:13-88
+//   field update          src/suspensions/update.c:36-59
+//   collisions            src/suspensions/collision.c:75-367
+//   increments / update   src/suspensions/increment.c:10-72, update.c:8-34
+//
+// Work decomposition (gfx950):
+//   * one workgroup per particle for the IBM integrals; its three periodic
+//     images' boxes are flattened and strided over the workgroup, and the
+//     per-particle sums are reduced in a fixed tree;
+//   * the signed distance of every box cell is cached per particle with the
+//     position it belongs to; the reference evaluates it three times at
+//     identical positions in every RK stage (inertia(0), exchange and the
+//     first inertia(1), src/main.c:21-38), the cache serves the repeats with
+//     the same bits;
+//   * the exchange writes each box cell's forcing to box-local scratch and
+//     registers the box with the 64x4 tiles it touches; a gather over active
+//     tiles then adds the contributions cell by cell in the reference's
+//     particle/image order and applies them to ux/uy (no atomics, no
+//     full-field pass);
+//   * collisions: one thread per particle pair finds the colliding pairs,
+//     one thread per particle gathers its partners' forces in the
+//     reference's pair order.
+#include "eif_internal.hpp"
+#include <cfloat>
+#include <cmath>
+
+namespace eif {
+
+#define NEXTRA 3
+#define BETA 2.
+
+// x^3 to nearly correct rounding (error-free square, then product), the
+// value glibc's pow(x, 3.) returns.
+__device__ inline double cube(double x) {
+  const double p = x * x;
+  const double e = fma(x, x, -p);
+  const double h = p * x;
+  const double eh = fma(p, x, -h);
+  return h + (eh + e * x);
+}
+
+// src/suspensions/auxiliary.c:11-17
+__device__ inline void loop_size(int lb, int ub, double h, double r, double c,
+                                 int &mn, int &mx) {
+  mn = (int)(floor((c - r) / h) - NEXTRA);
+  mx = (int)(ceil((c + r) / h) + NEXTRA);
+  mn = mn < lb ? lb : mn;
+  mx = mx > ub ? ub : mx;
+}
+
+struct EllipseShape {
+  double a, b;
+  double kex, key;  // a(1-(b/a)^2), b(1-(a/b)^2)   (ellipse.c:7-13)
+  double a2b2;      // a*a + b*b                     (ellipse.c:49)
+};
+
+__device__ inline EllipseShape make_shape(double a, double b) {
+  EllipseShape s;
+  s.a = a;
+  s.b = b;
+  const double ba = b / a, ab = a / b;
+  s.kex = a * (1. - ba * ba);
+  s.key = b * (1. - ab * ab);
+  s.a2b2 = a * a + b * b;
+  return s;
+}
+
+// src/ellipse.c:30-65
+__device__ double find_normal_t(const EllipseShape &e, double x_, double y_) {
+  const double px = fabs(x_), py = fabs(y_);
+  double t = 0.25 * EIF_PI;
+  for (int n = 0;; n++) {
+    double s, c;
+    sincos(t, &s, &c);
+    const double x = e.a * c, y = e.b * s;
+    const double ex = e.kex * cube(c), ey = e.key * cube(s);
+    const double rx = x - ex, ry = y - ey;
+    const double qx = px - ex, qy = py - ey;
+    const double r = sqrt(ry * ry + rx * rx);
+    const double q = sqrt(qy * qy + qx * qx);
+    const double dc = r * asin((rx * qy - ry * qx) / (r * q));
+    const double dt = dc / sqrt(e.a2b2 - x * x - y * y);
+    t += dt;
+    t = fmin(0.5 * EIF_PI, t);
+    t = fmax(0., t);
+    if (fabs(dt) < 1.e-8) break;
+    if (n > 100) break;
+  }
+  if (x_ < 0.) t = EIF_PI - t;
+  if (y_ < 0.) t = -t;
+  return t;
+}
+
+// compute_signed_dist, src/suspensions/auxiliary.c:19-47; (cr, sr) are
+// cos(-paz), sin(-paz)
+__device__ inline double signed_dist(const EllipseShape &e, double grid_size,
+                                     double px, double py, double cr, double sr,
+                                     double x, double y) {
+  const double ddx = x - px, ddy = y - py;
+  const double x_ = cr * ddx - sr * ddy;
+  const double y_ = sr * ddx + cr * ddy;
+  const double xa = x_ / e.a, yb = y_ / e.b;
+  const double f0 = 1. - xa * xa - yb * yb;
+  const double sign = f0 > 0. ? +1. : -1.;
+  const double t = find_normal_t(e, x_, y_);
+  double st, ct;
+  sincos(t, &st, &ct);
+  const double ex = e.a * ct - x_, ey = e.b * st - y_;
+  const double dist = sign * sqrt(+ex * ex + ey * ey);
+  return dist / grid_size;
+}
+
+__device__ inline double v_weight(double d) { return 0.5 * (1. + tanh(BETA * d)); }
+__device__ inline double s_weight(double d) {
+  const double th = tanh(BETA * d);
+  return 0.5 * BETA * (1. - th * th);
+}
+
+// mass and moment of inertia, src/suspensions/auxiliary.c:63-75
+__device__ inline double p_mass(const double *q) {
+  const double vol = EIF_PI * q[R_A] * q[R_B];
+  return q[R_DEN] * vol;
+}
+__device__ inline double p_moi(const double *q) {
+  const double m = p_mass(q);
+  return 0.25 * m * (q[R_A] * q[R_A] + q[R_B] * q[R_B]);
+}
+
+// ---------------------------------------------------------------------
+// IBM integrals
+// ---------------------------------------------------------------------
+constexpr int IBM_THREADS = 256;
+
+struct IbmArgs {
+  Layout L;
+  int nx, ny;
+  double ly, dx, dy, grid_size;
+  const double *xc, *yc;
+  const double *ux, *uy;
+  double *prt;
+  const double *dscal;
+  Box *boxes;
+  const long *box_off;
+  double *sc_fx, *sc_fy, *sc_dist, *sc_pos;
+  int *tile_cnt, *tile_list, *active;
+  int ntx, nty;
+  int *flags;
+};
+
+template <int N>
+__device__ inline void block_sum(double (&v)[N], double *sh) {
+  // fixed-order tree over IBM_THREADS partial sums
+  for (int k = 0; k < N; k++) sh[k * IBM_THREADS + threadIdx.x] = v[k];
+  __syncthreads();
+  for (int s = IBM_THREADS / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s)
+      for (int k = 0; k < N; k++)
+        sh[k * IBM_THREADS + threadIdx.x] =
+            sh[k * IBM_THREADS + threadIdx.x] + sh[k * IBM_THREADS + threadIdx.x + s];
+    __syncthreads();
+  }
+  for (int k = 0; k < N; k++) v[k] = sh[k * IBM_THREADS];
+}
+
+struct BoxSet {
+  int imin[3], imax[3], jmin[3], jmax[3];
+  long cnt[3];
+  long total;
+};
+
+__device__ inline void make_boxes(const IbmArgs &a, double px, double py,
+                                  double r, BoxSet &bs) {
+  bs.total = 0;
+  for (int k = 0; k < 3; k++) {
+    const double py_ = py + a.ly * (k - 1);
+    loop_size(1, a.nx, a.dx, r, px, bs.imin[k], bs.imax[k]);
+    loop_size(1, a.ny, a.dy, r, py_ - 0., bs.jmin[k], bs.jmax[k]);
+    const long w = bs.imax[k] - bs.imin[k] + 1, h = bs.jmax[k] - bs.jmin[k] + 1;
+    bs.cnt[k] = (w > 0 && h > 0) ? w * h : 0;
+    bs.total += bs.cnt[k];
+  }
+}
+
+__device__ inline void cell_of(const BoxSet &bs, long t, int &k, int &i, int &j) {
+  k = 0;
+  while (t >= bs.cnt[k]) {
+    t -= bs.cnt[k];
+    k++;
+  }
+  const int w = bs.imax[k] - bs.imin[k] + 1;
+  j = bs.jmin[k] + (int)(t / w);
+  i = bs.imin[k] + (int)(t % w);
+}
+
+// src/suspensions/inertia.c:9-70
+__global__ __launch_bounds__(IBM_THREADS) void k_inertia(IbmArgs a, int cnstep) {
+  __shared__ double sh[3 * IBM_THREADS];
+  __shared__ int use_cache;
+  const int n = blockIdx.x;
+  double *q = a.prt + (long)n * R_N;
+  const double pa = q[R_A], pb = q[R_B];
+  const double px = q[R_X] + q[R_DX];
+  const double py = q[R_Y] + q[R_DY];
+  const double paz = q[R_AZ] + q[R_DAZ];
+  const double pm = p_mass(q), pim = p_moi(q);
+  const EllipseShape e = make_shape(pa, pb);
+  BoxSet bs;
+  make_boxes(a, px, py, fmax(pa, pb), bs);
+  const long cap = a.box_off[n + 1] - a.box_off[n];
+  if (bs.total > cap) {
+    if (threadIdx.x == 0) atomicOr(&a.flags[0], 1);
+    return;
+  }
+  if (threadIdx.x == 0) {
+    const double *cp = a.sc_pos + 4 * (long)n;
+    use_cache = cp[3] != 0. && cp[0] == px && cp[1] == py && cp[2] == paz;
+  }
+  __syncthreads();
+  double *dist = a.sc_dist + a.box_off[n];
+  double cr, sr;
+  sincos(-paz, &sr, &cr);
+  const double dxdy = a.dx * a.dy;
+  double acc[3] = {0., 0., 0.};  // iux, iuy, ivz
+  for (long t = threadIdx.x; t < bs.total; t += IBM_THREADS) {
+    int k, i, j;
+    cell_of(bs, t, k, i, j);
+    const double py_ = py + a.ly * (k - 1);
+    const double x = a.xc[i], y = a.yc[j - 1];
+    double d;
+    if (use_cache) {
+      d = dist[t];
+    } else {
+      d = signed_dist(e, a.grid_size, px, py_, cr, sr, x, y);
+      dist[t] = d;
+    }
+    const double w = v_weight(d);
+    const double valx = w * 0.5 * (a.ux[a.L.ux(i, j)] + a.ux[a.L.ux(i + 1, j)]) * dxdy;
+    const double valy = w * 0.5 * (a.uy[a.L.p(i, j)] + a.uy[a.L.p(i, j + 1)]) * dxdy;
+    acc[0] += valx / pm;
+    acc[2] += -(y - py_) * valx / pim;
+    acc[1] += valy / pm;
+    acc[2] += +(x - px) * valy / pim;
+  }
+  block_sum<3>(acc, sh);
+  if (threadIdx.x == 0) {
+    q[R_IUX + cnstep] = acc[0];
+    q[R_IUY + cnstep] = acc[1];
+    q[R_IVZ + cnstep] = acc[2];
+    double *cp = a.sc_pos + 4 * (long)n;
+    cp[0] = px;
+    cp[1] = py;
+    cp[2] = paz;
+    cp[3] = 1.;
+  }
+}
+
+// registers the cells a box's forcing reaches in the field update with the
+// 64x4 tiles (update.c:42-57: UX(i) reads DUX(i-1), UY(j) reads DUY(j-1),
+// and DUY(i,0) is the periodic copy of DUY(i,jtot))
+__device__ void register_tiles(const IbmArgs &a, int box_id, int imin, int imax,
+                               int jmin, int jmax) {
+  auto reg = [&](int i_lo, int i_hi, int j_lo, int j_hi) {
+    const int tx0 = (i_lo - 1) / TILE_X, tx1 = (i_hi - 1) / TILE_X;
+    const int ty0 = (j_lo - 1) / TILE_Y, ty1 = (j_hi - 1) / TILE_Y;
+    for (int ty = ty0; ty <= ty1; ty++)
+      for (int tx = tx0; tx <= tx1; tx++) {
+        const int tile = ty * a.ntx + tx;
+        const int slot = atomicAdd(&a.tile_cnt[tile], 1);
+        if (slot == 0) {
+          const int pos = atomicAdd(&a.active[0], 1);
+          a.active[1 + pos] = tile;
+        }
+        if (slot < TILE_CAP) {
+          a.tile_list[(long)tile * TILE_CAP + slot] = box_id;
+        } else {
+          atomicOr(&a.flags[0], 2);
+        }
+      }
+  };
+  const int i_hi = imax + 1 > a.nx ? a.nx : imax + 1;
+  const int j_hi = jmax + 1 > a.ny ? a.ny : jmax + 1;
+  reg(imin, i_hi, jmin, j_hi);
+  if (jmax == a.ny && jmin > 1) reg(imin, imax, 1, 1);
+}
+
+// src/suspensions/exchange.c:13-88
+__global__ __launch_bounds__(IBM_THREADS) void k_exchange(IbmArgs a) {
+  __shared__ double sh[3 * IBM_THREADS];
+  __shared__ int use_cache;
+  const int n = blockIdx.x;
+  double *q = a.prt + (long)n * R_N;
+  const double dt = a.dscal[S_DT];
+  const double pa = q[R_A], pb = q[R_B];
+  const double pm = p_mass(q), pim = p_moi(q);
+  const double px = q[R_X], py = q[R_Y], paz = q[R_AZ];
+  const double pux = q[R_UX], puy = q[R_UY], pvz = q[R_VZ];
+  const EllipseShape e = make_shape(pa, pb);
+  BoxSet bs;
+  make_boxes(a, px, py, fmax(pa, pb), bs);
+  const long cap = a.box_off[n + 1] - a.box_off[n];
+  if (bs.total > cap) {
+    if (threadIdx.x == 0) atomicOr(&a.flags[0], 1);
+    return;
+  }
+  if (threadIdx.x < 3) {
+    const int k = threadIdx.x;
+    long off = a.box_off[n];
+    for (int kk = 0; kk < k; kk++) off += bs.cnt[kk];
+    Box b;
+    b.imin = bs.imin[k];
+    b.imax = bs.imax[k];
+    b.jmin = bs.jmin[k];
+    b.jmax = bs.jmax[k];
+    b.off = off;
+    a.boxes[(long)n * 3 + k] = b;
+    if (bs.cnt[k] > 0) register_tiles(a, n * 3 + k, b.imin, b.imax, b.jmin, b.jmax);
+  }
+  if (threadIdx.x == 0) {
+    const double *cp = a.sc_pos + 4 * (long)n;
+    use_cache = cp[3] != 0. && cp[0] == px && cp[1] == py && cp[2] == paz;
+  }
+  __syncthreads();
+  double *dist = a.sc_dist + a.box_off[n];
+  double *fxo = a.sc_fx + a.box_off[n];
+  double *fyo = a.sc_fy + a.box_off[n];
+  double cr, sr;
+  sincos(-paz, &sr, &cr);
+  const double dx = a.dx, dy = a.dy;
+  double acc[3] = {0., 0., 0.};  // fux, fuy, tvz
+  for (long t = threadIdx.x; t < bs.total; t += IBM_THREADS) {
+    int k, i, j;
+    cell_of(bs, t, k, i, j);
+    const double py_ = py + a.ly * (k - 1);
+    const double x = a.xc[i], y = a.yc[j - 1];
+    double d;
+    if (use_cache) {
+      d = dist[t];
+    } else {
+      d = signed_dist(e, a.grid_size, px, py_, cr, sr, x, y);
+      dist[t] = d;
+    }
+    const double w = s_weight(d);
+    const double ux_p = pux - pvz * (y - py_);
+    const double uy_p = puy + pvz * (x - px);
+    const double ux_f = 0.5 * (a.ux[a.L.ux(i, j)] + a.ux[a.L.ux(i + 1, j)]);
+    const double uy_f = 0.5 * (a.uy[a.L.p(i, j)] + a.uy[a.L.p(i, j + 1)]);
+    const double fx = w * (ux_p - ux_f) / dt;
+    const double fy = w * (uy_p - uy_f) / dt;
+    fxo[t] = fx * dt;
+    fyo[t] = fy * dt;
+    acc[0] -= +(fx * dx * dy) / pm;
+    acc[2] -= -(y - py_) * (fx * dx * dy) / pim;
+    acc[1] -= +(fy * dx * dy) / pm;
+    acc[2] -= +(x - px) * (fy * dx * dy) / pim;
+  }
+  block_sum<3>(acc, sh);
+  if (threadIdx.x == 0) {
+    q[R_FUX] = acc[0];
+    q[R_FUY] = acc[1];
+    q[R_TVZ] = acc[2];
+    double *cp = a.sc_pos + 4 * (long)n;
+    cp[0] = px;
+    cp[1] = py;
+    cp[2] = paz;
+    cp[3] = 1.;
+  }
+}
+
+// ---------------------------------------------------------------------
+// deposit + apply over active tiles (update.c:36-59)
+// ---------------------------------------------------------------------
+struct TileArgs {
+  Layout L;
+  int nx, ny, ntx;
+  const Box *boxes;
+  const double *sc_fx, *sc_fy;
+  const int *tile_cnt, *tile_list, *active;
+  double *ux, *uy;
+  double *dux, *duy;  // materialisation target (or null)
+  int apply;
+};
+
+__device__ inline double box_value(const Box &b, const double *sc, int i, int j) {
+  if (i < b.imin || i > b.imax || j < b.jmin || j > b.jmax) return 0.;
+  const long w = b.imax - b.imin + 1;
+  return sc[b.off + (long)(j - b.jmin) * w + (i - b.imin)];
+}
+
+__global__ __launch_bounds__(TILE_X * TILE_Y) void k_tiles(TileArgs a) {
+  __shared__ int list[TILE_CAP];
+  __shared__ int nlist;
+  __shared__ double dxr[(TILE_X + 1) * TILE_Y];
+  __shared__ double dyr[TILE_X * (TILE_Y + 1)];
+  const int nact = a.active[0];
+  const int tid = threadIdx.x;
+  for (int at = blockIdx.x; at < nact; at += gridDim.x) {
+    const int tile = a.active[1 + at];
+    const int tx = tile % a.ntx, ty = tile / a.ntx;
+    const int i0 = tx * TILE_X + 1, j0 = ty * TILE_Y + 1;
+    if (tid == 0) {
+      int cnt = a.tile_cnt[tile];
+      cnt = cnt > TILE_CAP ? TILE_CAP : cnt;
+      for (int s = 0; s < cnt; s++) list[s] = a.tile_list[(long)tile * TILE_CAP + s];
+      // insertion sort (particle index, then image: the reference's order)
+      for (int s = 1; s < cnt; s++) {
+        const int v = list[s];
+        int r = s - 1;
+        while (r >= 0 && list[r] > v) {
+          list[r + 1] = list[r];
+          r--;
+        }
+        list[r + 1] = v;
+      }
+      int m = 0;
+      for (int s = 0; s < cnt; s++)
+        if (m == 0 || list[m - 1] != list[s]) list[m++] = list[s];
+      nlist = m;
+    }
+    __syncthreads();
+    const int nl = nlist;
+    // DUX on (i0-1 .. i0+63) x (j0 .. j0+3)
+    for (int e = tid; e < (TILE_X + 1) * TILE_Y; e += blockDim.x) {
+      const int ii = i0 - 1 + e % (TILE_X + 1), jj = j0 + e / (TILE_X + 1);
+      double v = 0.;
+      if (ii >= 1 && ii <= a.nx && jj <= a.ny)
+        for (int s = 0; s < nl; s++) {
+          const Box b = a.boxes[list[s]];
+          v += box_value(b, a.sc_fx, ii, jj);
+        }
+      dxr[e] = v;
+    }
+    // DUY on (i0 .. i0+63) x (j0-1 .. j0+3), row 0 is row jtot
+    for (int e = tid; e < TILE_X * (TILE_Y + 1); e += blockDim.x) {
+      const int ii = i0 + e % TILE_X;
+      int jj = j0 - 1 + e / TILE_X;
+      if (jj == 0) jj = a.ny;
+      double v = 0.;
+      if (ii <= a.nx && jj <= a.ny)
+        for (int s = 0; s < nl; s++) {
+          const Box b = a.boxes[list[s]];
+          v += box_value(b, a.sc_fy, ii, jj);
+        }
+      dyr[e] = v;
+    }
+    __syncthreads();
+    const int li = tid % TILE_X, lj = tid / TILE_X;
+    const int i = i0 + li, j = j0 + lj;
+    if (i <= a.nx && j <= a.ny) {
+      const double dux_c = dxr[lj * (TILE_X + 1) + li + 1];
+      const double dux_w = dxr[lj * (TILE_X + 1) + li];
+      const double duy_c = dyr[(lj + 1) * TILE_X + li];
+      const double duy_s = dyr[lj * TILE_X + li];
+      if (a.apply) {
+        const Layout L = a.L;
+        if (i >= 2) {
+          const long k = L.ux(i, j);
+          const double v = a.ux[k] + 0.5 * (dux_w + dux_c);
+          a.ux[k] = v;
+          if (j == a.ny) a.ux[L.ux(i, 0)] = v;
+          if (j == 1) a.ux[L.ux(i, a.ny + 1)] = v;
+        }
+        {
+          const long k = L.p(i, j);
+          const double v = a.uy[k] + 0.5 * (duy_s + duy_c);
+          a.uy[k] = v;
+          if (j == a.ny) a.uy[L.p(i, 0)] = v;
+          if (j == 1) a.uy[L.p(i, a.ny + 1)] = v;
+        }
+      } else {
+        a.dux[a.L.p(i, j)] = dux_c;
+        a.duy[a.L.p(i, j)] = duy_c;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------
+// collisions, src/suspensions/collision.c:75-367
+// ---------------------------------------------------------------------
+struct EllD {
+  double a, b, x, y, angle;
+};
+
+__device__ double curvature(double a, double b, double t) {
+  double st, ct;
+  sincos(t, &st, &ct);
+  const double ct2 = ct * ct, st2 = st * st;
+  const double a2 = a * a, b2 = b * b;
+  const double num = a * b * (ct2 + st2);
+  const double den = pow(+a2 * st2 + b2 * ct2, 1.5);
+  return num / den;
+}
+
+__device__ void evolute_fit(const EllD &e, double x, double y, double &ex,
+                            double &ey, double &ix, double &iy, double &r) {
+  double sm, cm, sp, cp;
+  sincos(-e.angle, &sm, &cm);
+  sincos(+e.angle, &sp, &cp);
+  double x_, y_;
+  {
+    const double x__ = x - e.x, y__ = y - e.y;
+    x_ = cm * x__ - sm * y__;
+    y_ = sm * x__ + cm * y__;
+  }
+  const EllipseShape s = make_shape(e.a, e.b);
+  const double t = find_normal_t(s, x_, y_);
+  r = 1. / fmax(curvature(e.a, e.b, t), DBL_EPSILON);
+  double st, ct;
+  sincos(t, &st, &ct);
+  x_ = s.kex * cube(ct);
+  y_ = s.key * cube(st);
+  ex = cp * x_ - sp * y_;
+  ey = sp * x_ + cp * y_;
+  ex = ex + e.x;
+  ey = ey + e.y;
+  x_ = e.a * ct;
+  y_ = e.b * st;
+  ix = cp * x_ - sp * y_;
+  iy = sp * x_ + cp * y_;
+}
+
+struct CollArgs {
+  int np;
+  double lx, ly;
+  const double *prt;
+  int cnstep;
+  int *pair_cnt;
+  int pair_cap;
+  int *pair_ids;
+  double *pair_val;
+  double *wall_val;
+  int *wall_hit;
+  int *flags;
+};
+
+__device__ inline double harmonic(double v0, double v1) {
+  const double r = 0.5 * (1. / v0 + 1. / v1);
+  return 1. / r;
+}
+
+__global__ void k_collide_pairs(CollArgs a) {
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long np = a.np;
+  if (t >= np * np) return;
+  const int n0 = (int)(t / np), n1 = (int)(t % np);
+  if (n1 <= n0) return;
+  const double *p0 = a.prt + (long)n0 * R_N, *p1 = a.prt + (long)n1 * R_N;
+  double yoffset = 0.;
+  {
+    const double p0y = p0[R_Y] + p0[R_DY];
+    const double p1y = p1[R_Y] + p1[R_DY];
+    double minval = DBL_MAX;
+    for (int periodic = -1; periodic <= 1; periodic++) {
+      const double val = fabs((p1y + a.ly * periodic) - p0y);
+      if (val < minval) {
+        minval = val;
+        yoffset = a.ly * periodic;
+      }
+    }
+  }
+  {
+    const double x0 = p0[R_X] + p0[R_DX], y0 = p0[R_Y] + p0[R_DY];
+    const double r0 = fmax(p0[R_A], p0[R_B]);
+    const double x1 = p1[R_X] + p1[R_DX], y1 = p1[R_Y] + p1[R_DY] + yoffset;
+    const double r1 = fmax(p1[R_A], p1[R_B]);
+    const double norm = fmax(hypot(x1 - x0, y1 - y0), DBL_EPSILON);
+    if (r0 + r1 - norm < 0.) return;
+  }
+  const EllD e0 = {p0[R_A], p0[R_B], p0[R_X] + p0[R_DX], p0[R_Y] + p0[R_DY], p0[R_AZ]};
+  const EllD e1 = {p1[R_A], p1[R_B], p1[R_X] + p1[R_DX], p1[R_Y] + p1[R_DY] + yoffset,
+                   p1[R_AZ]};
+  // find_equivalent_circles, collision.c:111-135 (guarded iteration count)
+  double c0x = e0.x, c0y = e0.y, c1x = e1.x, c1y = e1.y, c0r = 0., c1r = 0.;
+  double ix0 = 0., iy0 = 0., ix1 = 0., iy1 = 0.;
+  for (int it = 0;; it++) {
+    double n0x, n0y, n1x, n1y;
+    evolute_fit(e0, c1x, c1y, n0x, n0y, ix0, iy0, c0r);
+    evolute_fit(e1, c0x, c0y, n1x, n1y, ix1, iy1, c1r);
+    const double d0x = n0x - c0x, d0y = n0y - c0y;
+    const double d1x = n1x - c1x, d1y = n1y - c1y;
+    c0x += d0x;
+    c0y += d0y;
+    c1x += d1x;
+    c1y += d1y;
+    if (fmax(fmax(fabs(d0x), fabs(d0y)), fmax(fabs(d1x), fabs(d1y))) < 1.e-8) break;
+    if (it >= EQC_ITER_CAP) {
+      atomicOr(&a.flags[0], 8);
+      break;
+    }
+  }
+  const double m0 = p_mass(p0), m1 = p_mass(p1);
+  const double i0 = p_moi(p0), i1 = p_moi(p1);
+  double k;
+  {
+    const double mass = harmonic(m0, m1);
+    const double r0 = sqrt(p0[R_A] * p0[R_B]);
+    const double r1 = sqrt(p1[R_A] * p1[R_B]);
+    const double reft = pow(harmonic(r0, r1), 1.5);
+    k = mass * (EIF_PI * EIF_PI) / (reft * reft);
+  }
+  double nxv = c1x - c0x, nyv = c1y - c0y;
+  const double norm = fmax(hypot(nxv, nyv), DBL_EPSILON);
+  nxv /= norm;
+  nyv /= norm;
+  const double overlap = c0r + c1r - norm;
+  if (overlap > 0.) {
+    const double cfx = k * overlap * nxv;
+    const double cfy = k * overlap * nyv;
+    const int slot = atomicAdd(a.pair_cnt, 1);
+    if (slot >= a.pair_cap) {
+      atomicOr(&a.flags[0], 4);
+      return;
+    }
+    a.pair_ids[2 * slot] = n0;
+    a.pair_ids[2 * slot + 1] = n1;
+    double *v = a.pair_val + 6 * (long)slot;
+    v[0] = 1. / m0 * (-cfx);
+    v[1] = 1. / m0 * (-cfy);
+    v[2] = 1. / i0 * (ix0 * (-cfy) - iy0 * (-cfx));
+    v[3] = 1. / m1 * (+cfx);
+    v[4] = 1. / m1 * (+cfy);
+    v[5] = 1. / i1 * (ix1 * (+cfy) - iy1 * (+cfx));
+  }
+}
+
+// collision.c:249-304
+__global__ void k_collide_walls(CollArgs a) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 2 * a.np) return;
+  const int n = t >> 1, w = t & 1;
+  const double wallx = w == 0 ? 0. : a.lx;
+  const double *p = a.prt + (long)n * R_N;
+  a.wall_hit[t] = 0;
+  {
+    const double x = p[R_X] + p[R_DX];
+    const double r = fmax(p[R_A], p[R_B]);
+    const double norm = fabs(wallx - x);
+    if (r - norm < 0.) return;
+  }
+  const EllD e = {p[R_A], p[R_B], p[R_X] + p[R_DX], p[R_Y] + p[R_DY], p[R_AZ]};
+  double cx = e.x, cy = e.y, cr = 0., ix = 0., iy = 0.;
+  for (int it = 0;; it++) {
+    double nxc, nyc;
+    const double mirrorx = 2. * wallx - cx;
+    evolute_fit(e, mirrorx, cy, nxc, nyc, ix, iy, cr);
+    const double ddx = nxc - cx, ddy = nyc - cy;
+    cx += ddx;
+    cy += ddy;
+    if (fmax(fabs(ddx), fabs(ddy)) < 1.e-8) break;
+    if (it >= EQC_ITER_CAP) {
+      atomicOr(&a.flags[0], 8);
+      break;
+    }
+  }
+  const double m = p_mass(p), im = p_moi(p);
+  double k;
+  {
+    const double mass = m;
+    const double r = sqrt(p[R_A] * p[R_B]);
+    const double reft = pow(r, 1.5);
+    k = mass * (EIF_PI * EIF_PI) / (reft * reft);
+  }
+  double nxv = wallx - cx;
+  const double norm = fabs(nxv);
+  nxv /= norm;
+  const double overlap = cr - norm;
+  if (overlap > 0.) {
+    const double cfx = k * overlap * nxv;
+    const double cfy = 0.;
+    double *v = a.wall_val + 3 * (long)t;
+    v[0] = 1. / m * (-cfx);
+    v[1] = 1. / m * (-cfy);
+    v[2] = 1. / im * (ix * (-cfy) - iy * (-cfx));
+    a.wall_hit[t] = 1;
+  }
+}
+
+// per particle: partners in ascending index (the order of the reference's
+// flattened pair loop), then wall x=0, then wall x=lx
+__global__ void k_collide_gather(CollArgs a) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= a.np) return;
+  int partner[CONTACT_CAP];
+  int slot[CONTACT_CAP];
+  int cnt = 0;
+  const int npairs = min(*a.pair_cnt, a.pair_cap);
+  for (int s = 0; s < npairs; s++) {
+    const int n0 = a.pair_ids[2 * s], n1 = a.pair_ids[2 * s + 1];
+    if (n0 != m && n1 != m) continue;
+    if (cnt >= CONTACT_CAP) {
+      atomicOr(&a.flags[0], 16);
+      break;
+    }
+    const int other = n0 == m ? n1 : n0;
+    int r = cnt - 1;
+    while (r >= 0 && partner[r] > other) {
+      partner[r + 1] = partner[r];
+      slot[r + 1] = slot[r];
+      r--;
+    }
+    partner[r + 1] = other;
+    slot[r + 1] = s;
+    cnt++;
+  }
+  double fx = 0., fy = 0., tz = 0.;
+  for (int r = 0; r < cnt; r++) {
+    const int s = slot[r];
+    const double *v = a.pair_val + 6 * (long)s + (a.pair_ids[2 * s] == m ? 0 : 3);
+    fx += v[0];
+    fy += v[1];
+    tz += v[2];
+  }
+  for (int w = 0; w < 2; w++)
+    if (a.wall_hit[2 * m + w]) {
+      const double *v = a.wall_val + 3 * (long)(2 * m + w);
+      fx += v[0];
+      fy += v[1];
+      tz += v[2];
+    }
+  double *q = const_cast<double *>(a.prt) + (long)m * R_N;
+  q[R_CFX + a.cnstep] = fx;
+  q[R_CFY + a.cnstep] = fy;
+  q[R_CTZ + a.cnstep] = tz;
+}
+
+__global__ void k_zero_int(int *p, int n) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = 0;
+}
+
+// ---------------------------------------------------------------------
+// particle increments / update / reset
+// ---------------------------------------------------------------------
+__device__ inline void atomic_max_nonneg(double *addr, double v) {
+  atomicMax(reinterpret_cast<unsigned long long *>(addr),
+            (unsigned long long)__double_as_longlong(v));
+}
+
+__global__ void k_zero_residual(double *dscal) { dscal[S_RESIDUAL] = 0.; }
+
+// src/suspensions/increment.c:10-64
+__global__ void k_increment(double *prt, int np, const double *dscal, double ga,
+                            double Fr, double *res_out) {
+  __shared__ double red[256];
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  double res = 0.;
+  if (n < np) {
+    double *p = prt + (long)n * R_N;
+    const double dt = dscal[S_DT];
+    const double den = p[R_DEN];
+    const double dux_prev = p[R_DUX], duy_prev = p[R_DUY], dvz_prev = p[R_DVZ];
+    const double dux = +0.5 * ga * dt * (p[R_CFX] + p[R_CFX + 1]) + dt * p[R_FUX]
+                       + (p[R_IUX + 1] - p[R_IUX]) + 1.0 * ga * dt * (1. - den) / (Fr * Fr);
+    const double duy = +0.5 * ga * dt * (p[R_CFY] + p[R_CFY + 1]) + dt * p[R_FUY]
+                       + (p[R_IUY + 1] - p[R_IUY]);
+    const double dvz = +0.5 * ga * dt * (p[R_CTZ] + p[R_CTZ + 1]) + dt * p[R_TVZ]
+                       + (p[R_IVZ + 1] - p[R_IVZ]);
+    p[R_DUX] = dux;
+    p[R_DUY] = duy;
+    p[R_DVZ] = dvz;
+    res = fmax(res, fabs(dux - dux_prev));
+    res = fmax(res, fabs(duy - duy_prev));
+    res = fmax(res, fabs(dvz - dvz_prev));
+    const double ux0 = p[R_UX], uy0 = p[R_UY], vz0 = p[R_VZ];
+    const double ux1 = p[R_UX] + dux, uy1 = p[R_UY] + duy, vz1 = p[R_VZ] + dvz;
+    p[R_DX] = 0.5 * ga * dt * (ux0 + ux1);
+    p[R_DY] = 0.5 * ga * dt * (uy0 + uy1);
+    p[R_DAZ] = 0.5 * ga * dt * (vz0 + vz1);
+  }
+  red[threadIdx.x] = res;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomic_max_nonneg(res_out, red[0]);
+}
+
+// src/suspensions/update.c:8-34
+__global__ void k_update_particles(double *prt, int np, double ly) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= np) return;
+  double *p = prt + (long)n * R_N;
+  p[R_UX] += p[R_DUX];
+  p[R_UY] += p[R_DUY];
+  p[R_VZ] += p[R_DVZ];
+  p[R_X] += p[R_DX];
+  p[R_Y] += p[R_DY];
+  p[R_AZ] += p[R_DAZ];
+  if (p[R_Y] < 0.) p[R_Y] += ly;
+  if (p[R_Y] > ly) p[R_Y] -= ly;
+  if (p[R_AZ] < 0.) p[R_AZ] += 2. * EIF_PI;
+  if (p[R_AZ] > 2. * EIF_PI) p[R_AZ] -= 2. * EIF_PI;
+}
+
+// src/suspensions/reset.c:7-18
+__global__ void k_reset(double *prt, int np) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= np) return;
+  double *p = prt + (long)n * R_N;
+  p[R_DUX] = 0.;
+  p[R_DUY] = 0.;
+  p[R_DVZ] = 0.;
+  p[R_DX] = 0.;
+  p[R_DY] = 0.;
+  p[R_DAZ] = 0.;
+}
+
+// ---------------------------------------------------------------------
+// host launchers
+// ---------------------------------------------------------------------
+static IbmArgs ibm_args(Ctx *c) {
+  IbmArgs a;
+  a.L = c->L;
+  a.nx = c->nx;
+  a.ny = c->ny;
+  a.ly = c->ly;
+  a.dx = c->dx;
+  a.dy = c->dy;
+  a.grid_size = fmin(c->dx, c->dy);
+  a.xc = c->xc;
+  a.yc = c->yc;
+  a.ux = c->ux;
+  a.uy = c->uy;
+  a.prt = c->prt;
+  a.dscal = c->dscal;
+  a.boxes = c->boxes;
+  a.box_off = c->box_off;
+  a.sc_fx = c->sc_fx;
+  a.sc_fy = c->sc_fy;
+  a.sc_dist = c->sc_dist;
+  a.sc_pos = c->sc_pos;
+  a.tile_cnt = c->tile_cnt;
+  a.tile_list = c->tile_list;
+  a.active = c->active;
+  a.ntx = c->ntx;
+  a.nty = c->nty;
+  a.flags = c->flags;
+  return a;
+}
+
+int susp_init(Ctx *c) {
+  // per-particle scratch capacity: three images of a box no wider than
+  // floor(2r/h)+10 cells in each direction
+  std::vector<double> rec((size_t)c->np * R_N);
+  EIF_HIP(hipMemcpy(rec.data(), c->prt, rec.size() * sizeof(double), hipMemcpyDeviceToHost));
+  std::vector<long> off(c->np + 1, 0);
+  for (int n = 0; n < c->np; n++) {
+    const double r = fmax(rec[(size_t)n * R_N + R_A], rec[(size_t)n * R_N + R_B]);
+    const long cx = (long)(2. * r / c->dx) + 10, cy = (long)(2. * r / c->dy) + 10;
+    const long wx = cx < c->nx ? cx : c->nx, wy = cy < c->ny ? cy : c->ny;
+    off[n + 1] = off[n] + 3 * wx * wy;
+  }
+  c->scratch_cells = off[c->np];
+  EIF_HIP(hipMalloc(&c->box_off, (c->np + 1) * sizeof(long)));
+  EIF_HIP(hipMemcpy(c->box_off, off.data(), (c->np + 1) * sizeof(long), hipMemcpyHostToDevice));
+  const size_t sc = (size_t)(c->scratch_cells > 0 ? c->scratch_cells : 1) * sizeof(double);
+  EIF_HIP(hipMalloc(&c->sc_fx, sc));
+  EIF_HIP(hipMalloc(&c->sc_fy, sc));
+  EIF_HIP(hipMalloc(&c->sc_dist, sc));
+  EIF_HIP(hipMalloc(&c->sc_pos, 4 * (size_t)(c->np > 0 ? c->np : 1) * sizeof(double)));
+  EIF_HIP(hipMemset(c->sc_pos, 0, 4 * (size_t)(c->np > 0 ? c->np : 1) * sizeof(double)));
+  EIF_HIP(hipMalloc(&c->boxes, 3 * (size_t)(c->np > 0 ? c->np : 1) * sizeof(Box)));
+  EIF_HIP(hipMemset(c->boxes, 0, 3 * (size_t)(c->np > 0 ? c->np : 1) * sizeof(Box)));
+  c->ntx = (c->nx + TILE_X - 1) / TILE_X;
+  c->nty = (c->ny + TILE_Y - 1) / TILE_Y;
+  const size_t ntiles = (size_t)c->ntx * c->nty;
+  EIF_HIP(hipMalloc(&c->tile_cnt, ntiles * sizeof(int)));
+  EIF_HIP(hipMemset(c->tile_cnt, 0, ntiles * sizeof(int)));
+  EIF_HIP(hipMalloc(&c->tile_list, ntiles * TILE_CAP * sizeof(int)));
+  EIF_HIP(hipMalloc(&c->active, (ntiles + 1) * sizeof(int)));
+  EIF_HIP(hipMemset(c->active, 0, (ntiles + 1) * sizeof(int)));
+  // collisions
+  c->pair_cap = c->np * 8 + 64;
+  EIF_HIP(hipMalloc(&c->pair_cnt, sizeof(int)));
+  EIF_HIP(hipMemset(c->pair_cnt, 0, sizeof(int)));
+  EIF_HIP(hipMalloc(&c->pair_ids, 2 * (size_t)c->pair_cap * sizeof(int)));
+  EIF_HIP(hipMalloc(&c->pair_val, 6 * (size_t)c->pair_cap * sizeof(double)));
+  EIF_HIP(hipMalloc(&c->wall_val, 6 * (size_t)(c->np > 0 ? c->np : 1) * sizeof(double)));
+  EIF_HIP(hipMalloc(&c->wall_hit, 2 * (size_t)(c->np > 0 ? c->np : 1) * sizeof(int)));
+  EIF_HIP(hipMemset(c->wall_hit, 0, 2 * (size_t)(c->np > 0 ? c->np : 1) * sizeof(int)));
+  return EIF_OK;
+}
+
+int launch_reset(Ctx *c) {
+  if (c->np == 0) return EIF_OK;
+  hipLaunchKernelGGL(k_reset, dim3((c->np + 255) / 256), dim3(256), 0, c->stream, c->prt,
+                     c->np);
+  EIF_HIP(hipGetLastError());
+  return EIF_OK;
+}
+
+int launch_inertia(Ctx *c, int cnstep) {
+  if (c->np == 0) return EIF_OK;
+  ProfScope ps(c, "ibm_inertia");
+  IbmArgs a = ibm_args(c);
+  hipLaunchKernelGGL(k_inertia, dim3(c->np), dim3(IBM_THREADS), 0, c->stream, a, cnstep);
+  EIF_HIP(hipGetLastError());
+  return EIF_OK;
+}
+
+int launch_exchange(Ctx *c) {
+  if (c->np == 0) return EIF_OK;
+  ProfScope ps(c, "ibm_exchange");
+  const size_t ntiles = (size_t)c->ntx * c->nty;
+  // clear the previous call's tile registrations
+  hipLaunchKernelGGL(k_zero_int, dim3((unsigned)((ntiles + 255) / 256)), dim3(256), 0,
+                     c->stream, c->tile_cnt, (int)ntiles);
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, c->active, 1);
+  IbmArgs a = ibm_args(c);
+  hipLaunchKernelGGL(k_exchange, dim3(c->np), dim3(IBM_THREADS), 0, c->stream, a);
+  EIF_HIP(hipGetLastError());
+  return EIF_OK;
+}
+
+static TileArgs tile_args(Ctx *c, int apply) {
+  TileArgs t;
+  t.L = c->L;
+  t.nx = c->nx;
+  t.ny = c->ny;
+  t.ntx = c->ntx;
+  t.boxes = c->boxes;
+  t.sc_fx = c->sc_fx;
+  t.sc_fy = c->sc_fy;
+  t.tile_cnt = c->tile_cnt;
+  t.tile_list = c->tile_list;
+  t.active = c->active;
+  t.ux = c->ux;
+  t.uy = c->uy;
+  t.dux = c->dux;
+  t.duy = c->duy;
+  t.apply = apply;
+  return t;
+}
+
+static int tile_grid(Ctx *c) {
+  const long ntiles = (long)c->ntx * c->nty;
+  return (int)(ntiles < 2048 ? ntiles : 2048);
+}
+
+int launch_update_momentum(Ctx *c) {
+  if (c->np == 0) return EIF_OK;
+  ProfScope ps(c, "ibm_apply");
+  TileArgs t = tile_args(c, 1);
+  hipLaunchKernelGGL(k_tiles, dim3(tile_grid(c)), dim3(TILE_X * TILE_Y), 0, c->stream, t);
+  EIF_HIP(hipGetLastError());
+  return EIF_OK;
+}
+
+int launch_materialise_du(Ctx *c) {
+  const size_t bytes = (size_t)c->L.p_len() * sizeof(double);
+  EIF_HIP(hipMemsetAsync(c->dux, 0, bytes, c->stream));
+  EIF_HIP(hipMemsetAsync(c->duy, 0, bytes, c->stream));
+  if (c->np > 0) {
+    TileArgs t = tile_args(c, 0);
+    hipLaunchKernelGGL(k_tiles, dim3(tile_grid(c)), dim3(TILE_X * TILE_Y), 0, c->stream,
+                       t);
+    EIF_HIP(hipGetLastError());
+  }
+  int r = launch_bc_p(c, c->dux);
+  if (r) return r;
+  return launch_bc_p(c, c->duy);
+}
+
+int launch_collision(Ctx *c, int cnstep) {
+  if (c->np == 0) return EIF_OK;
+  ProfScope ps(c, "collision");
+  CollArgs a;
+  a.np = c->np;
+  a.lx = c->lx;
+  a.ly = c->ly;
+  a.prt = c->prt;
+  a.cnstep = cnstep;
+  a.pair_cnt = c->pair_cnt;
+  a.pair_cap = c->pair_cap;
+  a.pair_ids = c->pair_ids;
+  a.pair_val = c->pair_val;
+  a.wall_val = c->wall_val;
+  a.wall_hit = c->wall_hit;
+  a.flags = c->flags;
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, c->pair_cnt, 1);
+  const long npp = (long)c->np * c->np;
+  hipLaunchKernelGGL(k_collide_pairs, dim3((unsigned)((npp + 255) / 256)), dim3(256), 0,
+                     c->stream, a);
+  hipLaunchKernelGGL(k_collide_walls, dim3((2 * c->np + 255) / 256), dim3(256), 0,
+                     c->stream, a);
+  hipLaunchKernelGGL(k_collide_gather, dim3((c->np + 127) / 128), dim3(128), 0, c->stream,
+                     a);
+  EIF_HIP(hipGetLastError());
+  return EIF_OK;
+}
+
+int launch_increment(Ctx *c, int rk) {
+  hipLaunchKernelGGL(k_zero_residual, dim3(1), dim3(1), 0, c->stream, c->dscal);
+  if (c->np > 0)
+    hipLaunchKernelGGL(k_increment, dim3((c->np + 255) / 256), dim3(256), 0, c->stream,
+                       c->prt, c->np, c->dscal, c->gamma[rk], c->cfg.Fr,
+                       c->dscal + S_RESIDUAL);
+  EIF_HIP(hipGetLastError());
+  return EIF_OK;
+}
+
+int launch_update_particles(Ctx *c) {
+  if (c->np == 0) return EIF_OK;
+  hipLaunchKernelGGL(k_update_particles, dim3((c->np + 255) / 256), dim3(256), 0,
+                     c->stream, c->prt, c->np, c->ly);
+  EIF_HIP(hipGetLastError());
+  return EIF_OK;
+}
+
+}  // namespace eif

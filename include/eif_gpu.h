@@ -1,0 +1,172 @@
+/*
+ * eif_gpu.h -- C-ABI of the MI355X (gfx950) EllipsesInFlows time-step path.
+ *
+ * Drop-in boundary for the reference's internal C API
+ * (NaokiHori/EllipsesInFlows include/fluid.h:27-38,
+ * include/suspensions.h:44-63, include/param.h:55-58), called the way
+ * src/main.c:13-118 calls those functions.  One context holds one whole
+ * domain on one GPU; the reference's MPI halo exchanges, transposes and
+ * Allreduces have no counterpart here.
+ *
+ * Conventions
+ *   - every function returns EIF_OK (0) or a negative EIF_ERR_* code and
+ *     never aborts; eif_gpu_last_error() describes the last failure
+ *     (thread-local);
+ *   - host arrays use the reference's halo-padded layouts
+ *     (include/arrays/fluid.h:5-43, include/arrays/param.h:5-45):
+ *       ux          [(jtot+2) x (itot+1)], UX(i,j) = ux[j*(itot+1)+i-1]
+ *       uy, p, psi  [(jtot+2) x (itot+2)], P(i,j)  = p[j*(itot+2)+i]
+ *       srcuxa      [jtot x (itot-1)],     S(i,j)  = s[(j-1)*(itot-1)+i-2]
+ *       srcuya      [jtot x itot],         S(i,j)  = s[(j-1)*itot+i-1]
+ *   - particle records are 30 doubles each, in the member order of
+ *     particle_t (include/suspensions.h:8-31);
+ *   - calls are asynchronous on the context's stream; only calls that
+ *     return host values synchronise;
+ *   - dt lives in device memory: fluid_gpu_decide_dt() writes it there and
+ *     every kernel reads it there.
+ */
+#ifndef EIF_GPU_H
+#define EIF_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EIF_OK 0
+#define EIF_ERR_ARG (-1)      /* invalid argument or size */
+#define EIF_ERR_HIP (-2)      /* HIP runtime failure (message has detail) */
+#define EIF_ERR_CAPACITY (-3) /* a fixed device capacity was exceeded */
+#define EIF_ERR_STATE (-4)    /* call out of order (e.g. no particles) */
+#define EIF_ERR_NODEVICE (-5) /* no HIP device available */
+
+#define EIF_PARTICLE_NDOUBLES 30
+
+/* y-direction solve of the pressure Poisson equation */
+#define EIF_POISSON_THOMAS 0    /* src/tdm.c bordering Thomas, same arithmetic */
+#define EIF_POISSON_PARTITION 1 /* partitioned Thomas (spike), parallel in y */
+
+typedef struct eif_ctx eif_ctx;
+
+/* Physical and numerical configuration; field names and defaults follow
+ * the reference's environment variables (src/param/init.c:67-103). */
+typedef struct {
+  int itot, jtot;          /* grid points in x (wall-normal) and y          */
+  double ly;               /* domain length in y (lx is 1, param.h:14)      */
+  double Re, Fr, extfrcy;  /* Reynolds, Froude, body force in y             */
+  double uy_wall_xm;       /* uy on the wall x=0  (reference default 0)     */
+  double uy_wall_xp;       /* uy on the wall x=lx (reference default 0)     */
+  double safefactors[3];   /* dt safety factors: advective, diffusive, par. */
+  int poisson_mode;        /* EIF_POISSON_*                                 */
+} eif_config;
+
+/* Replaces param_init's grid/RK set-up (src/param/init.c:105-191) and
+ * fluid_init's allocation (src/fluid/init.c:87-93).  Fields start at zero
+ * with boundary values applied, as src/fluid/init.c:33-85 without restart. */
+int fluid_gpu_create(const eif_config *cfg, eif_ctx **out);
+/* Replaces fluid_finalise + suspensions_finalise
+ * (src/fluid/finalise.c:19-33, src/suspensions/finalise.c:5-20). */
+int fluid_gpu_destroy(eif_ctx *ctx);
+
+/* Grid vectors in the reference layouts (include/arrays/param.h:5-45):
+ * xf[itot+1], xc[itot+2], dxf[itot], dxc[itot+1], yf[jtot+1], yc[jtot]. */
+int fluid_gpu_get_grid(const eif_ctx *ctx, double *xf, double *xc, double *dxf,
+                       double *dxc, double *yf, double *yc);
+
+/* Host <-> device copies of ux, uy, p in the reference layouts (any pointer
+ * may be NULL to skip that field).  Used where the reference reads/writes
+ * restart files (src/fluid/init.c:33-85, src/save.c:41-46). */
+int fluid_gpu_upload(eif_ctx *ctx, const double *ux, const double *uy,
+                     const double *p);
+int fluid_gpu_download(eif_ctx *ctx, double *ux, double *uy, double *p);
+
+/* Auxiliary fields for inspection, reference layouts. */
+#define EIF_FIELD_PSI 0    /* fluid_t.psi   (include/fluid.h:22)        */
+#define EIF_FIELD_SRCUXA 1 /* fluid_t.srcuxa (include/fluid.h:23)       */
+#define EIF_FIELD_SRCUYA 2 /* fluid_t.srcuya (include/fluid.h:24)       */
+#define EIF_FIELD_DUX 3    /* suspensions_t.dux (include/suspensions.h:38) */
+#define EIF_FIELD_DUY 4    /* suspensions_t.duy                          */
+int fluid_gpu_download_field(eif_ctx *ctx, int which, double *host);
+int fluid_gpu_upload_field(eif_ctx *ctx, int which, const double *host);
+
+/* src/fluid/boundary_conditions.c:7-24, :26-43, :45-62 */
+int fluid_gpu_update_boundaries_ux(eif_ctx *ctx);
+int fluid_gpu_update_boundaries_uy(eif_ctx *ctx);
+int fluid_gpu_update_boundaries_p(eif_ctx *ctx);
+
+/* param_decide_dt, src/param/decide_dt.c:11-81.  dt stays on the device;
+ * dt_out (may be NULL) receives it after a synchronisation. */
+int fluid_gpu_decide_dt(eif_ctx *ctx, double *dt_out);
+int fluid_gpu_set_dt(eif_ctx *ctx, double dt);
+int fluid_gpu_get_dt(eif_ctx *ctx, double *dt_out);
+
+/* fluid_update_velocity, src/fluid/update_velocity.c:229-237 */
+int fluid_gpu_update_velocity(eif_ctx *ctx, int rkstep);
+/* fluid_compute_potential, src/fluid/compute_potential.c:48-138 */
+int poisson_gpu_compute_potential(eif_ctx *ctx, int rkstep);
+/* fluid_correct_velocity, src/fluid/correct_velocity.c:6-42 */
+int fluid_gpu_correct_velocity(eif_ctx *ctx, int rkstep);
+/* fluid_update_pressure, src/fluid/update_pressure.c:7-24 */
+int fluid_gpu_update_pressure(eif_ctx *ctx);
+
+/* Diagnostics of src/logging.c:40-172 (divergence, momentum, energy),
+ * computed on the device; synchronising. */
+int fluid_gpu_divergence(eif_ctx *ctx, double *divmax, double *divsum);
+int fluid_gpu_momentum(eif_ctx *ctx, double moms[2]);
+int fluid_gpu_energy(eif_ctx *ctx, double quantities[2]);
+
+/* suspensions_init's particle load (src/suspensions/init.c:24-86): the nine
+ * SoA arrays of the initparticles .npy files. */
+int suspensions_gpu_create(eif_ctx *ctx, int n_particles, const double *den,
+                           const double *a, const double *b, const double *x,
+                           const double *y, const double *az, const double *ux,
+                           const double *uy, const double *vz);
+int suspensions_gpu_n_particles(const eif_ctx *ctx);
+/* whole particle_t records, n x 30 doubles */
+int suspensions_gpu_download(eif_ctx *ctx, double *records);
+int suspensions_gpu_upload(eif_ctx *ctx, const double *records);
+
+/* src/suspensions/reset.c:7-18 */
+int suspensions_gpu_reset_particle_increments(eif_ctx *ctx);
+/* src/suspensions/inertia.c:96-102 */
+int suspensions_gpu_compute_inertia(eif_ctx *ctx, int cnstep);
+/* src/suspensions/exchange.c:114-120 */
+int suspensions_gpu_exchange_momentum(eif_ctx *ctx);
+/* src/suspensions/update.c:36-59 (name keeps the reference's spelling) */
+int suspensions_gpu_update_momentum_fleid(eif_ctx *ctx);
+/* src/suspensions/collision.c:306-367 */
+int suspensions_gpu_compute_collision_force(eif_ctx *ctx, int cnstep);
+/* src/suspensions/increment.c:66-72; residual (may be NULL) is the
+ * max-norm change of the velocity increments, after a synchronisation */
+int suspensions_gpu_increment_particles(eif_ctx *ctx, int rkstep,
+                                        double *residual);
+/* src/suspensions/update.c:61-65 */
+int suspensions_gpu_update_particles(eif_ctx *ctx);
+/* Colliding pairs of the last collision call (overlap > 0 at
+ * src/suspensions/collision.c:232 and :292), sorted: (n0, n1) with n0<n1,
+ * then (n, -1) for the wall x=0 and (n, -2) for x=lx. */
+int suspensions_gpu_colliding_pairs(eif_ctx *ctx, int capacity, int *pairs,
+                                    int *n_pairs);
+
+/* integrate(), src/main.c:13-55: three RK stages with the CN
+ * sub-iteration.  substeps (may be NULL) receives the sub-iteration count
+ * of each stage. */
+int eif_gpu_integrate(eif_ctx *ctx, int substeps[3]);
+/* One time step of src/main.c:71-76: decide_dt + integrate.  dt_out (may
+ * be NULL) receives the dt used. */
+int eif_gpu_step(eif_ctx *ctx, double *dt_out);
+
+int eif_gpu_synchronize(eif_ctx *ctx);
+const char *eif_gpu_last_error(void);
+
+/* Per-kernel timing with HIP events on the context's stream.  Names are
+ * the kernel names listed by eif_gpu_profile_names(). */
+int eif_gpu_profile_enable(eif_ctx *ctx, int enable);
+int eif_gpu_profile_reset(eif_ctx *ctx);
+int eif_gpu_profile_query(eif_ctx *ctx, const char *name, long *count,
+                          double *total_ms);
+const char *eif_gpu_profile_names(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
