@@ -25,7 +25,7 @@ int hip_fail(hipError_t e, const char *what, const char *file, int line) {
 // profiling: HIP events around each named launch on the context stream
 // ---------------------------------------------------------------------
 static const char *k_prof_names =
-    "bc,predictor,rhs_dct2,thomas,dct3_psi,correct_pressure,decide_dt,"
+    "bc,predictor,rhs_dct2,thomas,ysolve_partition,dct3_psi,correct_pressure,decide_dt,"
     "ibm_inertia,ibm_exchange,ibm_apply,collision";
 
 struct Profiler {
@@ -122,9 +122,9 @@ const char *eif_gpu_profile_names(void) { return k_prof_names; }
 static void free_ctx(Ctx *c) {
   double **bufs[] = {&c->xf, &c->xc, &c->dxf, &c->dxc, &c->yf, &c->yc, &c->ux, &c->uy,
                      &c->p, &c->psi, &c->ux_alt, &c->uy_alt, &c->srcuxa, &c->srcuya,
-                     &c->dux, &c->duy, &c->qhat, &c->ctab, &c->th_val, &c->th_u,
+                     &c->dux, &c->duy, &c->qhat, &c->ctab, &c->twid, &c->th_val, &c->th_u,
                      &c->th_q1, &c->th_den, &c->th_last, &c->pt_fwd, &c->pt_u,
-                     &c->pt_v, &c->pt_w, &c->pt_bnd, &c->pt_red, &c->dscal, &c->dred,
+                     &c->pt_v, &c->pt_w, &c->pt_bnd, &c->pt_red, &c->pt_lvl, &c->pt_ends, &c->pt_g, &c->dscal, &c->dred,
                      &c->prt, &c->pconst, &c->sc_fx, &c->sc_fy, &c->sc_dist, &c->sc_pos,
                      &c->pair_val, &c->wall_val};
   for (auto b : bufs)

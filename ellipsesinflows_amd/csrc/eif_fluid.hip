@@ -335,8 +335,14 @@ __global__ __launch_bounds__(256) void k_dt_adv(Layout L, const double *ux,
     const double vel = fmax(fabs(uy[L.p(i, j)]), small);
     m = fmin(m, dy / vel);
   }
-  for (int off = 32; off > 0; off >>= 1) m = fmin(m, __shfl_down(m, off, 64));
-  if ((threadIdx.x & 63) == 0) atomic_min_pos(dred, m);
+  __shared__ double red[256];
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomic_min_pos(dred, red[0]);
 }
 
 __global__ void k_dt_final(const double *dred, const double *prt, int np,
@@ -372,7 +378,7 @@ int launch_decide_dt(Ctx *c) {
   hipLaunchKernelGGL(k_dt_init, dim3(1), dim3(1), 0, c->stream, c->dred);
   const long n = (long)c->nx * c->ny;
   int blocks = (int)((n + 255) / 256);
-  if (blocks > 2048) blocks = 2048;
+  if (blocks > 1024) blocks = 1024;
   hipLaunchKernelGGL(k_dt_adv, dim3(blocks), dim3(256), 0, c->stream, c->L, c->ux,
                      c->uy, c->dxc, c->dy, c->dred);
   // diffusive limits are grid constants (decide_dt.c:50-62)

@@ -79,6 +79,7 @@ struct Ctx {
   // Poisson
   double *qhat = nullptr;      // spectral field, x-contiguous rows [ny][nx]
   double *ctab = nullptr;      // C[m] = cos(pi m/(2 nx)), m=0..2nx
+  double *twid = nullptr;      // per-stage radix-2 twiddles (entries of C)
   double *th_val = nullptr;    // Thomas tables [ny-1][nx] (forward factors)
   double *th_u = nullptr;      // [ny-1][nx] modified upper diagonal
   double *th_q1 = nullptr;     // [ny-1][nx] border solution
@@ -88,7 +89,8 @@ struct Ctx {
   // partition solver tables
   int part_L = 0, part_P = 0;
   double *pt_fwd = nullptr, *pt_u = nullptr, *pt_v = nullptr, *pt_w = nullptr;
-  double *pt_bnd = nullptr, *pt_red = nullptr;
+  double *pt_bnd = nullptr, *pt_red = nullptr, *pt_lvl = nullptr;
+  double *pt_ends = nullptr, *pt_g = nullptr;
   // scalars
   double *dscal = nullptr;
   double *hscal = nullptr;  // pinned

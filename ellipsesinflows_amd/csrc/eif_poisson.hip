@@ -64,6 +64,88 @@ __device__ void lds_fft(double *re, double *im, int n, int logn,
   }
 }
 
+// Register form of the same radix-2 DIT: n/8 threads each hold 8 complex
+// values; stages are taken three at a time (groups of 8 positions closed
+// under those stages), with LDS used only to regroup between phases.  Every
+// butterfly sees the same inputs and performs the same operations as
+// lds_fft, so the results are identical.
+struct RegMap {
+  int lg, lh0, lgpt;  // log2 of group size, first butterfly distance, groups/thread
+  __device__ int pos(int t, int slot) const {
+    const int G = (t << lgpt) + (slot >> lg);
+    return ((G >> lh0) << (lg + lh0)) + (G & ((1 << lh0) - 1)) + ((slot & ((1 << lg) - 1)) << lh0);
+  }
+};
+
+__device__ inline RegMap reg_phase(int p, int logn) {
+  RegMap m;
+  m.lg = min(3, logn - 3 * p);
+  m.lh0 = 3 * p;
+  m.lgpt = 3 - m.lg;
+  return m;
+}
+
+__device__ inline int reg_nphase(int logn) { return (logn + 2) / 3; }
+
+// LDS address of element p: XOR-swizzles the 16-double bank group so that
+// the bit-reversed, strided and natural access patterns of the transform
+// hit at most two-way bank conflicts
+__device__ inline int sw(int p) { return p ^ ((p >> 4) & 15) ^ ((p >> 8) & 15); }
+
+// W: per-stage twiddles, stage len occupies entries [len/2-1, len-1):
+// W[2e] = C[4m], W[2e+1] = C[|n-4m|] with m = k n/len (the same table values
+// lds_fft reads from C)
+template <int SIGN>
+__device__ void reg_fft(double (&xr)[8], double (&xi)[8], double *lre, double *lim, int n,
+                        int logn, const double *__restrict__ W) {
+  const int t = threadIdx.x;
+  const int np = reg_nphase(logn);
+  for (int p = 0; p < np; p++) {
+    const RegMap mp = reg_phase(p, logn);
+    if (p > 0) {
+      const RegMap mq = reg_phase(p - 1, logn);
+      __syncthreads();
+#pragma unroll
+      for (int s = 0; s < 8; s++) {
+        const int q = sw(mq.pos(t, s));
+        lre[q] = xr[s];
+        lim[q] = xi[s];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int s = 0; s < 8; s++) {
+        const int q = sw(mp.pos(t, s));
+        xr[s] = lre[q];
+        xi[s] = lim[q];
+      }
+    }
+    const int r = mp.lg;
+    const int gmask = (1 << r) - 1;
+    for (int ss = 0; ss < r; ss++) {
+      const int half = 1 << (mp.lh0 + ss), len = half << 1;
+      const double *Ws = W + 2 * (half - 1);
+#pragma unroll
+      for (int s = 0; s < 8; s++) {
+        const int e = s & gmask;
+        if (e & (1 << ss)) continue;
+        if ((e | (1 << ss)) > gmask) continue;
+        const int s1 = s + (1 << ss);
+        const int kk = mp.pos(t, s) & (len - 1);
+        const double wr = Ws[2 * kk];
+        const double wi = SIGN * Ws[2 * kk + 1];
+        const double r1 = xr[s1], m1 = xi[s1];
+        const double tr = wr * r1 - wi * m1;
+        const double ti = wr * m1 + wi * r1;
+        const double r0 = xr[s], m0 = xi[s];
+        xr[s1] = r0 - tr;
+        xi[s1] = m0 - ti;
+        xr[s] = r0 + tr;
+        xi[s] = m0 + ti;
+      }
+    }
+  }
+}
+
 __device__ inline double cos_lookup(const double *C, int n, long m) {
   m %= 4 * (long)n;
   return m <= 2 * n ? C[m] : C[4 * (long)n - m];
@@ -71,37 +153,65 @@ __device__ inline double cos_lookup(const double *C, int n, long m) {
 
 // ---------------------------------------------------------------------
 // RHS (compute_potential.c:70-82) + DCT-II along x (:84-92), one row per
-// workgroup.
+// workgroup.  mode 2: register FFT (blockDim = nx/8), mode 1: LDS FFT,
+// mode 0: direct sum (nx not a power of two).
 // ---------------------------------------------------------------------
-__global__ __launch_bounds__(DCT_THREADS) void k_rhs_dct2(
+__device__ inline double rhs_value(const Layout &L, const double *ux, const double *uy,
+                                   double fac, double dx, double dy, int i, int j) {
+  return fac * (+(ux[L.ux(i + 2, j)] - ux[L.ux(i + 1, j)]) / dx
+                + (uy[L.p(i + 1, j + 1)] - uy[L.p(i + 1, j)]) / dy);
+}
+
+__global__ __launch_bounds__(1024) void k_rhs_dct2(
     Layout L, const double *ux, const double *uy, const double *dscal, double ga,
-    double dx, double dy, const double *__restrict__ C, int logn, double *qhat) {
+    double dx, double dy, const double *__restrict__ C, const double *__restrict__ W,
+    int logn, int mode, double *qhat) {
   extern __shared__ double sh[];
   const int nx = L.nx;
   const int j = blockIdx.x + 1;
   double *re = sh, *im = sh + nx;
   const double dt = dscal[S_DT];
   const double fac = 1. / (ga * dt);
-  if (logn >= 0) {
+  double *out = qhat + (long)(j - 1) * nx;
+  if (mode == 2) {
+    const int T = blockDim.x;
+    for (int m = threadIdx.x; m < nx; m += T) {
+      const int i = m < nx / 2 ? 2 * m : 2 * (nx - 1 - m) + 1;  // Makhoul order
+      re[sw(bitrev(m, logn))] = rhs_value(L, ux, uy, fac, dx, dy, i, j);
+    }
+    __syncthreads();
+    double xr[8], xi[8];
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+      xr[s] = re[sw(8 * threadIdx.x + s)];
+      xi[s] = 0.;
+    }
+    reg_fft<-1>(xr, xi, re, im, nx, logn, W);
+    const RegMap ml = reg_phase(reg_nphase(logn) - 1, logn);
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+      const int q = sw(ml.pos(threadIdx.x, s));
+      re[q] = xr[s];
+      im[q] = xi[s];
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < nx; k += T)
+      out[k] = 2. * (C[k] * re[sw(k)] + C[nx - k] * im[sw(k)]);
+  } else if (mode == 1) {
     for (int i = threadIdx.x; i < nx; i += blockDim.x) {
-      const double q = fac * (+(ux[L.ux(i + 2, j)] - ux[L.ux(i + 1, j)]) / dx
-                              + (uy[L.p(i + 1, j + 1)] - uy[L.p(i + 1, j)]) / dy);
-      const int m = (i & 1) ? nx - 1 - (i >> 1) : (i >> 1);  // Makhoul order
+      const int m = (i & 1) ? nx - 1 - (i >> 1) : (i >> 1);
       const int pos = bitrev(m, logn);
-      re[pos] = q;
+      re[pos] = rhs_value(L, ux, uy, fac, dx, dy, i, j);
       im[pos] = 0.;
     }
     __syncthreads();
     lds_fft(re, im, nx, logn, C, -1.);
-    double *out = qhat + (long)(j - 1) * nx;
     for (int k = threadIdx.x; k < nx; k += blockDim.x)
       out[k] = 2. * (C[k] * re[k] + C[nx - k] * im[k]);
   } else {
-    for (int i = threadIdx.x; i < nx; i += blockDim.x)
-      re[i] = fac * (+(ux[L.ux(i + 2, j)] - ux[L.ux(i + 1, j)]) / dx
-                     + (uy[L.p(i + 1, j + 1)] - uy[L.p(i + 1, j)]) / dy);
+    for (int i = threadIdx.x; i < nx; i += blockDim.x) re[i] = rhs_value(L, ux, uy, fac, dx, dy, i, j);
     __syncthreads();
-    double *out = qhat + (long)(j - 1) * nx;
     for (int k = threadIdx.x; k < nx; k += blockDim.x) {
       double acc = 0.;
       for (int jj = 0; jj < nx; jj++)
@@ -122,7 +232,7 @@ struct CombineArgs {
   const double *last;   // Thomas: [nx]
   // partition
   int L, P;
-  const double *v, *w;  // [L][nx] spikes
+  const double *v;      // [L][nx] spike A_L^-1 e_0 (the other is its mirror)
   const double *bnd;    // [P][2][nx] chunk-boundary neighbour values
   double coef;          // l (= u) of the matrix
 };
@@ -137,18 +247,41 @@ __device__ inline double combined(const CombineArgs &a, const double *qhat,
   const int c = j0 / a.L, r = j0 - c * a.L;
   const double xl = a.bnd[((long)c * 2 + 0) * nx + k];
   const double xr = a.bnd[((long)c * 2 + 1) * nx + k];
-  return q - a.coef * xl * a.v[(long)r * nx + k] - a.coef * xr * a.w[(long)r * nx + k];
+  return q - a.coef * xl * a.v[(long)r * nx + k] - a.coef * xr * a.v[(long)(a.L - 1 - r) * nx + k];
 }
 
-__global__ __launch_bounds__(DCT_THREADS) void k_dct3_psi(
+__global__ __launch_bounds__(1024) void k_dct3_psi(
     Layout L, const double *qhat, CombineArgs ca, const double *__restrict__ C,
-    int logn, double *psi) {
+    const double *__restrict__ W, int logn, int mode, double *psi) {
   extern __shared__ double sh[];
   const int nx = L.nx, ny = L.ny;
   const int j0 = blockIdx.x;
   double *re = sh, *im = sh + nx;
   const double norm = 2. * nx;
-  if (logn >= 0) {
+  if (mode == 2) {
+    const int T = blockDim.x;
+    for (int k = threadIdx.x; k < nx; k += T) re[sw(k)] = combined(ca, qhat, nx, ny, j0, k);
+    __syncthreads();
+    double xr[8], xi[8];
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+      const int k = bitrev(8 * threadIdx.x + s, logn);
+      const double vr = re[sw(k)];
+      const double vi = k == 0 ? 0. : -re[sw(nx - k)];
+      xr[s] = C[k] * vr - C[nx - k] * vi;
+      xi[s] = C[k] * vi + C[nx - k] * vr;
+    }
+    reg_fft<+1>(xr, xi, re, im, nx, logn, W);
+    const RegMap ml = reg_phase(reg_nphase(logn) - 1, logn);
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < 8; s++) re[sw(ml.pos(threadIdx.x, s))] = xr[s];
+    __syncthreads();
+    for (int i = threadIdx.x; i < nx; i += T) {
+      const double v = (i & 1) ? re[sw(nx - 1 - (i >> 1))] : re[sw(i >> 1)];
+      psi[L.p(i + 1, j0 + 1)] = v / norm;
+    }
+  } else if (mode == 1) {
     for (int k = threadIdx.x; k < nx; k += blockDim.x) {
       const double xr = combined(ca, qhat, nx, ny, j0, k);
       const double xi = k == 0 ? 0. : -combined(ca, qhat, nx, ny, j0, nx - k);
@@ -221,6 +354,193 @@ __global__ __launch_bounds__(64) void k_thomas(int nx, int ny, double *q,
   last[k] = xl;
 }
 
+// ---------------------------------------------------------------------
+// Partition mode (spike method).  Chunk c holds rows [cL, cL+L).  With the
+// chunk-local solution y = A_L^-1 d, v = A_L^-1 e_0 and w = A_L^-1 e_{L-1}
+// (w[r] = v[L-1-r] by symmetry), the periodic solution is
+//   x[cL+r] = y_c[r] - l*beta_{c-1}*v[r] - l*alpha_{c+1}*w[r]
+// where alpha_c, beta_c are the first/last values of chunk c.  The pairs
+// u_c = (beta_c, alpha_{c+1}) satisfy a periodic block-tridiagonal system
+//   Lb u_{c-1} + D u_c + Ub u_{c+1} = (y_c[L-1], y_{c+1}[0])
+//   D = [[1, l v0], [l v0, 1]], Lb = [[l vL, 0], [0, 0]], Ub = [[0, 0], [0, l vL]]
+// solved per wavenumber by block elimination with u_{P-1} as the border
+// unknown; all matrix-only factors are tabulated per context.
+// ---------------------------------------------------------------------
+constexpr int PART_LANES = 64;
+
+template <int L>
+__global__ __launch_bounds__(PART_LANES) void k_part_local(
+    int nx, double *q, const double *__restrict__ fwd,
+    const double *__restrict__ um, double coef, double *ends) {
+  const int k = blockIdx.x * PART_LANES + threadIdx.x;
+  const int c = blockIdx.y;
+  if (k >= nx) return;
+  const long j0 = (long)c * L;
+  const double l = coef;
+  double y[L];
+  double prev = 0.;
+#pragma unroll
+  for (int r = 0; r < L; r++) {
+    const double d = q[(j0 + r) * nx + k];
+    const double v = fwd[(long)r * nx + k] * (d - l * prev);
+    y[r] = v;
+    prev = v;
+  }
+  double next = y[L - 1];
+  q[(j0 + L - 1) * nx + k] = next;
+#pragma unroll
+  for (int r = L - 2; r >= 0; r--) {
+    next = y[r] - um[(long)r * nx + k] * next;
+    q[(j0 + r) * nx + k] = next;
+  }
+  ends[((long)c * 2 + 0) * nx + k] = next;   // y_c[0]
+  ends[((long)c * 2 + 1) * nx + k] = y[L - 1];  // y_c[L-1]
+}
+
+// Reduced system: the blocks are the same for every chunk, so the periodic
+// block system is block-circulant and the length-P DFT over chunks
+// diagonalises it.  For mode m (theta = 2 pi m / P):
+//   M_m = [[1 + l vL e^{-i theta}, l v0], [l v0, 1 + l vL e^{i theta}]]
+//   det_m = 1 + 2 l vL cos(theta) + (l vL)^2 - (l v0)^2   (real)
+// Mode 0 of wavenumber 0 is singular (constant null vector); its
+// minimum-norm solution fixes the gauge.  One workgroup handles
+// PART_KG wavenumbers with 64 threads each.
+constexpr int PART_KG = 4;
+
+__global__ __launch_bounds__(64 * PART_KG) void k_part_reduced(
+    int nx, int P, const double *__restrict__ ends, const double *__restrict__ lv,
+    const double *__restrict__ twc, const double *__restrict__ tws,
+    double *__restrict__ bnd) {
+  extern __shared__ double sh[];
+  const int kk = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int k = blockIdx.x * PART_KG + kk;
+  double *r0 = sh + (long)kk * 6 * P, *r1 = r0 + P;
+  double *u0r = r1 + P, *u0i = u0r + P, *u1r = u0i + P, *u1i = u1r + P;
+  const bool on = k < nx;
+  if (on)
+    for (int c = lane; c < P; c += 64) {
+      r0[c] = ends[((long)c * 2 + 1) * nx + k];             // y_c[L-1]
+      r1[c] = ends[((long)((c + 1) % P) * 2 + 0) * nx + k];  // y_{c+1}[0]
+    }
+  __syncthreads();
+  if (on) {
+    const double lv0 = lv[k], lvL = lv[nx + k];
+    const bool gauge = lv[2 * (long)nx + k] != 0.;
+    for (int m = lane; m < P; m += 64) {
+      double a0r = 0., a0i = 0., a1r = 0., a1i = 0.;
+      for (int c = 0; c < P; c++) {
+        const int t = (int)(((long)m * c) % P);
+        const double cs = twc[t], sn = tws[t];
+        a0r += r0[c] * cs;
+        a0i -= r0[c] * sn;
+        a1r += r1[c] * cs;
+        a1i -= r1[c] * sn;
+      }
+      const double ct = twc[m], st = tws[m];
+      double x0r, x0i, x1r, x1i;
+      if (gauge && m == 0) {
+        const double f = 1. / (2. * ((1. + lvL) - lv0));
+        x0r = (a0r - a1r) * f;
+        x0i = (a0i - a1i) * f;
+        x1r = -x0r;
+        x1i = -x0i;
+      } else {
+        const double det = 1. + 2. * lvL * ct + lvL * lvL - lv0 * lv0;
+        // adj(M) = [[1 + lvL e^{i th}, -lv0], [-lv0, 1 + lvL e^{-i th}]]
+        const double p0r = 1. + lvL * ct, p0i = lvL * st;   // 1 + lvL e^{i th}
+        const double p1r = 1. + lvL * ct, p1i = -lvL * st;  // 1 + lvL e^{-i th}
+        x0r = (p0r * a0r - p0i * a0i - lv0 * a1r) / det;
+        x0i = (p0r * a0i + p0i * a0r - lv0 * a1i) / det;
+        x1r = (-lv0 * a0r + p1r * a1r - p1i * a1i) / det;
+        x1i = (-lv0 * a0i + p1r * a1i + p1i * a1r) / det;
+      }
+      u0r[m] = x0r; u0i[m] = x0i; u1r[m] = x1r; u1i[m] = x1i;
+    }
+  }
+  __syncthreads();
+  if (on) {
+    const double invP = 1. / P;
+    for (int c = lane; c < P; c += 64) {
+      double b0 = 0., b1 = 0.;
+      for (int m = 0; m < P; m++) {
+        const int t = (int)(((long)m * c) % P);
+        const double cs = twc[t], sn = tws[t];
+        b0 += u0r[m] * cs - u0i[m] * sn;
+        b1 += u1r[m] * cs - u1i[m] * sn;
+      }
+      // u_c = (beta_c, alpha_{c+1}): chunk c+1's left value, chunk c's right
+      bnd[((long)((c + 1) % P) * 2 + 0) * nx + k] = b0 * invP;
+      bnd[((long)c * 2 + 1) * nx + k] = b1 * invP;
+    }
+  }
+}
+
+static int part_chunk(int ny) {
+  for (int L = 64; L >= 8; L >>= 1)
+    if (ny % L == 0 && ny / L >= 2 && ny / L <= 1024) return L;
+  return 0;
+}
+
+static int part_init(Ctx *c) {
+  const int nx = c->nx, ny = c->ny;
+  const int L = part_chunk(ny);
+  if (!L) return EIF_OK;  // partition mode unavailable for this jtot
+  const int P = ny / L;
+  c->part_L = L;
+  c->part_P = P;
+  const double dx = c->dx, dy = c->dy;
+  const double l = 1. / dy / dy;
+  std::vector<double> fwd((size_t)L * nx), um((size_t)L * nx), v((size_t)L * nx);
+  std::vector<double> lv(3 * (size_t)nx), twc(P), tws(P);
+  std::vector<long double> vv(L), uu(L), ff(L);
+  for (int k = 0; k < nx; k++) {
+    const double sn = sin(EIF_PI * k / (2. * nx));
+    const double eig = -4. / (dx * dx) * (sn * sn);
+    const double cc = -l - l + eig;
+    // chunk-local Thomas factors (Dirichlet truncation A_L)
+    double up = 0.;
+    for (int r = 0; r < L; r++) {
+      const double f = 1. / (cc - l * up);
+      fwd[(size_t)r * nx + k] = f;
+      up = f * l;
+      um[(size_t)r * nx + k] = up;
+    }
+    // v = A_L^-1 e_0 in long double
+    {
+      long double u = 0., q = 0.;
+      for (int r = 0; r < L; r++) {
+        const long double f = 1.0L / ((long double)cc - (long double)l * u);
+        ff[r] = f;
+        q = f * ((r == 0 ? 1.0L : 0.0L) - (long double)l * q);
+        vv[r] = q;
+        u = f * (long double)l;
+        uu[r] = u;
+      }
+      for (int r = L - 2; r >= 0; r--) vv[r] = vv[r] - uu[r] * vv[r + 1];
+      for (int r = 0; r < L; r++) v[(size_t)r * nx + k] = (double)vv[r];
+    }
+    lv[k] = (double)((long double)l * vv[0]);
+    lv[nx + k] = (double)((long double)l * vv[L - 1]);
+    lv[2 * (size_t)nx + k] = (k == 0 && eig == 0.) ? 1. : 0.;
+  }
+  for (int t = 0; t < P; t++) {
+    twc[t] = cos(2. * EIF_PI * t / P);
+    tws[t] = sin(2. * EIF_PI * t / P);
+  }
+  auto up = [&](double **d, const std::vector<double> &h) -> int {
+    EIF_HIP(hipMalloc(d, h.size() * sizeof(double)));
+    EIF_HIP(hipMemcpy(*d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    return EIF_OK;
+  };
+  int rc;
+  if ((rc = up(&c->pt_fwd, fwd)) || (rc = up(&c->pt_u, um)) || (rc = up(&c->pt_v, v)) ||
+      (rc = up(&c->pt_red, twc)) || (rc = up(&c->pt_w, tws)) || (rc = up(&c->pt_lvl, lv)))
+    return rc;
+  EIF_HIP(hipMalloc(&c->pt_bnd, (size_t)P * 2 * nx * sizeof(double)));
+  EIF_HIP(hipMalloc(&c->pt_ends, (size_t)P * 2 * nx * sizeof(double)));
+  return EIF_OK;
+}
+
 int poisson_init(Ctx *c) {
   const int nx = c->nx, ny = c->ny;
   // cosine table, same expression as the host restatement
@@ -228,6 +548,20 @@ int poisson_init(Ctx *c) {
   for (int m = 0; m <= 2 * nx; m++) C[m] = cos(EIF_PI * m / (2. * nx));
   EIF_HIP(hipMalloc(&c->ctab, C.size() * sizeof(double)));
   EIF_HIP(hipMemcpy(c->ctab, C.data(), C.size() * sizeof(double), hipMemcpyHostToDevice));
+  // per-stage twiddles of the radix-2 transform (copies of C entries)
+  {
+    std::vector<double> W(2 * (size_t)(nx > 1 ? nx - 1 : 1), 0.);
+    if ((nx & (nx - 1)) == 0)
+      for (int len = 2; len <= nx; len <<= 1)
+        for (int k = 0; k < len / 2; k++) {
+          const int m = k * (nx / len);
+          const size_t e = (size_t)(len / 2 - 1 + k);
+          W[2 * e] = C[4 * m];
+          W[2 * e + 1] = C[nx >= 4 * m ? nx - 4 * m : 4 * m - nx];
+        }
+    EIF_HIP(hipMalloc(&c->twid, W.size() * sizeof(double)));
+    EIF_HIP(hipMemcpy(c->twid, W.data(), W.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
   EIF_HIP(hipMalloc(&c->qhat, (size_t)nx * ny * sizeof(double)));
   const double dx = c->dx, dy = c->dy;
   const double l = 1. / dy / dy;  // compute_potential.c:108-111
@@ -285,7 +619,7 @@ int poisson_init(Ctx *c) {
   EIF_HIP(hipMemcpy(c->th_flag, flg.data(), flg.size(), hipMemcpyHostToDevice));
   EIF_HIP(hipMemcpy(c->th_den, den.data(), nx * sizeof(double), hipMemcpyHostToDevice));
   EIF_HIP(hipMemcpy(c->th_den + nx, cdiag.data(), nx * sizeof(double), hipMemcpyHostToDevice));
-  return EIF_OK;
+  return part_init(c);
 }
 
 static int ilog2_exact(int n) {
@@ -299,17 +633,39 @@ int launch_poisson(Ctx *c, int rk) {
   const int nx = c->nx, ny = c->ny;
   const int logn = ilog2_exact(nx);
   const size_t shmem = 2 * (size_t)nx * sizeof(double);
+  // register FFT for 64 <= nx <= 8192, LDS FFT for other powers of two
+  const int mode = logn < 0 ? 0 : ((nx >= 64 && nx <= 8192) ? 2 : 1);
+  const int threads = mode == 2 ? nx / 8 : DCT_THREADS;
   {
     ProfScope ps(c, "rhs_dct2");
-    hipLaunchKernelGGL(k_rhs_dct2, dim3(ny), dim3(DCT_THREADS), shmem, c->stream, c->L,
+    hipLaunchKernelGGL(k_rhs_dct2, dim3(ny), dim3(threads), shmem, c->stream, c->L,
                        c->ux, c->uy, c->dscal, c->gamma[rk], c->dx, c->dy, c->ctab,
-                       logn, c->qhat);
+                       c->twid, logn, mode, c->qhat);
     EIF_HIP(hipGetLastError());
   }
   const double l = 1. / c->dy / c->dy;
   CombineArgs ca{};
   ca.coef = l;
-  {
+  if (c->cfg.poisson_mode == EIF_POISSON_PARTITION && c->part_L) {
+    ProfScope ps(c, "ysolve_partition");
+    const int Lc = c->part_L, P = c->part_P;
+    dim3 g1((nx + PART_LANES - 1) / PART_LANES, P);
+    switch (Lc) {
+      case 64: hipLaunchKernelGGL(k_part_local<64>, g1, dim3(PART_LANES), 0, c->stream, nx, c->qhat, c->pt_fwd, c->pt_u, l, c->pt_ends); break;
+      case 32: hipLaunchKernelGGL(k_part_local<32>, g1, dim3(PART_LANES), 0, c->stream, nx, c->qhat, c->pt_fwd, c->pt_u, l, c->pt_ends); break;
+      case 16: hipLaunchKernelGGL(k_part_local<16>, g1, dim3(PART_LANES), 0, c->stream, nx, c->qhat, c->pt_fwd, c->pt_u, l, c->pt_ends); break;
+      default: hipLaunchKernelGGL(k_part_local<8>, g1, dim3(PART_LANES), 0, c->stream, nx, c->qhat, c->pt_fwd, c->pt_u, l, c->pt_ends); break;
+    }
+    hipLaunchKernelGGL(k_part_reduced, dim3((nx + PART_KG - 1) / PART_KG), dim3(64 * PART_KG),
+                       (size_t)PART_KG * 6 * P * sizeof(double), c->stream, nx, P, c->pt_ends,
+                       c->pt_lvl, c->pt_red, c->pt_w, c->pt_bnd);
+    EIF_HIP(hipGetLastError());
+    ca.mode = 1;
+    ca.L = Lc;
+    ca.P = P;
+    ca.v = c->pt_v;
+    ca.bnd = c->pt_bnd;
+  } else {
     ProfScope ps(c, "thomas");
     hipLaunchKernelGGL(k_thomas, dim3((nx + 63) / 64), dim3(64), 0, c->stream, nx, ny,
                        c->qhat, c->th_val, c->th_u, c->th_flag, c->th_den + nx,
@@ -321,8 +677,8 @@ int launch_poisson(Ctx *c, int rk) {
   }
   {
     ProfScope ps(c, "dct3_psi");
-    hipLaunchKernelGGL(k_dct3_psi, dim3(ny), dim3(DCT_THREADS), shmem, c->stream, c->L,
-                       c->qhat, ca, c->ctab, logn, c->psi);
+    hipLaunchKernelGGL(k_dct3_psi, dim3(ny), dim3(threads), shmem, c->stream, c->L,
+                       c->qhat, ca, c->ctab, c->twid, logn, mode, c->psi);
     EIF_HIP(hipGetLastError());
   }
   return launch_bc_p(c, c->psi);

@@ -262,33 +262,35 @@ __global__ __launch_bounds__(IBM_THREADS) void k_inertia(IbmArgs a, int cnstep) 
   }
 }
 
-// registers the cells a box's forcing reaches in the field update with the
-// 64x4 tiles (update.c:42-57: UX(i) reads DUX(i-1), UY(j) reads DUY(j-1),
-// and DUY(i,0) is the periodic copy of DUY(i,jtot))
-__device__ void register_tiles(const IbmArgs &a, int box_id, int imin, int imax,
-                               int jmin, int jmax) {
-  auto reg = [&](int i_lo, int i_hi, int j_lo, int j_hi) {
-    const int tx0 = (i_lo - 1) / TILE_X, tx1 = (i_hi - 1) / TILE_X;
-    const int ty0 = (j_lo - 1) / TILE_Y, ty1 = (j_hi - 1) / TILE_Y;
-    for (int ty = ty0; ty <= ty1; ty++)
-      for (int tx = tx0; tx <= tx1; tx++) {
-        const int tile = ty * a.ntx + tx;
-        const int slot = atomicAdd(&a.tile_cnt[tile], 1);
-        if (slot == 0) {
-          const int pos = atomicAdd(&a.active[0], 1);
-          a.active[1 + pos] = tile;
-        }
-        if (slot < TILE_CAP) {
-          a.tile_list[(long)tile * TILE_CAP + slot] = box_id;
-        } else {
-          atomicOr(&a.flags[0], 2);
-        }
-      }
-  };
+// Tiles reached by a box's forcing in the field update (update.c:42-57:
+// UX(i) reads DUX(i-1), UY(j) reads DUY(j-1), and DUY(i,0) is the periodic
+// copy of DUY(i,jtot)).  Up to two rectangles of tiles per box.
+struct TileRange {
+  int tx0, tx1, ty0, ty1;
+  __device__ int count() const { return tx1 < tx0 || ty1 < ty0 ? 0 : (tx1 - tx0 + 1) * (ty1 - ty0 + 1); }
+};
+
+__device__ inline void box_tile_ranges(const IbmArgs &a, int imin, int imax, int jmin,
+                                       int jmax, TileRange &r0, TileRange &r1) {
   const int i_hi = imax + 1 > a.nx ? a.nx : imax + 1;
   const int j_hi = jmax + 1 > a.ny ? a.ny : jmax + 1;
-  reg(imin, i_hi, jmin, j_hi);
-  if (jmax == a.ny && jmin > 1) reg(imin, imax, 1, 1);
+  r0 = {(imin - 1) / TILE_X, (i_hi - 1) / TILE_X, (jmin - 1) / TILE_Y, (j_hi - 1) / TILE_Y};
+  if (jmax == a.ny && jmin > 1)
+    r1 = {(imin - 1) / TILE_X, (imax - 1) / TILE_X, 0, 0};
+  else
+    r1 = {0, -1, 0, -1};
+}
+
+__device__ inline void register_tile(const IbmArgs &a, int tile, int box_id) {
+  const int slot = atomicAdd(&a.tile_cnt[tile], 1);
+  if (slot == 0) {
+    const int pos = atomicAdd(&a.active[0], 1);
+    a.active[1 + pos] = tile;
+  }
+  if (slot < TILE_CAP)
+    a.tile_list[(long)tile * TILE_CAP + slot] = box_id;
+  else
+    atomicOr(&a.flags[0], 2);
 }
 
 // src/suspensions/exchange.c:13-88
@@ -321,7 +323,30 @@ __global__ __launch_bounds__(IBM_THREADS) void k_exchange(IbmArgs a) {
     b.jmax = bs.jmax[k];
     b.off = off;
     a.boxes[(long)n * 3 + k] = b;
-    if (bs.cnt[k] > 0) register_tiles(a, n * 3 + k, b.imin, b.imax, b.jmin, b.jmax);
+  }
+  {
+    // register the boxes with their tiles, one tile per thread
+    TileRange tr[6];
+    int tot = 0;
+    for (int k = 0; k < 3; k++) {
+      if (bs.cnt[k] > 0) {
+        box_tile_ranges(a, bs.imin[k], bs.imax[k], bs.jmin[k], bs.jmax[k], tr[2 * k], tr[2 * k + 1]);
+      } else {
+        tr[2 * k] = {0, -1, 0, -1};
+        tr[2 * k + 1] = {0, -1, 0, -1};
+      }
+      tot += tr[2 * k].count() + tr[2 * k + 1].count();
+    }
+    for (int t = threadIdx.x; t < tot; t += IBM_THREADS) {
+      int e = t, r = 0;
+      while (e >= tr[r].count()) {
+        e -= tr[r].count();
+        r++;
+      }
+      const int w = tr[r].tx1 - tr[r].tx0 + 1;
+      const int tile = (tr[r].ty0 + e / w) * a.ntx + tr[r].tx0 + e % w;
+      register_tile(a, tile, n * 3 + r / 2);
+    }
   }
   if (threadIdx.x == 0) {
     const double *cp = a.sc_pos + 4 * (long)n;

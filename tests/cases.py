@@ -96,3 +96,34 @@ def touching_pairs(ly=1.0):
         y=[0.20, 0.25, 0.02, ly - 0.03, 0.60, 0.40, 0.8],
         az=[0.3, 1.2, 0.1, 2.0, 1.5707963267948966, 0.7, 0.0],
     )
+
+
+def contact_pairs(n_pairs, ly, seed, overlap=(0.85, 1.02), a_range=(0.02, 0.04), ba=(0.4, 1.0)):
+    """Pairs of ellipses at controlled, shallow contact (centre distance
+    between 0.85 and 1.02 of the sum of the bounding radii), each pair in
+    its own cell of a grid of slots so that pairs do not interact; one pair
+    straddles the periodic boundary and one touches each wall."""
+    rng = np.random.default_rng(seed)
+    out = {k: [] for k in ("den", "a", "b", "x", "y", "az")}
+    nslot_y = max(1, n_pairs // 2)
+    for k in range(n_pairs):
+        a0, a1 = rng.uniform(*a_range, size=2)
+        b0, b1 = a0 * rng.uniform(*ba), a1 * rng.uniform(*ba)
+        r0, r1 = max(a0, b0), max(a1, b1)
+        d = (r0 + r1) * rng.uniform(*overlap)
+        th = rng.uniform(0, 2 * np.pi)
+        cx = 0.25 + 0.5 * (k % 2)
+        cy = (k // 2 + 0.5) * ly / nslot_y
+        if k == 0:
+            cy = 0.0  # across the periodic boundary
+        x0, y0 = cx - 0.5 * d * np.cos(th), cy - 0.5 * d * np.sin(th)
+        x1, y1 = cx + 0.5 * d * np.cos(th), cy + 0.5 * d * np.sin(th)
+        if k == 1:
+            x0 = min(a0, b0) * rng.uniform(0.8, 0.95)  # overlapping the wall x = 0
+        for name, v in (("den", rng.uniform(1, 1.5)), ("a", a0), ("b", b0), ("x", x0),
+                        ("y", y0 % ly), ("az", rng.uniform(0, np.pi))):
+            out[name].append(v)
+        for name, v in (("den", rng.uniform(1, 1.5)), ("a", a1), ("b", b1), ("x", x1),
+                        ("y", y1 % ly), ("az", rng.uniform(0, np.pi))):
+            out[name].append(v)
+    return {k: np.array(v) for k, v in out.items()}

@@ -1,0 +1,307 @@
+"""Parity of the gfx950 path (through the C-ABI, libeif_gpu.so) with the
+oracle on the same seeded inputs.
+
+Bars (north_star): per-step relative L2 of ux, uy <= 1e-12 and of p (less
+its spatial mean: the reference's pressure gauge is round-off determined,
+src/tdm.c:100-102) <= 1e-12; particle x, y, az <= 1e-9 relative after 1000
+steps of c0; colliding-pair sets identical.  The fluid functions are
+expected to be bit-identical in Thomas mode (same operations in the same
+order, same cosine table); the IBM weights differ from the oracle only by
+the last-bit differences of the device libm (sin, cos, asin, tanh)."""
+import numpy as np
+import pytest
+
+import cases
+from oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+eif = pytest.importorskip("ellipsesinflows_amd")
+
+
+def pair(mode=0, **kw):
+    g = eif.Solver(poisson_mode=mode, **kw)
+    o = Oracle(**kw)
+    return g, o
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def rel_mod_mean(a, b):
+    a = a[1:-1, 1:-1]
+    b = b[1:-1, 1:-1]
+    return rel(a - a.mean(), b - b.mean())
+
+
+def both(g, o, fn, *args):
+    return getattr(g, fn)(*args), getattr(o, fn)(*args)
+
+
+# ----------------------------------------------------------------- fluid
+@pytest.mark.parametrize("nx,ny,ly", [(64, 96, 1.0), (48, 40, 1.0), (256, 128, 2.0), (128, 36, 0.5)])
+def test_fluid_functions_bit_exact_thomas(nx, ny, ly):
+    g, o = pair(0, itot=nx, jtot=ny, ly=ly, Re=120.0, extfrcy=0.3, uy_wall_xm=-0.1, uy_wall_xp=0.2)
+    ux, uy, p = cases.random_fields(nx, ny, nx + ny)
+    for s in (g, o):
+        s.upload(ux, uy, p)
+        s.update_boundaries()
+    dg, do = both(g, o, "decide_dt")
+    assert dg == do
+    for rk in range(3):
+        both(g, o, "update_velocity", rk)
+        a, b = g.download(), o.download()
+        for k in ("ux", "uy"):
+            assert np.array_equal(a[k], b[k]), (rk, k)
+        for k in ("srcuxa", "srcuya"):
+            assert np.array_equal(g.field(k), o.field(k)), (rk, k)
+        both(g, o, "compute_potential", rk)
+        assert np.array_equal(g.field("psi"), o.field("psi")), rk
+        both(g, o, "correct_velocity", rk)
+        both(g, o, "update_pressure")
+        a, b = g.download(), o.download()
+        for k in ("ux", "uy", "p"):
+            assert np.array_equal(a[k], b[k]), (rk, k)
+    dg, do = both(g, o, "decide_dt")
+    assert dg == do
+    g.close()
+
+
+@pytest.mark.parametrize("nx,ny,ly", [(64, 96, 1.0), (256, 512, 2.0)])
+def test_fluid_partition_mode_within_tolerance(nx, ny, ly):
+    """The partitioned y-solve changes rounding only; psi agrees modulo its
+    mean to the conditioning floor of the periodic system (~cond * eps),
+    velocities to 1e-12."""
+    g, o = pair(1, itot=nx, jtot=ny, ly=ly, Re=120.0, extfrcy=0.3)
+    ux, uy, p = cases.random_fields(nx, ny, 3)
+    for s in (g, o):
+        s.upload(ux, uy, p)
+        s.update_boundaries()
+        s.set_dt(2e-3)
+    for rk in range(3):
+        both(g, o, "update_velocity", rk)
+        both(g, o, "compute_potential", rk)
+        assert rel_mod_mean(g.field("psi"), o.field("psi")) < 1e-10
+        both(g, o, "correct_velocity", rk)
+        both(g, o, "update_pressure")
+        a, b = g.download(), o.download()
+        assert rel(a["ux"], b["ux"]) < 1e-12 and rel(a["uy"], b["uy"]) < 1e-12
+        assert rel_mod_mean(a["p"], b["p"]) < 1e-10
+    dmax, _ = g.divergence()
+    assert dmax < 1e-9
+    g.close()
+
+
+def test_diagnostics_match_logging_formulas():
+    nx, ny = 96, 80
+    g, o = pair(0, itot=nx, jtot=ny, ly=1.3)
+    ux, uy, p = cases.random_fields(nx, ny, 8)
+    for s in (g, o):
+        s.upload(ux, uy, p)
+        s.update_boundaries()
+    dg, do = g.divergence(), o.divergence()
+    assert dg[0] == do[0]
+    assert dg[1] == pytest.approx(do[1], abs=1e-12 * np.abs(ux).max() * nx * ny)
+    assert np.allclose(g.momentum(), o.momentum(), rtol=1e-12, atol=1e-14)
+    assert np.allclose(g.energy(), o.energy(), rtol=1e-12, atol=1e-14)
+    g.close()
+
+
+# ------------------------------------------------------------------- IBM
+def _ibm_pair(parts, nx=128, ny=128, ly=1.0, seed=1):
+    g, o = pair(0, itot=nx, jtot=ny, ly=ly, Re=100.0, extfrcy=0.0)
+    ux, uy, p = cases.random_fields(nx, ny, seed, amp=0.3)
+    for s in (g, o):
+        s.upload(ux, uy, p)
+        s.update_boundaries()
+        s.set_particles(**parts)
+        s.set_dt(1e-3)
+    return g, o
+
+
+IBM_CASES = {
+    "c0": cases.c0_config()[1],
+    # images across y = 0 / ly and boxes that overlap each other
+    "periodic_overlap": dict(den=[1.0, 1.3, 1.1], a=[0.08, 0.06, 0.05], b=[0.04, 0.05, 0.02],
+                             x=[0.3, 0.42, 0.8], y=[0.02, 0.1, 0.97], az=[0.4, 2.0, 1.0],
+                             ux=[0.1, -0.2, 0.0], uy=[0.0, 0.3, -0.1], vz=[1.0, 0.0, -2.0]),
+}
+
+
+@pytest.mark.parametrize("name", sorted(IBM_CASES))
+def test_ibm_inertia_exchange_update(name):
+    parts = IBM_CASES[name]
+    g, o = _ibm_pair(parts)
+    for s in (g, o):
+        s.reset_particle_increments()
+        s.compute_inertia(0)
+    rg, ro = g.particles(), o.particles()
+    assert np.allclose(rg[:, 18:24], ro[:, 18:24], rtol=1e-13, atol=1e-15)
+    for s in (g, o):
+        s.exchange_momentum()
+    rg, ro = g.particles(), o.particles()
+    scale = np.abs(ro[:, 15:18]).max()
+    assert np.max(np.abs(rg[:, 15:18] - ro[:, 15:18])) < 1e-13 * scale
+    for k in ("dux", "duy"):
+        fg, fo = g.field(k), o.field(k)
+        assert np.max(np.abs(fg - fo)) < 1e-14 * max(np.abs(fo).max(), 1e-300), k
+        assert np.array_equal(fg == 0, fo == 0) or np.max(np.abs(fg - fo)) < 1e-300
+    for s in (g, o):
+        s.update_momentum_field()
+    a, b = g.download(), o.download()
+    assert rel(a["ux"], b["ux"]) < 1e-14 and rel(a["uy"], b["uy"]) < 1e-14
+    # moved particles: inertia at x + dx (inertia.c:35-37)
+    rec = ro.copy()
+    rec[:, 6:9] = [[1e-3, -2e-3, 0.05]] * len(rec)
+    for s in (g, o):
+        s.set_particle_records(rec)
+        s.compute_inertia(1)
+    rg, ro = g.particles(), o.particles()
+    assert np.allclose(rg[:, 18:24], ro[:, 18:24], rtol=1e-13, atol=1e-15)
+    g.close()
+
+
+# ------------------------------------------------------------- collisions
+@pytest.mark.parametrize("seed", [3, 4, 5])
+def test_collision_pairs_and_forces(seed):
+    parts = cases.contact_pairs(10, 1.0, seed, overlap=(0.7, 0.95))
+    g, o = pair(0, itot=32, jtot=32, ly=1.0)
+    for s in (g, o):
+        s.set_particles(**parts)
+    for cn in (0, 1):
+        for s in (g, o):
+            s.compute_collision_force(cn)
+        pg, po = g.colliding_pairs(), o.colliding_pairs()
+        assert pg == po
+        assert len(pg) >= 3
+        assert any(q[1] == -1 for q in pg)  # the wall contact
+        rg, ro = g.particles(), o.particles()
+        sl = slice(24 + cn, 30, 2)
+        scale = np.abs(ro[:, sl]).max()
+        assert np.max(np.abs(rg[:, sl] - ro[:, sl])) < 1e-12 * scale
+    g.close()
+
+
+def test_increment_and_update_particles():
+    parts = cases.contact_pairs(4, 1.0, 9)
+    g, o = pair(0, itot=32, jtot=32, ly=1.0, Fr=3.0)
+    rng = np.random.default_rng(2)
+    rec = np.zeros((8, 30))
+    for k, name in enumerate(("den", "a", "b", "x", "y", "az")):
+        rec[:, k] = parts[name]
+    rec[:, 9:30] = rng.uniform(-1, 1, size=(8, 21))
+    rec[0, 4] = 0.9999
+    rec[0, 10] = 5.0
+    for s in (g, o):
+        s.set_particles(**parts)
+        s.set_particle_records(rec)
+        s.set_dt(1e-3)
+    for rk in range(3):
+        rgd, rod = g.increment_particles(rk), o.increment_particles(rk)
+        assert rgd == rod
+        assert np.array_equal(g.particles(), o.particles())
+    for s in (g, o):
+        s.update_particles()
+    assert np.array_equal(g.particles(), o.particles())
+    g.close()
+
+
+# ----------------------------------------------------------- whole steps
+def test_c0_1000_steps():
+    """config c0 for 1000 steps: particle x, y, az within 1e-9 relative."""
+    cfg, parts = cases.c0_config()
+    g, o = pair(0, **cfg)
+    xc = g.grid()["xc"]
+    ux, uy, p = cases.couette_fields(cfg["itot"], cfg["jtot"], xc)
+    for s in (g, o):
+        s.upload(ux, uy, p)
+        s.update_boundaries()
+        s.set_particles(**parts)
+    worst = [0.0, 0.0, 0.0]
+    for k in range(1000):
+        dg, do = g.step(), o.step()
+        assert dg == do, k
+        if k % 100 == 99 or k < 3:
+            a, b = g.download(), o.download()
+            worst[0] = max(worst[0], rel(a["ux"], b["ux"]), rel(a["uy"], b["uy"]))
+            worst[1] = max(worst[1], rel_mod_mean(a["p"], b["p"]))
+    rg, ro = g.particles(), o.particles()
+    worst[2] = np.max(np.abs(rg[:, 3:6] - ro[:, 3:6]) / np.maximum(np.abs(ro[:, 3:6]), 1e-300))
+    assert worst[0] < 1e-12, worst
+    assert worst[1] < 1e-12, worst
+    assert worst[2] < 1e-9, worst
+    assert ro[0, 5] > 0.5  # the ellipse has rotated
+    g.close()
+
+
+def test_c1_full_size_steps():
+    """config c1 at its real shape (512x1024, ly=2, Re=500, 64 ellipses
+    a~U[.03,.05], b/a=.5, gap .01): 5 steps against the oracle."""
+    parts = cases.place_ellipses(64, 2.0, (0.03, 0.05), (0.5, 0.5), 0.01, 1)
+    kw = dict(itot=512, jtot=1024, ly=2.0, Re=500.0, extfrcy=8.0 / 500.0)
+    g, o = pair(0, **kw)
+    xc = g.grid()["xc"]
+    uy = np.zeros((1026, 514))
+    uy[1:-1, 1:-1] = 4.0 * xc[None, 1:-1] * (1 - xc[None, 1:-1])
+    for s in (g, o):
+        s.upload(uy=uy)
+        s.update_boundaries()
+        s.set_particles(**parts)
+    for k in range(5):
+        dg, do = g.step(), o.step()
+        assert dg == do
+        a, b = g.download(), o.download()
+        assert rel(a["ux"], b["ux"]) < 1e-12 and rel(a["uy"], b["uy"]) < 1e-12, k
+        assert rel_mod_mean(a["p"], b["p"]) < 1e-12, k
+    rg, ro = g.particles(), o.particles()
+    assert np.max(np.abs(rg[:, 3:6] - ro[:, 3:6])) < 1e-12
+    g.close()
+
+
+def test_contacts_through_whole_steps():
+    """ellipses in contact (collisions active) over several steps."""
+    parts = cases.contact_pairs(6, 1.0, 21, overlap=(0.9, 0.98))
+    kw = dict(itot=128, jtot=128, ly=1.0, Re=50.0, extfrcy=0.5)
+    g, o = pair(0, **kw)
+    for s in (g, o):
+        s.set_particles(**parts)
+    for k in range(8):
+        dg, do = g.step(), o.step()
+        assert dg == do
+    a, b = g.download(), o.download()
+    assert rel(a["uy"], b["uy"]) < 1e-12
+    rg, ro = g.particles(), o.particles()
+    assert np.max(np.abs(rg[:, 3:6] - ro[:, 3:6])) < 1e-12
+    assert np.max(np.abs(rg[:, 9:12] - ro[:, 9:12])) < 1e-9 * max(np.abs(ro[:, 9:12]).max(), 1.0)
+    g.close()
+
+
+def test_gpu_run_is_deterministic():
+    parts = cases.contact_pairs(6, 1.0, 22)
+    outs = []
+    for _ in range(2):
+        g = eif.Solver(itot=128, jtot=128, ly=1.0, Re=50.0, extfrcy=0.5, poisson_mode=1)
+        g.set_particles(**parts)
+        for _ in range(4):
+            g.step()
+        outs.append((g.download(), g.particles()))
+        g.close()
+    for k in ("ux", "uy", "p"):
+        assert np.array_equal(outs[0][0][k], outs[1][0][k])
+    assert np.array_equal(outs[0][1], outs[1][1])
+
+
+def test_errors_are_returned_not_raised_in_native_code():
+    with pytest.raises(eif.EIFError):
+        eif.Solver(itot=1, jtot=8)
+    g = eif.Solver(itot=16, jtot=16)
+    with pytest.raises(eif.EIFError):
+        g.update_velocity(3)
+    with pytest.raises(eif.EIFError):
+        g.set_dt(-1.0)
+    with pytest.raises(ValueError):
+        g.upload(ux=np.zeros(5))
+    with pytest.raises(eif.EIFError):
+        g.set_particles(den=[1.0], a=[-0.1], b=[0.1], x=[0.5], y=[0.5], az=[0.0])
+    g.close()
