@@ -48,7 +48,9 @@ EXPORTED = (
     "suspensions_gpu_compute_collision_force", "suspensions_gpu_increment_particles",
     "suspensions_gpu_update_particles", "suspensions_gpu_colliding_pairs",
     "eif_gpu_integrate", "eif_gpu_step", "eif_gpu_synchronize", "eif_gpu_last_error",
-    "eif_gpu_profile_enable", "eif_gpu_profile_reset", "eif_gpu_profile_query",
+    "eif_gpu_ibm_stats",
+    "eif_gpu_profile_enable", "eif_gpu_profile_select", "eif_gpu_profile_reset",
+    "eif_gpu_profile_query",
     "eif_gpu_profile_names",
 )
 
@@ -297,12 +299,21 @@ class Solver:
         self._check(self.lib.eif_gpu_step(self.h, ctypes.byref(dt)))
         return dt.value
 
+    def ibm_stats(self, reset=False):
+        """(cells visited, cells evaluated, Newton iterations, IBM calls)."""
+        out = (ctypes.c_ulonglong * 4)()
+        self._check(self.lib.eif_gpu_ibm_stats(self.h, out, int(bool(reset))))
+        return tuple(int(v) for v in out)
+
     def synchronize(self):
         self._check(self.lib.eif_gpu_synchronize(self.h))
 
     # -- profiling ------------------------------------------------------------
     def profile(self, enable=True):
         self._check(self.lib.eif_gpu_profile_enable(self.h, int(bool(enable))))
+
+    def profile_select(self, name=None):
+        self._check(self.lib.eif_gpu_profile_select(self.h, None if not name else name.encode()))
 
     def profile_reset(self):
         self._check(self.lib.eif_gpu_profile_reset(self.h))

@@ -30,6 +30,7 @@ static const char *k_prof_names =
 
 struct Profiler {
   bool on = false;
+  std::string only;
   struct Rec {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
     size_t used = 0;
@@ -58,6 +59,7 @@ static int prof_flush(Ctx *c) {
 
 void prof_begin(Ctx *c, const char *name) {
   if (!c->prof || !c->prof->on) return;
+  if (!c->prof->only.empty() && c->prof->only != name) return;
   auto &r = c->prof->recs[name];
   if (r.used == r.ev.size()) {
     hipEvent_t a, b;
@@ -69,6 +71,7 @@ void prof_begin(Ctx *c, const char *name) {
 
 void prof_end(Ctx *c, const char *name) {
   if (!c->prof || !c->prof->on) return;
+  if (!c->prof->only.empty() && c->prof->only != name) return;
   auto &r = c->prof->recs[name];
   if (r.used >= r.ev.size()) return;
   (void)hipEventRecord(r.ev[r.used].second, c->stream);
@@ -130,7 +133,8 @@ static void free_ctx(Ctx *c) {
   for (auto b : bufs)
     if (*b) (void)hipFree(*b);
   void *others[] = {c->th_flag, c->boxes, c->box_off, c->tile_cnt, c->tile_list,
-                    c->active, c->flags, c->pair_cnt, c->pair_ids, c->wall_hit};
+                    c->active, c->flags, c->pair_cnt, c->pair_ids, c->wall_hit,
+                    c->ibm_stats};
   for (auto o : others)
     if (o) (void)hipFree(o);
   if (c->hscal) (void)hipHostFree(c->hscal);
@@ -252,6 +256,8 @@ int fluid_gpu_create(const eif_config *cfg, eif_ctx **out) {
     if (e != hipSuccess) return fail(hip_fail(e, "hipHostMalloc", __FILE__, __LINE__));
     e = hipMalloc(&c->flags, 4 * sizeof(int));
     if (e == hipSuccess) e = hipMemset(c->flags, 0, 4 * sizeof(int));
+    if (e == hipSuccess) e = hipMalloc(&c->ibm_stats, 4 * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMemset(c->ibm_stats, 0, 4 * sizeof(unsigned long long));
     if (e != hipSuccess) return fail(hip_fail(e, "hipMalloc", __FILE__, __LINE__));
   }
   if ((rc = poisson_init(c))) return fail(rc);
@@ -614,9 +620,29 @@ int eif_gpu_synchronize(eif_ctx *c) {
   return check_flags(c);
 }
 
+int eif_gpu_ibm_stats(eif_ctx *c, unsigned long long out[4], int reset) {
+  CHECK_CTX(c);
+  unsigned long long h[4] = {0, 0, 0, 0};
+  EIF_HIP(hipMemcpyAsync(h, c->ibm_stats, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  EIF_HIP(hipStreamSynchronize(c->stream));
+  if (out)
+    for (int k = 0; k < 4; k++) out[k] = h[k];
+  if (reset) {
+    EIF_HIP(hipMemsetAsync(c->ibm_stats, 0, sizeof(h), c->stream));
+    EIF_HIP(hipStreamSynchronize(c->stream));
+  }
+  return EIF_OK;
+}
+
 int eif_gpu_profile_enable(eif_ctx *c, int enable) {
   CHECK_CTX(c);
   c->prof->on = enable != 0;
+  return EIF_OK;
+}
+
+int eif_gpu_profile_select(eif_ctx *c, const char *name) {
+  CHECK_CTX(c);
+  c->prof->only = name ? name : "";
   return EIF_OK;
 }
 

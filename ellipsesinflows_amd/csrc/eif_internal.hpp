@@ -111,6 +111,7 @@ struct Ctx {
   int ntx = 0, nty = 0;
   int *active = nullptr;   // [0] count, then tile ids
   int *flags = nullptr;    // device error flags
+  unsigned long long *ibm_stats = nullptr;  // [4] see eif_gpu_ibm_stats
   // collisions
   int *pair_cnt = nullptr;
   int pair_cap = 0;

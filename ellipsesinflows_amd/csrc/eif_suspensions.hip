@@ -74,10 +74,11 @@ __device__ inline EllipseShape make_shape(double a, double b) {
 }
 
 // src/ellipse.c:30-65
-__device__ double find_normal_t(const EllipseShape &e, double x_, double y_) {
+__device__ double find_normal_t(const EllipseShape &e, double x_, double y_, int &iters) {
   const double px = fabs(x_), py = fabs(y_);
   double t = 0.25 * EIF_PI;
-  for (int n = 0;; n++) {
+  int n;
+  for (n = 0;; n++) {
     double s, c;
     sincos(t, &s, &c);
     const double x = e.a * c, y = e.b * s;
@@ -94,6 +95,7 @@ __device__ double find_normal_t(const EllipseShape &e, double x_, double y_) {
     if (fabs(dt) < 1.e-8) break;
     if (n > 100) break;
   }
+  iters += n + 1;
   if (x_ < 0.) t = EIF_PI - t;
   if (y_ < 0.) t = -t;
   return t;
@@ -103,14 +105,14 @@ __device__ double find_normal_t(const EllipseShape &e, double x_, double y_) {
 // cos(-paz), sin(-paz)
 __device__ inline double signed_dist(const EllipseShape &e, double grid_size,
                                      double px, double py, double cr, double sr,
-                                     double x, double y) {
+                                     double x, double y, int &iters) {
   const double ddx = x - px, ddy = y - py;
   const double x_ = cr * ddx - sr * ddy;
   const double y_ = sr * ddx + cr * ddy;
   const double xa = x_ / e.a, yb = y_ / e.b;
   const double f0 = 1. - xa * xa - yb * yb;
   const double sign = f0 > 0. ? +1. : -1.;
-  const double t = find_normal_t(e, x_, y_);
+  const double t = find_normal_t(e, x_, y_, iters);
   double st, ct;
   sincos(t, &st, &ct);
   const double ex = e.a * ct - x_, ey = e.b * st - y_;
@@ -153,7 +155,25 @@ struct IbmArgs {
   int *tile_cnt, *tile_list, *active;
   int ntx, nty;
   int *flags;
+  unsigned long long *stats;  // cells (algorithmic), cells evaluated, Newton iterations, calls
 };
+
+__device__ inline void add_stats(const IbmArgs &a, long total, long evaluated, long iters,
+                                 unsigned long long *sh) {
+  // one atomic per statistic per workgroup; sh still holds the block sums
+  // being read by every thread, so wait before reusing it
+  __syncthreads();
+  if (threadIdx.x == 0) { sh[0] = 0; sh[1] = 0; }
+  __syncthreads();
+  atomicAdd(&sh[0], (unsigned long long)iters);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    atomicAdd(&a.stats[0], (unsigned long long)total);
+    atomicAdd(&a.stats[1], (unsigned long long)evaluated);
+    atomicAdd(&a.stats[2], sh[0]);
+    if (blockIdx.x == 0) atomicAdd(&a.stats[3], 1ull);
+  }
+}
 
 template <int N>
 __device__ inline void block_sum(double (&v)[N], double *sh) {
@@ -229,6 +249,7 @@ __global__ __launch_bounds__(IBM_THREADS) void k_inertia(IbmArgs a, int cnstep) 
   sincos(-paz, &sr, &cr);
   const double dxdy = a.dx * a.dy;
   double acc[3] = {0., 0., 0.};  // iux, iuy, ivz
+  int iters = 0;
   for (long t = threadIdx.x; t < bs.total; t += IBM_THREADS) {
     int k, i, j;
     cell_of(bs, t, k, i, j);
@@ -238,7 +259,7 @@ __global__ __launch_bounds__(IBM_THREADS) void k_inertia(IbmArgs a, int cnstep) 
     if (use_cache) {
       d = dist[t];
     } else {
-      d = signed_dist(e, a.grid_size, px, py_, cr, sr, x, y);
+      d = signed_dist(e, a.grid_size, px, py_, cr, sr, x, y, iters);
       dist[t] = d;
     }
     const double w = v_weight(d);
@@ -250,6 +271,7 @@ __global__ __launch_bounds__(IBM_THREADS) void k_inertia(IbmArgs a, int cnstep) 
     acc[2] += +(x - px) * valy / pim;
   }
   block_sum<3>(acc, sh);
+  add_stats(a, bs.total, use_cache ? 0 : bs.total, iters, reinterpret_cast<unsigned long long *>(sh));
   if (threadIdx.x == 0) {
     q[R_IUX + cnstep] = acc[0];
     q[R_IUY + cnstep] = acc[1];
@@ -360,6 +382,7 @@ __global__ __launch_bounds__(IBM_THREADS) void k_exchange(IbmArgs a) {
   sincos(-paz, &sr, &cr);
   const double dx = a.dx, dy = a.dy;
   double acc[3] = {0., 0., 0.};  // fux, fuy, tvz
+  int iters = 0;
   for (long t = threadIdx.x; t < bs.total; t += IBM_THREADS) {
     int k, i, j;
     cell_of(bs, t, k, i, j);
@@ -369,7 +392,7 @@ __global__ __launch_bounds__(IBM_THREADS) void k_exchange(IbmArgs a) {
     if (use_cache) {
       d = dist[t];
     } else {
-      d = signed_dist(e, a.grid_size, px, py_, cr, sr, x, y);
+      d = signed_dist(e, a.grid_size, px, py_, cr, sr, x, y, iters);
       dist[t] = d;
     }
     const double w = s_weight(d);
@@ -387,6 +410,7 @@ __global__ __launch_bounds__(IBM_THREADS) void k_exchange(IbmArgs a) {
     acc[2] -= +(x - px) * (fy * dx * dy) / pim;
   }
   block_sum<3>(acc, sh);
+  add_stats(a, bs.total, use_cache ? 0 : bs.total, iters, reinterpret_cast<unsigned long long *>(sh));
   if (threadIdx.x == 0) {
     q[R_FUX] = acc[0];
     q[R_FUY] = acc[1];
@@ -537,7 +561,8 @@ __device__ void evolute_fit(const EllD &e, double x, double y, double &ex,
     y_ = sm * x__ + cm * y__;
   }
   const EllipseShape s = make_shape(e.a, e.b);
-  const double t = find_normal_t(s, x_, y_);
+  int iters = 0;
+  const double t = find_normal_t(s, x_, y_, iters);
   r = 1. / fmax(curvature(e.a, e.b, t), DBL_EPSILON);
   double st, ct;
   sincos(t, &st, &ct);
@@ -870,6 +895,7 @@ static IbmArgs ibm_args(Ctx *c) {
   a.ntx = c->ntx;
   a.nty = c->nty;
   a.flags = c->flags;
+  a.stats = c->ibm_stats;
   return a;
 }
 

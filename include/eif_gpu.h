@@ -156,11 +156,20 @@ int eif_gpu_integrate(eif_ctx *ctx, int substeps[3]);
 int eif_gpu_step(eif_ctx *ctx, double *dt_out);
 
 int eif_gpu_synchronize(eif_ctx *ctx);
+
+/* IBM work counters since the last reset: out[0] box cells visited
+ * (the reference evaluates every one, inertia.c:45-63 / exchange.c:55-79),
+ * out[1] cells whose signed distance was computed (the rest were served by
+ * the per-particle distance cache), out[2] Newton iterations of
+ * find_normal_t, out[3] IBM kernel calls.  reset != 0 clears them. */
+int eif_gpu_ibm_stats(eif_ctx *ctx, unsigned long long out[4], int reset);
 const char *eif_gpu_last_error(void);
 
 /* Per-kernel timing with HIP events on the context's stream.  Names are
  * the kernel names listed by eif_gpu_profile_names(). */
 int eif_gpu_profile_enable(eif_ctx *ctx, int enable);
+/* restrict timing to one kernel name (NULL or "" times all of them) */
+int eif_gpu_profile_select(eif_ctx *ctx, const char *name);
 int eif_gpu_profile_reset(eif_ctx *ctx);
 int eif_gpu_profile_query(eif_ctx *ctx, const char *name, long *count,
                           double *total_ms);
