@@ -48,7 +48,8 @@ EXPORTED = (
     "suspensions_gpu_compute_collision_force", "suspensions_gpu_increment_particles",
     "suspensions_gpu_update_particles", "suspensions_gpu_colliding_pairs",
     "eif_gpu_integrate", "eif_gpu_step", "eif_gpu_synchronize", "eif_gpu_last_error",
-    "eif_gpu_ibm_stats",
+    "eif_gpu_ibm_stats", "statistics_gpu_collect", "statistics_gpu_reset",
+    "statistics_gpu_download",
     "eif_gpu_profile_enable", "eif_gpu_profile_select", "eif_gpu_profile_reset",
     "eif_gpu_profile_query",
     "eif_gpu_profile_names",
@@ -307,6 +308,23 @@ class Solver:
 
     def synchronize(self):
         self._check(self.lib.eif_gpu_synchronize(self.h))
+
+    # -- statistics (src/statistics/collect.c, output.c) -------------------------
+    def statistics_collect(self):
+        self._check(self.lib.statistics_gpu_collect(self.h))
+
+    def statistics_reset(self):
+        self._check(self.lib.statistics_gpu_reset(self.h))
+
+    def statistics(self):
+        """Accumulators ux1, ux2 (ux layout), uy1, uy2, phi (uy layout) and num."""
+        out = {k: np.empty(self.shapes["ux" if k[:2] == "ux" else "uy"])
+               for k in ("ux1", "ux2", "uy1", "uy2", "phi")}
+        num = ctypes.c_int()
+        self._check(self.lib.statistics_gpu_download(
+            self.h, *[_ptr(out[k]) for k in ("ux1", "ux2", "uy1", "uy2", "phi")], ctypes.byref(num)))
+        out["num"] = num.value
+        return out
 
     # -- profiling ------------------------------------------------------------
     def profile(self, enable=True):

@@ -1,8 +1,10 @@
 """Build recipes for the native libraries (run on a machine with hipcc).
 
 ``build_gpu()`` compiles the HIP sources into ``ellipsesinflows_amd/libeif_gpu.so``
-for gfx950; ``build_host()`` compiles the C driver ``eif_gpu_main`` (a
-drop-in for the reference's ``a.out``) against it.  Both build in-tree.
+for gfx950; ``build_host()`` compiles the host modules (parameters, npy
+save/restart, logging) into ``libeif_host.so`` and the C driver
+``eif_gpu_main`` (a drop-in for the reference's ``a.out``) against
+``libeif_gpu.so``.  Everything builds in-tree.
 """
 import os
 import subprocess
@@ -13,9 +15,12 @@ CSRC = os.path.join(HERE, "csrc")
 HOST = os.path.join(HERE, "host")
 LIB = os.path.join(HERE, "libeif_gpu.so")
 EXE = os.path.join(HERE, "eif_gpu_main")
+HOSTLIB = os.path.join(HERE, "libeif_host.so")
 
-HIP_SOURCES = ["eif_api.hip", "eif_fluid.hip", "eif_poisson.hip", "eif_suspensions.hip"]
-HOST_SOURCES = ["main.c", "param.c", "npyio.c", "save.c", "logging.c", "particles.c"]
+HIP_SOURCES = ["eif_api.hip", "eif_fluid.hip", "eif_poisson.hip", "eif_suspensions.hip",
+               "eif_statistics.hip"]
+HOST_SOURCES = ["param.c", "npyio.c", "save.c", "logging.c"]
+HOST_CFLAGS = ["-O2", "-std=c99", "-Wall", "-Wextra", "-Werror"]
 
 # -ffp-contract=off: expressions round like the reference's ISO C99 build.
 # packed FP32 instructions are disabled on this platform.
@@ -43,12 +48,15 @@ def build_gpu(hipcc="hipcc"):
 
 
 def build_host(cc="gcc"):
-    srcs = [s for s in HOST_SOURCES if os.path.exists(os.path.join(HOST, s))]
-    if not srcs:
-        return None
-    cmd = [cc, "-O2", "-std=c99", "-Wall", "-Wextra", "-I", os.path.join(ROOT, "include"),
-           "-o", EXE] + srcs + ["-L", HERE, "-leif_gpu", "-Wl,-rpath,$ORIGIN", "-lm"]
+    """libeif_host.so (no GPU dependency) and eif_gpu_main (needs libeif_gpu.so)."""
+    cmd = [cc] + HOST_CFLAGS + ["-fPIC", "-shared", "-o", HOSTLIB + ".tmp"] + HOST_SOURCES + ["-lm"]
     _run(cmd, HOST)
+    os.replace(HOSTLIB + ".tmp", HOSTLIB)
+    cmd = [cc] + HOST_CFLAGS + ["-I", os.path.join(ROOT, "include"), "-o", EXE + ".tmp",
+                                "main.c"] + HOST_SOURCES + [
+        "-L", HERE, "-leif_gpu", "-Wl,-rpath,$ORIGIN", "-lm"]
+    _run(cmd, HOST)
+    os.replace(EXE + ".tmp", EXE)
     return EXE
 
 

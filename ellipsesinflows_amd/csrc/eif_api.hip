@@ -129,7 +129,8 @@ static void free_ctx(Ctx *c) {
                      &c->th_q1, &c->th_den, &c->th_last, &c->pt_fwd, &c->pt_u,
                      &c->pt_v, &c->pt_w, &c->pt_bnd, &c->pt_red, &c->pt_lvl, &c->pt_ends, &c->pt_g, &c->dscal, &c->dred,
                      &c->prt, &c->pconst, &c->sc_fx, &c->sc_fy, &c->sc_dist, &c->sc_pos,
-                     &c->pair_val, &c->wall_val};
+                     &c->pair_val, &c->wall_val, &c->st_ux1, &c->st_ux2, &c->st_uy1,
+                     &c->st_uy2, &c->st_phi};
   for (auto b : bufs)
     if (*b) (void)hipFree(*b);
   void *others[] = {c->th_flag, c->boxes, c->box_off, c->tile_cnt, c->tile_list,
@@ -438,6 +439,31 @@ int fluid_gpu_momentum(eif_ctx *c, double moms[2]) {
 int fluid_gpu_energy(eif_ctx *c, double q[2]) {
   CHECK_CTX(c);
   return launch_diagnostics(c, 2, q);
+}
+
+int statistics_gpu_collect(eif_ctx *c) {
+  CHECK_CTX(c);
+  return launch_statistics(c);
+}
+
+int statistics_gpu_reset(eif_ctx *c) {
+  CHECK_CTX(c);
+  return stat_reset(c);
+}
+
+int statistics_gpu_download(eif_ctx *c, double *ux1, double *ux2, double *uy1,
+                            double *uy2, double *phi, int *num) {
+  CHECK_CTX(c);
+  TRY(stat_alloc(c));
+  const size_t nu = c->L.ux_len() * sizeof(double), np = c->L.p_len() * sizeof(double);
+  if (ux1) EIF_HIP(hipMemcpyAsync(ux1, c->st_ux1, nu, hipMemcpyDeviceToHost, c->stream));
+  if (ux2) EIF_HIP(hipMemcpyAsync(ux2, c->st_ux2, nu, hipMemcpyDeviceToHost, c->stream));
+  if (uy1) EIF_HIP(hipMemcpyAsync(uy1, c->st_uy1, np, hipMemcpyDeviceToHost, c->stream));
+  if (uy2) EIF_HIP(hipMemcpyAsync(uy2, c->st_uy2, np, hipMemcpyDeviceToHost, c->stream));
+  if (phi) EIF_HIP(hipMemcpyAsync(phi, c->st_phi, np, hipMemcpyDeviceToHost, c->stream));
+  EIF_HIP(hipStreamSynchronize(c->stream));
+  if (num) *num = c->st_num;
+  return EIF_OK;
 }
 
 int suspensions_gpu_create(eif_ctx *c, int n, const double *den, const double *a,

@@ -120,6 +120,10 @@ struct Ctx {
   double *wall_val = nullptr;     // np x 2 walls x 3
   int *wall_hit = nullptr;        // np x 2
   int last_npairs = 0;
+  // statistics accumulators (allocated on first use), reference layouts
+  double *st_ux1 = nullptr, *st_ux2 = nullptr;                   // like ux
+  double *st_uy1 = nullptr, *st_uy2 = nullptr, *st_phi = nullptr;  // like uy
+  int st_num = 0;
   Profiler *prof = nullptr;
 };
 
@@ -163,5 +167,8 @@ int launch_increment(Ctx *c, int rk);
 int launch_update_particles(Ctx *c);
 int launch_reset(Ctx *c);
 int check_flags(Ctx *c);
+int launch_statistics(Ctx *c);
+int stat_alloc(Ctx *c);
+int stat_reset(Ctx *c);
 
 }  // namespace eif

@@ -114,6 +114,20 @@ int fluid_gpu_divergence(eif_ctx *ctx, double *divmax, double *divsum);
 int fluid_gpu_momentum(eif_ctx *ctx, double moms[2]);
 int fluid_gpu_energy(eif_ctx *ctx, double quantities[2]);
 
+/* statistics_collect, src/statistics/collect.c:85-95: adds ux, ux^2, uy,
+ * uy^2 and the particle indicator phi (collect.c:48-83) to accumulators
+ * kept on the device (allocated and zeroed on first use, as
+ * src/statistics/init.c:8-44). */
+int statistics_gpu_collect(eif_ctx *ctx);
+/* Zero the accumulators and the sample count. */
+int statistics_gpu_reset(eif_ctx *ctx);
+/* The accumulators written by statistics_output
+ * (src/statistics/output.c:45-57) in the reference layouts
+ * (include/arrays/statistics.h): ux1, ux2 like ux; uy1, uy2, phi like uy.
+ * num receives the number of samples.  Any pointer may be NULL. */
+int statistics_gpu_download(eif_ctx *ctx, double *ux1, double *ux2,
+                            double *uy1, double *uy2, double *phi, int *num);
+
 /* suspensions_init's particle load (src/suspensions/init.c:24-86): the nine
  * SoA arrays of the initparticles .npy files. */
 int suspensions_gpu_create(eif_ctx *ctx, int n_particles, const double *den,

@@ -1257,6 +1257,49 @@ int orc_momentum(const orc_t *s, double moms[2]) {
   return 0;
 }
 
+/* src/statistics/collect.c:10-95, whole-grid loops as in the reference;
+ * pow(v, 1.) is v and pow(v, 2.) is v*v (GCC folds both in the reference
+ * build) */
+int orc_statistics_collect(const orc_t *s, double *ux1, double *ux2, double *uy1,
+                           double *uy2, double *phi) {
+  const int nx = s->itot, ny = s->jtot;
+  const double *ux = s->f[ORC_UX], *uy = s->f[ORC_UY];
+  for (int j = 1; j <= ny; j++)
+    for (int i = 1; i <= nx + 1; i++) {
+      const double v = ux[IX_UX(i, j)];
+      ux1[IX_UX(i, j)] += v;
+      ux2[IX_UX(i, j)] += v * v;
+    }
+  for (int j = 1; j <= ny; j++)
+    for (int i = 0; i <= nx + 1; i++) {
+      const double v = uy[IX_P(i, j)];
+      uy1[IX_P(i, j)] += v;
+      uy2[IX_P(i, j)] += v * v;
+    }
+  for (int n = 0; n < s->np; n++) {
+    const double *p = s->prt + (size_t)n * NREC;
+    const double pa = p[P_A], pb = p[P_B], px = p[P_X], py = p[P_Y], paz = p[P_AZ];
+    for (int periodic = -1; periodic <= 1; periodic++) {
+      const double py_ = py + s->ly * periodic;
+      for (int j = 1; j <= ny; j++) {
+        const double y = YC(j);
+        for (int i = 1; i <= nx; i++) {
+          const double x = XC(i);
+          const double dx = x - px;
+          const double dy = y - py_;
+          const double c = cos(-paz);
+          const double sn = sin(-paz);
+          const double x_ = c * dx - sn * dy;
+          const double y_ = sn * dx + c * dy;
+          const double ex = x_ / pa, ey = y_ / pb;
+          phi[IX_P(i, j)] += 1. - ex * ex - ey * ey > 0. ? 1. : 0.;
+        }
+      }
+    }
+  }
+  return 0;
+}
+
 int orc_energy(const orc_t *s, double q[2]) {
   const int nx = s->itot, ny = s->jtot;
   const double *ux = s->f[ORC_UX], *uy = s->f[ORC_UY];

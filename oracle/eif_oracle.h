@@ -96,6 +96,12 @@ int orc_divergence(const orc_t *s, double *divmax, double *divsum);
 int orc_momentum(const orc_t *s, double moms[2]);
 int orc_energy(const orc_t *s, double quantities[2]);
 
+/* statistics_collect (src/statistics/collect.c:10-95): adds the current
+ * sample to caller-owned accumulators in the reference layouts
+ * (include/arrays/statistics.h): ux1, ux2 like ux; uy1, uy2, phi like uy */
+int orc_statistics_collect(const orc_t *s, double *ux1, double *ux2, double *uy1,
+                           double *uy2, double *phi);
+
 /* pure kernels (no state) */
 double orc_find_normal_t(double a, double b, double x, double y);
 double orc_s_weight(double grid_size, double pa, double pb, double px,

@@ -174,3 +174,54 @@ int ref_load_particles(const char *dir, int cap, double *records) {
   suspensions_finalise(s);
   return n;
 }
+
+/* param_init (src/param/init.c:218-239) with the current environment:
+ * out = {timemax, wtimemax, log.rate, log.after, log.next, save.rate,
+ *        save.after, save.next, stat.rate, stat.after, stat.next, itot, jtot,
+ *        lx, ly, sf[0], sf[1], sf[2], Re, Fr, extfrcy, step, time,
+ *        load_flow_field} (24 values) */
+int ref_param_values(double *out) {
+  ensure_mpi();
+  FILE *saved = stdout;
+  stdout = fopen("/dev/null", "w");
+  param_t *p = param_init();
+  fclose(stdout);
+  stdout = saved;
+  const double v[24] = {p->timemax, p->wtimemax, p->log.rate, p->log.after, p->log.next,
+                        p->save.rate, p->save.after, p->save.next, p->stat.rate,
+                        p->stat.after, p->stat.next, p->itot, p->jtot, p->lx, p->ly,
+                        p->safefactors[0], p->safefactors[1], p->safefactors[2], p->Re,
+                        p->Fr, p->extfrcy, p->step, p->time, p->load_flow_field};
+  memcpy(out, v, sizeof(v));
+  param_finalise(p);
+  return 0;
+}
+
+/* src/fileio.c:283-365: 2-d fields through the reference's MPI-IO writer
+ * and reader for one rank; kind 0 ux-like, 1 uy-like, 2 p-like; the data
+ * are halo-padded arrays of the matching layout */
+static int field_io(int write, const char *dir, const char *name, int kind, int itot,
+                    int jtot, double *data) {
+  ensure_mpi();
+  param_t param;
+  memset(&param, 0, sizeof(param));
+  param.itot = itot;
+  param.jtot = jtot;
+  parallel_t parallel = {1, 0, 0, 0};
+  if (write) {
+    if (kind == 0) return fileio_w_ux_like_parallel(dir, name, &param, &parallel, data);
+    if (kind == 1) return fileio_w_uy_like_parallel(dir, name, &param, &parallel, data);
+    return fileio_w_p_like_parallel(dir, name, &param, &parallel, data);
+  }
+  if (kind == 0) return fileio_r_ux_like_parallel(dir, name, &param, &parallel, data);
+  if (kind == 1) return fileio_r_uy_like_parallel(dir, name, &param, &parallel, data);
+  return fileio_r_p_like_parallel(dir, name, &param, &parallel, data);
+}
+int ref_w_field(const char *dir, const char *name, int kind, int itot, int jtot,
+                double *data) {
+  return field_io(1, dir, name, kind, itot, jtot, data);
+}
+int ref_r_field(const char *dir, const char *name, int kind, int itot, int jtot,
+                double *data) {
+  return field_io(0, dir, name, kind, itot, jtot, data);
+}

@@ -238,6 +238,19 @@ class Oracle:
         self.lib.orc_step(self.h)
         return self.lib.orc_get_dt(self.h)
 
+    def statistics_collect(self):
+        if not hasattr(self, "_stats"):
+            self._stats = {k: np.zeros(self.shapes["ux" if k[:2] == "ux" else "uy"])
+                           for k in ("ux1", "ux2", "uy1", "uy2", "phi")}
+            self._stats["num"] = 0
+        st = self._stats
+        self.lib.orc_statistics_collect(self.h, *[_ptr(st[k]) for k in ("ux1", "ux2", "uy1",
+                                                                       "uy2", "phi")])
+        st["num"] += 1
+
+    def statistics(self):
+        return {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in self._stats.items()}
+
 
 def ref_lib():
     """The reference's own sources built by `make -C oracle ref`, or None."""

@@ -376,3 +376,39 @@ def test_update_wraps_position_and_angle():
     r = o.particles()[0]
     assert r[4] == pytest.approx(0.001, abs=1e-15)
     assert r[5] == pytest.approx(6.29 - 2 * 3.1415926535897932, abs=1e-15)
+
+
+def test_statistics_accumulate_and_indicator():
+    """statistics_collect (src/statistics/collect.c:10-95): collecting the
+    same fields k times gives k*u and k*u^2 (k <= 3 rounds identically),
+    and for an axis-aligned ellipse (az = 0, so the rotation is the
+    identity) phi is the textbook indicator of cell centres inside
+    ((x-px)/a)^2 + ((y-py)/b)^2 < 1, including the periodic image of an
+    ellipse that crosses y = 0."""
+    nx, ny, ly = 40, 50, 1.0
+    o = Oracle(itot=nx, jtot=ny, ly=ly, Re=100.0)
+    ux, uy, p = cases.random_fields(nx, ny, 5)
+    o.upload(ux, uy, p)
+    o.update_boundaries()
+    parts = dict(den=[1.0, 1.0], a=[0.2, 0.1], b=[0.1, 0.15], x=[0.4, 0.7], y=[0.5, 0.05],
+                 az=[0.0, 0.0])
+    o.set_particles(**parts)
+    for _ in range(3):
+        o.statistics_collect()
+    st = o.statistics()
+    f = o.download()
+    assert st["num"] == 3
+    rows = slice(1, ny + 1)
+    np.testing.assert_array_equal(st["ux1"][rows], 3 * f["ux"][rows])
+    np.testing.assert_array_equal(st["ux2"][rows], 3 * (f["ux"][rows] * f["ux"][rows]))
+    np.testing.assert_array_equal(st["uy1"][rows], 3 * f["uy"][rows])
+    assert not st["ux1"][0].any() and not st["uy2"][ny + 1].any()
+    xc = o.grid()["xc"]
+    yc = o.grid()["yc"]
+    X, Y = np.meshgrid(xc[1:nx + 1], yc)
+    inside = np.zeros((ny, nx))
+    for a, b, px, py in zip(parts["a"], parts["b"], parts["x"], parts["y"]):
+        for per in (-1, 0, 1):
+            inside += (1.0 - ((X - px) / a) ** 2 - ((Y - (py + ly * per)) / b) ** 2 > 0)
+    np.testing.assert_array_equal(st["phi"][1:ny + 1, 1:nx + 1], 3 * inside)
+    assert inside[:3].any() and inside[-3:].any()  # the second ellipse wraps
