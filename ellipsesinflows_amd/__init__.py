@@ -48,7 +48,7 @@ EXPORTED = (
     "suspensions_gpu_compute_collision_force", "suspensions_gpu_increment_particles",
     "suspensions_gpu_update_particles", "suspensions_gpu_colliding_pairs",
     "eif_gpu_integrate", "eif_gpu_step", "eif_gpu_synchronize", "eif_gpu_last_error",
-    "eif_gpu_ibm_stats", "statistics_gpu_collect", "statistics_gpu_reset",
+    "eif_gpu_ibm_stats", "eif_gpu_ibm_reference_iterations", "statistics_gpu_collect", "statistics_gpu_reset",
     "statistics_gpu_download",
     "eif_gpu_profile_enable", "eif_gpu_profile_select", "eif_gpu_profile_reset",
     "eif_gpu_profile_query",
@@ -301,10 +301,17 @@ class Solver:
         return dt.value
 
     def ibm_stats(self, reset=False):
-        """(cells visited, cells evaluated, Newton iterations, IBM calls)."""
-        out = (ctypes.c_ulonglong * 4)()
+        """(cells visited, cells evaluated, FP64 iterations, IBM calls, FP32 iterations)."""
+        out = (ctypes.c_ulonglong * 5)()
         self._check(self.lib.eif_gpu_ibm_stats(self.h, out, int(bool(reset))))
         return tuple(int(v) for v in out)
+
+    def ibm_reference_iterations(self):
+        """(cells, iterations) of the reference's FP64 find_normal_t over the
+        current IBM boxes."""
+        out = (ctypes.c_ulonglong * 2)()
+        self._check(self.lib.eif_gpu_ibm_reference_iterations(self.h, out))
+        return int(out[0]), int(out[1])
 
     def synchronize(self):
         self._check(self.lib.eif_gpu_synchronize(self.h))

@@ -130,12 +130,12 @@ static void free_ctx(Ctx *c) {
                      &c->pt_v, &c->pt_w, &c->pt_bnd, &c->pt_red, &c->pt_lvl, &c->pt_ends, &c->pt_g, &c->dscal, &c->dred,
                      &c->prt, &c->pconst, &c->sc_fx, &c->sc_fy, &c->sc_dist, &c->sc_pos,
                      &c->pair_val, &c->wall_val, &c->st_ux1, &c->st_ux2, &c->st_uy1,
-                     &c->st_uy2, &c->st_phi};
+                     &c->st_uy2, &c->st_phi, &c->slice_part};
   for (auto b : bufs)
     if (*b) (void)hipFree(*b);
   void *others[] = {c->th_flag, c->boxes, c->box_off, c->tile_cnt, c->tile_list,
                     c->active, c->flags, c->pair_cnt, c->pair_ids, c->wall_hit,
-                    c->ibm_stats};
+                    c->ibm_stats, c->slice_first, c->slice_info, c->ibm_pp, c->slice_owner};
   for (auto o : others)
     if (o) (void)hipFree(o);
   if (c->hscal) (void)hipHostFree(c->hscal);
@@ -257,8 +257,8 @@ int fluid_gpu_create(const eif_config *cfg, eif_ctx **out) {
     if (e != hipSuccess) return fail(hip_fail(e, "hipHostMalloc", __FILE__, __LINE__));
     e = hipMalloc(&c->flags, 4 * sizeof(int));
     if (e == hipSuccess) e = hipMemset(c->flags, 0, 4 * sizeof(int));
-    if (e == hipSuccess) e = hipMalloc(&c->ibm_stats, 4 * sizeof(unsigned long long));
-    if (e == hipSuccess) e = hipMemset(c->ibm_stats, 0, 4 * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMalloc(&c->ibm_stats, 8 * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMemset(c->ibm_stats, 0, 8 * sizeof(unsigned long long));
     if (e != hipSuccess) return fail(hip_fail(e, "hipMalloc", __FILE__, __LINE__));
   }
   if ((rc = poisson_init(c))) return fail(rc);
@@ -646,18 +646,50 @@ int eif_gpu_synchronize(eif_ctx *c) {
   return check_flags(c);
 }
 
-int eif_gpu_ibm_stats(eif_ctx *c, unsigned long long out[4], int reset) {
+int eif_gpu_ibm_stats(eif_ctx *c, unsigned long long out[5], int reset) {
   CHECK_CTX(c);
-  unsigned long long h[4] = {0, 0, 0, 0};
-  EIF_HIP(hipMemcpyAsync(h, c->ibm_stats, sizeof(h), hipMemcpyDeviceToHost, c->stream));
-  EIF_HIP(hipStreamSynchronize(c->stream));
-  if (out)
-    for (int k = 0; k < 4; k++) out[k] = h[k];
-  if (reset) {
-    EIF_HIP(hipMemsetAsync(c->ibm_stats, 0, sizeof(h), c->stream));
+  unsigned long long h[5] = {0, 0, 0, c->ibm_calls, 0};
+  if (c->np > 0) {
+    std::vector<unsigned long long> pp(4 * (size_t)c->np);
+    EIF_HIP(hipMemcpyAsync(pp.data(), c->ibm_pp, pp.size() * sizeof(unsigned long long),
+                           hipMemcpyDeviceToHost, c->stream));
     EIF_HIP(hipStreamSynchronize(c->stream));
+    for (int n = 0; n < c->np; n++) {
+      h[0] += pp[4 * (size_t)n];
+      h[1] += pp[4 * (size_t)n + 1];
+      h[2] += pp[4 * (size_t)n + 2];
+      h[4] += pp[4 * (size_t)n + 3];
+    }
+  }
+  if (out)
+    for (int k = 0; k < 5; k++) out[k] = h[k];
+  if (reset) {
+    c->ibm_calls = 0;
+    if (c->np > 0) {
+      EIF_HIP(hipMemsetAsync(c->ibm_pp, 0, 4 * (size_t)c->np * sizeof(unsigned long long),
+                             c->stream));
+      EIF_HIP(hipStreamSynchronize(c->stream));
+    }
   }
   return EIF_OK;
+}
+
+int eif_gpu_ibm_reference_iterations(eif_ctx *c, unsigned long long out[2]) {
+  CHECK_CTX(c);
+  if (!out) {
+    set_error("null output");
+    return EIF_ERR_ARG;
+  }
+  // two counters in the unused tail of the statistics array
+  unsigned long long *d = c->ibm_stats + 6;
+  EIF_HIP(hipMemsetAsync(d, 0, 2 * sizeof(unsigned long long), c->stream));
+  TRY(launch_ref_iterations(c, d));
+  unsigned long long h[2];
+  EIF_HIP(hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  EIF_HIP(hipStreamSynchronize(c->stream));
+  out[0] = h[0];
+  out[1] = h[1];
+  return check_flags(c);
 }
 
 int eif_gpu_profile_enable(eif_ctx *c, int enable) {

@@ -107,11 +107,18 @@ struct Ctx {
   double *sc_fx = nullptr, *sc_fy = nullptr;  // per box cell forcing*dt
   double *sc_dist = nullptr;                  // cached signed distance
   double *sc_pos = nullptr;                   // np x 3: positions of cache
+  int *slice_first = nullptr;  // IBM work slices
+  long *slice_info = nullptr;
+  int *slice_owner = nullptr;
+  double *slice_part = nullptr;
+  int nslices = 0;
   int *tile_cnt = nullptr, *tile_list = nullptr;
   int ntx = 0, nty = 0;
   int *active = nullptr;   // [0] count, then tile ids
   int *flags = nullptr;    // device error flags
-  unsigned long long *ibm_stats = nullptr;  // [4] see eif_gpu_ibm_stats
+  unsigned long long *ibm_stats = nullptr;  // [8] measurement counters
+  unsigned long long *ibm_pp = nullptr;     // np x 4 IBM work counters
+  unsigned long long ibm_calls = 0;
   // collisions
   int *pair_cnt = nullptr;
   int pair_cap = 0;
@@ -168,6 +175,7 @@ int launch_update_particles(Ctx *c);
 int launch_reset(Ctx *c);
 int check_flags(Ctx *c);
 int launch_statistics(Ctx *c);
+int launch_ref_iterations(Ctx *c, unsigned long long *dout);
 int stat_alloc(Ctx *c);
 int stat_reset(Ctx *c);
 

@@ -120,9 +120,188 @@ __device__ inline double signed_dist(const EllipseShape &e, double grid_size,
   return dist / grid_size;
 }
 
-__device__ inline double v_weight(double d) { return 0.5 * (1. + tanh(BETA * d)); }
+// ---------------------------------------------------------------------
+// Foot of the normal for the IBM weights.
+//
+// src/ellipse.c:30-65 iterates t <- clamp(t + dt) from t = pi/4 until
+// |dt| < 1e-8, every iteration in FP64 with sin, cos, asin and three
+// square roots.  The iteration converges quadratically to the unique foot
+// on the first-quadrant arc, so here it starts in FP32 (hardware sin/cos,
+// stopped at |dt| < 1e-3, where the FP32 rounding noise takes over) and
+// finishes with the reference's FP64 iteration and stopping rule, which
+// then needs one or two iterations instead of about four.  After the first
+// FP64 step every update of t is below 2^-10, so cos and sin of the new t
+// follow from the old ones by the addition theorem with the Taylor series
+// of cos(delta) and sin(delta) (truncation below 1e-21), and only one FP64
+// sincos is evaluated per cell.  The converged (cos t, sin t) agree with
+// the reference's to a few units in the last place; the distance to the
+// foot is stationary in t, so it agrees to rounding.
+// ---------------------------------------------------------------------
+struct EllipseShape32 {
+  float a, b, kex, key, a2b2;
+};
+
+__device__ inline EllipseShape32 shape32(const EllipseShape &e) {
+  return {(float)e.a, (float)e.b, (float)e.kex, (float)e.key, (float)e.a2b2};
+}
+
+// sin and cos on [0, pi/2] (slightly beyond is fine): Taylor polynomials
+// on |u| <= pi/4 (truncation below 1e-20) after u = pi/2 - t for t > pi/4,
+// with pi/2 split into two doubles so that the reduction is exact to
+// rounding.  The general libm routines carry large-argument reduction
+// that is never needed here and costs registers in the cell loop.
+__device__ inline double sin_poly(double u) {
+  const double u2 = u * u;
+  double p = 1. / 355687428096000.;
+  p = fma(p, u2, -1. / 1307674368000.);
+  p = fma(p, u2, 1. / 6227020800.);
+  p = fma(p, u2, -1. / 39916800.);
+  p = fma(p, u2, 1. / 362880.);
+  p = fma(p, u2, -1. / 5040.);
+  p = fma(p, u2, 1. / 120.);
+  p = fma(p, u2, -1. / 6.);
+  return fma(u * u2, p, u);
+}
+__device__ inline double cos_poly(double u) {
+  const double u2 = u * u;
+  double p = -1. / 6402373705728000.;
+  p = fma(p, u2, 1. / 20922789888000.);
+  p = fma(p, u2, -1. / 87178291200.);
+  p = fma(p, u2, 1. / 479001600.);
+  p = fma(p, u2, -1. / 3628800.);
+  p = fma(p, u2, 1. / 40320.);
+  p = fma(p, u2, -1. / 720.);
+  p = fma(p, u2, 1. / 24.);
+  p = fma(p, u2, -0.5);
+  return fma(u2, p, 1.);
+}
+__device__ inline void sincos_q1(double t, double &s, double &c) {
+  constexpr double PIO2_HI = 1.5707963267948966;      // nearest double to pi/2
+  constexpr double PIO2_LO = 6.123233995736766e-17;   // pi/2 - PIO2_HI
+  if (t <= 0.78539816339744831) {
+    s = sin_poly(t);
+    c = cos_poly(t);
+  } else {
+    const double u = (PIO2_HI - t) + PIO2_LO;
+    s = cos_poly(u);
+    c = sin_poly(u);
+  }
+}
+
+// asin by its Maclaurin series, exact to rounding for |x| <= 2^-6; for
+// larger |x| a monotone under-estimate, which only shortens a step of the
+// fixed-point iteration (its fixed point is x = 0)
+__device__ inline double asin_series(double x) {
+  const double x2 = x * x;
+  double p = 0.022372159090909091;  // 63/2816
+  p = fma(p, x2, 0.030381944444444444);  // 35/1152
+  p = fma(p, x2, 0.044642857142857144);  // 5/112
+  p = fma(p, x2, 0.075);                 // 3/40
+  p = fma(p, x2, 1. / 6.);
+  return fma(x * x2, p, x);
+}
+
+// cos and sin of t + delta from c = cos t, s = sin t, |delta| <= 2^-10
+__device__ inline void rotate_small(double &c, double &s, double delta) {
+  const double d2 = delta * delta;
+  const double hc = d2 * (0.5 - d2 * (1. / 24.));           // 1 - cos(delta)
+  const double sd = delta * (1. - d2 * (1. / 6. - d2 * (1. / 120.)));  // sin(delta)
+  const double cn = c - (s * sd + c * hc);
+  const double sn = s + (c * sd - s * hc);
+  c = cn;
+  s = sn;
+}
+
+// (c, s) = (cos t, sin t) of the foot on the first-quadrant arc for the
+// folded point (px, py) >= 0
+__device__ inline void foot_point(const EllipseShape &e, const EllipseShape32 &f,
+                                  double px, double py, double &c, double &s,
+                                  int &it32, int &it64) {
+  constexpr double HALF_PI = 0.5 * EIF_PI;
+  float tf = (float)(0.25 * EIF_PI);
+  {
+    const float fx = (float)px, fy = (float)py;
+    for (int n = 0; n < 8; n++) {
+      float sn, cs;
+      __sincosf(tf, &sn, &cs);
+      const float x = f.a * cs, y = f.b * sn;
+      const float ex = f.kex * (cs * cs * cs), ey = f.key * (sn * sn * sn);
+      const float rx = x - ex, ry = y - ey;
+      const float qx = fx - ex, qy = fy - ey;
+      const float rr = rx * rx + ry * ry, qq = qx * qx + qy * qy;
+      float arg = (rx * qy - ry * qx) * __builtin_amdgcn_rsqf(rr * qq);
+      arg = fminf(fmaxf(arg, -1.f), 1.f);
+      const float den = fmaxf(f.a2b2 - x * x - y * y, 1e-30f);
+      const float a2 = arg * arg;
+      const float as = arg + arg * a2 * (0.16666667f + a2 * (0.075f + a2 * 0.044642857f));
+      const float dt = __builtin_amdgcn_sqrtf(rr) * as * __builtin_amdgcn_rsqf(den);
+      it32++;
+      if (!(fabsf(dt) < 1e30f)) break;  // no usable step: hand over to FP64
+      tf = fminf(fmaxf(tf + dt, 0.f), 1.5707963f);
+      if (fabsf(dt) < 1e-3f) break;
+    }
+  }
+  double t = (double)tf;
+  sincos_q1(t, s, c);
+  for (int n = 0;; n++) {
+    const double x = e.a * c, y = e.b * s;
+    const double ex = e.kex * cube(c), ey = e.key * cube(s);
+    const double rx = x - ex, ry = y - ey;
+    const double qx = px - ex, qy = py - ey;
+    const double r = sqrt(ry * ry + rx * rx);
+    const double q = sqrt(qy * qy + qx * qx);
+    const double arg = (rx * qy - ry * qx) / (r * q);
+    const double dc = r * asin_series(arg);
+    const double dt = dc / sqrt(e.a2b2 - x * x - y * y);
+    double tn = t + dt;
+    tn = fmin(HALF_PI, tn);
+    tn = fmax(0., tn);
+    const double delta = tn - t;
+    if (fabs(delta) <= 0x1p-10 && tn != HALF_PI && tn != 0.)
+      rotate_small(c, s, delta);
+    else
+      sincos_q1(tn, s, c);
+    t = tn;
+    it64++;
+    if (fabs(dt) < 1.e-8) break;
+    if (n > 100) break;
+  }
+}
+
+// compute_signed_dist (src/suspensions/auxiliary.c:19-47) with the foot
+// from foot_point; the unfolding of ellipse.c:62-63 maps (cos t, sin t) of
+// the folded foot to (-cos t, sin t) for x_ < 0 and (cos t, -sin t) for
+// y_ < 0
+__device__ inline double signed_dist_fast(const EllipseShape &e, const EllipseShape32 &f,
+                                          double grid_size, double px, double py, double cr,
+                                          double sr, double x, double y, int &it32,
+                                          int &it64) {
+  const double ddx = x - px, ddy = y - py;
+  const double x_ = cr * ddx - sr * ddy;
+  const double y_ = sr * ddx + cr * ddy;
+  const double xa = x_ / e.a, yb = y_ / e.b;
+  const double f0 = 1. - xa * xa - yb * yb;
+  const double sign = f0 > 0. ? +1. : -1.;
+  double ct, st;
+  foot_point(e, f, fabs(x_), fabs(y_), ct, st, it32, it64);
+  const double fx = x_ < 0. ? -(e.a * ct) : e.a * ct;
+  const double fy = y_ < 0. ? -(e.b * st) : e.b * st;
+  const double ex = fx - x_, ey = fy - y_;
+  const double dist = sign * sqrt(+ex * ex + ey * ey);
+  return dist / grid_size;
+}
+
+// src/suspensions/auxiliary.c:49-59.  tanh(x) as 1 - 2 / (exp(2|x|) + 1)
+// with the sign of x: like the libm routine it rounds to exactly +-1 once
+// 2 / (exp(2|x|) + 1) drops below half an ulp of 1, so the weights are
+// exactly 0 or 1 on the same cells.
+__device__ inline double tanh_w(double x) {
+  const double th = 1. - 2. / (exp(2. * fabs(x)) + 1.);
+  return copysign(th, x);
+}
+__device__ inline double v_weight(double d) { return 0.5 * (1. + tanh_w(BETA * d)); }
 __device__ inline double s_weight(double d) {
-  const double th = tanh(BETA * d);
+  const double th = tanh_w(BETA * d);
   return 0.5 * BETA * (1. - th * th);
 }
 
@@ -140,6 +319,8 @@ __device__ inline double p_moi(const double *q) {
 // IBM integrals
 // ---------------------------------------------------------------------
 constexpr int IBM_THREADS = 256;
+constexpr int IBM_SLICE = 256;  // box cells per wave in the inertia / exchange kernels
+constexpr int SLICE_VALS = 5;   // per-slice partials: 3 sums, FP64 and FP32 iterations
 
 struct IbmArgs {
   Layout L;
@@ -155,39 +336,25 @@ struct IbmArgs {
   int *tile_cnt, *tile_list, *active;
   int ntx, nty;
   int *flags;
-  unsigned long long *stats;  // cells (algorithmic), cells evaluated, Newton iterations, calls
+  unsigned long long *stats;  // per particle: cells, cells evaluated, FP64 and FP32 iterations
+  int np;
+  const int *slice_first;  // np+1: first slice of each particle
+  const int *slice_owner;  // particle of each slice
+  long *slice_info;        // np x 2: box cells, cells evaluated (this call)
+  double *slice_part;      // per-slice partial sums (3 each)
 };
 
-__device__ inline void add_stats(const IbmArgs &a, long total, long evaluated, long iters,
-                                 unsigned long long *sh) {
-  // one atomic per statistic per workgroup; sh still holds the block sums
-  // being read by every thread, so wait before reusing it
-  __syncthreads();
-  if (threadIdx.x == 0) { sh[0] = 0; sh[1] = 0; }
-  __syncthreads();
-  atomicAdd(&sh[0], (unsigned long long)iters);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    atomicAdd(&a.stats[0], (unsigned long long)total);
-    atomicAdd(&a.stats[1], (unsigned long long)evaluated);
-    atomicAdd(&a.stats[2], sh[0]);
-    if (blockIdx.x == 0) atomicAdd(&a.stats[3], 1ull);
-  }
+// fixed-order tree over the 64 lanes of a wave (lane 0 receives the sum)
+__device__ inline double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
 }
 
-template <int N>
-__device__ inline void block_sum(double (&v)[N], double *sh) {
-  // fixed-order tree over IBM_THREADS partial sums
-  for (int k = 0; k < N; k++) sh[k * IBM_THREADS + threadIdx.x] = v[k];
-  __syncthreads();
-  for (int s = IBM_THREADS / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s)
-      for (int k = 0; k < N; k++)
-        sh[k * IBM_THREADS + threadIdx.x] =
-            sh[k * IBM_THREADS + threadIdx.x] + sh[k * IBM_THREADS + threadIdx.x + s];
-    __syncthreads();
-  }
-  for (int k = 0; k < N; k++) v[k] = sh[k * IBM_THREADS];
+// A slice's partial sums (lane 0); k_ibm_finalize adds them per particle in
+// slice order after the kernel has finished, so no fence or atomic is needed.
+__device__ inline void slice_store(const IbmArgs &a, int slice, const double v[SLICE_VALS]) {
+  if (threadIdx.x == 0)
+    for (int k = 0; k < SLICE_VALS; k++) a.slice_part[SLICE_VALS * (long)slice + k] = v[k];
 }
 
 struct BoxSet {
@@ -220,67 +387,135 @@ __device__ inline void cell_of(const BoxSet &bs, long t, int &k, int &i, int &j)
   i = bs.imin[k] + (int)(t % w);
 }
 
-// src/suspensions/inertia.c:9-70
-__global__ __launch_bounds__(IBM_THREADS) void k_inertia(IbmArgs a, int cnstep) {
-  __shared__ double sh[3 * IBM_THREADS];
-  __shared__ int use_cache;
-  const int n = blockIdx.x;
+// src/suspensions/inertia.c:9-70.  One wave per slice of IBM_SLICE box
+// cells of one particle (its three images' boxes flattened); the last
+// slice of a particle to finish adds the slices' sums in slice order.
+__global__ __launch_bounds__(64) void k_inertia(IbmArgs a, int cnstep) {
+  const int slice = blockIdx.x;
+  const int n = a.slice_owner[slice];
+  const int s0 = a.slice_first[n], nsl = a.slice_first[n + 1] - s0;
   double *q = a.prt + (long)n * R_N;
   const double pa = q[R_A], pb = q[R_B];
   const double px = q[R_X] + q[R_DX];
   const double py = q[R_Y] + q[R_DY];
   const double paz = q[R_AZ] + q[R_DAZ];
-  const double pm = p_mass(q), pim = p_moi(q);
+  const double inv_pm = 1. / p_mass(q), inv_pim = 1. / p_moi(q);
   const EllipseShape e = make_shape(pa, pb);
+  const EllipseShape32 f = shape32(e);
   BoxSet bs;
   make_boxes(a, px, py, fmax(pa, pb), bs);
-  const long cap = a.box_off[n + 1] - a.box_off[n];
-  if (bs.total > cap) {
+  const long total = bs.total;
+  if (total > a.box_off[n + 1] - a.box_off[n] || total > (long)nsl * IBM_SLICE) {
     if (threadIdx.x == 0) atomicOr(&a.flags[0], 1);
     return;
   }
-  if (threadIdx.x == 0) {
-    const double *cp = a.sc_pos + 4 * (long)n;
-    use_cache = cp[3] != 0. && cp[0] == px && cp[1] == py && cp[2] == paz;
-  }
-  __syncthreads();
+  const double *cp = a.sc_pos + 4 * (long)n;
+  const bool cached = cp[3] != 0. && cp[0] == px && cp[1] == py && cp[2] == paz;
   double *dist = a.sc_dist + a.box_off[n];
   double cr, sr;
   sincos(-paz, &sr, &cr);
   const double dxdy = a.dx * a.dy;
   double acc[3] = {0., 0., 0.};  // iux, iuy, ivz
-  int iters = 0;
-  for (long t = threadIdx.x; t < bs.total; t += IBM_THREADS) {
+  int it64 = 0, it32 = 0;
+  const long t0 = (long)(slice - s0) * IBM_SLICE;
+  const long t1 = t0 + IBM_SLICE < total ? t0 + IBM_SLICE : total;
+  for (long t = t0 + threadIdx.x; t < t1; t += 64) {
     int k, i, j;
     cell_of(bs, t, k, i, j);
     const double py_ = py + a.ly * (k - 1);
     const double x = a.xc[i], y = a.yc[j - 1];
     double d;
-    if (use_cache) {
+    if (cached) {
       d = dist[t];
     } else {
-      d = signed_dist(e, a.grid_size, px, py_, cr, sr, x, y, iters);
+      d = signed_dist_fast(e, f, a.grid_size, px, py_, cr, sr, x, y, it32, it64);
       dist[t] = d;
     }
     const double w = v_weight(d);
     const double valx = w * 0.5 * (a.ux[a.L.ux(i, j)] + a.ux[a.L.ux(i + 1, j)]) * dxdy;
     const double valy = w * 0.5 * (a.uy[a.L.p(i, j)] + a.uy[a.L.p(i, j + 1)]) * dxdy;
-    acc[0] += valx / pm;
-    acc[2] += -(y - py_) * valx / pim;
-    acc[1] += valy / pm;
-    acc[2] += +(x - px) * valy / pim;
+    acc[0] += valx * inv_pm;
+    acc[2] += -(y - py_) * valx * inv_pim;
+    acc[1] += valy * inv_pm;
+    acc[2] += +(x - px) * valy * inv_pim;
   }
-  block_sum<3>(acc, sh);
-  add_stats(a, bs.total, use_cache ? 0 : bs.total, iters, reinterpret_cast<unsigned long long *>(sh));
+  double part[SLICE_VALS] = {wave_sum(acc[0]), wave_sum(acc[1]), wave_sum(acc[2]),
+                             wave_sum((double)it64), wave_sum((double)it32)};
+  slice_store(a, slice, part);
+  if (slice == s0 && threadIdx.x == 0) {
+    a.slice_info[2 * (long)n] = total;
+    a.slice_info[2 * (long)n + 1] = cached ? 0 : total;
+  }
+}
+
+// Per-particle totals of the sliced inertia / exchange kernels, added in
+// slice order; also records the position whose distances are now cached
+// and the work counters.  which: 0..1 inertia (cnstep), 2 exchange.
+__global__ void k_ibm_finalize(IbmArgs a, int which) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= a.np) return;
+  double tot[SLICE_VALS] = {0., 0., 0., 0., 0.};
+  for (int sl = a.slice_first[n]; sl < a.slice_first[n + 1]; sl++)
+    for (int k = 0; k < SLICE_VALS; k++) tot[k] += a.slice_part[SLICE_VALS * (long)sl + k];
+  double *q = a.prt + (long)n * R_N;
+  double px = q[R_X], py = q[R_Y], paz = q[R_AZ];
+  if (which < 2) {
+    q[R_IUX + which] = tot[0];
+    q[R_IUY + which] = tot[1];
+    q[R_IVZ + which] = tot[2];
+    px += q[R_DX];
+    py += q[R_DY];
+    paz += q[R_DAZ];
+  } else {
+    q[R_FUX] = tot[0];
+    q[R_FUY] = tot[1];
+    q[R_TVZ] = tot[2];
+  }
+  double *cw = a.sc_pos + 4 * (long)n;
+  cw[0] = px;
+  cw[1] = py;
+  cw[2] = paz;
+  cw[3] = 1.;
+  unsigned long long *c = a.stats + 4 * (long)n;
+  c[0] += (unsigned long long)a.slice_info[2 * (long)n];
+  c[1] += (unsigned long long)a.slice_info[2 * (long)n + 1];
+  c[2] += (unsigned long long)tot[3];
+  c[3] += (unsigned long long)tot[4];
+}
+
+// Measurement only: the reference's FP64 iteration (find_normal_t,
+// src/ellipse.c:30-65) at every cell of the particles' current inertia
+// boxes, counting cells and iterations into out[0], out[1].
+__global__ __launch_bounds__(IBM_THREADS) void k_ref_iterations(IbmArgs a,
+                                                                unsigned long long *out) {
+  __shared__ BoxSet bs;
+  __shared__ unsigned long long sum;
+  const int n = blockIdx.x;
+  const double *q = a.prt + (long)n * R_N;
+  const double pa = q[R_A], pb = q[R_B];
+  const double px = q[R_X] + q[R_DX];
+  const double py = q[R_Y] + q[R_DY];
+  const double paz = q[R_AZ] + q[R_DAZ];
+  const EllipseShape e = make_shape(pa, pb);
   if (threadIdx.x == 0) {
-    q[R_IUX + cnstep] = acc[0];
-    q[R_IUY + cnstep] = acc[1];
-    q[R_IVZ + cnstep] = acc[2];
-    double *cp = a.sc_pos + 4 * (long)n;
-    cp[0] = px;
-    cp[1] = py;
-    cp[2] = paz;
-    cp[3] = 1.;
+    make_boxes(a, px, py, fmax(pa, pb), bs);
+    sum = 0;
+  }
+  __syncthreads();
+  double cr, sr;
+  sincos(-paz, &sr, &cr);
+  int iters = 0;
+  for (long t = threadIdx.x; t < bs.total; t += IBM_THREADS) {
+    int k, i, j;
+    cell_of(bs, t, k, i, j);
+    const double py_ = py + a.ly * (k - 1);
+    (void)signed_dist(e, a.grid_size, px, py_, cr, sr, a.xc[i], a.yc[j - 1], iters);
+  }
+  if (iters) atomicAdd(&sum, (unsigned long long)iters);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    atomicAdd(&out[0], (unsigned long long)bs.total);
+    atomicAdd(&out[1], sum);
   }
 }
 
@@ -315,66 +550,57 @@ __device__ inline void register_tile(const IbmArgs &a, int tile, int box_id) {
     atomicOr(&a.flags[0], 2);
 }
 
-// src/suspensions/exchange.c:13-88
-__global__ __launch_bounds__(IBM_THREADS) void k_exchange(IbmArgs a) {
-  __shared__ double sh[3 * IBM_THREADS];
-  __shared__ int use_cache;
-  const int n = blockIdx.x;
+// src/suspensions/exchange.c:13-88, sliced like k_inertia; the first
+// slice of a particle also publishes its boxes and registers them with
+// the tiles they reach.
+__global__ __launch_bounds__(64) void k_exchange(IbmArgs a) {
+  const int slice = blockIdx.x;
+  const int n = a.slice_owner[slice];
+  const int s0 = a.slice_first[n], nsl = a.slice_first[n + 1] - s0;
   double *q = a.prt + (long)n * R_N;
   const double dt = a.dscal[S_DT];
+  const double inv_dt = 1. / dt;
   const double pa = q[R_A], pb = q[R_B];
-  const double pm = p_mass(q), pim = p_moi(q);
+  const double inv_pm = 1. / p_mass(q), inv_pim = 1. / p_moi(q);
   const double px = q[R_X], py = q[R_Y], paz = q[R_AZ];
   const double pux = q[R_UX], puy = q[R_UY], pvz = q[R_VZ];
   const EllipseShape e = make_shape(pa, pb);
+  const EllipseShape32 f = shape32(e);
   BoxSet bs;
   make_boxes(a, px, py, fmax(pa, pb), bs);
-  const long cap = a.box_off[n + 1] - a.box_off[n];
-  if (bs.total > cap) {
+  const long total = bs.total;
+  if (total > a.box_off[n + 1] - a.box_off[n] || total > (long)nsl * IBM_SLICE) {
     if (threadIdx.x == 0) atomicOr(&a.flags[0], 1);
     return;
   }
-  if (threadIdx.x < 3) {
-    const int k = threadIdx.x;
-    long off = a.box_off[n];
-    for (int kk = 0; kk < k; kk++) off += bs.cnt[kk];
-    Box b;
-    b.imin = bs.imin[k];
-    b.imax = bs.imax[k];
-    b.jmin = bs.jmin[k];
-    b.jmax = bs.jmax[k];
-    b.off = off;
-    a.boxes[(long)n * 3 + k] = b;
-  }
-  {
-    // register the boxes with their tiles, one tile per thread
-    TileRange tr[6];
-    int tot = 0;
-    for (int k = 0; k < 3; k++) {
-      if (bs.cnt[k] > 0) {
-        box_tile_ranges(a, bs.imin[k], bs.imax[k], bs.jmin[k], bs.jmax[k], tr[2 * k], tr[2 * k + 1]);
-      } else {
-        tr[2 * k] = {0, -1, 0, -1};
-        tr[2 * k + 1] = {0, -1, 0, -1};
-      }
-      tot += tr[2 * k].count() + tr[2 * k + 1].count();
+  if (slice == s0) {
+    if (threadIdx.x < 3) {
+      const int k = threadIdx.x;
+      long off = a.box_off[n];
+      for (int kk = 0; kk < k; kk++) off += bs.cnt[kk];
+      Box b;
+      b.imin = bs.imin[k];
+      b.imax = bs.imax[k];
+      b.jmin = bs.jmin[k];
+      b.jmax = bs.jmax[k];
+      b.off = off;
+      a.boxes[(long)n * 3 + k] = b;
     }
-    for (int t = threadIdx.x; t < tot; t += IBM_THREADS) {
-      int e = t, r = 0;
-      while (e >= tr[r].count()) {
-        e -= tr[r].count();
-        r++;
-      }
-      const int w = tr[r].tx1 - tr[r].tx0 + 1;
-      const int tile = (tr[r].ty0 + e / w) * a.ntx + tr[r].tx0 + e % w;
-      register_tile(a, tile, n * 3 + r / 2);
+    // register the boxes with their tiles, one tile per lane
+    for (int r = 0; r < 6; r++) {
+      const int k = r / 2;
+      if (bs.cnt[k] <= 0) continue;
+      TileRange tr[2];
+      box_tile_ranges(a, bs.imin[k], bs.imax[k], bs.jmin[k], bs.jmax[k], tr[0], tr[1]);
+      const TileRange &rg = tr[r & 1];
+      const int cnt = rg.count();
+      const int w = rg.tx1 - rg.tx0 + 1;
+      for (int e_ = threadIdx.x; e_ < cnt; e_ += 64)
+        register_tile(a, (rg.ty0 + e_ / w) * a.ntx + rg.tx0 + e_ % w, n * 3 + k);
     }
   }
-  if (threadIdx.x == 0) {
-    const double *cp = a.sc_pos + 4 * (long)n;
-    use_cache = cp[3] != 0. && cp[0] == px && cp[1] == py && cp[2] == paz;
-  }
-  __syncthreads();
+  const double *cp = a.sc_pos + 4 * (long)n;
+  const bool cached = cp[3] != 0. && cp[0] == px && cp[1] == py && cp[2] == paz;
   double *dist = a.sc_dist + a.box_off[n];
   double *fxo = a.sc_fx + a.box_off[n];
   double *fyo = a.sc_fy + a.box_off[n];
@@ -382,17 +608,19 @@ __global__ __launch_bounds__(IBM_THREADS) void k_exchange(IbmArgs a) {
   sincos(-paz, &sr, &cr);
   const double dx = a.dx, dy = a.dy;
   double acc[3] = {0., 0., 0.};  // fux, fuy, tvz
-  int iters = 0;
-  for (long t = threadIdx.x; t < bs.total; t += IBM_THREADS) {
+  int it64 = 0, it32 = 0;
+  const long t0 = (long)(slice - s0) * IBM_SLICE;
+  const long t1 = t0 + IBM_SLICE < total ? t0 + IBM_SLICE : total;
+  for (long t = t0 + threadIdx.x; t < t1; t += 64) {
     int k, i, j;
     cell_of(bs, t, k, i, j);
     const double py_ = py + a.ly * (k - 1);
     const double x = a.xc[i], y = a.yc[j - 1];
     double d;
-    if (use_cache) {
+    if (cached) {
       d = dist[t];
     } else {
-      d = signed_dist(e, a.grid_size, px, py_, cr, sr, x, y, iters);
+      d = signed_dist_fast(e, f, a.grid_size, px, py_, cr, sr, x, y, it32, it64);
       dist[t] = d;
     }
     const double w = s_weight(d);
@@ -400,26 +628,21 @@ __global__ __launch_bounds__(IBM_THREADS) void k_exchange(IbmArgs a) {
     const double uy_p = puy + pvz * (x - px);
     const double ux_f = 0.5 * (a.ux[a.L.ux(i, j)] + a.ux[a.L.ux(i + 1, j)]);
     const double uy_f = 0.5 * (a.uy[a.L.p(i, j)] + a.uy[a.L.p(i, j + 1)]);
-    const double fx = w * (ux_p - ux_f) / dt;
-    const double fy = w * (uy_p - uy_f) / dt;
+    const double fx = w * (ux_p - ux_f) * inv_dt;
+    const double fy = w * (uy_p - uy_f) * inv_dt;
     fxo[t] = fx * dt;
     fyo[t] = fy * dt;
-    acc[0] -= +(fx * dx * dy) / pm;
-    acc[2] -= -(y - py_) * (fx * dx * dy) / pim;
-    acc[1] -= +(fy * dx * dy) / pm;
-    acc[2] -= +(x - px) * (fy * dx * dy) / pim;
+    acc[0] -= +(fx * dx * dy) * inv_pm;
+    acc[2] -= -(y - py_) * (fx * dx * dy) * inv_pim;
+    acc[1] -= +(fy * dx * dy) * inv_pm;
+    acc[2] -= +(x - px) * (fy * dx * dy) * inv_pim;
   }
-  block_sum<3>(acc, sh);
-  add_stats(a, bs.total, use_cache ? 0 : bs.total, iters, reinterpret_cast<unsigned long long *>(sh));
-  if (threadIdx.x == 0) {
-    q[R_FUX] = acc[0];
-    q[R_FUY] = acc[1];
-    q[R_TVZ] = acc[2];
-    double *cp = a.sc_pos + 4 * (long)n;
-    cp[0] = px;
-    cp[1] = py;
-    cp[2] = paz;
-    cp[3] = 1.;
+  double part[SLICE_VALS] = {wave_sum(acc[0]), wave_sum(acc[1]), wave_sum(acc[2]),
+                             wave_sum((double)it64), wave_sum((double)it32)};
+  slice_store(a, slice, part);
+  if (slice == s0 && threadIdx.x == 0) {
+    a.slice_info[2 * (long)n] = total;
+    a.slice_info[2 * (long)n + 1] = cached ? 0 : total;
   }
 }
 
@@ -895,8 +1118,21 @@ static IbmArgs ibm_args(Ctx *c) {
   a.ntx = c->ntx;
   a.nty = c->nty;
   a.flags = c->flags;
-  a.stats = c->ibm_stats;
+  a.stats = c->ibm_pp;
+  a.np = c->np;
+  a.slice_first = c->slice_first;
+  a.slice_owner = c->slice_owner;
+  a.slice_info = c->slice_info;
+  a.slice_part = c->slice_part;
   return a;
+}
+
+int launch_ref_iterations(Ctx *c, unsigned long long *dout) {
+  if (c->np == 0) return EIF_OK;
+  IbmArgs a = ibm_args(c);
+  hipLaunchKernelGGL(k_ref_iterations, dim3(c->np), dim3(IBM_THREADS), 0, c->stream, a, dout);
+  EIF_HIP(hipGetLastError());
+  return EIF_OK;
 }
 
 int susp_init(Ctx *c) {
@@ -905,13 +1141,34 @@ int susp_init(Ctx *c) {
   std::vector<double> rec((size_t)c->np * R_N);
   EIF_HIP(hipMemcpy(rec.data(), c->prt, rec.size() * sizeof(double), hipMemcpyDeviceToHost));
   std::vector<long> off(c->np + 1, 0);
+  // slices of IBM_SLICE cells per particle: the boxes of at most two images
+  // meet the domain when ly > 2r + 8 dy, and two clipped images together
+  // are at most two rows taller than one box; otherwise allow three images
+  std::vector<int> first(c->np + 1, 0);
   for (int n = 0; n < c->np; n++) {
     const double r = fmax(rec[(size_t)n * R_N + R_A], rec[(size_t)n * R_N + R_B]);
     const long cx = (long)(2. * r / c->dx) + 10, cy = (long)(2. * r / c->dy) + 10;
     const long wx = cx < c->nx ? cx : c->nx, wy = cy < c->ny ? cy : c->ny;
     off[n + 1] = off[n] + 3 * wx * wy;
+    const long cells = c->ly > 2. * r + 16. * c->dy ? wx * (wy + 4) : 3 * wx * wy;
+    first[n + 1] = first[n] + (int)((cells + IBM_SLICE - 1) / IBM_SLICE);
   }
   c->scratch_cells = off[c->np];
+  c->nslices = first[c->np];
+  EIF_HIP(hipMalloc(&c->slice_first, (c->np + 1) * sizeof(int)));
+  EIF_HIP(hipMemcpy(c->slice_first, first.data(), (c->np + 1) * sizeof(int), hipMemcpyHostToDevice));
+  {
+    std::vector<int> owner(c->nslices > 0 ? c->nslices : 1, 0);
+    for (int n = 0; n < c->np; n++)
+      for (int sl = first[n]; sl < first[n + 1]; sl++) owner[sl] = n;
+    EIF_HIP(hipMalloc(&c->slice_owner, owner.size() * sizeof(int)));
+    EIF_HIP(hipMemcpy(c->slice_owner, owner.data(), owner.size() * sizeof(int), hipMemcpyHostToDevice));
+  }
+  EIF_HIP(hipMalloc(&c->slice_info, 2 * (size_t)(c->np > 0 ? c->np : 1) * sizeof(long)));
+  EIF_HIP(hipMemset(c->slice_info, 0, 2 * (size_t)(c->np > 0 ? c->np : 1) * sizeof(long)));
+  EIF_HIP(hipMalloc(&c->slice_part, SLICE_VALS * (size_t)(c->nslices > 0 ? c->nslices : 1) * sizeof(double)));
+  EIF_HIP(hipMalloc(&c->ibm_pp, 4 * (size_t)(c->np > 0 ? c->np : 1) * sizeof(unsigned long long)));
+  EIF_HIP(hipMemset(c->ibm_pp, 0, 4 * (size_t)(c->np > 0 ? c->np : 1) * sizeof(unsigned long long)));
   EIF_HIP(hipMalloc(&c->box_off, (c->np + 1) * sizeof(long)));
   EIF_HIP(hipMemcpy(c->box_off, off.data(), (c->np + 1) * sizeof(long), hipMemcpyHostToDevice));
   const size_t sc = (size_t)(c->scratch_cells > 0 ? c->scratch_cells : 1) * sizeof(double);
@@ -954,7 +1211,9 @@ int launch_inertia(Ctx *c, int cnstep) {
   if (c->np == 0) return EIF_OK;
   ProfScope ps(c, "ibm_inertia");
   IbmArgs a = ibm_args(c);
-  hipLaunchKernelGGL(k_inertia, dim3(c->np), dim3(IBM_THREADS), 0, c->stream, a, cnstep);
+  hipLaunchKernelGGL(k_inertia, dim3(c->nslices), dim3(64), 0, c->stream, a, cnstep);
+  hipLaunchKernelGGL(k_ibm_finalize, dim3((c->np + 63) / 64), dim3(64), 0, c->stream, a, cnstep);
+  c->ibm_calls++;
   EIF_HIP(hipGetLastError());
   return EIF_OK;
 }
@@ -968,7 +1227,9 @@ int launch_exchange(Ctx *c) {
                      c->stream, c->tile_cnt, (int)ntiles);
   hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, c->active, 1);
   IbmArgs a = ibm_args(c);
-  hipLaunchKernelGGL(k_exchange, dim3(c->np), dim3(IBM_THREADS), 0, c->stream, a);
+  hipLaunchKernelGGL(k_exchange, dim3(c->nslices), dim3(64), 0, c->stream, a);
+  hipLaunchKernelGGL(k_ibm_finalize, dim3((c->np + 63) / 64), dim3(64), 0, c->stream, a, 2);
+  c->ibm_calls++;
   EIF_HIP(hipGetLastError());
   return EIF_OK;
 }

@@ -174,9 +174,16 @@ int eif_gpu_synchronize(eif_ctx *ctx);
 /* IBM work counters since the last reset: out[0] box cells visited
  * (the reference evaluates every one, inertia.c:45-63 / exchange.c:55-79),
  * out[1] cells whose signed distance was computed (the rest were served by
- * the per-particle distance cache), out[2] Newton iterations of
- * find_normal_t, out[3] IBM kernel calls.  reset != 0 clears them. */
-int eif_gpu_ibm_stats(eif_ctx *ctx, unsigned long long out[4], int reset);
+ * the per-particle distance cache), out[2] FP64 iterations of the foot
+ * point search, out[3] IBM kernel calls, out[4] FP32 iterations of the
+ * foot point search.  reset != 0 clears them. */
+int eif_gpu_ibm_stats(eif_ctx *ctx, unsigned long long out[5], int reset);
+/* Measurement of the reference's work (no counterpart in the reference):
+ * runs the reference's own FP64 iteration of find_normal_t
+ * (src/ellipse.c:30-65: start pi/4, stop at |dt| < 1e-8) at every cell of
+ * the particles' current IBM boxes and returns out[0] cells and out[1]
+ * iterations.  Synchronising; does not change the simulation state. */
+int eif_gpu_ibm_reference_iterations(eif_ctx *ctx, unsigned long long out[2]);
 const char *eif_gpu_last_error(void);
 
 /* Per-kernel timing with HIP events on the context's stream.  Names are
