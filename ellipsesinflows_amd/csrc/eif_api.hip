@@ -135,7 +135,8 @@ static void free_ctx(Ctx *c) {
     if (*b) (void)hipFree(*b);
   void *others[] = {c->th_flag, c->boxes, c->box_off, c->tile_cnt, c->tile_list,
                     c->active, c->flags, c->pair_cnt, c->pair_ids, c->wall_hit,
-                    c->ibm_stats, c->slice_first, c->slice_info, c->ibm_pp, c->slice_owner};
+                    c->ibm_stats, c->slice_first, c->slice_info, c->ibm_pp, c->slice_owner,
+                    c->cand};
   for (auto o : others)
     if (o) (void)hipFree(o);
   if (c->hscal) (void)hipHostFree(c->hscal);

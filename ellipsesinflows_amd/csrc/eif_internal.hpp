@@ -120,8 +120,10 @@ struct Ctx {
   unsigned long long *ibm_pp = nullptr;     // np x 4 IBM work counters
   unsigned long long ibm_calls = 0;
   // collisions
-  int *pair_cnt = nullptr;
+  int *pair_cnt = nullptr;  // [0] colliding pairs, [1] candidate pairs
   int pair_cap = 0;
+  int *cand = nullptr;      // cand_cap x 2 candidate pairs
+  int cand_cap = 0;
   int *pair_ids = nullptr;        // pair_cap x 2
   double *pair_val = nullptr;     // pair_cap x 6 contributions
   double *wall_val = nullptr;     // np x 2 walls x 3

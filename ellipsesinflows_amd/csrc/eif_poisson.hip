@@ -146,6 +146,64 @@ __device__ void reg_fft(double (&xr)[8], double (&xi)[8], double *lre, double *l
   }
 }
 
+// The same transform with n = 2^LOGN known at compile time: phases and
+// stages unroll, so the slot -> position maps and twiddle offsets fold to
+// constants plus a per-thread base.  Butterflies, operand order and twiddle
+// values are those of reg_fft.
+template <int LOGN, int SIGN>
+__device__ inline void reg_fft_t(double (&xr)[8], double (&xi)[8], double *lre, double *lim,
+                                 const double *__restrict__ W) {
+  const int t = threadIdx.x;
+  constexpr int NP = (LOGN + 2) / 3;
+#pragma unroll
+  for (int p = 0; p < NP; p++) {
+    const RegMap mp = reg_phase(p, LOGN);
+    if (p > 0) {
+      const RegMap mq = reg_phase(p - 1, LOGN);
+      __syncthreads();
+#pragma unroll
+      for (int s = 0; s < 8; s++) {
+        const int q = sw(mq.pos(t, s));
+        lre[q] = xr[s];
+        lim[q] = xi[s];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int s = 0; s < 8; s++) {
+        const int q = sw(mp.pos(t, s));
+        xr[s] = lre[q];
+        xi[s] = lim[q];
+      }
+    }
+    const int r = mp.lg;
+    const int gmask = (1 << r) - 1;
+#pragma unroll
+    for (int ss = 0; ss < 3; ss++) {
+      if (ss >= r) break;
+      const int half = 1 << (mp.lh0 + ss), len = half << 1;
+      const double *Ws = W + 2 * (half - 1);
+#pragma unroll
+      for (int s = 0; s < 8; s++) {
+        const int e = s & gmask;
+        if (e & (1 << ss)) continue;
+        if ((e | (1 << ss)) > gmask) continue;
+        const int s1 = s + (1 << ss);
+        const int kk = mp.pos(t, s) & (len - 1);
+        const double wr = Ws[2 * kk];
+        const double wi = SIGN * Ws[2 * kk + 1];
+        const double r1 = xr[s1], m1 = xi[s1];
+        const double tr = wr * r1 - wi * m1;
+        const double ti = wr * m1 + wi * r1;
+        const double r0 = xr[s], m0 = xi[s];
+        xr[s1] = r0 - tr;
+        xi[s1] = m0 - ti;
+        xr[s] = r0 + tr;
+        xi[s] = m0 + ti;
+      }
+    }
+  }
+}
+
 __device__ inline double cos_lookup(const double *C, int n, long m) {
   m %= 4 * (long)n;
   return m <= 2 * n ? C[m] : C[4 * (long)n - m];
@@ -305,6 +363,92 @@ __global__ __launch_bounds__(1024) void k_dct3_psi(
         acc = acc + 2. * re[jj] * cos_lookup(C, nx, (long)jj * (2 * i + 1));
       psi[L.p(i + 1, j0 + 1)] = acc / norm;
     }
+  }
+}
+
+// Register-FFT kernels specialised for nx = 2^LOGN (nx/8 threads); the
+// arithmetic is that of the mode-2 branches of k_rhs_dct2 / k_dct3_psi.
+template <int LOGN>
+__global__ __launch_bounds__(1024) void k_rhs_dct2_t(
+    Layout L, const double *ux, const double *uy, const double *dscal, double ga,
+    double dx, double dy, const double *__restrict__ C, const double *__restrict__ W,
+    double *qhat) {
+  extern __shared__ double sh[];
+  constexpr int nx = 1 << LOGN;
+  constexpr int T = nx / 8;
+  const int j = blockIdx.x + 1;
+  double *re = sh, *im = sh + nx;
+  const double dt = dscal[S_DT];
+  const double fac = 1. / (ga * dt);
+  double *out = qhat + (long)(j - 1) * nx;
+#pragma unroll
+  for (int s = 0; s < 8; s++) {
+    const int m = threadIdx.x + s * T;
+    const int i = m < nx / 2 ? 2 * m : 2 * (nx - 1 - m) + 1;  // Makhoul order
+    re[sw(bitrev(m, LOGN))] = rhs_value(L, ux, uy, fac, dx, dy, i, j);
+  }
+  __syncthreads();
+  double xr[8], xi[8];
+#pragma unroll
+  for (int s = 0; s < 8; s++) {
+    xr[s] = re[sw(8 * threadIdx.x + s)];
+    xi[s] = 0.;
+  }
+  reg_fft_t<LOGN, -1>(xr, xi, re, im, W);
+  const RegMap ml = reg_phase(reg_nphase(LOGN) - 1, LOGN);
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < 8; s++) {
+    const int q = sw(ml.pos(threadIdx.x, s));
+    re[q] = xr[s];
+    im[q] = xi[s];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < 8; s++) {
+    const int k = threadIdx.x + s * T;
+    out[k] = 2. * (C[k] * re[sw(k)] + C[nx - k] * im[sw(k)]);
+  }
+}
+
+template <int LOGN>
+__global__ __launch_bounds__(1024) void k_dct3_psi_t(
+    Layout L, const double *qhat, CombineArgs ca, const double *__restrict__ C,
+    const double *__restrict__ W, double *psi) {
+  extern __shared__ double sh[];
+  constexpr int nx = 1 << LOGN;
+  constexpr int T = nx / 8;
+  const int ny = L.ny;
+  const int j0 = blockIdx.x;
+  double *re = sh, *im = sh + nx;
+  // division by 2 nx, a power of two, is the exact product with its inverse
+  constexpr double inv_norm = 1. / (2. * nx);
+#pragma unroll
+  for (int s = 0; s < 8; s++) {
+    const int k = threadIdx.x + s * T;
+    re[sw(k)] = combined(ca, qhat, nx, ny, j0, k);
+  }
+  __syncthreads();
+  double xr[8], xi[8];
+#pragma unroll
+  for (int s = 0; s < 8; s++) {
+    const int k = bitrev(8 * threadIdx.x + s, LOGN);
+    const double vr = re[sw(k)];
+    const double vi = k == 0 ? 0. : -re[sw(nx - k)];
+    xr[s] = C[k] * vr - C[nx - k] * vi;
+    xi[s] = C[k] * vi + C[nx - k] * vr;
+  }
+  reg_fft_t<LOGN, +1>(xr, xi, re, im, W);
+  const RegMap ml = reg_phase(reg_nphase(LOGN) - 1, LOGN);
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < 8; s++) re[sw(ml.pos(threadIdx.x, s))] = xr[s];
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < 8; s++) {
+    const int i = threadIdx.x + s * T;
+    const double v = (i & 1) ? re[sw(nx - 1 - (i >> 1))] : re[sw(i >> 1)];
+    psi[L.p(i + 1, j0 + 1)] = v * inv_norm;
   }
 }
 
@@ -638,9 +782,21 @@ int launch_poisson(Ctx *c, int rk) {
   const int threads = mode == 2 ? nx / 8 : DCT_THREADS;
   {
     ProfScope ps(c, "rhs_dct2");
-    hipLaunchKernelGGL(k_rhs_dct2, dim3(ny), dim3(threads), shmem, c->stream, c->L,
-                       c->ux, c->uy, c->dscal, c->gamma[rk], c->dx, c->dy, c->ctab,
-                       c->twid, logn, mode, c->qhat);
+#define EIF_RHS_T(LG)                                                                    \
+  case LG:                                                                               \
+    hipLaunchKernelGGL(k_rhs_dct2_t<LG>, dim3(ny), dim3(threads), shmem, c->stream, c->L, \
+                       c->ux, c->uy, c->dscal, c->gamma[rk], c->dx, c->dy, c->ctab,      \
+                       c->twid, c->qhat);                                                \
+    break;
+    switch (mode == 2 ? logn : -1) {
+      EIF_RHS_T(6) EIF_RHS_T(7) EIF_RHS_T(8) EIF_RHS_T(9) EIF_RHS_T(10) EIF_RHS_T(11)
+      EIF_RHS_T(12) EIF_RHS_T(13)
+      default:
+        hipLaunchKernelGGL(k_rhs_dct2, dim3(ny), dim3(threads), shmem, c->stream, c->L,
+                           c->ux, c->uy, c->dscal, c->gamma[rk], c->dx, c->dy, c->ctab,
+                           c->twid, logn, mode, c->qhat);
+    }
+#undef EIF_RHS_T
     EIF_HIP(hipGetLastError());
   }
   const double l = 1. / c->dy / c->dy;
@@ -677,8 +833,19 @@ int launch_poisson(Ctx *c, int rk) {
   }
   {
     ProfScope ps(c, "dct3_psi");
-    hipLaunchKernelGGL(k_dct3_psi, dim3(ny), dim3(threads), shmem, c->stream, c->L,
-                       c->qhat, ca, c->ctab, c->twid, logn, mode, c->psi);
+#define EIF_DCT3_T(LG)                                                                  \
+  case LG:                                                                              \
+    hipLaunchKernelGGL(k_dct3_psi_t<LG>, dim3(ny), dim3(threads), shmem, c->stream, c->L, \
+                       c->qhat, ca, c->ctab, c->twid, c->psi);                          \
+    break;
+    switch (mode == 2 ? logn : -1) {
+      EIF_DCT3_T(6) EIF_DCT3_T(7) EIF_DCT3_T(8) EIF_DCT3_T(9) EIF_DCT3_T(10) EIF_DCT3_T(11)
+      EIF_DCT3_T(12) EIF_DCT3_T(13)
+      default:
+        hipLaunchKernelGGL(k_dct3_psi, dim3(ny), dim3(threads), shmem, c->stream, c->L,
+                           c->qhat, ca, c->ctab, c->twid, logn, mode, c->psi);
+    }
+#undef EIF_DCT3_T
     EIF_HIP(hipGetLastError());
   }
   return launch_bc_p(c, c->psi);

@@ -212,36 +212,38 @@ __device__ inline void rotate_small(double &c, double &s, double delta) {
   s = sn;
 }
 
-// (c, s) = (cos t, sin t) of the foot on the first-quadrant arc for the
-// folded point (px, py) >= 0
-__device__ inline void foot_point(const EllipseShape &e, const EllipseShape32 &f,
-                                  double px, double py, double &c, double &s,
-                                  int &it32, int &it64) {
-  constexpr double HALF_PI = 0.5 * EIF_PI;
+// FP32 part: from t = pi/4 until |dt| < 1e-3 (at most 8 iterations)
+__device__ inline float foot_fp32(const EllipseShape32 &f, double px, double py, int &it32) {
   float tf = (float)(0.25 * EIF_PI);
-  {
-    const float fx = (float)px, fy = (float)py;
-    for (int n = 0; n < 8; n++) {
-      float sn, cs;
-      __sincosf(tf, &sn, &cs);
-      const float x = f.a * cs, y = f.b * sn;
-      const float ex = f.kex * (cs * cs * cs), ey = f.key * (sn * sn * sn);
-      const float rx = x - ex, ry = y - ey;
-      const float qx = fx - ex, qy = fy - ey;
-      const float rr = rx * rx + ry * ry, qq = qx * qx + qy * qy;
-      float arg = (rx * qy - ry * qx) * __builtin_amdgcn_rsqf(rr * qq);
-      arg = fminf(fmaxf(arg, -1.f), 1.f);
-      const float den = fmaxf(f.a2b2 - x * x - y * y, 1e-30f);
-      const float a2 = arg * arg;
-      const float as = arg + arg * a2 * (0.16666667f + a2 * (0.075f + a2 * 0.044642857f));
-      const float dt = __builtin_amdgcn_sqrtf(rr) * as * __builtin_amdgcn_rsqf(den);
-      it32++;
-      if (!(fabsf(dt) < 1e30f)) break;  // no usable step: hand over to FP64
-      tf = fminf(fmaxf(tf + dt, 0.f), 1.5707963f);
-      if (fabsf(dt) < 1e-3f) break;
-    }
+  const float fx = (float)px, fy = (float)py;
+  for (int n = 0; n < 8; n++) {
+    float sn, cs;
+    __sincosf(tf, &sn, &cs);
+    const float x = f.a * cs, y = f.b * sn;
+    const float ex = f.kex * (cs * cs * cs), ey = f.key * (sn * sn * sn);
+    const float rx = x - ex, ry = y - ey;
+    const float qx = fx - ex, qy = fy - ey;
+    const float rr = rx * rx + ry * ry, qq = qx * qx + qy * qy;
+    float arg = (rx * qy - ry * qx) * __builtin_amdgcn_rsqf(rr * qq);
+    arg = fminf(fmaxf(arg, -1.f), 1.f);
+    const float den = fmaxf(f.a2b2 - x * x - y * y, 1e-30f);
+    const float a2 = arg * arg;
+    const float as = arg + arg * a2 * (0.16666667f + a2 * (0.075f + a2 * 0.044642857f));
+    const float dt = __builtin_amdgcn_sqrtf(rr) * as * __builtin_amdgcn_rsqf(den);
+    it32++;
+    if (!(fabsf(dt) < 1e30f)) break;  // no usable step: hand over to FP64
+    tf = fminf(fmaxf(tf + dt, 0.f), 1.5707963f);
+    if (fabsf(dt) < 1e-3f) break;
   }
-  double t = (double)tf;
+  return tf;
+}
+
+// FP64 part: the reference's iteration and stopping rule from t (updated
+// in place); (c, s) = (cos t, sin t) of the foot on the first-quadrant arc
+// for the folded point (px, py) >= 0
+__device__ inline void foot_fp64(const EllipseShape &e, double px, double py, double &t,
+                                 double &c, double &s, int &it64) {
+  constexpr double HALF_PI = 0.5 * EIF_PI;
   sincos_q1(t, s, c);
   for (int n = 0;; n++) {
     const double x = e.a * c, y = e.b * s;
@@ -266,6 +268,13 @@ __device__ inline void foot_point(const EllipseShape &e, const EllipseShape32 &f
     if (fabs(dt) < 1.e-8) break;
     if (n > 100) break;
   }
+}
+
+__device__ inline void foot_point(const EllipseShape &e, const EllipseShape32 &f,
+                                  double px, double py, double &c, double &s,
+                                  int &it32, int &it64) {
+  double t = (double)foot_fp32(f, px, py, it32);
+  foot_fp64(e, px, py, t, c, s, it64);
 }
 
 // compute_signed_dist (src/suspensions/auxiliary.c:19-47) with the foot
@@ -762,33 +771,37 @@ struct EllD {
   double a, b, x, y, angle;
 };
 
-__device__ double curvature(double a, double b, double t) {
-  double st, ct;
-  sincos(t, &st, &ct);
-  const double ct2 = ct * ct, st2 = st * st;
-  const double a2 = a * a, b2 = b * b;
-  const double num = a * b * (ct2 + st2);
-  const double den = pow(+a2 * st2 + b2 * ct2, 1.5);
-  return num / den;
-}
-
-__device__ void evolute_fit(const EllD &e, double x, double y, double &ex,
-                            double &ey, double &ix, double &iy, double &r) {
-  double sm, cm, sp, cp;
-  sincos(-e.angle, &sm, &cm);
-  sincos(+e.angle, &sp, &cp);
+// compute_evolute (collision.c:75-109) with the foot from foot_fp64,
+// started at tw, the folded parameter of the previous fit in the same
+// fixed-point iteration (or the FP32 estimate when tw < 0); the root on
+// the first-quadrant arc is unique, so the start only changes the number
+// of iterations.  (cm, sm) = cos/sin(-angle), (cp, sp) = cos/sin(+angle).
+__device__ void evolute_fit_fast(const EllD &e, const EllipseShape &s,
+                                 const EllipseShape32 &f, double cm, double sm, double cp,
+                                 double sp, double x, double y, double &ex, double &ey,
+                                 double &ix, double &iy, double &r, double &tw) {
   double x_, y_;
   {
     const double x__ = x - e.x, y__ = y - e.y;
     x_ = cm * x__ - sm * y__;
     y_ = sm * x__ + cm * y__;
   }
-  const EllipseShape s = make_shape(e.a, e.b);
-  int iters = 0;
-  const double t = find_normal_t(s, x_, y_, iters);
-  r = 1. / fmax(curvature(e.a, e.b, t), DBL_EPSILON);
-  double st, ct;
-  sincos(t, &st, &ct);
+  const double px = fabs(x_), py = fabs(y_);
+  int it32 = 0, it64 = 0;
+  if (tw < 0.) tw = (double)foot_fp32(f, px, py, it32);
+  double c, sn;
+  foot_fp64(s, px, py, tw, c, sn, it64);
+  // unfolded foot (ellipse.c:62-63)
+  const double ct = x_ < 0. ? -c : c;
+  const double st = y_ < 0. ? -sn : sn;
+  {
+    // compute_curvature, ellipse.c:15-25
+    const double ct2 = ct * ct, st2 = st * st;
+    const double a2 = e.a * e.a, b2 = e.b * e.b;
+    const double num = e.a * e.b * (ct2 + st2);
+    const double den = +a2 * st2 + b2 * ct2;
+    r = 1. / fmax(num / (den * sqrt(den)), DBL_EPSILON);
+  }
   x_ = s.kex * cube(ct);
   y_ = s.key * cube(st);
   ex = cp * x_ - sp * y_;
@@ -802,6 +815,8 @@ __device__ void evolute_fit(const EllD &e, double x, double y, double &ex,
 }
 
 struct CollArgs {
+  int *cand;     // candidate pairs (n0, n1)
+  int cand_cap;
   int np;
   double lx, ly;
   const double *prt;
@@ -820,44 +835,87 @@ __device__ inline double harmonic(double v0, double v1) {
   return 1. / r;
 }
 
-__global__ void k_collide_pairs(CollArgs a) {
-  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  const long np = a.np;
-  if (t >= np * np) return;
-  const int n0 = (int)(t / np), n1 = (int)(t % np);
-  if (n1 <= n0) return;
-  const double *p0 = a.prt + (long)n0 * R_N, *p1 = a.prt + (long)n1 * R_N;
-  double yoffset = 0.;
-  {
-    const double p0y = p0[R_Y] + p0[R_DY];
-    const double p1y = p1[R_Y] + p1[R_DY];
-    double minval = DBL_MAX;
-    for (int periodic = -1; periodic <= 1; periodic++) {
-      const double val = fabs((p1y + a.ly * periodic) - p0y);
-      if (val < minval) {
-        minval = val;
-        yoffset = a.ly * periodic;
-      }
+// Candidate pairs (n0 < n1) whose bounding circles (radius max(a, b)) of
+// the minimum image overlap, the early exit of collision.c:156-189; tiles
+// of 64 x 64 pairs read their particles once into LDS.
+constexpr int COLL_TILE = 64;
+
+__device__ inline double min_image_offset(double p0y, double p1y, double ly) {
+  double yoffset = 0., minval = DBL_MAX;
+  for (int periodic = -1; periodic <= 1; periodic++) {
+    const double val = fabs((p1y + ly * periodic) - p0y);
+    if (val < minval) {
+      minval = val;
+      yoffset = ly * periodic;
     }
   }
-  {
-    const double x0 = p0[R_X] + p0[R_DX], y0 = p0[R_Y] + p0[R_DY];
-    const double r0 = fmax(p0[R_A], p0[R_B]);
-    const double x1 = p1[R_X] + p1[R_DX], y1 = p1[R_Y] + p1[R_DY] + yoffset;
-    const double r1 = fmax(p1[R_A], p1[R_B]);
-    const double norm = fmax(hypot(x1 - x0, y1 - y0), DBL_EPSILON);
-    if (r0 + r1 - norm < 0.) return;
+  return yoffset;
+}
+
+__global__ __launch_bounds__(256) void k_collide_search(CollArgs a) {
+  __shared__ double sx[2][COLL_TILE], sy[2][COLL_TILE], sr[2][COLL_TILE];
+  const int nt = (a.np + COLL_TILE - 1) / COLL_TILE;
+  int b = blockIdx.x, tb0 = 0;
+  while (b >= nt - tb0) {
+    b -= nt - tb0;
+    tb0++;
   }
+  const int tb1 = tb0 + b;
+  for (int e = threadIdx.x; e < 2 * COLL_TILE; e += blockDim.x) {
+    const int side = e / COLL_TILE, m = e % COLL_TILE;
+    const int n = (side ? tb1 : tb0) * COLL_TILE + m;
+    if (n < a.np) {
+      const double *p = a.prt + (long)n * R_N;
+      sx[side][m] = p[R_X] + p[R_DX];
+      sy[side][m] = p[R_Y] + p[R_DY];
+      sr[side][m] = fmax(p[R_A], p[R_B]);
+    }
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < COLL_TILE * COLL_TILE; e += blockDim.x) {
+    const int m0 = e / COLL_TILE, m1 = e % COLL_TILE;
+    const int n0 = tb0 * COLL_TILE + m0, n1 = tb1 * COLL_TILE + m1;
+    if (n1 <= n0 || n1 >= a.np) continue;
+    const double y0 = sy[0][m0];
+    const double y1 = sy[1][m1] + min_image_offset(y0, sy[1][m1], a.ly);
+    const double norm = fmax(hypot(sx[1][m1] - sx[0][m0], y1 - y0), DBL_EPSILON);
+    if (sr[0][m0] + sr[1][m1] - norm < 0.) continue;
+    const int slot = atomicAdd(&a.pair_cnt[1], 1);
+    if (slot < a.cand_cap) {
+      a.cand[2 * slot] = n0;
+      a.cand[2 * slot + 1] = n1;
+    } else {
+      atomicOr(&a.flags[0], 4);
+    }
+  }
+}
+
+// collision.c:156-247 for one candidate pair per thread
+__global__ void k_collide_pairs(CollArgs a) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ncand = min(a.pair_cnt[1], a.cand_cap);
+  if (t >= ncand) return;
+  const int n0 = a.cand[2 * t], n1 = a.cand[2 * t + 1];
+  const double *p0 = a.prt + (long)n0 * R_N, *p1 = a.prt + (long)n1 * R_N;
+  const double yoffset = min_image_offset(p0[R_Y] + p0[R_DY], p1[R_Y] + p1[R_DY], a.ly);
   const EllD e0 = {p0[R_A], p0[R_B], p0[R_X] + p0[R_DX], p0[R_Y] + p0[R_DY], p0[R_AZ]};
   const EllD e1 = {p1[R_A], p1[R_B], p1[R_X] + p1[R_DX], p1[R_Y] + p1[R_DY] + yoffset,
                    p1[R_AZ]};
   // find_equivalent_circles, collision.c:111-135 (guarded iteration count)
+  const EllipseShape s0 = make_shape(e0.a, e0.b), s1 = make_shape(e1.a, e1.b);
+  const EllipseShape32 f0 = shape32(s0), f1 = shape32(s1);
+  double cm0, sm0, cp0, sp0, cm1, sm1, cp1, sp1;
+  sincos(-e0.angle, &sm0, &cm0);
+  sincos(+e0.angle, &sp0, &cp0);
+  sincos(-e1.angle, &sm1, &cm1);
+  sincos(+e1.angle, &sp1, &cp1);
+  double tw0 = -1., tw1 = -1.;
   double c0x = e0.x, c0y = e0.y, c1x = e1.x, c1y = e1.y, c0r = 0., c1r = 0.;
   double ix0 = 0., iy0 = 0., ix1 = 0., iy1 = 0.;
   for (int it = 0;; it++) {
     double n0x, n0y, n1x, n1y;
-    evolute_fit(e0, c1x, c1y, n0x, n0y, ix0, iy0, c0r);
-    evolute_fit(e1, c0x, c0y, n1x, n1y, ix1, iy1, c1r);
+    evolute_fit_fast(e0, s0, f0, cm0, sm0, cp0, sp0, c1x, c1y, n0x, n0y, ix0, iy0, c0r, tw0);
+    evolute_fit_fast(e1, s1, f1, cm1, sm1, cp1, sp1, c0x, c0y, n1x, n1y, ix1, iy1, c1r, tw1);
     const double d0x = n0x - c0x, d0y = n0y - c0y;
     const double d1x = n1x - c1x, d1y = n1y - c1y;
     c0x += d0x;
@@ -888,7 +946,7 @@ __global__ void k_collide_pairs(CollArgs a) {
   if (overlap > 0.) {
     const double cfx = k * overlap * nxv;
     const double cfy = k * overlap * nyv;
-    const int slot = atomicAdd(a.pair_cnt, 1);
+    const int slot = atomicAdd(&a.pair_cnt[0], 1);
     if (slot >= a.pair_cap) {
       atomicOr(&a.flags[0], 4);
       return;
@@ -920,11 +978,17 @@ __global__ void k_collide_walls(CollArgs a) {
     if (r - norm < 0.) return;
   }
   const EllD e = {p[R_A], p[R_B], p[R_X] + p[R_DX], p[R_Y] + p[R_DY], p[R_AZ]};
+  const EllipseShape sh = make_shape(e.a, e.b);
+  const EllipseShape32 f = shape32(sh);
+  double cm, sm, cp, sp;
+  sincos(-e.angle, &sm, &cm);
+  sincos(+e.angle, &sp, &cp);
+  double tw = -1.;
   double cx = e.x, cy = e.y, cr = 0., ix = 0., iy = 0.;
   for (int it = 0;; it++) {
     double nxc, nyc;
     const double mirrorx = 2. * wallx - cx;
-    evolute_fit(e, mirrorx, cy, nxc, nyc, ix, iy, cr);
+    evolute_fit_fast(e, sh, f, cm, sm, cp, sp, mirrorx, cy, nxc, nyc, ix, iy, cr, tw);
     const double ddx = nxc - cx, ddy = nyc - cy;
     cx += ddx;
     cy += ddy;
@@ -965,7 +1029,7 @@ __global__ void k_collide_gather(CollArgs a) {
   int partner[CONTACT_CAP];
   int slot[CONTACT_CAP];
   int cnt = 0;
-  const int npairs = min(*a.pair_cnt, a.pair_cap);
+  const int npairs = min(a.pair_cnt[0], a.pair_cap);
   for (int s = 0; s < npairs; s++) {
     const int n0 = a.pair_ids[2 * s], n1 = a.pair_ids[2 * s + 1];
     if (n0 != m && n1 != m) continue;
@@ -1189,8 +1253,10 @@ int susp_init(Ctx *c) {
   EIF_HIP(hipMemset(c->active, 0, (ntiles + 1) * sizeof(int)));
   // collisions
   c->pair_cap = c->np * 8 + 64;
-  EIF_HIP(hipMalloc(&c->pair_cnt, sizeof(int)));
-  EIF_HIP(hipMemset(c->pair_cnt, 0, sizeof(int)));
+  c->cand_cap = c->np * 16 + 64;
+  EIF_HIP(hipMalloc(&c->pair_cnt, 2 * sizeof(int)));
+  EIF_HIP(hipMemset(c->pair_cnt, 0, 2 * sizeof(int)));
+  EIF_HIP(hipMalloc(&c->cand, 2 * (size_t)c->cand_cap * sizeof(int)));
   EIF_HIP(hipMalloc(&c->pair_ids, 2 * (size_t)c->pair_cap * sizeof(int)));
   EIF_HIP(hipMalloc(&c->pair_val, 6 * (size_t)c->pair_cap * sizeof(double)));
   EIF_HIP(hipMalloc(&c->wall_val, 6 * (size_t)(c->np > 0 ? c->np : 1) * sizeof(double)));
@@ -1299,9 +1365,12 @@ int launch_collision(Ctx *c, int cnstep) {
   a.wall_val = c->wall_val;
   a.wall_hit = c->wall_hit;
   a.flags = c->flags;
-  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, c->pair_cnt, 1);
-  const long npp = (long)c->np * c->np;
-  hipLaunchKernelGGL(k_collide_pairs, dim3((unsigned)((npp + 255) / 256)), dim3(256), 0,
+  a.cand = c->cand;
+  a.cand_cap = c->cand_cap;
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, c->pair_cnt, 2);
+  const int nt = (c->np + COLL_TILE - 1) / COLL_TILE;
+  hipLaunchKernelGGL(k_collide_search, dim3(nt * (nt + 1) / 2), dim3(256), 0, c->stream, a);
+  hipLaunchKernelGGL(k_collide_pairs, dim3((c->cand_cap + 63) / 64), dim3(64), 0,
                      c->stream, a);
   hipLaunchKernelGGL(k_collide_walls, dim3((2 * c->np + 255) / 256), dim3(256), 0,
                      c->stream, a);
