@@ -114,12 +114,26 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--poisson", choices=("partition", "thomas"), default="partition")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibration-copy-mib", type=int, default=0,
+                    help="before the workload, copy a buffer of this size once on the GPU "
+                         "(a kernel of known traffic for scaling FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
+    if args.calibration_copy_mib > 0:
+        import torch
+        n = args.calibration_copy_mib * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("calibration copy: %d MiB read + %d MiB written" % (args.calibration_copy_mib,
+                                                                args.calibration_copy_mib))
+        del src, dst
     parts = workload_inputs()
     mode = eif.POISSON_PARTITION if args.poisson == "partition" else eif.POISSON_THOMAS
     g = eif.Solver(itot=NX, jtot=NY, ly=LY, Re=RE, extfrcy=8.0 / RE, poisson_mode=mode)

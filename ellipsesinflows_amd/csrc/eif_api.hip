@@ -214,6 +214,18 @@ int fluid_gpu_create(const eif_config *cfg, eif_ctx **out) {
     for (int j = 1; j <= ny + 1; j++) c->h_yf[j - 1] = yoffset + 1. * (j - 1) * dy;
     for (int j = 1; j <= ny; j++) c->h_yc[j - 1] = yoffset + 0.5 * (2 * j - 1) * dy;
   }
+  // uniform power-of-two spacings make every grid division exact as a
+  // product with the reciprocal
+  {
+    auto p2 = [](double v) {
+      int e;
+      return v > 0. && frexp(v, &e) == 0.5;
+    };
+    bool ok = p2(c->dx) && p2(c->dy);
+    for (double v : c->h_dxf) ok = ok && p2(v);
+    for (double v : c->h_dxc) ok = ok && p2(v);
+    c->pow2_grid = ok;
+  }
   // RK coefficients, src/param/init.c:169-191
   c->alpha[0] = 32. / 60.;
   c->beta[0] = 0. / 60.;

@@ -105,6 +105,7 @@ struct PredArgs {
   int use_b;
 };
 
+template <bool P2>
 __global__ __launch_bounds__(PRED_BX) void k_predictor(PredArgs a) {
   const Layout L = a.L;
   const int nx = L.nx, ny = L.ny;
@@ -122,12 +123,14 @@ __global__ __launch_bounds__(PRED_BX) void k_predictor(PredArgs a) {
   const double dxf_i = a.dxf[i - 1];             // DXF(i)
   const double dxc_i = a.dxc[i - 1];             // DXC(i)
   const double dxc_ip = a.dxc[i];                // DXC(i+1)
+  const double idxf_im = hx ? 1. / dxf_im : 0., idxf_i = 1. / dxf_i;
+  const double idxc_i = 1. / dxc_i, idxc_ip = 1. / dxc_ip, idy = 1. / dy;
   // ux window: columns i-1, i, i+1 (ux index i-1 exists for i >= 2)
   const double *ux = a.ux, *uy = a.uy, *p = a.p;
   auto ldux = [&](int ii, int jj) { return ux[L.ux(ii, jj)]; };
-  double uxm_w = hx ? ldux(i - 1, j0 - 1) : 0., uxm_c = ldux(i, j0 - 1), uxm_e = ldux(i + 1, j0 - 1);
+  double uxm_c = ldux(i, j0 - 1), uxm_e = ldux(i + 1, j0 - 1);
   double uxc_w = hx ? ldux(i - 1, j0) : 0., uxc_c = ldux(i, j0), uxc_e = ldux(i + 1, j0);
-  double uym_w = uy[L.p(i - 1, j0 - 1)], uym_c = uy[L.p(i, j0 - 1)], uym_e = uy[L.p(i + 1, j0 - 1)];
+  double uym_c = uy[L.p(i, j0 - 1)];
   double uyc_w = uy[L.p(i - 1, j0)], uyc_c = uy[L.p(i, j0)], uyc_e = uy[L.p(i + 1, j0)];
   double pm_c = p[L.p(i, j0 - 1)];
   for (int j = j0; j <= j1; j++) {
@@ -138,10 +141,10 @@ __global__ __launch_bounds__(PRED_BX) void k_predictor(PredArgs a) {
     const double pc_w = p[L.p(i - 1, j)], pc_c = p[L.p(i, j)];
     // ---- x-momentum at face (i, j), src/fluid/update_velocity.c:28-88
     if (hx) {
-      const double g_w = (-uxc_w + uxc_c) / dxf_im;
-      const double g_e = (-uxc_c + uxc_e) / dxf_i;
-      const double g_s = (-uxm_c + uxc_c) / dy;
-      const double g_n = (-uxc_c + uxp_c) / dy;
+      const double g_w = qdiv<P2>(-uxc_w + uxc_c, dxf_im, idxf_im);
+      const double g_e = qdiv<P2>(-uxc_c + uxc_e, dxf_i, idxf_i);
+      const double g_s = qdiv<P2>(-uxm_c + uxc_c, dy, idy);
+      const double g_n = qdiv<P2>(-uxc_c + uxp_c, dy, idy);
       const double cw = dxf_im / (2. * dxc_i);
       const double ce = dxf_i / (2. * dxc_i);
       const double aw = 0.5 * uxc_w + 0.5 * uxc_c;
@@ -150,9 +153,9 @@ __global__ __launch_bounds__(PRED_BX) void k_predictor(PredArgs a) {
       const double vs = cw * uyc_w + ce * uyc_c;
       const double vn = cw * uyp_w + ce * uyp_c;
       const double adv2 = -0.5 * vs * g_s - 0.5 * vn * g_n;
-      const double dif1 = rre * (g_e - g_w) / dxc_i;
-      const double dif2 = rre * (g_n - g_s) / dy;
-      const double pre = -(pc_c - pc_w) / dxc_i;
+      const double dif1 = qdiv<P2>(rre * (g_e - g_w), dxc_i, idxc_i);
+      const double dif2 = qdiv<P2>(rre * (g_n - g_s), dy, idy);
+      const double pre = qdiv<P2>(-(pc_c - pc_w), dxc_i, idxc_i);
       const double A = (adv1 + adv2) + (dif1 + dif2);
       const long si = L.sx(i, j);
       const double bterm = a.use_b ? bedt * a.sa_x[si] : 0.;
@@ -164,10 +167,10 @@ __global__ __launch_bounds__(PRED_BX) void k_predictor(PredArgs a) {
     }
     // ---- y-momentum at face (i, j), src/fluid/update_velocity.c:112-171
     {
-      const double g_w = (-uyc_w + uyc_c) / dxc_i;
-      const double g_e = (-uyc_c + uyc_e) / dxc_ip;
-      const double g_s = (-uym_c + uyc_c) / dy;
-      const double g_n = (-uyc_c + uyp_c) / dy;
+      const double g_w = qdiv<P2>(-uyc_w + uyc_c, dxc_i, idxc_i);
+      const double g_e = qdiv<P2>(-uyc_c + uyc_e, dxc_ip, idxc_ip);
+      const double g_s = qdiv<P2>(-uym_c + uyc_c, dy, idy);
+      const double g_n = qdiv<P2>(-uyc_c + uyp_c, dy, idy);
       const double cw = dxc_i / (2. * dxf_i);
       const double ce = dxc_ip / (2. * dxf_i);
       const double uw = 0.5 * uxm_c + 0.5 * uxc_c;
@@ -176,9 +179,9 @@ __global__ __launch_bounds__(PRED_BX) void k_predictor(PredArgs a) {
       const double as = 0.5 * uym_c + 0.5 * uyc_c;
       const double an = 0.5 * uyc_c + 0.5 * uyp_c;
       const double adv2 = -0.5 * as * g_s - 0.5 * an * g_n;
-      const double dif1 = rre * (g_e - g_w) / dxf_i;
-      const double dif2 = rre * (g_n - g_s) / dy;
-      const double pre = -(pc_c - pm_c) / dy;
+      const double dif1 = qdiv<P2>(rre * (g_e - g_w), dxf_i, idxf_i);
+      const double dif2 = qdiv<P2>(rre * (g_n - g_s), dy, idy);
+      const double pre = qdiv<P2>(-(pc_c - pm_c), dy, idy);
       const double A = (adv1 + adv2) + (dif1 + dif2) + a.ext;
       const long si = L.sy(i, j);
       const double bterm = a.use_b ? bedt * a.sa_y[si] : 0.;
@@ -210,9 +213,9 @@ __global__ __launch_bounds__(PRED_BX) void k_predictor(PredArgs a) {
       }
     }
     // slide the window
-    uxm_w = uxc_w; uxm_c = uxc_c; uxm_e = uxc_e;
+    uxm_c = uxc_c; uxm_e = uxc_e;
     uxc_w = uxp_w; uxc_c = uxp_c; uxc_e = uxp_e;
-    uym_w = uyc_w; uym_c = uyc_c; uym_e = uyc_e;
+    uym_c = uyc_c;
     uyc_w = uyp_w; uyc_c = uyp_c; uyc_e = uyp_e;
     pm_c = pc_c;
   }
@@ -232,7 +235,10 @@ int launch_predictor(Ctx *c, int rk) {
   a.wxm = c->cfg.uy_wall_xm; a.wxp = c->cfg.uy_wall_xp;
   a.use_b = c->beta[rk] != 0.;
   dim3 grid((c->nx + PRED_BX - 1) / PRED_BX, (c->ny + PRED_ROWS - 1) / PRED_ROWS);
-  hipLaunchKernelGGL(k_predictor, grid, dim3(PRED_BX), 0, c->stream, a);
+  if (c->pow2_grid)
+    hipLaunchKernelGGL(k_predictor<true>, grid, dim3(PRED_BX), 0, c->stream, a);
+  else
+    hipLaunchKernelGGL(k_predictor<false>, grid, dim3(PRED_BX), 0, c->stream, a);
   EIF_HIP(hipGetLastError());
   std::swap(c->ux, c->ux_alt);
   std::swap(c->uy, c->uy_alt);
@@ -245,6 +251,7 @@ int launch_predictor(Ctx *c, int rk) {
 // values of ux, uy and p refreshed by the threads that own the source
 // interior values.
 // ---------------------------------------------------------------------
+template <bool P2>
 __global__ void k_correct_pressure(Layout L, double *ux, double *uy, double *p,
                                    const double *psi, const double *dxc,
                                    const double *dscal, double ga, double dy,
@@ -266,7 +273,7 @@ __global__ void k_correct_pressure(Layout L, double *ux, double *uy, double *p,
     }
     {
       const long k = L.p(i, j);
-      const double v = uy[k] - ga * dt * (ps_c - psi[L.p(i, j - 1)]) / dy;
+      const double v = uy[k] - qdiv<P2>(ga * dt * (ps_c - psi[L.p(i, j - 1)]), dy, 1. / dy);
       uy[k] = v;
       if (j == ny) uy[L.p(i, 0)] = v;
       if (j == 1) uy[L.p(i, ny + 1)] = v;
@@ -294,9 +301,14 @@ __global__ void k_correct_pressure(Layout L, double *ux, double *uy, double *p,
 int launch_correct_pressure(Ctx *c, int rk, bool correct, bool pressure) {
   ProfScope ps(c, "correct_pressure");
   dim3 grid((c->nx + 255) / 256, c->ny);
-  hipLaunchKernelGGL(k_correct_pressure, grid, dim3(256), 0, c->stream, c->L, c->ux,
-                     c->uy, c->p, c->psi, c->dxc, c->dscal, c->gamma[rk], c->dy,
-                     (int)correct, (int)pressure);
+  if (c->pow2_grid)
+    hipLaunchKernelGGL(k_correct_pressure<true>, grid, dim3(256), 0, c->stream, c->L, c->ux,
+                       c->uy, c->p, c->psi, c->dxc, c->dscal, c->gamma[rk], c->dy,
+                       (int)correct, (int)pressure);
+  else
+    hipLaunchKernelGGL(k_correct_pressure<false>, grid, dim3(256), 0, c->stream, c->L, c->ux,
+                       c->uy, c->p, c->psi, c->dxc, c->dscal, c->gamma[rk], c->dy,
+                       (int)correct, (int)pressure);
   EIF_HIP(hipGetLastError());
   return EIF_OK;
 }

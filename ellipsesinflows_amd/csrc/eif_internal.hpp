@@ -58,6 +58,13 @@ constexpr int EQC_ITER_CAP = 100000;  // guard of the uncapped fixed point
 
 struct Profiler;
 
+// x / d; with POW2 the divisor is known to be a power of two and the
+// division is the exact product with inv = 1/d (identical result)
+template <bool POW2>
+__host__ __device__ inline double qdiv(double x, double d, double inv) {
+  return POW2 ? x * inv : x / d;
+}
+
 struct Ctx {
   eif_config cfg;
   int nx, ny;
@@ -68,6 +75,7 @@ struct Ctx {
   // host copies of the grid (exact reference arithmetic)
   std::vector<double> h_xf, h_xc, h_dxf, h_dxc, h_yf, h_yc;
   double alpha[3], beta[3], gamma[3];
+  bool pow2_grid = false;  // dx, dy and every DXF, DXC are powers of two
   // device grid
   double *xf = nullptr, *xc = nullptr, *dxf = nullptr, *dxc = nullptr,
          *yf = nullptr, *yc = nullptr;

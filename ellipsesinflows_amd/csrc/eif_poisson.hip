@@ -214,10 +214,11 @@ __device__ inline double cos_lookup(const double *C, int n, long m) {
 // workgroup.  mode 2: register FFT (blockDim = nx/8), mode 1: LDS FFT,
 // mode 0: direct sum (nx not a power of two).
 // ---------------------------------------------------------------------
+template <bool P2 = false>
 __device__ inline double rhs_value(const Layout &L, const double *ux, const double *uy,
                                    double fac, double dx, double dy, int i, int j) {
-  return fac * (+(ux[L.ux(i + 2, j)] - ux[L.ux(i + 1, j)]) / dx
-                + (uy[L.p(i + 1, j + 1)] - uy[L.p(i + 1, j)]) / dy);
+  return fac * (+qdiv<P2>(ux[L.ux(i + 2, j)] - ux[L.ux(i + 1, j)], dx, 1. / dx)
+                + qdiv<P2>(uy[L.p(i + 1, j + 1)] - uy[L.p(i + 1, j)], dy, 1. / dy));
 }
 
 __global__ __launch_bounds__(1024) void k_rhs_dct2(
@@ -368,7 +369,7 @@ __global__ __launch_bounds__(1024) void k_dct3_psi(
 
 // Register-FFT kernels specialised for nx = 2^LOGN (nx/8 threads); the
 // arithmetic is that of the mode-2 branches of k_rhs_dct2 / k_dct3_psi.
-template <int LOGN>
+template <int LOGN, bool P2>
 __global__ __launch_bounds__(1024) void k_rhs_dct2_t(
     Layout L, const double *ux, const double *uy, const double *dscal, double ga,
     double dx, double dy, const double *__restrict__ C, const double *__restrict__ W,
@@ -385,7 +386,7 @@ __global__ __launch_bounds__(1024) void k_rhs_dct2_t(
   for (int s = 0; s < 8; s++) {
     const int m = threadIdx.x + s * T;
     const int i = m < nx / 2 ? 2 * m : 2 * (nx - 1 - m) + 1;  // Makhoul order
-    re[sw(bitrev(m, LOGN))] = rhs_value(L, ux, uy, fac, dx, dy, i, j);
+    re[sw(bitrev(m, LOGN))] = rhs_value<P2>(L, ux, uy, fac, dx, dy, i, j);
   }
   __syncthreads();
   double xr[8], xi[8];
@@ -784,9 +785,12 @@ int launch_poisson(Ctx *c, int rk) {
     ProfScope ps(c, "rhs_dct2");
 #define EIF_RHS_T(LG)                                                                    \
   case LG:                                                                               \
-    hipLaunchKernelGGL(k_rhs_dct2_t<LG>, dim3(ny), dim3(threads), shmem, c->stream, c->L, \
-                       c->ux, c->uy, c->dscal, c->gamma[rk], c->dx, c->dy, c->ctab,      \
-                       c->twid, c->qhat);                                                \
+    {                                                                                    \
+      auto kern = c->pow2_grid ? k_rhs_dct2_t<LG, true> : k_rhs_dct2_t<LG, false>;       \
+      hipLaunchKernelGGL(kern, dim3(ny), dim3(threads), shmem, c->stream, c->L, c->ux,   \
+                         c->uy, c->dscal, c->gamma[rk], c->dx, c->dy, c->ctab, c->twid,  \
+                         c->qhat);                                                       \
+    }                                                                                    \
     break;
     switch (mode == 2 ? logn : -1) {
       EIF_RHS_T(6) EIF_RHS_T(7) EIF_RHS_T(8) EIF_RHS_T(9) EIF_RHS_T(10) EIF_RHS_T(11)
