@@ -573,13 +573,14 @@ __global__ __launch_bounds__(64 * PART_KG) void k_part_reduced(
     const bool gauge = lv[2 * (long)nx + k] != 0.;
     for (int m = lane; m < P; m += 64) {
       double a0r = 0., a0i = 0., a1r = 0., a1i = 0.;
-      for (int c = 0; c < P; c++) {
-        const int t = (int)(((long)m * c) % P);
+      for (int c = 0, t = 0; c < P; c++) {  // t = m c mod P
         const double cs = twc[t], sn = tws[t];
         a0r += r0[c] * cs;
         a0i -= r0[c] * sn;
         a1r += r1[c] * cs;
         a1i -= r1[c] * sn;
+        t += m;
+        if (t >= P) t -= P;
       }
       const double ct = twc[m], st = tws[m];
       double x0r, x0i, x1r, x1i;
@@ -607,11 +608,12 @@ __global__ __launch_bounds__(64 * PART_KG) void k_part_reduced(
     const double invP = 1. / P;
     for (int c = lane; c < P; c += 64) {
       double b0 = 0., b1 = 0.;
-      for (int m = 0; m < P; m++) {
-        const int t = (int)(((long)m * c) % P);
+      for (int m = 0, t = 0; m < P; m++) {  // t = m c mod P
         const double cs = twc[t], sn = tws[t];
         b0 += u0r[m] * cs - u0i[m] * sn;
         b1 += u1r[m] * cs - u1i[m] * sn;
+        t += c;
+        if (t >= P) t -= P;
       }
       // u_c = (beta_c, alpha_{c+1}): chunk c+1's left value, chunk c's right
       bnd[((long)((c + 1) % P) * 2 + 0) * nx + k] = b0 * invP;

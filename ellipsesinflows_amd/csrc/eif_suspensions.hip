@@ -60,6 +60,7 @@ struct EllipseShape {
   double a, b;
   double kex, key;  // a(1-(b/a)^2), b(1-(a/b)^2)   (ellipse.c:7-13)
   double a2b2;      // a*a + b*b                     (ellipse.c:49)
+  double ia, ib;    // 1/a, 1/b
 };
 
 __device__ inline EllipseShape make_shape(double a, double b) {
@@ -70,6 +71,8 @@ __device__ inline EllipseShape make_shape(double a, double b) {
   s.kex = a * (1. - ba * ba);
   s.key = b * (1. - ab * ab);
   s.a2b2 = a * a + b * b;
+  s.ia = 1. / a;
+  s.ib = 1. / b;
   return s;
 }
 
@@ -188,6 +191,14 @@ __device__ inline void sincos_q1(double t, double &s, double &c) {
   }
 }
 
+// 1/sqrt(x) from the hardware estimate and one Newton step (relative error
+// near 1e-16); used where only the size of a step depends on it
+__device__ inline double rsqrt_nr(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  const double e = fma(-(x * y), y, 1.);
+  return fma(0.5 * y, e, y);
+}
+
 // asin by its Maclaurin series, exact to rounding for |x| <= 2^-6; for
 // larger |x| a monotone under-estimate, which only shortens a step of the
 // fixed-point iteration (its fixed point is x = 0)
@@ -250,11 +261,13 @@ __device__ inline void foot_fp64(const EllipseShape &e, double px, double py, do
     const double ex = e.kex * cube(c), ey = e.key * cube(s);
     const double rx = x - ex, ry = y - ey;
     const double qx = px - ex, qy = py - ey;
-    const double r = sqrt(ry * ry + rx * rx);
-    const double q = sqrt(qy * qy + qx * qx);
-    const double arg = (rx * qy - ry * qx) / (r * q);
-    const double dc = r * asin_series(arg);
-    const double dt = dc / sqrt(e.a2b2 - x * x - y * y);
+    // dt = r asin(cross / (r q)) / sqrt(a^2 + b^2 - x^2 - y^2): the
+    // residual (cross) is formed exactly as in ellipse.c:41-46; the scale
+    // factors only set the step length, so reciprocal square roots do
+    const double rr = ry * ry + rx * rx, qq = qy * qy + qx * qx;
+    const double cross = rx * qy - ry * qx;
+    const double arg = cross * rsqrt_nr(rr * qq);
+    const double dt = asin_series(arg) * (rr * rsqrt_nr(rr * (e.a2b2 - x * x - y * y)));
     double tn = t + dt;
     tn = fmin(HALF_PI, tn);
     tn = fmax(0., tn);
@@ -282,13 +295,13 @@ __device__ inline void foot_point(const EllipseShape &e, const EllipseShape32 &f
 // the folded foot to (-cos t, sin t) for x_ < 0 and (cos t, -sin t) for
 // y_ < 0
 __device__ inline double signed_dist_fast(const EllipseShape &e, const EllipseShape32 &f,
-                                          double grid_size, double px, double py, double cr,
+                                          double inv_grid, double px, double py, double cr,
                                           double sr, double x, double y, int &it32,
                                           int &it64) {
   const double ddx = x - px, ddy = y - py;
   const double x_ = cr * ddx - sr * ddy;
   const double y_ = sr * ddx + cr * ddy;
-  const double xa = x_ / e.a, yb = y_ / e.b;
+  const double xa = x_ * e.ia, yb = y_ * e.ib;
   const double f0 = 1. - xa * xa - yb * yb;
   const double sign = f0 > 0. ? +1. : -1.;
   double ct, st;
@@ -297,7 +310,7 @@ __device__ inline double signed_dist_fast(const EllipseShape &e, const EllipseSh
   const double fy = y_ < 0. ? -(e.b * st) : e.b * st;
   const double ex = fx - x_, ey = fy - y_;
   const double dist = sign * sqrt(+ex * ex + ey * ey);
-  return dist / grid_size;
+  return dist * inv_grid;
 }
 
 // src/suspensions/auxiliary.c:49-59.  tanh(x) as 1 - 2 / (exp(2|x|) + 1)
@@ -366,34 +379,67 @@ __device__ inline void slice_store(const IbmArgs &a, int slice, const double v[S
     for (int k = 0; k < SLICE_VALS; k++) a.slice_part[SLICE_VALS * (long)slice + k] = v[k];
 }
 
-struct BoxSet {
-  int imin[3], imax[3], jmin[3], jmax[3];
-  long cnt[3];
-  long total;
+// box of one periodic image (auxiliary.c:11-17); three separate members
+// rather than an array, so that selecting an image stays in registers
+struct Img {
+  int imin, imax, jmin, jmax;
+  int cnt;
+  float inv_w;  // 1 / box width, for the quotient in cell_of
 };
+struct BoxSet {
+  Img b0, b1, b2;
+  int total;
+};
+
+__device__ inline Img make_img(const IbmArgs &a, double px, double py_, double r) {
+  Img m;
+  loop_size(1, a.nx, a.dx, r, px, m.imin, m.imax);
+  loop_size(1, a.ny, a.dy, r, py_ - 0., m.jmin, m.jmax);
+  const int w = m.imax - m.imin + 1, h = m.jmax - m.jmin + 1;
+  m.cnt = (w > 0 && h > 0) ? w * h : 0;
+  m.inv_w = w > 0 ? 1.f / (float)w : 0.f;
+  return m;
+}
 
 __device__ inline void make_boxes(const IbmArgs &a, double px, double py,
                                   double r, BoxSet &bs) {
-  bs.total = 0;
-  for (int k = 0; k < 3; k++) {
-    const double py_ = py + a.ly * (k - 1);
-    loop_size(1, a.nx, a.dx, r, px, bs.imin[k], bs.imax[k]);
-    loop_size(1, a.ny, a.dy, r, py_ - 0., bs.jmin[k], bs.jmax[k]);
-    const long w = bs.imax[k] - bs.imin[k] + 1, h = bs.jmax[k] - bs.jmin[k] + 1;
-    bs.cnt[k] = (w > 0 && h > 0) ? w * h : 0;
-    bs.total += bs.cnt[k];
-  }
+  bs.b0 = make_img(a, px, py + a.ly * (0 - 1), r);
+  bs.b1 = make_img(a, px, py + a.ly * (1 - 1), r);
+  bs.b2 = make_img(a, px, py + a.ly * (2 - 1), r);
+  bs.total = bs.b0.cnt + bs.b1.cnt + bs.b2.cnt;
 }
 
-__device__ inline void cell_of(const BoxSet &bs, long t, int &k, int &i, int &j) {
-  k = 0;
-  while (t >= bs.cnt[k]) {
-    t -= bs.cnt[k];
-    k++;
+__device__ inline Img pick(const BoxSet &bs, int k) {
+  Img m = bs.b0;
+  if (k == 1) m = bs.b1;
+  if (k == 2) m = bs.b2;
+  return m;
+}
+
+// image k and cell (i, j) of flattened box cell t.  The image's fields are
+// blended arithmetically (a select between struct members can be turned
+// into a load from a stack copy), and the quotient by the box width comes
+// from a float reciprocal, corrected once (exact for t < 2^24).
+__device__ inline void cell_of(const BoxSet &bs, int t, int &k, int &i, int &j) {
+  const int c0 = bs.b0.cnt, c01 = c0 + bs.b1.cnt;
+  const int g1 = t >= c0, g2 = t >= c01;
+  k = g1 + g2;
+  const int tk = t - g1 * c0 - g2 * bs.b1.cnt;
+  const int imin = bs.b0.imin + g1 * (bs.b1.imin - bs.b0.imin) + g2 * (bs.b2.imin - bs.b1.imin);
+  const int imax = bs.b0.imax + g1 * (bs.b1.imax - bs.b0.imax) + g2 * (bs.b2.imax - bs.b1.imax);
+  const int jmin = bs.b0.jmin + g1 * (bs.b1.jmin - bs.b0.jmin) + g2 * (bs.b2.jmin - bs.b1.jmin);
+  const int w = imax - imin + 1;
+  int q = (int)((float)tk * __builtin_amdgcn_rcpf((float)w));
+  int r = tk - q * w;
+  if (r < 0) {
+    q--;
+    r += w;
+  } else if (r >= w) {
+    q++;
+    r -= w;
   }
-  const int w = bs.imax[k] - bs.imin[k] + 1;
-  j = bs.jmin[k] + (int)(t / w);
-  i = bs.imin[k] + (int)(t % w);
+  j = jmin + q;
+  i = imin + r;
 }
 
 // src/suspensions/inertia.c:9-70.  One wave per slice of IBM_SLICE box
@@ -413,8 +459,8 @@ __global__ __launch_bounds__(64) void k_inertia(IbmArgs a, int cnstep) {
   const EllipseShape32 f = shape32(e);
   BoxSet bs;
   make_boxes(a, px, py, fmax(pa, pb), bs);
-  const long total = bs.total;
-  if (total > a.box_off[n + 1] - a.box_off[n] || total > (long)nsl * IBM_SLICE) {
+  const int total = bs.total;
+  if (total > a.box_off[n + 1] - a.box_off[n] || total > nsl * IBM_SLICE) {
     if (threadIdx.x == 0) atomicOr(&a.flags[0], 1);
     return;
   }
@@ -426,9 +472,9 @@ __global__ __launch_bounds__(64) void k_inertia(IbmArgs a, int cnstep) {
   const double dxdy = a.dx * a.dy;
   double acc[3] = {0., 0., 0.};  // iux, iuy, ivz
   int it64 = 0, it32 = 0;
-  const long t0 = (long)(slice - s0) * IBM_SLICE;
-  const long t1 = t0 + IBM_SLICE < total ? t0 + IBM_SLICE : total;
-  for (long t = t0 + threadIdx.x; t < t1; t += 64) {
+  const int t0 = (slice - s0) * IBM_SLICE;
+  const int t1 = t0 + IBM_SLICE < total ? t0 + IBM_SLICE : total;
+  for (int t = t0 + (int)threadIdx.x; t < t1; t += 64) {
     int k, i, j;
     cell_of(bs, t, k, i, j);
     const double py_ = py + a.ly * (k - 1);
@@ -437,7 +483,7 @@ __global__ __launch_bounds__(64) void k_inertia(IbmArgs a, int cnstep) {
     if (cached) {
       d = dist[t];
     } else {
-      d = signed_dist_fast(e, f, a.grid_size, px, py_, cr, sr, x, y, it32, it64);
+      d = signed_dist_fast(e, f, 1. / a.grid_size, px, py_, cr, sr, x, y, it32, it64);
       dist[t] = d;
     }
     const double w = v_weight(d);
@@ -577,35 +623,39 @@ __global__ __launch_bounds__(64) void k_exchange(IbmArgs a) {
   const EllipseShape32 f = shape32(e);
   BoxSet bs;
   make_boxes(a, px, py, fmax(pa, pb), bs);
-  const long total = bs.total;
-  if (total > a.box_off[n + 1] - a.box_off[n] || total > (long)nsl * IBM_SLICE) {
+  const int total = bs.total;
+  if (total > a.box_off[n + 1] - a.box_off[n] || total > nsl * IBM_SLICE) {
     if (threadIdx.x == 0) atomicOr(&a.flags[0], 1);
     return;
   }
   if (slice == s0) {
-    if (threadIdx.x < 3) {
-      const int k = threadIdx.x;
+    if (threadIdx.x == 0) {
       long off = a.box_off[n];
-      for (int kk = 0; kk < k; kk++) off += bs.cnt[kk];
-      Box b;
-      b.imin = bs.imin[k];
-      b.imax = bs.imax[k];
-      b.jmin = bs.jmin[k];
-      b.jmax = bs.jmax[k];
-      b.off = off;
-      a.boxes[(long)n * 3 + k] = b;
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        const Img m = pick(bs, k);
+        Box b;
+        b.imin = m.imin;
+        b.imax = m.imax;
+        b.jmin = m.jmin;
+        b.jmax = m.jmax;
+        b.off = off;
+        a.boxes[(long)n * 3 + k] = b;
+        off += m.cnt;
+      }
     }
     // register the boxes with their tiles, one tile per lane
+#pragma unroll
     for (int r = 0; r < 6; r++) {
-      const int k = r / 2;
-      if (bs.cnt[k] <= 0) continue;
+      const Img m = pick(bs, r / 2);
+      if (m.cnt <= 0) continue;
       TileRange tr[2];
-      box_tile_ranges(a, bs.imin[k], bs.imax[k], bs.jmin[k], bs.jmax[k], tr[0], tr[1]);
+      box_tile_ranges(a, m.imin, m.imax, m.jmin, m.jmax, tr[0], tr[1]);
       const TileRange &rg = tr[r & 1];
       const int cnt = rg.count();
       const int w = rg.tx1 - rg.tx0 + 1;
       for (int e_ = threadIdx.x; e_ < cnt; e_ += 64)
-        register_tile(a, (rg.ty0 + e_ / w) * a.ntx + rg.tx0 + e_ % w, n * 3 + k);
+        register_tile(a, (rg.ty0 + e_ / w) * a.ntx + rg.tx0 + e_ % w, n * 3 + r / 2);
     }
   }
   const double *cp = a.sc_pos + 4 * (long)n;
@@ -618,9 +668,9 @@ __global__ __launch_bounds__(64) void k_exchange(IbmArgs a) {
   const double dx = a.dx, dy = a.dy;
   double acc[3] = {0., 0., 0.};  // fux, fuy, tvz
   int it64 = 0, it32 = 0;
-  const long t0 = (long)(slice - s0) * IBM_SLICE;
-  const long t1 = t0 + IBM_SLICE < total ? t0 + IBM_SLICE : total;
-  for (long t = t0 + threadIdx.x; t < t1; t += 64) {
+  const int t0 = (slice - s0) * IBM_SLICE;
+  const int t1 = t0 + IBM_SLICE < total ? t0 + IBM_SLICE : total;
+  for (int t = t0 + (int)threadIdx.x; t < t1; t += 64) {
     int k, i, j;
     cell_of(bs, t, k, i, j);
     const double py_ = py + a.ly * (k - 1);
@@ -629,7 +679,7 @@ __global__ __launch_bounds__(64) void k_exchange(IbmArgs a) {
     if (cached) {
       d = dist[t];
     } else {
-      d = signed_dist_fast(e, f, a.grid_size, px, py_, cr, sr, x, y, it32, it64);
+      d = signed_dist_fast(e, f, 1. / a.grid_size, px, py_, cr, sr, x, y, it32, it64);
       dist[t] = d;
     }
     const double w = s_weight(d);
