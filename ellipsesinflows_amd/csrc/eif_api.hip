@@ -136,10 +136,11 @@ static void free_ctx(Ctx *c) {
   void *others[] = {c->th_flag, c->boxes, c->box_off, c->tile_cnt, c->tile_list,
                     c->active, c->flags, c->pair_cnt, c->pair_ids, c->wall_hit,
                     c->ibm_stats, c->slice_first, c->slice_info, c->ibm_pp, c->slice_owner,
-                    c->cand};
+                    c->cand, c->gate};
   for (auto o : others)
     if (o) (void)hipFree(o);
   if (c->hscal) (void)hipHostFree(c->hscal);
+  if (c->hgate) (void)hipHostFree(c->hgate);
   if (c->prof) {
     for (auto &kv : c->prof->recs)
       for (auto &e : kv.second.ev) {
@@ -270,6 +271,9 @@ int fluid_gpu_create(const eif_config *cfg, eif_ctx **out) {
     if (e != hipSuccess) return fail(hip_fail(e, "hipHostMalloc", __FILE__, __LINE__));
     e = hipMalloc(&c->flags, 4 * sizeof(int));
     if (e == hipSuccess) e = hipMemset(c->flags, 0, 4 * sizeof(int));
+    if (e == hipSuccess) e = hipMalloc(&c->gate, 4 * sizeof(int));
+    if (e == hipSuccess) e = hipMemset(c->gate, 0, 4 * sizeof(int));
+    if (e == hipSuccess) e = hipHostMalloc((void **)&c->hgate, 4 * sizeof(int), 0);
     if (e == hipSuccess) e = hipMalloc(&c->ibm_stats, 8 * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMemset(c->ibm_stats, 0, 8 * sizeof(unsigned long long));
     if (e != hipSuccess) return fail(hip_fail(e, "hipMalloc", __FILE__, __LINE__));
@@ -629,16 +633,32 @@ int eif_gpu_integrate(eif_ctx *c, int substeps[3]) {
     TRY(launch_poisson(c, rk));
     TRY(launch_correct_pressure(c, rk, true, true));
     TRY(launch_collision(c, 0));
-    int substep;
-    for (substep = 0;; substep++) {
-      TRY(launch_inertia(c, 1));
-      TRY(launch_collision(c, 1));
-      double residual = 0.;
-      TRY(suspensions_gpu_increment_particles(c, rk, &residual));
-      if (residual < 1.e-8) break;
-      if (substep > 100) break;
+    // Crank-Nicolson sub-iterations (src/main.c:37-50).  They are queued in
+    // batches; a device-side gate before each one applies the loop's exit
+    // rule, so sub-iterations queued past convergence do nothing, and the
+    // host reads the count and the residual once per batch.
+    int queued = 0, done = 0;
+    int batch = c->substep_guess > 0 ? c->substep_guess : 1;
+    for (;;) {
+      for (int b = 0; b < batch; b++) {
+        TRY(launch_substep_gate(c));
+        TRY(launch_inertia(c, 1, true));
+        TRY(launch_collision(c, 1, true));
+        TRY(launch_increment(c, rk, true));
+      }
+      queued += batch;
+      EIF_HIP(hipMemcpyAsync(c->hgate, c->gate, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+      EIF_HIP(hipMemcpyAsync(c->hscal + S_RESIDUAL, c->dscal + S_RESIDUAL, sizeof(double),
+                             hipMemcpyDeviceToHost, c->stream));
+      EIF_HIP(hipStreamSynchronize(c->stream));
+      TRY(check_flags(c));
+      done = c->hgate[1];
+      const double residual = c->hscal[S_RESIDUAL];
+      if (done < queued || residual < 1.e-8 || done - 1 > 100) break;
+      batch = 1;
     }
-    if (substeps) substeps[rk] = substep + 1;
+    c->substep_guess = done;
+    if (substeps) substeps[rk] = done;
     TRY(launch_update_particles(c));
   }
   return EIF_OK;
@@ -661,9 +681,9 @@ int eif_gpu_synchronize(eif_ctx *c) {
 
 int eif_gpu_ibm_stats(eif_ctx *c, unsigned long long out[5], int reset) {
   CHECK_CTX(c);
-  unsigned long long h[5] = {0, 0, 0, c->ibm_calls, 0};
+  unsigned long long h[5] = {0, 0, 0, 0, 0};
   if (c->np > 0) {
-    std::vector<unsigned long long> pp(4 * (size_t)c->np);
+    std::vector<unsigned long long> pp(4 * (size_t)c->np + 1);
     EIF_HIP(hipMemcpyAsync(pp.data(), c->ibm_pp, pp.size() * sizeof(unsigned long long),
                            hipMemcpyDeviceToHost, c->stream));
     EIF_HIP(hipStreamSynchronize(c->stream));
@@ -673,13 +693,13 @@ int eif_gpu_ibm_stats(eif_ctx *c, unsigned long long out[5], int reset) {
       h[2] += pp[4 * (size_t)n + 2];
       h[4] += pp[4 * (size_t)n + 3];
     }
+    h[3] = pp[4 * (size_t)c->np];
   }
   if (out)
     for (int k = 0; k < 5; k++) out[k] = h[k];
   if (reset) {
-    c->ibm_calls = 0;
     if (c->np > 0) {
-      EIF_HIP(hipMemsetAsync(c->ibm_pp, 0, 4 * (size_t)c->np * sizeof(unsigned long long),
+      EIF_HIP(hipMemsetAsync(c->ibm_pp, 0, (4 * (size_t)c->np + 1) * sizeof(unsigned long long),
                              c->stream));
       EIF_HIP(hipStreamSynchronize(c->stream));
     }

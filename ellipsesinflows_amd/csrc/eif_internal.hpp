@@ -125,8 +125,10 @@ struct Ctx {
   int *active = nullptr;   // [0] count, then tile ids
   int *flags = nullptr;    // device error flags
   unsigned long long *ibm_stats = nullptr;  // [8] measurement counters
-  unsigned long long *ibm_pp = nullptr;     // np x 4 IBM work counters
-  unsigned long long ibm_calls = 0;
+  unsigned long long *ibm_pp = nullptr;     // np x 4 IBM work counters, then calls
+  int *gate = nullptr;   // [0] sub-iteration open, [1] sub-iterations run
+  int *hgate = nullptr;  // pinned copy
+  int substep_guess = 3; // sub-iterations queued before the first check
   // collisions
   int *pair_cnt = nullptr;  // [0] colliding pairs, [1] candidate pairs
   int pair_cap = 0;
@@ -175,12 +177,13 @@ int launch_diagnostics(Ctx *c, int which, double *out);
 int poisson_init(Ctx *c);
 int launch_poisson(Ctx *c, int rk);
 int susp_init(Ctx *c);
-int launch_inertia(Ctx *c, int cnstep);
+int launch_inertia(Ctx *c, int cnstep, bool gated = false);
 int launch_exchange(Ctx *c);
 int launch_update_momentum(Ctx *c);
 int launch_materialise_du(Ctx *c);
-int launch_collision(Ctx *c, int cnstep);
-int launch_increment(Ctx *c, int rk);
+int launch_collision(Ctx *c, int cnstep, bool gated = false);
+int launch_increment(Ctx *c, int rk, bool gated = false);
+int launch_substep_gate(Ctx *c);
 int launch_update_particles(Ctx *c);
 int launch_reset(Ctx *c);
 int check_flags(Ctx *c);

@@ -363,6 +363,7 @@ struct IbmArgs {
   const int *slice_first;  // np+1: first slice of each particle
   const int *slice_owner;  // particle of each slice
   long *slice_info;        // np x 2: box cells, cells evaluated (this call)
+  const int *gate;         // speculative sub-iteration gate (null: always run)
   double *slice_part;      // per-slice partial sums (3 each)
 };
 
@@ -446,6 +447,7 @@ __device__ inline void cell_of(const BoxSet &bs, int t, int &k, int &i, int &j) 
 // cells of one particle (its three images' boxes flattened); the last
 // slice of a particle to finish adds the slices' sums in slice order.
 __global__ __launch_bounds__(64) void k_inertia(IbmArgs a, int cnstep) {
+  if (a.gate && !a.gate[0]) return;
   const int slice = blockIdx.x;
   const int n = a.slice_owner[slice];
   const int s0 = a.slice_first[n], nsl = a.slice_first[n + 1] - s0;
@@ -507,6 +509,7 @@ __global__ __launch_bounds__(64) void k_inertia(IbmArgs a, int cnstep) {
 // slice order; also records the position whose distances are now cached
 // and the work counters.  which: 0..1 inertia (cnstep), 2 exchange.
 __global__ void k_ibm_finalize(IbmArgs a, int which) {
+  if (a.gate && !a.gate[0]) return;
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= a.np) return;
   double tot[SLICE_VALS] = {0., 0., 0., 0., 0.};
@@ -531,6 +534,7 @@ __global__ void k_ibm_finalize(IbmArgs a, int which) {
   cw[1] = py;
   cw[2] = paz;
   cw[3] = 1.;
+  if (n == 0) a.stats[4 * (long)a.np] += 1ull;  // IBM calls
   unsigned long long *c = a.stats + 4 * (long)n;
   c[0] += (unsigned long long)a.slice_info[2 * (long)n];
   c[1] += (unsigned long long)a.slice_info[2 * (long)n + 1];
@@ -865,6 +869,7 @@ __device__ void evolute_fit_fast(const EllD &e, const EllipseShape &s,
 }
 
 struct CollArgs {
+  const int *gate;  // speculative sub-iteration gate (null: always run)
   int *cand;     // candidate pairs (n0, n1)
   int cand_cap;
   int np;
@@ -903,6 +908,7 @@ __device__ inline double min_image_offset(double p0y, double p1y, double ly) {
 }
 
 __global__ __launch_bounds__(256) void k_collide_search(CollArgs a) {
+  if (a.gate && !a.gate[0]) return;
   __shared__ double sx[2][COLL_TILE], sy[2][COLL_TILE], sr[2][COLL_TILE];
   const int nt = (a.np + COLL_TILE - 1) / COLL_TILE;
   int b = blockIdx.x, tb0 = 0;
@@ -941,8 +947,7 @@ __global__ __launch_bounds__(256) void k_collide_search(CollArgs a) {
 }
 
 // collision.c:156-247 for one candidate pair per thread
-__global__ void k_collide_pairs(CollArgs a) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ void collide_pair(const CollArgs &a, int t) {
   const int ncand = min(a.pair_cnt[1], a.cand_cap);
   if (t >= ncand) return;
   const int n0 = a.cand[2 * t], n1 = a.cand[2 * t + 1];
@@ -1013,9 +1018,8 @@ __global__ void k_collide_pairs(CollArgs a) {
   }
 }
 
-// collision.c:249-304
-__global__ void k_collide_walls(CollArgs a) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+// collision.c:249-304, thread t: particle t/2, wall t%2
+__device__ void collide_wall(const CollArgs &a, int t) {
   if (t >= 2 * a.np) return;
   const int n = t >> 1, w = t & 1;
   const double wallx = w == 0 ? 0. : a.lx;
@@ -1071,9 +1075,20 @@ __global__ void k_collide_walls(CollArgs a) {
   }
 }
 
+// candidate pairs in the first pair_blocks workgroups, particle-wall pairs
+// in the rest
+__global__ void k_collide_pairs_walls(CollArgs a, int pair_blocks) {
+  if (a.gate && !a.gate[0]) return;
+  if ((int)blockIdx.x < pair_blocks)
+    collide_pair(a, blockIdx.x * blockDim.x + threadIdx.x);
+  else
+    collide_wall(a, (blockIdx.x - pair_blocks) * blockDim.x + threadIdx.x);
+}
+
 // per particle: partners in ascending index (the order of the reference's
 // flattened pair loop), then wall x=0, then wall x=lx
 __global__ void k_collide_gather(CollArgs a) {
+  if (a.gate && !a.gate[0]) return;
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= a.np) return;
   int partner[CONTACT_CAP];
@@ -1124,6 +1139,26 @@ __global__ void k_zero_int(int *p, int n) {
   if (t < n) p[t] = 0;
 }
 
+// Gate of one speculatively queued Crank-Nicolson sub-iteration
+// (src/main.c:37-50): the loop ends after a sub-iteration whose residual
+// is below 1e-8, or after sub-iteration 101; a closed gate makes every
+// kernel of the later sub-iterations return at once.  An open gate also
+// clears what the sub-iteration accumulates into (residual, pair counts)
+// and counts the sub-iteration.
+__global__ void k_substep_gate(double *dscal, int *gate, int *pair_cnt) {
+  const int s = gate[1];
+  const bool open = s == 0 || (!(dscal[S_RESIDUAL] < 1.e-8) && !(s - 1 > 100));
+  gate[0] = open ? 1 : 0;
+  if (open) {
+    gate[1] = s + 1;
+    dscal[S_RESIDUAL] = 0.;
+    if (pair_cnt) {
+      pair_cnt[0] = 0;
+      pair_cnt[1] = 0;
+    }
+  }
+}
+
 // ---------------------------------------------------------------------
 // particle increments / update / reset
 // ---------------------------------------------------------------------
@@ -1136,7 +1171,8 @@ __global__ void k_zero_residual(double *dscal) { dscal[S_RESIDUAL] = 0.; }
 
 // src/suspensions/increment.c:10-64
 __global__ void k_increment(double *prt, int np, const double *dscal, double ga,
-                            double Fr, double *res_out) {
+                            double Fr, double *res_out, const int *gate) {
+  if (gate && !gate[0]) return;
   __shared__ double red[256];
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   double res = 0.;
@@ -1190,8 +1226,12 @@ __global__ void k_update_particles(double *prt, int np, double ly) {
 }
 
 // src/suspensions/reset.c:7-18
-__global__ void k_reset(double *prt, int np) {
+__global__ void k_reset(double *prt, int np, int *gate) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n == 0 && gate) {
+    gate[0] = 1;
+    gate[1] = 0;
+  }
   if (n >= np) return;
   double *p = prt + (long)n * R_N;
   p[R_DUX] = 0.;
@@ -1237,6 +1277,7 @@ static IbmArgs ibm_args(Ctx *c) {
   a.slice_first = c->slice_first;
   a.slice_owner = c->slice_owner;
   a.slice_info = c->slice_info;
+  a.gate = nullptr;
   a.slice_part = c->slice_part;
   return a;
 }
@@ -1281,8 +1322,8 @@ int susp_init(Ctx *c) {
   EIF_HIP(hipMalloc(&c->slice_info, 2 * (size_t)(c->np > 0 ? c->np : 1) * sizeof(long)));
   EIF_HIP(hipMemset(c->slice_info, 0, 2 * (size_t)(c->np > 0 ? c->np : 1) * sizeof(long)));
   EIF_HIP(hipMalloc(&c->slice_part, SLICE_VALS * (size_t)(c->nslices > 0 ? c->nslices : 1) * sizeof(double)));
-  EIF_HIP(hipMalloc(&c->ibm_pp, 4 * (size_t)(c->np > 0 ? c->np : 1) * sizeof(unsigned long long)));
-  EIF_HIP(hipMemset(c->ibm_pp, 0, 4 * (size_t)(c->np > 0 ? c->np : 1) * sizeof(unsigned long long)));
+  EIF_HIP(hipMalloc(&c->ibm_pp, (4 * (size_t)c->np + 1) * sizeof(unsigned long long)));
+  EIF_HIP(hipMemset(c->ibm_pp, 0, (4 * (size_t)c->np + 1) * sizeof(unsigned long long)));
   EIF_HIP(hipMalloc(&c->box_off, (c->np + 1) * sizeof(long)));
   EIF_HIP(hipMemcpy(c->box_off, off.data(), (c->np + 1) * sizeof(long), hipMemcpyHostToDevice));
   const size_t sc = (size_t)(c->scratch_cells > 0 ? c->scratch_cells : 1) * sizeof(double);
@@ -1316,20 +1357,20 @@ int susp_init(Ctx *c) {
 }
 
 int launch_reset(Ctx *c) {
-  if (c->np == 0) return EIF_OK;
-  hipLaunchKernelGGL(k_reset, dim3((c->np + 255) / 256), dim3(256), 0, c->stream, c->prt,
-                     c->np);
+  // also rewinds the sub-iteration gate, so it runs without particles too
+  hipLaunchKernelGGL(k_reset, dim3(c->np > 0 ? (c->np + 255) / 256 : 1), dim3(256), 0,
+                     c->stream, c->prt, c->np, c->gate);
   EIF_HIP(hipGetLastError());
   return EIF_OK;
 }
 
-int launch_inertia(Ctx *c, int cnstep) {
+int launch_inertia(Ctx *c, int cnstep, bool gated) {
   if (c->np == 0) return EIF_OK;
   ProfScope ps(c, "ibm_inertia");
   IbmArgs a = ibm_args(c);
+  if (gated) a.gate = c->gate;
   hipLaunchKernelGGL(k_inertia, dim3(c->nslices), dim3(64), 0, c->stream, a, cnstep);
   hipLaunchKernelGGL(k_ibm_finalize, dim3((c->np + 63) / 64), dim3(64), 0, c->stream, a, cnstep);
-  c->ibm_calls++;
   EIF_HIP(hipGetLastError());
   return EIF_OK;
 }
@@ -1345,7 +1386,6 @@ int launch_exchange(Ctx *c) {
   IbmArgs a = ibm_args(c);
   hipLaunchKernelGGL(k_exchange, dim3(c->nslices), dim3(64), 0, c->stream, a);
   hipLaunchKernelGGL(k_ibm_finalize, dim3((c->np + 63) / 64), dim3(64), 0, c->stream, a, 2);
-  c->ibm_calls++;
   EIF_HIP(hipGetLastError());
   return EIF_OK;
 }
@@ -1399,7 +1439,7 @@ int launch_materialise_du(Ctx *c) {
   return launch_bc_p(c, c->duy);
 }
 
-int launch_collision(Ctx *c, int cnstep) {
+int launch_collision(Ctx *c, int cnstep, bool gated) {
   if (c->np == 0) return EIF_OK;
   ProfScope ps(c, "collision");
   CollArgs a;
@@ -1417,25 +1457,34 @@ int launch_collision(Ctx *c, int cnstep) {
   a.flags = c->flags;
   a.cand = c->cand;
   a.cand_cap = c->cand_cap;
-  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, c->pair_cnt, 2);
+  a.gate = gated ? c->gate : nullptr;
+  if (!gated)  // an open gate has cleared the counts
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, c->pair_cnt, 2);
   const int nt = (c->np + COLL_TILE - 1) / COLL_TILE;
   hipLaunchKernelGGL(k_collide_search, dim3(nt * (nt + 1) / 2), dim3(256), 0, c->stream, a);
-  hipLaunchKernelGGL(k_collide_pairs, dim3((c->cand_cap + 63) / 64), dim3(64), 0,
-                     c->stream, a);
-  hipLaunchKernelGGL(k_collide_walls, dim3((2 * c->np + 255) / 256), dim3(256), 0,
-                     c->stream, a);
+  const int pair_blocks = (c->cand_cap + 63) / 64, wall_blocks = (2 * c->np + 63) / 64;
+  hipLaunchKernelGGL(k_collide_pairs_walls, dim3(pair_blocks + wall_blocks), dim3(64), 0,
+                     c->stream, a, pair_blocks);
   hipLaunchKernelGGL(k_collide_gather, dim3((c->np + 127) / 128), dim3(128), 0, c->stream,
                      a);
   EIF_HIP(hipGetLastError());
   return EIF_OK;
 }
 
-int launch_increment(Ctx *c, int rk) {
-  hipLaunchKernelGGL(k_zero_residual, dim3(1), dim3(1), 0, c->stream, c->dscal);
+int launch_increment(Ctx *c, int rk, bool gated) {
+  if (!gated)  // an open gate has cleared the residual
+    hipLaunchKernelGGL(k_zero_residual, dim3(1), dim3(1), 0, c->stream, c->dscal);
   if (c->np > 0)
     hipLaunchKernelGGL(k_increment, dim3((c->np + 255) / 256), dim3(256), 0, c->stream,
                        c->prt, c->np, c->dscal, c->gamma[rk], c->cfg.Fr,
-                       c->dscal + S_RESIDUAL);
+                       c->dscal + S_RESIDUAL, gated ? c->gate : nullptr);
+  EIF_HIP(hipGetLastError());
+  return EIF_OK;
+}
+
+int launch_substep_gate(Ctx *c) {
+  hipLaunchKernelGGL(k_substep_gate, dim3(1), dim3(1), 0, c->stream, c->dscal, c->gate,
+                     c->np > 0 ? c->pair_cnt : nullptr);
   EIF_HIP(hipGetLastError());
   return EIF_OK;
 }
