@@ -1412,7 +1412,7 @@ static TileArgs tile_args(Ctx *c, int apply) {
 
 static int tile_grid(Ctx *c) {
   const long ntiles = (long)c->ntx * c->nty;
-  return (int)(ntiles < 2048 ? ntiles : 2048);
+  return (int)ntiles;  // one workgroup per tile; idle tiles return at once
 }
 
 int launch_update_momentum(Ctx *c) {
