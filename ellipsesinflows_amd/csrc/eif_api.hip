@@ -129,14 +129,14 @@ static void free_ctx(Ctx *c) {
                      &c->th_q1, &c->th_den, &c->th_last, &c->pt_fwd, &c->pt_u,
                      &c->pt_v, &c->pt_w, &c->pt_bnd, &c->pt_red, &c->pt_lvl, &c->pt_ends, &c->pt_g, &c->dscal, &c->dred,
                      &c->prt, &c->pconst, &c->sc_fx, &c->sc_fy, &c->sc_dist, &c->sc_pos,
-                     &c->pair_val, &c->wall_val, &c->st_ux1, &c->st_ux2, &c->st_uy1,
+                     &c->pair_val, &c->wall_val, &c->sc_t, &c->st_ux1, &c->st_ux2, &c->st_uy1,
                      &c->st_uy2, &c->st_phi, &c->slice_part};
   for (auto b : bufs)
     if (*b) (void)hipFree(*b);
   void *others[] = {c->th_flag, c->boxes, c->box_off, c->tile_cnt, c->tile_list,
                     c->active, c->flags, c->pair_cnt, c->pair_ids, c->wall_hit,
                     c->ibm_stats, c->slice_first, c->slice_info, c->ibm_pp, c->slice_owner,
-                    c->cand, c->gate};
+                    c->cand, c->gate, c->sc_key};
   for (auto o : others)
     if (o) (void)hipFree(o);
   if (c->hscal) (void)hipHostFree(c->hscal);
@@ -528,6 +528,10 @@ int suspensions_gpu_upload(eif_ctx *c, const double *records) {
   CHECK_CTX(c);
   if (c->np == 0) return EIF_OK;
   EIF_HIP(hipMemcpyAsync(c->prt, records, (size_t)c->np * R_N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  // the records may change shapes and positions: drop the per-particle
+  // distance caches
+  EIF_HIP(hipMemsetAsync(c->sc_pos, 0, 4 * (size_t)c->np * sizeof(double), c->stream));
+  EIF_HIP(hipMemsetAsync(c->sc_key, 0, 8 * (size_t)c->np * sizeof(int), c->stream));
   EIF_HIP(hipStreamSynchronize(c->stream));
   return EIF_OK;
 }

@@ -313,6 +313,33 @@ __device__ inline double signed_dist_fast(const EllipseShape &e, const EllipseSh
   return dist * inv_grid;
 }
 
+// signed_dist_fast with the foot iteration started from the parameter this
+// cell's foot had at the particle's previous evaluation in the same RK
+// stage (warm), which the particle has barely moved from; the root on the
+// first-quadrant arc is unique, so the start changes only the number of
+// iterations.  The new parameter is stored back.
+__device__ inline double signed_dist_warm(const EllipseShape &e, const EllipseShape32 &f,
+                                          double inv_grid, double px, double py, double cr,
+                                          double sr, double x, double y, bool warm,
+                                          double &tslot, int &it32, int &it64) {
+  const double ddx = x - px, ddy = y - py;
+  const double x_ = cr * ddx - sr * ddy;
+  const double y_ = sr * ddx + cr * ddy;
+  const double xa = x_ * e.ia, yb = y_ * e.ib;
+  const double f0 = 1. - xa * xa - yb * yb;
+  const double sign = f0 > 0. ? +1. : -1.;
+  const double ax = fabs(x_), ay = fabs(y_);
+  double t = warm ? tslot : (double)foot_fp32(f, ax, ay, it32);
+  double ct, st;
+  foot_fp64(e, ax, ay, t, ct, st, it64);
+  tslot = t;
+  const double fx = x_ < 0. ? -(e.a * ct) : e.a * ct;
+  const double fy = y_ < 0. ? -(e.b * st) : e.b * st;
+  const double ex = fx - x_, ey = fy - y_;
+  const double dist = sign * sqrt(+ex * ex + ey * ey);
+  return dist * inv_grid;
+}
+
 // src/suspensions/auxiliary.c:49-59.  tanh(x) as 1 - 2 / (exp(2|x|) + 1)
 // with the sign of x: like the libm routine it rounds to exactly +-1 once
 // 2 / (exp(2|x|) + 1) drops below half an ulp of 1, so the weights are
@@ -355,6 +382,8 @@ struct IbmArgs {
   Box *boxes;
   const long *box_off;
   double *sc_fx, *sc_fy, *sc_dist, *sc_pos;
+  double *sc_t;   // per box cell: foot parameter of the last evaluation
+  int *sc_key;    // per particle: boxes of that evaluation (8 ints)
   int *tile_cnt, *tile_list, *active;
   int ntx, nty;
   int *flags;
@@ -408,6 +437,24 @@ __device__ inline void make_boxes(const IbmArgs &a, double px, double py,
   bs.b1 = make_img(a, px, py + a.ly * (1 - 1), r);
   bs.b2 = make_img(a, px, py + a.ly * (2 - 1), r);
   bs.total = bs.b0.cnt + bs.b1.cnt + bs.b2.cnt;
+}
+
+// boxes of the last distance evaluation (cells map to the same flattened
+// index only when every image's box is unchanged)
+__device__ inline bool same_boxes(const int *key, const BoxSet &bs) {
+  return key[7] != 0 && key[0] == bs.b0.imin && key[1] == bs.b0.jmin &&
+         key[2] == bs.b1.imin && key[3] == bs.b1.jmin && key[4] == bs.b2.imin &&
+         key[5] == bs.b2.jmin && key[6] == bs.total;
+}
+__device__ inline void store_boxes(int *key, const BoxSet &bs) {
+  key[0] = bs.b0.imin;
+  key[1] = bs.b0.jmin;
+  key[2] = bs.b1.imin;
+  key[3] = bs.b1.jmin;
+  key[4] = bs.b2.imin;
+  key[5] = bs.b2.jmin;
+  key[6] = bs.total;
+  key[7] = 1;
 }
 
 __device__ inline Img pick(const BoxSet &bs, int k) {
@@ -468,7 +515,9 @@ __global__ __launch_bounds__(64) void k_inertia(IbmArgs a, int cnstep) {
   }
   const double *cp = a.sc_pos + 4 * (long)n;
   const bool cached = cp[3] != 0. && cp[0] == px && cp[1] == py && cp[2] == paz;
+  const bool warm = same_boxes(a.sc_key + 8 * (long)n, bs);
   double *dist = a.sc_dist + a.box_off[n];
+  double *tcell = a.sc_t + a.box_off[n];
   double cr, sr;
   sincos(-paz, &sr, &cr);
   const double dxdy = a.dx * a.dy;
@@ -485,7 +534,8 @@ __global__ __launch_bounds__(64) void k_inertia(IbmArgs a, int cnstep) {
     if (cached) {
       d = dist[t];
     } else {
-      d = signed_dist_fast(e, f, 1. / a.grid_size, px, py_, cr, sr, x, y, it32, it64);
+      d = signed_dist_warm(e, f, 1. / a.grid_size, px, py_, cr, sr, x, y, warm, tcell[t], it32,
+                           it64);
       dist[t] = d;
     }
     const double w = v_weight(d);
@@ -534,6 +584,11 @@ __global__ void k_ibm_finalize(IbmArgs a, int which) {
   cw[1] = py;
   cw[2] = paz;
   cw[3] = 1.;
+  {
+    BoxSet bs;
+    make_boxes(a, px, py, fmax(q[R_A], q[R_B]), bs);
+    store_boxes(a.sc_key + 8 * (long)n, bs);
+  }
   if (n == 0) a.stats[4 * (long)a.np] += 1ull;  // IBM calls
   unsigned long long *c = a.stats + 4 * (long)n;
   c[0] += (unsigned long long)a.slice_info[2 * (long)n];
@@ -664,7 +719,9 @@ __global__ __launch_bounds__(64) void k_exchange(IbmArgs a) {
   }
   const double *cp = a.sc_pos + 4 * (long)n;
   const bool cached = cp[3] != 0. && cp[0] == px && cp[1] == py && cp[2] == paz;
+  const bool warm = same_boxes(a.sc_key + 8 * (long)n, bs);
   double *dist = a.sc_dist + a.box_off[n];
+  double *tcell = a.sc_t + a.box_off[n];
   double *fxo = a.sc_fx + a.box_off[n];
   double *fyo = a.sc_fy + a.box_off[n];
   double cr, sr;
@@ -683,7 +740,8 @@ __global__ __launch_bounds__(64) void k_exchange(IbmArgs a) {
     if (cached) {
       d = dist[t];
     } else {
-      d = signed_dist_fast(e, f, 1. / a.grid_size, px, py_, cr, sr, x, y, it32, it64);
+      d = signed_dist_warm(e, f, 1. / a.grid_size, px, py_, cr, sr, x, y, warm, tcell[t], it32,
+                           it64);
       dist[t] = d;
     }
     const double w = s_weight(d);
@@ -1226,13 +1284,16 @@ __global__ void k_update_particles(double *prt, int np, double ly) {
 }
 
 // src/suspensions/reset.c:7-18
-__global__ void k_reset(double *prt, int np, int *gate) {
+__global__ void k_reset(double *prt, int np, int *gate, int *sc_key) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n == 0 && gate) {
     gate[0] = 1;
     gate[1] = 0;
   }
   if (n >= np) return;
+  // foot-point warm starts reach back no further than the stage start, so
+  // that a run restarted from a save repeats the same operations
+  if (sc_key) sc_key[8 * (long)n + 7] = 0;
   double *p = prt + (long)n * R_N;
   p[R_DUX] = 0.;
   p[R_DUY] = 0.;
@@ -1266,6 +1327,8 @@ static IbmArgs ibm_args(Ctx *c) {
   a.sc_fy = c->sc_fy;
   a.sc_dist = c->sc_dist;
   a.sc_pos = c->sc_pos;
+  a.sc_t = c->sc_t;
+  a.sc_key = c->sc_key;
   a.tile_cnt = c->tile_cnt;
   a.tile_list = c->tile_list;
   a.active = c->active;
@@ -1330,6 +1393,9 @@ int susp_init(Ctx *c) {
   EIF_HIP(hipMalloc(&c->sc_fx, sc));
   EIF_HIP(hipMalloc(&c->sc_fy, sc));
   EIF_HIP(hipMalloc(&c->sc_dist, sc));
+  EIF_HIP(hipMalloc(&c->sc_t, sc));
+  EIF_HIP(hipMalloc(&c->sc_key, 8 * (size_t)(c->np > 0 ? c->np : 1) * sizeof(int)));
+  EIF_HIP(hipMemset(c->sc_key, 0, 8 * (size_t)(c->np > 0 ? c->np : 1) * sizeof(int)));
   EIF_HIP(hipMalloc(&c->sc_pos, 4 * (size_t)(c->np > 0 ? c->np : 1) * sizeof(double)));
   EIF_HIP(hipMemset(c->sc_pos, 0, 4 * (size_t)(c->np > 0 ? c->np : 1) * sizeof(double)));
   EIF_HIP(hipMalloc(&c->boxes, 3 * (size_t)(c->np > 0 ? c->np : 1) * sizeof(Box)));
@@ -1359,7 +1425,7 @@ int susp_init(Ctx *c) {
 int launch_reset(Ctx *c) {
   // also rewinds the sub-iteration gate, so it runs without particles too
   hipLaunchKernelGGL(k_reset, dim3(c->np > 0 ? (c->np + 255) / 256 : 1), dim3(256), 0,
-                     c->stream, c->prt, c->np, c->gate);
+                     c->stream, c->prt, c->np, c->gate, c->sc_key);
   EIF_HIP(hipGetLastError());
   return EIF_OK;
 }
