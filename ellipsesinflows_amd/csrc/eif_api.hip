@@ -632,7 +632,7 @@ int eif_gpu_integrate(eif_ctx *c, int substeps[3]) {
     TRY(launch_bc_p(c, c->p));
     TRY(launch_inertia(c, 0));
     TRY(launch_predictor(c, rk));
-    TRY(launch_exchange(c));
+    TRY(launch_exchange(c, true));  // same position as inertia(0): dx = 0 after the reset
     TRY(launch_update_momentum(c));
     TRY(launch_poisson(c, rk));
     TRY(launch_correct_pressure(c, rk, true, true));

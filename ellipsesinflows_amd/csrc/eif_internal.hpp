@@ -180,7 +180,9 @@ int poisson_init(Ctx *c);
 int launch_poisson(Ctx *c, int rk);
 int susp_init(Ctx *c);
 int launch_inertia(Ctx *c, int cnstep, bool gated = false);
-int launch_exchange(Ctx *c);
+// cached_known: the distance cache holds the exchange position (it follows
+// inertia(0) of the same stage), so the distance pass is skipped
+int launch_exchange(Ctx *c, bool cached_known = false);
 int launch_update_momentum(Ctx *c);
 int launch_materialise_du(Ctx *c);
 int launch_collision(Ctx *c, int cnstep, bool gated = false);

@@ -664,10 +664,63 @@ __device__ inline void register_tile(const IbmArgs &a, int tile, int box_id) {
     atomicOr(&a.flags[0], 2);
 }
 
-// src/suspensions/exchange.c:13-88, sliced like k_inertia; the first
-// slice of a particle also publishes its boxes and registers them with
-// the tiles they reach.
-__global__ __launch_bounds__(64) void k_exchange(IbmArgs a) {
+// Signed distances of the box cells at the particles' positions (with the
+// increments dx, dy, daz when with_dx) where the per-particle cache does not
+// already hold them; sliced like k_inertia.  Used before k_exchange when
+// the cache may be stale.
+__global__ __launch_bounds__(64) void k_ibm_dist(IbmArgs a, int with_dx) {
+  if (a.gate && !a.gate[0]) return;
+  const int slice = blockIdx.x;
+  const int n = a.slice_owner[slice];
+  const int s0 = a.slice_first[n], nsl = a.slice_first[n + 1] - s0;
+  const double *q = a.prt + (long)n * R_N;
+  const double pa = q[R_A], pb = q[R_B];
+  const double px = q[R_X] + (with_dx ? q[R_DX] : 0.);
+  const double py = q[R_Y] + (with_dx ? q[R_DY] : 0.);
+  const double paz = q[R_AZ] + (with_dx ? q[R_DAZ] : 0.);
+  BoxSet bs;
+  make_boxes(a, px, py, fmax(pa, pb), bs);
+  const int total = bs.total;
+  if (total > a.box_off[n + 1] - a.box_off[n] || total > nsl * IBM_SLICE) {
+    if (threadIdx.x == 0) atomicOr(&a.flags[0], 1);
+    return;
+  }
+  const double *cp = a.sc_pos + 4 * (long)n;
+  const bool cached = cp[3] != 0. && cp[0] == px && cp[1] == py && cp[2] == paz;
+  int it64 = 0, it32 = 0;
+  if (!cached) {
+    const EllipseShape e = make_shape(pa, pb);
+    const EllipseShape32 f = shape32(e);
+    const bool warm = same_boxes(a.sc_key + 8 * (long)n, bs);
+    double *dist = a.sc_dist + a.box_off[n];
+    double *tcell = a.sc_t + a.box_off[n];
+    double cr, sr;
+    sincos(-paz, &sr, &cr);
+    const int t0 = (slice - s0) * IBM_SLICE;
+    const int t1 = t0 + IBM_SLICE < total ? t0 + IBM_SLICE : total;
+    for (int t = t0 + (int)threadIdx.x; t < t1; t += 64) {
+      int k, i, j;
+      cell_of(bs, t, k, i, j);
+      const double py_ = py + a.ly * (k - 1);
+      dist[t] = signed_dist_warm(e, f, 1. / a.grid_size, px, py_, cr, sr, a.xc[i],
+                                 a.yc[j - 1], warm, tcell[t], it32, it64);
+    }
+  }
+  const double c64 = wave_sum((double)it64), c32 = wave_sum((double)it32);
+  if (threadIdx.x == 0) {
+    a.slice_part[SLICE_VALS * (long)slice + 3] = c64;
+    a.slice_part[SLICE_VALS * (long)slice + 4] = c32;
+    if (slice == s0) a.slice_info[2 * (long)n + 1] = cached ? 0 : total;
+  }
+}
+
+// src/suspensions/exchange.c:13-88, sliced like k_inertia, on distances
+// already in the per-particle cache (the position is the one of the
+// preceding inertia(0) or k_ibm_dist).  The first slice of a particle also
+// publishes its boxes and registers them with the tiles they reach.
+// own_counts: no k_ibm_dist ran before, so this kernel reports no
+// distance work.
+__global__ __launch_bounds__(64) void k_exchange(IbmArgs a, int own_counts) {
   const int slice = blockIdx.x;
   const int n = a.slice_owner[slice];
   const int s0 = a.slice_first[n], nsl = a.slice_first[n + 1] - s0;
@@ -676,10 +729,8 @@ __global__ __launch_bounds__(64) void k_exchange(IbmArgs a) {
   const double inv_dt = 1. / dt;
   const double pa = q[R_A], pb = q[R_B];
   const double inv_pm = 1. / p_mass(q), inv_pim = 1. / p_moi(q);
-  const double px = q[R_X], py = q[R_Y], paz = q[R_AZ];
+  const double px = q[R_X], py = q[R_Y];
   const double pux = q[R_UX], puy = q[R_UY], pvz = q[R_VZ];
-  const EllipseShape e = make_shape(pa, pb);
-  const EllipseShape32 f = shape32(e);
   BoxSet bs;
   make_boxes(a, px, py, fmax(pa, pb), bs);
   const int total = bs.total;
@@ -717,18 +768,11 @@ __global__ __launch_bounds__(64) void k_exchange(IbmArgs a) {
         register_tile(a, (rg.ty0 + e_ / w) * a.ntx + rg.tx0 + e_ % w, n * 3 + r / 2);
     }
   }
-  const double *cp = a.sc_pos + 4 * (long)n;
-  const bool cached = cp[3] != 0. && cp[0] == px && cp[1] == py && cp[2] == paz;
-  const bool warm = same_boxes(a.sc_key + 8 * (long)n, bs);
-  double *dist = a.sc_dist + a.box_off[n];
-  double *tcell = a.sc_t + a.box_off[n];
+  const double *dist = a.sc_dist + a.box_off[n];
   double *fxo = a.sc_fx + a.box_off[n];
   double *fyo = a.sc_fy + a.box_off[n];
-  double cr, sr;
-  sincos(-paz, &sr, &cr);
   const double dx = a.dx, dy = a.dy;
   double acc[3] = {0., 0., 0.};  // fux, fuy, tvz
-  int it64 = 0, it32 = 0;
   const int t0 = (slice - s0) * IBM_SLICE;
   const int t1 = t0 + IBM_SLICE < total ? t0 + IBM_SLICE : total;
   for (int t = t0 + (int)threadIdx.x; t < t1; t += 64) {
@@ -736,15 +780,7 @@ __global__ __launch_bounds__(64) void k_exchange(IbmArgs a) {
     cell_of(bs, t, k, i, j);
     const double py_ = py + a.ly * (k - 1);
     const double x = a.xc[i], y = a.yc[j - 1];
-    double d;
-    if (cached) {
-      d = dist[t];
-    } else {
-      d = signed_dist_warm(e, f, 1. / a.grid_size, px, py_, cr, sr, x, y, warm, tcell[t], it32,
-                           it64);
-      dist[t] = d;
-    }
-    const double w = s_weight(d);
+    const double w = s_weight(dist[t]);
     const double ux_p = pux - pvz * (y - py_);
     const double uy_p = puy + pvz * (x - px);
     const double ux_f = 0.5 * (a.ux[a.L.ux(i, j)] + a.ux[a.L.ux(i + 1, j)]);
@@ -758,12 +794,20 @@ __global__ __launch_bounds__(64) void k_exchange(IbmArgs a) {
     acc[1] -= +(fy * dx * dy) * inv_pm;
     acc[2] -= +(x - px) * (fy * dx * dy) * inv_pim;
   }
-  double part[SLICE_VALS] = {wave_sum(acc[0]), wave_sum(acc[1]), wave_sum(acc[2]),
-                             wave_sum((double)it64), wave_sum((double)it32)};
-  slice_store(a, slice, part);
-  if (slice == s0 && threadIdx.x == 0) {
-    a.slice_info[2 * (long)n] = total;
-    a.slice_info[2 * (long)n + 1] = cached ? 0 : total;
+  const double s0v = wave_sum(acc[0]), s1v = wave_sum(acc[1]), s2v = wave_sum(acc[2]);
+  if (threadIdx.x == 0) {
+    double *pp = a.slice_part + SLICE_VALS * (long)slice;
+    pp[0] = s0v;
+    pp[1] = s1v;
+    pp[2] = s2v;
+    if (own_counts) {
+      pp[3] = 0.;
+      pp[4] = 0.;
+    }
+    if (slice == s0) {
+      a.slice_info[2 * (long)n] = total;
+      if (own_counts) a.slice_info[2 * (long)n + 1] = 0;
+    }
   }
 }
 
@@ -1441,7 +1485,7 @@ int launch_inertia(Ctx *c, int cnstep, bool gated) {
   return EIF_OK;
 }
 
-int launch_exchange(Ctx *c) {
+int launch_exchange(Ctx *c, bool cached_known) {
   if (c->np == 0) return EIF_OK;
   ProfScope ps(c, "ibm_exchange");
   const size_t ntiles = (size_t)c->ntx * c->nty;
@@ -1450,7 +1494,10 @@ int launch_exchange(Ctx *c) {
                      c->stream, c->tile_cnt, (int)ntiles);
   hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, c->active, 1);
   IbmArgs a = ibm_args(c);
-  hipLaunchKernelGGL(k_exchange, dim3(c->nslices), dim3(64), 0, c->stream, a);
+  if (!cached_known)
+    hipLaunchKernelGGL(k_ibm_dist, dim3(c->nslices), dim3(64), 0, c->stream, a, 0);
+  hipLaunchKernelGGL(k_exchange, dim3(c->nslices), dim3(64), 0, c->stream, a,
+                     cached_known ? 1 : 0);
   hipLaunchKernelGGL(k_ibm_finalize, dim3((c->np + 63) / 64), dim3(64), 0, c->stream, a, 2);
   EIF_HIP(hipGetLastError());
   return EIF_OK;
