@@ -57,7 +57,7 @@ static int prof_flush(Ctx *c) {
   return EIF_OK;
 }
 
-void prof_begin(Ctx *c, const char *name) {
+void prof_begin(Ctx *c, const char *name, hipStream_t st) {
   if (!c->prof || !c->prof->on) return;
   if (!c->prof->only.empty() && c->prof->only != name) return;
   auto &r = c->prof->recs[name];
@@ -66,15 +66,15 @@ void prof_begin(Ctx *c, const char *name) {
     if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return;
     r.ev.push_back({a, b});
   }
-  (void)hipEventRecord(r.ev[r.used].first, c->stream);
+  (void)hipEventRecord(r.ev[r.used].first, st);
 }
 
-void prof_end(Ctx *c, const char *name) {
+void prof_end(Ctx *c, const char *name, hipStream_t st) {
   if (!c->prof || !c->prof->on) return;
   if (!c->prof->only.empty() && c->prof->only != name) return;
   auto &r = c->prof->recs[name];
   if (r.used >= r.ev.size()) return;
-  (void)hipEventRecord(r.ev[r.used].second, c->stream);
+  (void)hipEventRecord(r.ev[r.used].second, st);
   r.used++;
   if (r.used >= 4096) prof_flush(c);
 }
@@ -149,6 +149,11 @@ static void free_ctx(Ctx *c) {
       }
     delete c->prof;
   }
+  if (c->side) (void)hipStreamSynchronize(c->side);
+  hipEvent_t evs[] = {c->ev_fork, c->ev_i0, c->ev_c0, c->ev_gate, c->ev_coll};
+  for (auto ev : evs)
+    if (ev) (void)hipEventDestroy(ev);
+  if (c->side) (void)hipStreamDestroy(c->side);
   if (c->stream) (void)hipStreamDestroy(c->stream);
 }
 
@@ -188,6 +193,10 @@ int fluid_gpu_create(const eif_config *cfg, eif_ctx **out) {
   if (hipGetDevice(&c->device) != hipSuccess) return fail(hip_fail(hipErrorUnknown, "hipGetDevice", __FILE__, __LINE__));
   {
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
+    hipEvent_t *evs[] = {&c->ev_fork, &c->ev_i0, &c->ev_c0, &c->ev_gate, &c->ev_coll};
+    for (auto ev : evs)
+      if (e == hipSuccess) e = hipEventCreateWithFlags(ev, hipEventDisableTiming);
     if (e != hipSuccess) return fail(hip_fail(e, "hipStreamCreate", __FILE__, __LINE__));
   }
   // grid, src/param/init.c:105-167 (one process)
@@ -625,18 +634,30 @@ int suspensions_gpu_colliding_pairs(eif_ctx *c, int capacity, int *pairs, int *n
 // integrate(), src/main.c:13-55
 int eif_gpu_integrate(eif_ctx *c, int substeps[3]) {
   CHECK_CTX(c);
+  // Two streams.  Main: fluid kernels, exchange, deposit, inertia(1),
+  // increments.  Side: inertia(0) and the collisions, which read only the
+  // particles and the stage-start velocity, so they overlap the predictor,
+  // the pressure solve and inertia(1).  Events join the side work back
+  // before anything that reads its results or reuses its buffers.
   for (int rk = 0; rk < 3; rk++) {
     TRY(launch_reset(c));
     TRY(launch_bc_ux(c, c->ux));
     TRY(launch_bc_uy(c, c->uy));
     TRY(launch_bc_p(c, c->p));
-    TRY(launch_inertia(c, 0));
-    TRY(launch_predictor(c, rk));
+    EIF_HIP(hipEventRecord(c->ev_fork, c->stream));
+    EIF_HIP(hipStreamWaitEvent(c->side, c->ev_fork, 0));
+    TRY(launch_inertia(c, 0, false, c->side));  // reads ux, uy before the predictor
+    EIF_HIP(hipEventRecord(c->ev_i0, c->side));
+    TRY(launch_collision(c, 0, false, c->side));
+    EIF_HIP(hipEventRecord(c->ev_c0, c->side));
+    TRY(launch_predictor(c, rk));  // writes the other ux, uy buffers
+    EIF_HIP(hipStreamWaitEvent(c->stream, c->ev_i0, 0));
     TRY(launch_exchange(c, true));  // same position as inertia(0): dx = 0 after the reset
     TRY(launch_update_momentum(c));
     TRY(launch_poisson(c, rk));
     TRY(launch_correct_pressure(c, rk, true, true));
-    TRY(launch_collision(c, 0));
+    // the sub-iteration gate clears the pair counters collision(0) used
+    EIF_HIP(hipStreamWaitEvent(c->stream, c->ev_c0, 0));
     // Crank-Nicolson sub-iterations (src/main.c:37-50).  They are queued in
     // batches; a device-side gate before each one applies the loop's exit
     // rule, so sub-iterations queued past convergence do nothing, and the
@@ -646,8 +667,12 @@ int eif_gpu_integrate(eif_ctx *c, int substeps[3]) {
     for (;;) {
       for (int b = 0; b < batch; b++) {
         TRY(launch_substep_gate(c));
+        EIF_HIP(hipEventRecord(c->ev_gate, c->stream));
+        EIF_HIP(hipStreamWaitEvent(c->side, c->ev_gate, 0));
+        TRY(launch_collision(c, 1, true, c->side));
+        EIF_HIP(hipEventRecord(c->ev_coll, c->side));
         TRY(launch_inertia(c, 1, true));
-        TRY(launch_collision(c, 1, true));
+        EIF_HIP(hipStreamWaitEvent(c->stream, c->ev_coll, 0));
         TRY(launch_increment(c, rk, true));
       }
       queued += batch;
@@ -668,7 +693,6 @@ int eif_gpu_integrate(eif_ctx *c, int substeps[3]) {
   return EIF_OK;
 }
 
-// src/main.c:71-76 (decide_dt + integrate)
 int eif_gpu_step(eif_ctx *c, double *dt_out) {
   CHECK_CTX(c);
   TRY(launch_decide_dt(c));

@@ -71,6 +71,11 @@ struct Ctx {
   double lx, ly, dx, dy;
   Layout L;
   hipStream_t stream = nullptr;
+  // second stream (default priority) for particle work that can overlap the
+  // fluid kernels, and the events that join it back
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_i0 = nullptr, ev_c0 = nullptr;
+  hipEvent_t ev_gate = nullptr, ev_coll = nullptr;
   int device = 0;
   // host copies of the grid (exact reference arithmetic)
   std::vector<double> h_xf, h_xc, h_dxf, h_dxc, h_yf, h_yc;
@@ -159,13 +164,17 @@ int hip_fail(hipError_t e, const char *what, const char *file, int line);
   } while (0)
 
 // profiling helpers (host)
-void prof_begin(Ctx *c, const char *name);
-void prof_end(Ctx *c, const char *name);
-struct ProfScope {
+void prof_begin(Ctx *c, const char *name, hipStream_t st);
+void prof_end(Ctx *c, const char *name, hipStream_t st);
+struct ProfScope {  // events on the stream the scoped kernels run on
   Ctx *c;
   const char *n;
-  ProfScope(Ctx *c_, const char *n_) : c(c_), n(n_) { prof_begin(c, n); }
-  ~ProfScope() { prof_end(c, n); }
+  hipStream_t st;
+  ProfScope(Ctx *c_, const char *n_) : c(c_), n(n_), st(c_->stream) { prof_begin(c, n, st); }
+  ProfScope(Ctx *c_, const char *n_, hipStream_t s_) : c(c_), n(n_), st(s_) {
+    prof_begin(c, n, st);
+  }
+  ~ProfScope() { prof_end(c, n, st); }
 };
 
 // launchers implemented in the .hip files
@@ -179,13 +188,13 @@ int launch_diagnostics(Ctx *c, int which, double *out);
 int poisson_init(Ctx *c);
 int launch_poisson(Ctx *c, int rk);
 int susp_init(Ctx *c);
-int launch_inertia(Ctx *c, int cnstep, bool gated = false);
+int launch_inertia(Ctx *c, int cnstep, bool gated = false, hipStream_t st = nullptr);
 // cached_known: the distance cache holds the exchange position (it follows
 // inertia(0) of the same stage), so the distance pass is skipped
 int launch_exchange(Ctx *c, bool cached_known = false);
 int launch_update_momentum(Ctx *c);
 int launch_materialise_du(Ctx *c);
-int launch_collision(Ctx *c, int cnstep, bool gated = false);
+int launch_collision(Ctx *c, int cnstep, bool gated = false, hipStream_t st = nullptr);
 int launch_increment(Ctx *c, int rk, bool gated = false);
 int launch_substep_gate(Ctx *c);
 int launch_update_particles(Ctx *c);

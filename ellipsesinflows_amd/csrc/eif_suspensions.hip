@@ -1474,13 +1474,14 @@ int launch_reset(Ctx *c) {
   return EIF_OK;
 }
 
-int launch_inertia(Ctx *c, int cnstep, bool gated) {
+int launch_inertia(Ctx *c, int cnstep, bool gated, hipStream_t st) {
   if (c->np == 0) return EIF_OK;
-  ProfScope ps(c, "ibm_inertia");
+  if (!st) st = c->stream;
+  ProfScope ps(c, "ibm_inertia", st);
   IbmArgs a = ibm_args(c);
   if (gated) a.gate = c->gate;
-  hipLaunchKernelGGL(k_inertia, dim3(c->nslices), dim3(64), 0, c->stream, a, cnstep);
-  hipLaunchKernelGGL(k_ibm_finalize, dim3((c->np + 63) / 64), dim3(64), 0, c->stream, a, cnstep);
+  hipLaunchKernelGGL(k_inertia, dim3(c->nslices), dim3(64), 0, st, a, cnstep);
+  hipLaunchKernelGGL(k_ibm_finalize, dim3((c->np + 63) / 64), dim3(64), 0, st, a, cnstep);
   EIF_HIP(hipGetLastError());
   return EIF_OK;
 }
@@ -1552,9 +1553,10 @@ int launch_materialise_du(Ctx *c) {
   return launch_bc_p(c, c->duy);
 }
 
-int launch_collision(Ctx *c, int cnstep, bool gated) {
+int launch_collision(Ctx *c, int cnstep, bool gated, hipStream_t st) {
   if (c->np == 0) return EIF_OK;
-  ProfScope ps(c, "collision");
+  if (!st) st = c->stream;
+  ProfScope ps(c, "collision", st);
   CollArgs a;
   a.np = c->np;
   a.lx = c->lx;
@@ -1572,13 +1574,13 @@ int launch_collision(Ctx *c, int cnstep, bool gated) {
   a.cand_cap = c->cand_cap;
   a.gate = gated ? c->gate : nullptr;
   if (!gated)  // an open gate has cleared the counts
-    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, c->pair_cnt, 2);
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, c->pair_cnt, 2);
   const int nt = (c->np + COLL_TILE - 1) / COLL_TILE;
-  hipLaunchKernelGGL(k_collide_search, dim3(nt * (nt + 1) / 2), dim3(256), 0, c->stream, a);
+  hipLaunchKernelGGL(k_collide_search, dim3(nt * (nt + 1) / 2), dim3(256), 0, st, a);
   const int pair_blocks = (c->cand_cap + 63) / 64, wall_blocks = (2 * c->np + 63) / 64;
   hipLaunchKernelGGL(k_collide_pairs_walls, dim3(pair_blocks + wall_blocks), dim3(64), 0,
-                     c->stream, a, pair_blocks);
-  hipLaunchKernelGGL(k_collide_gather, dim3((c->np + 127) / 128), dim3(128), 0, c->stream,
+                     st, a, pair_blocks);
+  hipLaunchKernelGGL(k_collide_gather, dim3((c->np + 127) / 128), dim3(128), 0, st,
                      a);
   EIF_HIP(hipGetLastError());
   return EIF_OK;
