@@ -671,9 +671,9 @@ int eif_gpu_integrate(eif_ctx *c, int substeps[3]) {
         EIF_HIP(hipStreamWaitEvent(c->side, c->ev_gate, 0));
         TRY(launch_collision(c, 1, true, c->side));
         EIF_HIP(hipEventRecord(c->ev_coll, c->side));
-        TRY(launch_inertia(c, 1, true));
+        TRY(launch_inertia(c, 1, true, c->stream, true));
         EIF_HIP(hipStreamWaitEvent(c->stream, c->ev_coll, 0));
-        TRY(launch_increment(c, rk, true));
+        TRY(launch_increment(c, rk, true, true));
       }
       queued += batch;
       EIF_HIP(hipMemcpyAsync(c->hgate, c->gate, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));

@@ -188,14 +188,17 @@ int launch_diagnostics(Ctx *c, int which, double *out);
 int poisson_init(Ctx *c);
 int launch_poisson(Ctx *c, int rk);
 int susp_init(Ctx *c);
-int launch_inertia(Ctx *c, int cnstep, bool gated = false, hipStream_t st = nullptr);
+// defer_finalize: the slice sums are added by the next launch_increment
+// with fin_inertia (one launch fewer per sub-iteration)
+int launch_inertia(Ctx *c, int cnstep, bool gated = false, hipStream_t st = nullptr,
+                   bool defer_finalize = false);
 // cached_known: the distance cache holds the exchange position (it follows
 // inertia(0) of the same stage), so the distance pass is skipped
 int launch_exchange(Ctx *c, bool cached_known = false);
 int launch_update_momentum(Ctx *c);
 int launch_materialise_du(Ctx *c);
 int launch_collision(Ctx *c, int cnstep, bool gated = false, hipStream_t st = nullptr);
-int launch_increment(Ctx *c, int rk, bool gated = false);
+int launch_increment(Ctx *c, int rk, bool gated = false, bool fin_inertia = false);
 int launch_substep_gate(Ctx *c);
 int launch_update_particles(Ctx *c);
 int launch_reset(Ctx *c);

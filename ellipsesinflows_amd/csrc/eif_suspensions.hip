@@ -558,10 +558,7 @@ __global__ __launch_bounds__(64) void k_inertia(IbmArgs a, int cnstep) {
 // Per-particle totals of the sliced inertia / exchange kernels, added in
 // slice order; also records the position whose distances are now cached
 // and the work counters.  which: 0..1 inertia (cnstep), 2 exchange.
-__global__ void k_ibm_finalize(IbmArgs a, int which) {
-  if (a.gate && !a.gate[0]) return;
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= a.np) return;
+__device__ void finalize_particle(const IbmArgs &a, int n, int which) {
   double tot[SLICE_VALS] = {0., 0., 0., 0., 0.};
   for (int sl = a.slice_first[n]; sl < a.slice_first[n + 1]; sl++)
     for (int k = 0; k < SLICE_VALS; k++) tot[k] += a.slice_part[SLICE_VALS * (long)sl + k];
@@ -595,6 +592,12 @@ __global__ void k_ibm_finalize(IbmArgs a, int which) {
   c[1] += (unsigned long long)a.slice_info[2 * (long)n + 1];
   c[2] += (unsigned long long)tot[3];
   c[3] += (unsigned long long)tot[4];
+}
+
+__global__ void k_ibm_finalize(IbmArgs a, int which) {
+  if (a.gate && !a.gate[0]) return;
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n < a.np) finalize_particle(a, n, which);
 }
 
 // Measurement only: the reference's FP64 iteration (find_normal_t,
@@ -1272,13 +1275,17 @@ __device__ inline void atomic_max_nonneg(double *addr, double v) {
 __global__ void k_zero_residual(double *dscal) { dscal[S_RESIDUAL] = 0.; }
 
 // src/suspensions/increment.c:10-64
+// fin: also finalises the preceding inertia(1) call (its slice sums and
+// cache records) for this particle first, saving a launch per sub-iteration
 __global__ void k_increment(double *prt, int np, const double *dscal, double ga,
-                            double Fr, double *res_out, const int *gate) {
+                            double Fr, double *res_out, const int *gate, IbmArgs ia,
+                            int fin) {
   if (gate && !gate[0]) return;
   __shared__ double red[256];
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   double res = 0.;
   if (n < np) {
+    if (fin) finalize_particle(ia, n, 1);
     double *p = prt + (long)n * R_N;
     const double dt = dscal[S_DT];
     const double den = p[R_DEN];
@@ -1474,14 +1481,15 @@ int launch_reset(Ctx *c) {
   return EIF_OK;
 }
 
-int launch_inertia(Ctx *c, int cnstep, bool gated, hipStream_t st) {
+int launch_inertia(Ctx *c, int cnstep, bool gated, hipStream_t st, bool defer_finalize) {
   if (c->np == 0) return EIF_OK;
   if (!st) st = c->stream;
   ProfScope ps(c, "ibm_inertia", st);
   IbmArgs a = ibm_args(c);
   if (gated) a.gate = c->gate;
   hipLaunchKernelGGL(k_inertia, dim3(c->nslices), dim3(64), 0, st, a, cnstep);
-  hipLaunchKernelGGL(k_ibm_finalize, dim3((c->np + 63) / 64), dim3(64), 0, st, a, cnstep);
+  if (!defer_finalize)
+    hipLaunchKernelGGL(k_ibm_finalize, dim3((c->np + 63) / 64), dim3(64), 0, st, a, cnstep);
   EIF_HIP(hipGetLastError());
   return EIF_OK;
 }
@@ -1586,13 +1594,14 @@ int launch_collision(Ctx *c, int cnstep, bool gated, hipStream_t st) {
   return EIF_OK;
 }
 
-int launch_increment(Ctx *c, int rk, bool gated) {
+int launch_increment(Ctx *c, int rk, bool gated, bool fin_inertia) {
   if (!gated)  // an open gate has cleared the residual
     hipLaunchKernelGGL(k_zero_residual, dim3(1), dim3(1), 0, c->stream, c->dscal);
   if (c->np > 0)
     hipLaunchKernelGGL(k_increment, dim3((c->np + 255) / 256), dim3(256), 0, c->stream,
                        c->prt, c->np, c->dscal, c->gamma[rk], c->cfg.Fr,
-                       c->dscal + S_RESIDUAL, gated ? c->gate : nullptr);
+                       c->dscal + S_RESIDUAL, gated ? c->gate : nullptr, ibm_args(c),
+                       fin_inertia ? 1 : 0);
   EIF_HIP(hipGetLastError());
   return EIF_OK;
 }
