@@ -522,24 +522,24 @@ __global__ __launch_bounds__(PART_LANES) void k_part_local(
   if (k >= nx) return;
   const long j0 = (long)c * L;
   const double l = coef;
-  double y[L];
+  // forward sweep stores y in place; the backward sweep reads it back
   double prev = 0.;
-#pragma unroll
+#pragma unroll 8
   for (int r = 0; r < L; r++) {
     const double d = q[(j0 + r) * nx + k];
     const double v = fwd[(long)r * nx + k] * (d - l * prev);
-    y[r] = v;
+    q[(j0 + r) * nx + k] = v;
     prev = v;
   }
-  double next = y[L - 1];
-  q[(j0 + L - 1) * nx + k] = next;
-#pragma unroll
+  const double ylast = prev;
+  double next = prev;
+#pragma unroll 8
   for (int r = L - 2; r >= 0; r--) {
-    next = y[r] - um[(long)r * nx + k] * next;
+    next = q[(j0 + r) * nx + k] - um[(long)r * nx + k] * next;
     q[(j0 + r) * nx + k] = next;
   }
   ends[((long)c * 2 + 0) * nx + k] = next;   // y_c[0]
-  ends[((long)c * 2 + 1) * nx + k] = y[L - 1];  // y_c[L-1]
+  ends[((long)c * 2 + 1) * nx + k] = ylast;  // y_c[L-1]
 }
 
 // Reduced system: the blocks are the same for every chunk, so the periodic
@@ -559,6 +559,13 @@ __global__ __launch_bounds__(64 * PART_KG) void k_part_reduced(
   extern __shared__ double sh[];
   const int kk = threadIdx.x / 64, lane = threadIdx.x % 64;
   const int k = blockIdx.x * PART_KG + kk;
+  // chunk-DFT twiddles staged in LDS: the sums below read them P times per
+  // lane
+  double *tc = sh + (long)PART_KG * 6 * P, *ts = tc + P;
+  for (int e = threadIdx.x; e < P; e += blockDim.x) {
+    tc[e] = twc[e];
+    ts[e] = tws[e];
+  }
   double *r0 = sh + (long)kk * 6 * P, *r1 = r0 + P;
   double *u0r = r1 + P, *u0i = u0r + P, *u1r = u0i + P, *u1i = u1r + P;
   const bool on = k < nx;
@@ -574,7 +581,7 @@ __global__ __launch_bounds__(64 * PART_KG) void k_part_reduced(
     for (int m = lane; m < P; m += 64) {
       double a0r = 0., a0i = 0., a1r = 0., a1i = 0.;
       for (int c = 0, t = 0; c < P; c++) {  // t = m c mod P
-        const double cs = twc[t], sn = tws[t];
+        const double cs = tc[t], sn = ts[t];
         a0r += r0[c] * cs;
         a0i -= r0[c] * sn;
         a1r += r1[c] * cs;
@@ -582,7 +589,7 @@ __global__ __launch_bounds__(64 * PART_KG) void k_part_reduced(
         t += m;
         if (t >= P) t -= P;
       }
-      const double ct = twc[m], st = tws[m];
+      const double ct = tc[m], st = ts[m];
       double x0r, x0i, x1r, x1i;
       if (gauge && m == 0) {
         const double f = 1. / (2. * ((1. + lvL) - lv0));
@@ -609,7 +616,7 @@ __global__ __launch_bounds__(64 * PART_KG) void k_part_reduced(
     for (int c = lane; c < P; c += 64) {
       double b0 = 0., b1 = 0.;
       for (int m = 0, t = 0; m < P; m++) {  // t = m c mod P
-        const double cs = twc[t], sn = tws[t];
+        const double cs = tc[t], sn = ts[t];
         b0 += u0r[m] * cs - u0i[m] * sn;
         b1 += u1r[m] * cs - u1i[m] * sn;
         t += c;
@@ -819,7 +826,7 @@ int launch_poisson(Ctx *c, int rk) {
       default: hipLaunchKernelGGL(k_part_local<8>, g1, dim3(PART_LANES), 0, c->stream, nx, c->qhat, c->pt_fwd, c->pt_u, l, c->pt_ends); break;
     }
     hipLaunchKernelGGL(k_part_reduced, dim3((nx + PART_KG - 1) / PART_KG), dim3(64 * PART_KG),
-                       (size_t)PART_KG * 6 * P * sizeof(double), c->stream, nx, P, c->pt_ends,
+                       (size_t)(PART_KG * 6 + 2) * P * sizeof(double), c->stream, nx, P, c->pt_ends,
                        c->pt_lvl, c->pt_red, c->pt_w, c->pt_bnd);
     EIF_HIP(hipGetLastError());
     ca.mode = 1;
