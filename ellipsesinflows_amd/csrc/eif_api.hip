@@ -117,6 +117,13 @@ using namespace eif;
 
 struct eif_ctx : public Ctx {};
 
+// ux, uy or p changed outside eif_gpu_integrate: boundary values and the
+// fused advective dt limit are no longer known to be current
+static void fields_changed(Ctx *c) {
+  c->bc_valid = false;
+  c->dt_adv_valid = false;
+}
+
 extern "C" {
 
 const char *eif_gpu_last_error(void) { return g_err.c_str(); }
@@ -128,7 +135,7 @@ static void free_ctx(Ctx *c) {
                      &c->dux, &c->duy, &c->qhat, &c->ctab, &c->twid, &c->th_val, &c->th_u,
                      &c->th_q1, &c->th_den, &c->th_last, &c->pt_fwd, &c->pt_u,
                      &c->pt_v, &c->pt_w, &c->pt_bnd, &c->pt_red, &c->pt_lvl, &c->pt_ends, &c->pt_g, &c->dscal, &c->dred,
-                     &c->prt, &c->pconst, &c->sc_fx, &c->sc_fy, &c->sc_dist, &c->sc_pos,
+                     &c->prt, &c->pconst, &c->dt_part, &c->sc_fx, &c->sc_fy, &c->sc_dist, &c->sc_pos,
                      &c->pair_val, &c->wall_val, &c->sc_t, &c->st_ux1, &c->st_ux2, &c->st_uy1,
                      &c->st_uy2, &c->st_phi, &c->slice_part};
   for (auto b : bufs)
@@ -275,6 +282,7 @@ int fluid_gpu_create(const eif_config *cfg, eif_ctx **out) {
     return fail(rc);
   c->nred = 1024;
   if ((rc = dalloc(&c->dred, 16 + 2 * (size_t)c->nred, nullptr))) return fail(rc);
+  if ((rc = dalloc(&c->dt_part, DT_PARTS_CAP, nullptr))) return fail(rc);
   {
     hipError_t e = hipHostMalloc((void **)&c->hscal, S_NSCAL * sizeof(double), 0);
     if (e != hipSuccess) return fail(hip_fail(e, "hipHostMalloc", __FILE__, __LINE__));
@@ -325,6 +333,7 @@ int fluid_gpu_get_grid(const eif_ctx *c, double *xf, double *xc, double *dxf,
 
 int fluid_gpu_upload(eif_ctx *c, const double *ux, const double *uy, const double *p) {
   CHECK_CTX(c);
+  fields_changed(c);
   const Layout &L = c->L;
   if (ux) EIF_HIP(hipMemcpyAsync(c->ux, ux, L.ux_len() * sizeof(double), hipMemcpyHostToDevice, c->stream));
   if (uy) EIF_HIP(hipMemcpyAsync(c->uy, uy, L.p_len() * sizeof(double), hipMemcpyHostToDevice, c->stream));
@@ -373,6 +382,7 @@ int fluid_gpu_download_field(eif_ctx *c, int which, double *host) {
 
 int fluid_gpu_upload_field(eif_ctx *c, int which, const double *host) {
   CHECK_CTX(c);
+  fields_changed(c);
   if (!host || which == EIF_FIELD_DUX || which == EIF_FIELD_DUY) {
     set_error("field cannot be uploaded");
     return EIF_ERR_ARG;
@@ -385,9 +395,21 @@ int fluid_gpu_upload_field(eif_ctx *c, int which, const double *host) {
   return EIF_OK;
 }
 
-int fluid_gpu_update_boundaries_ux(eif_ctx *c) { CHECK_CTX(c); return launch_bc_ux(c, c->ux); }
-int fluid_gpu_update_boundaries_uy(eif_ctx *c) { CHECK_CTX(c); return launch_bc_uy(c, c->uy); }
-int fluid_gpu_update_boundaries_p(eif_ctx *c) { CHECK_CTX(c); return launch_bc_p(c, c->p); }
+int fluid_gpu_update_boundaries_ux(eif_ctx *c) {
+  CHECK_CTX(c);
+  fields_changed(c);
+  return launch_bc_ux(c, c->ux);
+}
+int fluid_gpu_update_boundaries_uy(eif_ctx *c) {
+  CHECK_CTX(c);
+  fields_changed(c);
+  return launch_bc_uy(c, c->uy);
+}
+int fluid_gpu_update_boundaries_p(eif_ctx *c) {
+  CHECK_CTX(c);
+  fields_changed(c);
+  return launch_bc_p(c, c->p);
+}
 
 int fluid_gpu_get_dt(eif_ctx *c, double *dt_out) {
   CHECK_CTX(c);
@@ -427,6 +449,7 @@ static int check_rk(int rk) {
 
 int fluid_gpu_update_velocity(eif_ctx *c, int rk) {
   CHECK_CTX(c);
+  fields_changed(c);
   TRY(check_rk(rk));
   return launch_predictor(c, rk);
 }
@@ -439,12 +462,14 @@ int poisson_gpu_compute_potential(eif_ctx *c, int rk) {
 
 int fluid_gpu_correct_velocity(eif_ctx *c, int rk) {
   CHECK_CTX(c);
+  fields_changed(c);
   TRY(check_rk(rk));
   return launch_correct_pressure(c, rk, true, false);
 }
 
 int fluid_gpu_update_pressure(eif_ctx *c) {
   CHECK_CTX(c);
+  fields_changed(c);
   return launch_correct_pressure(c, 0, false, true);
 }
 
@@ -566,6 +591,7 @@ int suspensions_gpu_exchange_momentum(eif_ctx *c) {
 
 int suspensions_gpu_update_momentum_fleid(eif_ctx *c) {
   CHECK_CTX(c);
+  fields_changed(c);
   return launch_update_momentum(c);
 }
 
@@ -641,9 +667,15 @@ int eif_gpu_integrate(eif_ctx *c, int substeps[3]) {
   // before anything that reads its results or reuses its buffers.
   for (int rk = 0; rk < 3; rk++) {
     TRY(launch_reset(c));
-    TRY(launch_bc_ux(c, c->ux));
-    TRY(launch_bc_uy(c, c->uy));
-    TRY(launch_bc_p(c, c->p));
+    // src/main.c:16-19: the kernels of every stage keep the boundary and
+    // halo values they change up to date, so this only runs after a
+    // host-side change of the fields
+    if (!c->bc_valid) {
+      TRY(launch_bc_ux(c, c->ux));
+      TRY(launch_bc_uy(c, c->uy));
+      TRY(launch_bc_p(c, c->p));
+      c->bc_valid = true;
+    }
     EIF_HIP(hipEventRecord(c->ev_fork, c->stream));
     EIF_HIP(hipStreamWaitEvent(c->side, c->ev_fork, 0));
     TRY(launch_inertia(c, 0, false, c->side));  // reads ux, uy before the predictor
@@ -655,7 +687,9 @@ int eif_gpu_integrate(eif_ctx *c, int substeps[3]) {
     TRY(launch_exchange(c, true));  // same position as inertia(0): dx = 0 after the reset
     TRY(launch_update_momentum(c));
     TRY(launch_poisson(c, rk));
-    TRY(launch_correct_pressure(c, rk, true, true));
+    // the last stage also reduces the advective dt limit of the final
+    // velocities for the next param_decide_dt
+    TRY(launch_correct_pressure(c, rk, true, true, rk == 2));
     // the sub-iteration gate clears the pair counters collision(0) used
     EIF_HIP(hipStreamWaitEvent(c->stream, c->ev_c0, 0));
     // Crank-Nicolson sub-iterations (src/main.c:37-50).  They are queued in
@@ -690,6 +724,7 @@ int eif_gpu_integrate(eif_ctx *c, int substeps[3]) {
     if (substeps) substeps[rk] = done;
     TRY(launch_update_particles(c));
   }
+  c->dt_adv_valid = c->dt_adv_parts > 0;
   return EIF_OK;
 }
 

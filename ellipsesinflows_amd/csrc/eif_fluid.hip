@@ -251,65 +251,82 @@ int launch_predictor(Ctx *c, int rk) {
 // values of ux, uy and p refreshed by the threads that own the source
 // interior values.
 // ---------------------------------------------------------------------
-template <bool P2>
-__global__ void k_correct_pressure(Layout L, double *ux, double *uy, double *p,
-                                   const double *psi, const double *dxc,
-                                   const double *dscal, double ga, double dy,
-                                   int do_correct, int do_pressure) {
+constexpr int CP_ROWS = 16;  // rows per workgroup of k_correct_pressure
+
+template <bool P2, bool DT>
+__global__ __launch_bounds__(256) void k_correct_pressure(
+    Layout L, double *ux, double *uy, double *p, const double *psi, const double *dxc,
+    const double *dscal, double ga, double dy, int do_correct, int do_pressure,
+    double *dt_part) {
   const int nx = L.nx, ny = L.ny;
   const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-  const int j = blockIdx.y + 1;
-  if (i > nx || j > ny) return;
   const double dt = dscal[S_DT];
-  const double ps_c = psi[L.p(i, j)];
-  if (do_correct) {
-    if (i >= 2) {
-      const double dxi = 1. / dxc[i - 1];
-      const long k = L.ux(i, j);
-      const double v = ux[k] - ga * dt * (ps_c - psi[L.p(i - 1, j)]) * dxi;
-      ux[k] = v;
-      if (j == ny) ux[L.ux(i, 0)] = v;
-      if (j == 1) ux[L.ux(i, ny + 1)] = v;
+  // with DT: the advective time-step limit of decide_dt.c:26-48 over the
+  // corrected velocities, as one partial minimum per workgroup
+  double m = 5.e-2;
+  for (int j = blockIdx.y * CP_ROWS + 1; j <= ny && j <= (int)(blockIdx.y + 1) * CP_ROWS; j++) {
+    if (i > nx) break;
+    const double ps_c = psi[L.p(i, j)];
+    if (do_correct) {
+      if (i >= 2) {
+        const double dxi = 1. / dxc[i - 1];
+        const long k = L.ux(i, j);
+        const double v = ux[k] - ga * dt * (ps_c - psi[L.p(i - 1, j)]) * dxi;
+        ux[k] = v;
+        if (j == ny) ux[L.ux(i, 0)] = v;
+        if (j == 1) ux[L.ux(i, ny + 1)] = v;
+        if (DT) m = fmin(m, dxc[i - 1] / fmax(fabs(v), 1.e-8));
+      }
+      {
+        const long k = L.p(i, j);
+        const double v = uy[k] - qdiv<P2>(ga * dt * (ps_c - psi[L.p(i, j - 1)]), dy, 1. / dy);
+        uy[k] = v;
+        if (j == ny) uy[L.p(i, 0)] = v;
+        if (j == 1) uy[L.p(i, ny + 1)] = v;
+        if (DT) m = fmin(m, dy / fmax(fabs(v), 1.e-8));
+      }
     }
-    {
+    if (do_pressure) {
       const long k = L.p(i, j);
-      const double v = uy[k] - qdiv<P2>(ga * dt * (ps_c - psi[L.p(i, j - 1)]), dy, 1. / dy);
-      uy[k] = v;
-      if (j == ny) uy[L.p(i, 0)] = v;
-      if (j == 1) uy[L.p(i, ny + 1)] = v;
+      const double v = p[k] + ps_c;
+      p[k] = v;
+      if (j == ny) p[L.p(i, 0)] = v;
+      if (j == 1) p[L.p(i, ny + 1)] = v;
+      if (i == 1) {
+        p[L.p(0, j)] = v;
+        if (j == ny) p[L.p(0, 0)] = v;
+        if (j == 1) p[L.p(0, ny + 1)] = v;
+      }
+      if (i == nx) {
+        p[L.p(nx + 1, j)] = v;
+        if (j == ny) p[L.p(nx + 1, 0)] = v;
+        if (j == 1) p[L.p(nx + 1, ny + 1)] = v;
+      }
     }
   }
-  if (do_pressure) {
-    const long k = L.p(i, j);
-    const double v = p[k] + ps_c;
-    p[k] = v;
-    if (j == ny) p[L.p(i, 0)] = v;
-    if (j == 1) p[L.p(i, ny + 1)] = v;
-    if (i == 1) {
-      p[L.p(0, j)] = v;
-      if (j == ny) p[L.p(0, 0)] = v;
-      if (j == 1) p[L.p(0, ny + 1)] = v;
+  if (DT) {
+    __shared__ double red[256];
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+      if ((int)threadIdx.x < s) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + s]);
+      __syncthreads();
     }
-    if (i == nx) {
-      p[L.p(nx + 1, j)] = v;
-      if (j == ny) p[L.p(nx + 1, 0)] = v;
-      if (j == 1) p[L.p(nx + 1, ny + 1)] = v;
-    }
+    if (threadIdx.x == 0) dt_part[blockIdx.y * gridDim.x + blockIdx.x] = red[0];
   }
 }
 
-int launch_correct_pressure(Ctx *c, int rk, bool correct, bool pressure) {
+int launch_correct_pressure(Ctx *c, int rk, bool correct, bool pressure, bool dt_fused) {
   ProfScope ps(c, "correct_pressure");
-  dim3 grid((c->nx + 255) / 256, c->ny);
-  if (c->pow2_grid)
-    hipLaunchKernelGGL(k_correct_pressure<true>, grid, dim3(256), 0, c->stream, c->L, c->ux,
-                       c->uy, c->p, c->psi, c->dxc, c->dscal, c->gamma[rk], c->dy,
-                       (int)correct, (int)pressure);
-  else
-    hipLaunchKernelGGL(k_correct_pressure<false>, grid, dim3(256), 0, c->stream, c->L, c->ux,
-                       c->uy, c->p, c->psi, c->dxc, c->dscal, c->gamma[rk], c->dy,
-                       (int)correct, (int)pressure);
+  dim3 grid((c->nx + 255) / 256, (c->ny + CP_ROWS - 1) / CP_ROWS);
+  const bool dt = dt_fused && correct && (long)grid.x * grid.y <= DT_PARTS_CAP;
+  auto kern = c->pow2_grid ? (dt ? k_correct_pressure<true, true> : k_correct_pressure<true, false>)
+                           : (dt ? k_correct_pressure<false, true> : k_correct_pressure<false, false>);
+  hipLaunchKernelGGL(kern, grid, dim3(256), 0, c->stream, c->L, c->ux, c->uy, c->p, c->psi,
+                     c->dxc, c->dscal, c->gamma[rk], c->dy, (int)correct, (int)pressure,
+                     c->dt_part);
   EIF_HIP(hipGetLastError());
+  c->dt_adv_parts = dt ? (int)(grid.x * grid.y) : 0;
   return EIF_OK;
 }
 
@@ -357,10 +374,28 @@ __global__ __launch_bounds__(256) void k_dt_adv(Layout L, const double *ux,
   if (threadIdx.x == 0) atomic_min_pos(dred, red[0]);
 }
 
-__global__ void k_dt_final(const double *dred, const double *prt, int np,
+// nparts > 0: the advective limit comes as partial minima part[0..nparts)
+// from the projection kernel of the previous step's last stage
+__global__ void k_dt_final(const double *dred, const double *part, const double *prt, int np,
                            double sf0, double sf2, double dt_dif_x,
-                           double dt_dif_y, double delta, double *dscal) {
+                           double dt_dif_y, double delta, double *dscal, int nparts) {
   __shared__ double red[256];
+  __shared__ double adv;
+  if (nparts > 0) {
+    double m = 5.e-2;
+    for (int b = threadIdx.x; b < nparts; b += blockDim.x) m = fmin(m, part[b]);
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+      if ((int)threadIdx.x < s) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + s]);
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) adv = red[0];
+    __syncthreads();
+  } else if (threadIdx.x == 0) {
+    adv = dred[0];
+  }
+  __syncthreads();
   const double small = 1.e-8;
   double dt_par = 1.7976931348623157e308;
   for (int n = threadIdx.x; n < np; n += blockDim.x) {
@@ -377,7 +412,7 @@ __global__ void k_dt_final(const double *dred, const double *prt, int np,
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    double dt_adv = dred[0];
+    double dt_adv = adv;
     dt_adv *= sf0;
     double dtp = red[0];
     dtp *= sf2;
@@ -387,12 +422,16 @@ __global__ void k_dt_final(const double *dred, const double *prt, int np,
 
 int launch_decide_dt(Ctx *c) {
   ProfScope ps(c, "decide_dt");
-  hipLaunchKernelGGL(k_dt_init, dim3(1), dim3(1), 0, c->stream, c->dred);
-  const long n = (long)c->nx * c->ny;
-  int blocks = (int)((n + 255) / 256);
-  if (blocks > 1024) blocks = 1024;
-  hipLaunchKernelGGL(k_dt_adv, dim3(blocks), dim3(256), 0, c->stream, c->L, c->ux,
-                     c->uy, c->dxc, c->dy, c->dred);
+  const int nparts = c->dt_adv_valid ? c->dt_adv_parts : 0;
+  c->dt_adv_valid = false;
+  if (nparts == 0) {
+    hipLaunchKernelGGL(k_dt_init, dim3(1), dim3(1), 0, c->stream, c->dred);
+    const long n = (long)c->nx * c->ny;
+    int blocks = (int)((n + 255) / 256);
+    if (blocks > 1024) blocks = 1024;
+    hipLaunchKernelGGL(k_dt_adv, dim3(blocks), dim3(256), 0, c->stream, c->L, c->ux,
+                       c->uy, c->dxc, c->dy, c->dred);
+  }
   // diffusive limits are grid constants (decide_dt.c:50-62)
   double dxmin = c->lx;
   for (int i = 2; i <= c->nx; i++) dxmin = fmin(dxmin, c->h_dxc[i - 1]);
@@ -401,9 +440,9 @@ int launch_decide_dt(Ctx *c) {
   dt_dif_x *= c->cfg.safefactors[1];
   dt_dif_y *= c->cfg.safefactors[1];
   const double delta = fmin(c->dx, c->dy);
-  hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(256), 0, c->stream, c->dred, c->prt,
+  hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(256), 0, c->stream, c->dred, c->dt_part, c->prt,
                      c->np, c->cfg.safefactors[0], c->cfg.safefactors[2], dt_dif_x,
-                     dt_dif_y, delta, c->dscal);
+                     dt_dif_y, delta, c->dscal, nparts);
   EIF_HIP(hipGetLastError());
   return EIF_OK;
 }

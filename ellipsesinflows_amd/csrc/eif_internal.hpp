@@ -55,6 +55,7 @@ constexpr int TILE_Y = 4;
 constexpr int TILE_CAP = 48;     // boxes registered per tile
 constexpr int CONTACT_CAP = 64;  // colliding partners per particle
 constexpr int EQC_ITER_CAP = 100000;  // guard of the uncapped fixed point
+constexpr int DT_PARTS_CAP = 4096;    // workgroups of the projection with fused dt
 
 struct Profiler;
 
@@ -81,6 +82,13 @@ struct Ctx {
   std::vector<double> h_xf, h_xc, h_dxf, h_dxc, h_yf, h_yc;
   double alpha[3], beta[3], gamma[3];
   bool pow2_grid = false;  // dx, dy and every DXF, DXC are powers of two
+  // ux, uy and p carry current boundary values (every kernel of a stage
+  // refreshes the ones it changes); host-side changes clear it
+  bool bc_valid = false;
+  // the advective dt limit of the current velocities is already reduced
+  // into dt_adv_parts partial minima (dred[16...])
+  bool dt_adv_valid = false;
+  int dt_adv_parts = 0;
   // device grid
   double *xf = nullptr, *xc = nullptr, *dxf = nullptr, *dxc = nullptr,
          *yf = nullptr, *yc = nullptr;
@@ -108,6 +116,7 @@ struct Ctx {
   double *dscal = nullptr;
   double *hscal = nullptr;  // pinned
   double *dred = nullptr;   // reduction partials
+  double *dt_part = nullptr;  // DT_PARTS_CAP partial minima of the fused dt
   int nred = 0;
   // particles
   int np = 0;
@@ -182,7 +191,9 @@ int launch_bc_ux(Ctx *c, double *ux);
 int launch_bc_uy(Ctx *c, double *uy);
 int launch_bc_p(Ctx *c, double *p);
 int launch_predictor(Ctx *c, int rk);
-int launch_correct_pressure(Ctx *c, int rk, bool correct, bool pressure);
+// dt_fused: also leave the next decide_dt's advective limit as partial minima
+int launch_correct_pressure(Ctx *c, int rk, bool correct, bool pressure,
+                            bool dt_fused = false);
 int launch_decide_dt(Ctx *c);
 int launch_diagnostics(Ctx *c, int which, double *out);
 int poisson_init(Ctx *c);
