@@ -79,20 +79,27 @@ void prof_end(Ctx *c, const char *name, hipStream_t st) {
   if (r.used >= 4096) prof_flush(c);
 }
 
+// message for nonzero device capacity flags
+int flags_error(int h) {
+  std::string m = "device capacity exceeded:";
+  if (h & 1) m += " IBM box scratch;";
+  if (h & 2) m += " tile box list;";
+  if (h & 4) m += " colliding pair list;";
+  if (h & 8) m += " equivalent-circle iteration cap;";
+  if (h & 16) m += " contacts per particle;";
+  set_error(m);
+  return EIF_ERR_CAPACITY;
+}
+
 int check_flags(Ctx *c) {
-  int h = 0;
-  EIF_HIP(hipMemcpyAsync(&h, c->flags, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  EIF_HIP(hipMemcpyAsync(c->hscal + S_NSCAL - 1, c->flags, sizeof(int), hipMemcpyDeviceToHost,
+                         c->stream));
   EIF_HIP(hipStreamSynchronize(c->stream));
+  int h;
+  memcpy(&h, c->hscal + S_NSCAL - 1, sizeof(int));
   if (h) {
-    std::string m = "device capacity exceeded:";
-    if (h & 1) m += " IBM box scratch;";
-    if (h & 2) m += " tile box list;";
-    if (h & 4) m += " colliding pair list;";
-    if (h & 8) m += " equivalent-circle iteration cap;";
-    if (h & 16) m += " contacts per particle;";
-    set_error(m);
     EIF_HIP(hipMemsetAsync(c->flags, 0, sizeof(int), c->stream));
-    return EIF_ERR_CAPACITY;
+    return flags_error(h);
   }
   return EIF_OK;
 }
@@ -147,7 +154,7 @@ static void free_ctx(Ctx *c) {
   for (auto o : others)
     if (o) (void)hipFree(o);
   if (c->hscal) (void)hipHostFree(c->hscal);
-  if (c->hgate) (void)hipHostFree(c->hgate);
+  if (c->hstat) (void)hipHostFree(c->hstat);
   if (c->prof) {
     for (auto &kv : c->prof->recs)
       for (auto &e : kv.second.ev) {
@@ -157,7 +164,7 @@ static void free_ctx(Ctx *c) {
     delete c->prof;
   }
   if (c->side) (void)hipStreamSynchronize(c->side);
-  hipEvent_t evs[] = {c->ev_fork, c->ev_i0, c->ev_c0, c->ev_gate, c->ev_coll};
+  hipEvent_t evs[] = {c->ev_fork, c->ev_i0, c->ev_c0, c->ev_gate, c->ev_coll, c->ev_x, c->ev_sub};
   for (auto ev : evs)
     if (ev) (void)hipEventDestroy(ev);
   if (c->side) (void)hipStreamDestroy(c->side);
@@ -201,7 +208,8 @@ int fluid_gpu_create(const eif_config *cfg, eif_ctx **out) {
   {
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
-    hipEvent_t *evs[] = {&c->ev_fork, &c->ev_i0, &c->ev_c0, &c->ev_gate, &c->ev_coll};
+    hipEvent_t *evs[] = {&c->ev_fork, &c->ev_i0, &c->ev_c0, &c->ev_gate, &c->ev_coll,
+                         &c->ev_x, &c->ev_sub};
     for (auto ev : evs)
       if (e == hipSuccess) e = hipEventCreateWithFlags(ev, hipEventDisableTiming);
     if (e != hipSuccess) return fail(hip_fail(e, "hipStreamCreate", __FILE__, __LINE__));
@@ -290,7 +298,8 @@ int fluid_gpu_create(const eif_config *cfg, eif_ctx **out) {
     if (e == hipSuccess) e = hipMemset(c->flags, 0, 4 * sizeof(int));
     if (e == hipSuccess) e = hipMalloc(&c->gate, 4 * sizeof(int));
     if (e == hipSuccess) e = hipMemset(c->gate, 0, 4 * sizeof(int));
-    if (e == hipSuccess) e = hipHostMalloc((void **)&c->hgate, 4 * sizeof(int), 0);
+    if (e == hipSuccess)
+      e = hipHostMalloc((void **)&c->hstat, sizeof(HostStatus), hipHostMallocCoherent);
     if (e == hipSuccess) e = hipMalloc(&c->ibm_stats, 8 * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMemset(c->ibm_stats, 0, 8 * sizeof(unsigned long long));
     if (e != hipSuccess) return fail(hip_fail(e, "hipMalloc", __FILE__, __LINE__));
@@ -658,33 +667,83 @@ int suspensions_gpu_colliding_pairs(eif_ctx *c, int capacity, int *pairs, int *n
 }
 
 // integrate(), src/main.c:13-55
+// One batch of n speculatively queued Crank-Nicolson sub-iterations
+// (src/main.c:37-50).  A device-side gate before each one applies the
+// loop's exit rule, so sub-iterations queued past convergence do nothing.
+static int queue_substeps(Ctx *c, int rk, int n) {
+  for (int b = 0; b < n; b++) {
+    TRY(launch_substep_gate(c));
+    EIF_HIP(hipEventRecord(c->ev_gate, c->stream));
+    EIF_HIP(hipStreamWaitEvent(c->side, c->ev_gate, 0));
+    TRY(launch_collision(c, 1, true, c->side));
+    EIF_HIP(hipEventRecord(c->ev_coll, c->side));
+    TRY(launch_inertia(c, 1, true, c->stream, true));
+    EIF_HIP(hipStreamWaitEvent(c->stream, c->ev_coll, 0));
+    TRY(launch_increment(c, rk, true, true));
+  }
+  // the count, residual, dt and capacity flags, for the host
+  TRY(launch_status(c));
+  EIF_HIP(hipEventRecord(c->ev_sub, c->stream));
+  return EIF_OK;
+}
+
 int eif_gpu_integrate(eif_ctx *c, int substeps[3]) {
   CHECK_CTX(c);
   // Two streams.  Main: fluid kernels, exchange, deposit, inertia(1),
-  // increments.  Side: inertia(0) and the collisions, which read only the
-  // particles and the stage-start velocity, so they overlap the predictor,
-  // the pressure solve and inertia(1).  Events join the side work back
-  // before anything that reads its results or reuses its buffers.
+  // increments.  Side: inertia(0), the collisions and the exchange's
+  // particle sums, which read only the particles and the stage-start
+  // velocity, so they overlap the predictor, the pressure solve and
+  // inertia(1).  Events join the side work back before anything that reads
+  // its results or reuses its buffers.
+  //
+  // The predictor of stage rk+1 reads only the fluid fields, which the
+  // sub-iterations of stage rk leave alone.  It is queued right behind the
+  // first batch of sub-iterations, so the GPU runs it while the host waits
+  // for that batch's count; the particle update of stage rk and the reset
+  // of stage rk+1 then go to the side stream.
+  bool pred_queued = false;  // this stage's predictor is already queued
   for (int rk = 0; rk < 3; rk++) {
-    TRY(launch_reset(c));
-    // src/main.c:16-19: the kernels of every stage keep the boundary and
-    // halo values they change up to date, so this only runs after a
-    // host-side change of the fields
-    if (!c->bc_valid) {
-      TRY(launch_bc_ux(c, c->ux));
-      TRY(launch_bc_uy(c, c->uy));
-      TRY(launch_bc_p(c, c->p));
-      c->bc_valid = true;
+    if (pred_queued) {
+      EIF_HIP(hipStreamWaitEvent(c->side, c->ev_sub, 0));
+      TRY(launch_update_particles(c, c->side));
+      TRY(launch_reset(c, c->side, true));
+    } else {
+      TRY(launch_reset(c, c->stream, true));
+      // src/main.c:16-19: the kernels of every stage keep the boundary and
+      // halo values they change up to date, so this only runs after a
+      // host-side change of the fields
+      if (!c->bc_valid) {
+        TRY(launch_bc_ux(c, c->ux));
+        TRY(launch_bc_uy(c, c->uy));
+        TRY(launch_bc_p(c, c->p));
+        c->bc_valid = true;
+      }
+      EIF_HIP(hipEventRecord(c->ev_fork, c->stream));
+      EIF_HIP(hipStreamWaitEvent(c->side, c->ev_fork, 0));
     }
-    EIF_HIP(hipEventRecord(c->ev_fork, c->stream));
-    EIF_HIP(hipStreamWaitEvent(c->side, c->ev_fork, 0));
-    TRY(launch_inertia(c, 0, false, c->side));  // reads ux, uy before the predictor
+    // inertia(0) reads the stage-start velocity: the buffers the predictor
+    // reads (already swapped out when the predictor is queued)
+    if (pred_queued) {
+      std::swap(c->ux, c->ux_alt);
+      std::swap(c->uy, c->uy_alt);
+    }
+    const int ri = launch_inertia(c, 0, false, c->side);
+    if (pred_queued) {
+      std::swap(c->ux, c->ux_alt);
+      std::swap(c->uy, c->uy_alt);
+    }
+    TRY(ri);
     EIF_HIP(hipEventRecord(c->ev_i0, c->side));
     TRY(launch_collision(c, 0, false, c->side));
-    EIF_HIP(hipEventRecord(c->ev_c0, c->side));
-    TRY(launch_predictor(c, rk));  // writes the other ux, uy buffers
+    if (!pred_queued) TRY(launch_predictor(c, rk));  // writes the other ux, uy buffers
     EIF_HIP(hipStreamWaitEvent(c->stream, c->ev_i0, 0));
-    TRY(launch_exchange(c, true));  // same position as inertia(0): dx = 0 after the reset
+    // same position as inertia(0) (dx = 0 after the reset); the reset
+    // cleared the tile registrations
+    TRY(launch_exchange(c, true, true, true));
+    EIF_HIP(hipEventRecord(c->ev_x, c->stream));
+    EIF_HIP(hipStreamWaitEvent(c->side, c->ev_x, 0));
+    TRY(launch_exchange_finalize(c, c->side));
+    EIF_HIP(hipEventRecord(c->ev_c0, c->side));
     TRY(launch_update_momentum(c));
     TRY(launch_poisson(c, rk));
     // the last stage also reduces the advective dt limit of the final
@@ -692,37 +751,43 @@ int eif_gpu_integrate(eif_ctx *c, int substeps[3]) {
     TRY(launch_correct_pressure(c, rk, true, true, rk == 2));
     // the sub-iteration gate clears the pair counters collision(0) used
     EIF_HIP(hipStreamWaitEvent(c->stream, c->ev_c0, 0));
-    // Crank-Nicolson sub-iterations (src/main.c:37-50).  They are queued in
-    // batches; a device-side gate before each one applies the loop's exit
-    // rule, so sub-iterations queued past convergence do nothing, and the
-    // host reads the count and the residual once per batch.
     int queued = 0, done = 0;
     int batch = c->substep_guess > 0 ? c->substep_guess : 1;
+    bool early = false;  // predictor of stage rk+1 queued
     for (;;) {
-      for (int b = 0; b < batch; b++) {
-        TRY(launch_substep_gate(c));
-        EIF_HIP(hipEventRecord(c->ev_gate, c->stream));
-        EIF_HIP(hipStreamWaitEvent(c->side, c->ev_gate, 0));
-        TRY(launch_collision(c, 1, true, c->side));
-        EIF_HIP(hipEventRecord(c->ev_coll, c->side));
-        TRY(launch_inertia(c, 1, true, c->stream, true));
-        EIF_HIP(hipStreamWaitEvent(c->stream, c->ev_coll, 0));
-        TRY(launch_increment(c, rk, true, true));
+      if (early) {  // the stage's own velocity fields for inertia(1)
+        std::swap(c->ux, c->ux_alt);
+        std::swap(c->uy, c->uy_alt);
       }
+      const int r = queue_substeps(c, rk, batch);
+      if (early) {
+        std::swap(c->ux, c->ux_alt);
+        std::swap(c->uy, c->uy_alt);
+      }
+      TRY(r);
       queued += batch;
-      EIF_HIP(hipMemcpyAsync(c->hgate, c->gate, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
-      EIF_HIP(hipMemcpyAsync(c->hscal + S_RESIDUAL, c->dscal + S_RESIDUAL, sizeof(double),
-                             hipMemcpyDeviceToHost, c->stream));
-      EIF_HIP(hipStreamSynchronize(c->stream));
-      TRY(check_flags(c));
-      done = c->hgate[1];
-      const double residual = c->hscal[S_RESIDUAL];
-      if (done < queued || residual < 1.e-8 || done - 1 > 100) break;
+      if (!early && rk < 2) {
+        TRY(launch_predictor(c, rk + 1));
+        early = true;
+      }
+      EIF_HIP(hipEventSynchronize(c->ev_sub));
+      const HostStatus st = *c->hstat;
+      if (st.flags) {
+        EIF_HIP(hipStreamSynchronize(c->side));
+        EIF_HIP(hipStreamSynchronize(c->stream));
+        EIF_HIP(hipMemsetAsync(c->flags, 0, sizeof(int), c->stream));
+        EIF_HIP(hipStreamSynchronize(c->stream));
+        return flags_error(st.flags);
+      }
+      c->hscal[S_DT] = st.dt;
+      done = st.substeps;
+      if (done < queued || st.residual < 1.e-8 || done - 1 > 100) break;
       batch = 1;
     }
     c->substep_guess = done;
     if (substeps) substeps[rk] = done;
-    TRY(launch_update_particles(c));
+    pred_queued = early;
+    if (!early) TRY(launch_update_particles(c));
   }
   c->dt_adv_valid = c->dt_adv_parts > 0;
   return EIF_OK;
@@ -732,7 +797,8 @@ int eif_gpu_step(eif_ctx *c, double *dt_out) {
   CHECK_CTX(c);
   TRY(launch_decide_dt(c));
   TRY(eif_gpu_integrate(c, nullptr));
-  if (dt_out) return fluid_gpu_get_dt(c, dt_out);
+  // dt is read back with the first sub-iteration count of the step
+  if (dt_out) *dt_out = c->hscal[S_DT];
   return EIF_OK;
 }
 

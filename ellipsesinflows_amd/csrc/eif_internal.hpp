@@ -66,6 +66,15 @@ __host__ __device__ inline double qdiv(double x, double d, double inv) {
   return POW2 ? x * inv : x / d;
 }
 
+// what the host reads after each batch of sub-iterations, written by one
+// kernel straight into pinned host memory (one readback per batch)
+struct HostStatus {
+  double residual;  // of the last sub-iteration run
+  double dt;        // of the current step
+  int substeps;     // sub-iterations run in this stage (gate[1])
+  int flags;        // device capacity flags
+};
+
 struct Ctx {
   eif_config cfg;
   int nx, ny;
@@ -77,6 +86,8 @@ struct Ctx {
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_i0 = nullptr, ev_c0 = nullptr;
   hipEvent_t ev_gate = nullptr, ev_coll = nullptr;
+  hipEvent_t ev_x = nullptr;    // exchange done (main), for its finalize on the side stream
+  hipEvent_t ev_sub = nullptr;  // sub-iteration batch and its status record done (main)
   int device = 0;
   // host copies of the grid (exact reference arithmetic)
   std::vector<double> h_xf, h_xc, h_dxf, h_dxc, h_yf, h_yc;
@@ -143,7 +154,7 @@ struct Ctx {
   unsigned long long *ibm_stats = nullptr;  // [8] measurement counters
   unsigned long long *ibm_pp = nullptr;     // np x 4 IBM work counters, then calls
   int *gate = nullptr;   // [0] sub-iteration open, [1] sub-iterations run
-  int *hgate = nullptr;  // pinned copy
+  HostStatus *hstat = nullptr;  // pinned, written by k_status
   int substep_guess = 3; // sub-iterations queued before the first check
   // collisions
   int *pair_cnt = nullptr;  // [0] colliding pairs, [1] candidate pairs
@@ -205,15 +216,22 @@ int launch_inertia(Ctx *c, int cnstep, bool gated = false, hipStream_t st = null
                    bool defer_finalize = false);
 // cached_known: the distance cache holds the exchange position (it follows
 // inertia(0) of the same stage), so the distance pass is skipped
-int launch_exchange(Ctx *c, bool cached_known = false);
+// tiles_clear: the tile registrations were cleared by launch_reset(clear_tiles);
+// defer_finalize: the per-particle sums are added by launch_exchange_finalize
+int launch_exchange(Ctx *c, bool cached_known = false, bool tiles_clear = false,
+                    bool defer_finalize = false);
+int launch_exchange_finalize(Ctx *c, hipStream_t st);
 int launch_update_momentum(Ctx *c);
 int launch_materialise_du(Ctx *c);
 int launch_collision(Ctx *c, int cnstep, bool gated = false, hipStream_t st = nullptr);
 int launch_increment(Ctx *c, int rk, bool gated = false, bool fin_inertia = false);
 int launch_substep_gate(Ctx *c);
-int launch_update_particles(Ctx *c);
-int launch_reset(Ctx *c);
+int launch_update_particles(Ctx *c, hipStream_t st = nullptr);
+// clear_tiles: also clear the IBM tile registrations for the next exchange
+int launch_reset(Ctx *c, hipStream_t st = nullptr, bool clear_tiles = false);
+int launch_status(Ctx *c);
 int check_flags(Ctx *c);
+int flags_error(int h);
 int launch_statistics(Ctx *c);
 int launch_ref_iterations(Ctx *c, unsigned long long *dout);
 int stat_alloc(Ctx *c);

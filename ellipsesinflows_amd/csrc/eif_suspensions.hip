@@ -29,6 +29,7 @@
 //     one thread per particle gathers its partners' forces in the
 //     reference's pair order.
 #include "eif_internal.hpp"
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 
@@ -1335,11 +1336,18 @@ __global__ void k_update_particles(double *prt, int np, double ly) {
 }
 
 // src/suspensions/reset.c:7-18
-__global__ void k_reset(double *prt, int np, int *gate, int *sc_key) {
+// tile_cnt (ntiles) and active[0]: cleared when given, ready for the next
+// exchange's tile registrations
+__global__ void k_reset(double *prt, int np, int *gate, int *sc_key, int *tile_cnt,
+                        int ntiles, int *active) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n == 0 && gate) {
     gate[0] = 1;
     gate[1] = 0;
+  }
+  if (tile_cnt) {
+    if (n < ntiles) tile_cnt[n] = 0;
+    if (n == 0) active[0] = 0;
   }
   if (n >= np) return;
   // foot-point warm starts reach back no further than the stage start, so
@@ -1473,10 +1481,13 @@ int susp_init(Ctx *c) {
   return EIF_OK;
 }
 
-int launch_reset(Ctx *c) {
+int launch_reset(Ctx *c, hipStream_t st, bool clear_tiles) {
+  if (!st) st = c->stream;
   // also rewinds the sub-iteration gate, so it runs without particles too
-  hipLaunchKernelGGL(k_reset, dim3(c->np > 0 ? (c->np + 255) / 256 : 1), dim3(256), 0,
-                     c->stream, c->prt, c->np, c->gate, c->sc_key);
+  const int ntiles = c->np > 0 && clear_tiles ? c->ntx * c->nty : 0;
+  const int n = std::max(std::max(c->np, ntiles), 1);
+  hipLaunchKernelGGL(k_reset, dim3((n + 255) / 256), dim3(256), 0, st, c->prt, c->np, c->gate,
+                     c->sc_key, ntiles > 0 ? c->tile_cnt : nullptr, ntiles, c->active);
   EIF_HIP(hipGetLastError());
   return EIF_OK;
 }
@@ -1494,20 +1505,30 @@ int launch_inertia(Ctx *c, int cnstep, bool gated, hipStream_t st, bool defer_fi
   return EIF_OK;
 }
 
-int launch_exchange(Ctx *c, bool cached_known) {
+int launch_exchange(Ctx *c, bool cached_known, bool tiles_clear, bool defer_finalize) {
   if (c->np == 0) return EIF_OK;
   ProfScope ps(c, "ibm_exchange");
   const size_t ntiles = (size_t)c->ntx * c->nty;
-  // clear the previous call's tile registrations
-  hipLaunchKernelGGL(k_zero_int, dim3((unsigned)((ntiles + 255) / 256)), dim3(256), 0,
-                     c->stream, c->tile_cnt, (int)ntiles);
-  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, c->active, 1);
+  if (!tiles_clear) {  // clear the previous call's tile registrations
+    hipLaunchKernelGGL(k_zero_int, dim3((unsigned)((ntiles + 255) / 256)), dim3(256), 0,
+                       c->stream, c->tile_cnt, (int)ntiles);
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, c->active, 1);
+  }
   IbmArgs a = ibm_args(c);
   if (!cached_known)
     hipLaunchKernelGGL(k_ibm_dist, dim3(c->nslices), dim3(64), 0, c->stream, a, 0);
   hipLaunchKernelGGL(k_exchange, dim3(c->nslices), dim3(64), 0, c->stream, a,
                      cached_known ? 1 : 0);
-  hipLaunchKernelGGL(k_ibm_finalize, dim3((c->np + 63) / 64), dim3(64), 0, c->stream, a, 2);
+  EIF_HIP(hipGetLastError());
+  if (!defer_finalize) return launch_exchange_finalize(c, c->stream);
+  return EIF_OK;
+}
+
+// the exchange's per-particle sums (slice order); nothing on the fluid side
+// reads them, so this may run on another stream after the exchange
+int launch_exchange_finalize(Ctx *c, hipStream_t st) {
+  if (c->np == 0) return EIF_OK;
+  hipLaunchKernelGGL(k_ibm_finalize, dim3((c->np + 63) / 64), dim3(64), 0, st, ibm_args(c), 2);
   EIF_HIP(hipGetLastError());
   return EIF_OK;
 }
@@ -1606,6 +1627,22 @@ int launch_increment(Ctx *c, int rk, bool gated, bool fin_inertia) {
   return EIF_OK;
 }
 
+// one thread copies the sub-iteration state into pinned host memory
+__global__ void k_status(const int *gate, const int *flags, const double *dscal,
+                         HostStatus *h) {
+  h->residual = dscal[S_RESIDUAL];
+  h->dt = dscal[S_DT];
+  h->substeps = gate[1];
+  h->flags = flags[0];
+}
+
+int launch_status(Ctx *c) {
+  hipLaunchKernelGGL(k_status, dim3(1), dim3(1), 0, c->stream, c->gate, c->flags, c->dscal,
+                     c->hstat);
+  EIF_HIP(hipGetLastError());
+  return EIF_OK;
+}
+
 int launch_substep_gate(Ctx *c) {
   hipLaunchKernelGGL(k_substep_gate, dim3(1), dim3(1), 0, c->stream, c->dscal, c->gate,
                      c->np > 0 ? c->pair_cnt : nullptr);
@@ -1613,10 +1650,10 @@ int launch_substep_gate(Ctx *c) {
   return EIF_OK;
 }
 
-int launch_update_particles(Ctx *c) {
+int launch_update_particles(Ctx *c, hipStream_t st) {
   if (c->np == 0) return EIF_OK;
   hipLaunchKernelGGL(k_update_particles, dim3((c->np + 255) / 256), dim3(256), 0,
-                     c->stream, c->prt, c->np, c->ly);
+                     st ? st : c->stream, c->prt, c->np, c->ly);
   EIF_HIP(hipGetLastError());
   return EIF_OK;
 }
