@@ -51,7 +51,7 @@ EXPORTED = (
     "eif_gpu_ibm_stats", "eif_gpu_ibm_reference_iterations", "statistics_gpu_collect", "statistics_gpu_reset",
     "statistics_gpu_download",
     "eif_gpu_profile_enable", "eif_gpu_profile_select", "eif_gpu_profile_reset",
-    "eif_gpu_profile_query",
+    "eif_gpu_profile_query", "eif_gpu_profile_sample",
     "eif_gpu_profile_names",
 )
 
@@ -339,6 +339,9 @@ class Solver:
 
     def profile_select(self, name=None):
         self._check(self.lib.eif_gpu_profile_select(self.h, None if not name else name.encode()))
+
+    def profile_sample(self, every):
+        self._check(self.lib.eif_gpu_profile_sample(self.h, int(every)))
 
     def profile_reset(self):
         self._check(self.lib.eif_gpu_profile_reset(self.h))

@@ -26,14 +26,16 @@ int hip_fail(hipError_t e, const char *what, const char *file, int line) {
 // ---------------------------------------------------------------------
 static const char *k_prof_names =
     "bc,predictor,rhs_dct2,thomas,ysolve_partition,dct3_psi,correct_pressure,decide_dt,"
-    "ibm_inertia,ibm_exchange,ibm_apply,collision";
+    "ibm_inertia,ibm_substep,ibm_exchange,ibm_apply,collision";
 
 struct Profiler {
   bool on = false;
   std::string only;
+  int every = 1;  // time one launch in `every` (each event pair idles the stream briefly)
   struct Rec {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
     size_t used = 0;
+    long seen = 0;
     long count = 0;
     double total_ms = 0.;
   };
@@ -57,21 +59,20 @@ static int prof_flush(Ctx *c) {
   return EIF_OK;
 }
 
-void prof_begin(Ctx *c, const char *name, hipStream_t st) {
-  if (!c->prof || !c->prof->on) return;
-  if (!c->prof->only.empty() && c->prof->only != name) return;
+bool prof_begin(Ctx *c, const char *name, hipStream_t st) {
+  if (!c->prof || !c->prof->on) return false;
+  if (!c->prof->only.empty() && c->prof->only != name) return false;
   auto &r = c->prof->recs[name];
+  if (r.seen++ % c->prof->every != 0) return false;
   if (r.used == r.ev.size()) {
     hipEvent_t a, b;
-    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return false;
     r.ev.push_back({a, b});
   }
-  (void)hipEventRecord(r.ev[r.used].first, st);
+  return hipEventRecord(r.ev[r.used].first, st) == hipSuccess;
 }
 
 void prof_end(Ctx *c, const char *name, hipStream_t st) {
-  if (!c->prof || !c->prof->on) return;
-  if (!c->prof->only.empty() && c->prof->only != name) return;
   auto &r = c->prof->recs[name];
   if (r.used >= r.ev.size()) return;
   (void)hipEventRecord(r.ev[r.used].second, st);
@@ -143,14 +144,14 @@ static void free_ctx(Ctx *c) {
                      &c->th_q1, &c->th_den, &c->th_last, &c->pt_fwd, &c->pt_u,
                      &c->pt_v, &c->pt_w, &c->pt_bnd, &c->pt_red, &c->pt_lvl, &c->pt_ends, &c->pt_g, &c->dscal, &c->dred,
                      &c->prt, &c->pconst, &c->dt_part, &c->sc_fx, &c->sc_fy, &c->sc_dist, &c->sc_pos,
-                     &c->pair_val, &c->wall_val, &c->sc_t, &c->st_ux1, &c->st_ux2, &c->st_uy1,
+                     &c->sc_t, &c->res_slots, &c->st_ux1, &c->st_ux2, &c->st_uy1,
                      &c->st_uy2, &c->st_phi, &c->slice_part};
   for (auto b : bufs)
     if (*b) (void)hipFree(*b);
   void *others[] = {c->th_flag, c->boxes, c->box_off, c->tile_cnt, c->tile_list,
-                    c->active, c->flags, c->pair_cnt, c->pair_ids, c->wall_hit,
+                    c->active, c->flags, c->rows, c->row_cnt, c->wall_hit,
                     c->ibm_stats, c->slice_first, c->slice_info, c->ibm_pp, c->slice_owner,
-                    c->cand, c->gate, c->sc_key};
+                    c->sc_key};
   for (auto o : others)
     if (o) (void)hipFree(o);
   if (c->hscal) (void)hipHostFree(c->hscal);
@@ -164,7 +165,7 @@ static void free_ctx(Ctx *c) {
     delete c->prof;
   }
   if (c->side) (void)hipStreamSynchronize(c->side);
-  hipEvent_t evs[] = {c->ev_fork, c->ev_i0, c->ev_c0, c->ev_gate, c->ev_coll, c->ev_x, c->ev_sub};
+  hipEvent_t evs[] = {c->ev_fork, c->ev_i0, c->ev_c0, c->ev_x, c->ev_sub};
   for (auto ev : evs)
     if (ev) (void)hipEventDestroy(ev);
   if (c->side) (void)hipStreamDestroy(c->side);
@@ -208,8 +209,7 @@ int fluid_gpu_create(const eif_config *cfg, eif_ctx **out) {
   {
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
-    hipEvent_t *evs[] = {&c->ev_fork, &c->ev_i0, &c->ev_c0, &c->ev_gate, &c->ev_coll,
-                         &c->ev_x, &c->ev_sub};
+    hipEvent_t *evs[] = {&c->ev_fork, &c->ev_i0, &c->ev_c0, &c->ev_x, &c->ev_sub};
     for (auto ev : evs)
       if (e == hipSuccess) e = hipEventCreateWithFlags(ev, hipEventDisableTiming);
     if (e != hipSuccess) return fail(hip_fail(e, "hipStreamCreate", __FILE__, __LINE__));
@@ -296,8 +296,8 @@ int fluid_gpu_create(const eif_config *cfg, eif_ctx **out) {
     if (e != hipSuccess) return fail(hip_fail(e, "hipHostMalloc", __FILE__, __LINE__));
     e = hipMalloc(&c->flags, 4 * sizeof(int));
     if (e == hipSuccess) e = hipMemset(c->flags, 0, 4 * sizeof(int));
-    if (e == hipSuccess) e = hipMalloc(&c->gate, 4 * sizeof(int));
-    if (e == hipSuccess) e = hipMemset(c->gate, 0, 4 * sizeof(int));
+    if (e == hipSuccess) e = hipMalloc(&c->res_slots, RES_SLOTS * sizeof(double));
+    if (e == hipSuccess) e = hipMemset(c->res_slots, 0, RES_SLOTS * sizeof(double));
     if (e == hipSuccess)
       e = hipHostMalloc((void **)&c->hstat, sizeof(HostStatus), hipHostMallocCoherent);
     if (e == hipSuccess) e = hipMalloc(&c->ibm_stats, 8 * sizeof(unsigned long long));
@@ -635,20 +635,20 @@ int suspensions_gpu_colliding_pairs(eif_ctx *c, int capacity, int *pairs, int *n
   CHECK_CTX(c);
   std::vector<std::pair<int, int>> out;
   if (c->np > 0) {
-    int cnt = 0;
-    EIF_HIP(hipMemcpyAsync(&cnt, c->pair_cnt, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    // pairs n < m from the per-particle partner rows of the last collision call
+    std::vector<int> cnt(c->np), rows((size_t)c->np * CONTACT_CAP), hits(2 * (size_t)c->np);
+    EIF_HIP(hipMemcpyAsync(cnt.data(), c->row_cnt, (size_t)c->np * sizeof(int),
+                           hipMemcpyDeviceToHost, c->stream));
+    EIF_HIP(hipMemcpyAsync(rows.data(), c->rows, rows.size() * sizeof(int),
+                           hipMemcpyDeviceToHost, c->stream));
+    EIF_HIP(hipMemcpyAsync(hits.data(), c->wall_hit, hits.size() * sizeof(int),
+                           hipMemcpyDeviceToHost, c->stream));
     EIF_HIP(hipStreamSynchronize(c->stream));
-    cnt = std::min(cnt, c->pair_cap);
-    std::vector<int> ids(2 * (size_t)(cnt > 0 ? cnt : 1));
-    std::vector<int> hits(2 * (size_t)c->np);
-    if (cnt > 0)
-      EIF_HIP(hipMemcpyAsync(ids.data(), c->pair_ids, 2 * (size_t)cnt * sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    EIF_HIP(hipMemcpyAsync(hits.data(), c->wall_hit, 2 * (size_t)c->np * sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    EIF_HIP(hipStreamSynchronize(c->stream));
-    std::vector<std::pair<int, int>> pp;
-    for (int s = 0; s < cnt; s++) pp.push_back({ids[2 * s], ids[2 * s + 1]});
-    std::sort(pp.begin(), pp.end());
-    out = pp;
+    for (int n = 0; n < c->np; n++)
+      for (int k = 0; k < std::min(cnt[n], CONTACT_CAP); k++)
+        if (rows[(size_t)n * CONTACT_CAP + k] > n)
+          out.push_back({n, rows[(size_t)n * CONTACT_CAP + k]});
+    std::sort(out.begin(), out.end());
     for (int w = 0; w < 2; w++)
       for (int n = 0; n < c->np; n++)
         if (hits[2 * n + w]) out.push_back({n, -1 - w});
@@ -667,34 +667,33 @@ int suspensions_gpu_colliding_pairs(eif_ctx *c, int capacity, int *pairs, int *n
 }
 
 // integrate(), src/main.c:13-55
-// One batch of n speculatively queued Crank-Nicolson sub-iterations
-// (src/main.c:37-50).  A device-side gate before each one applies the
-// loop's exit rule, so sub-iterations queued past convergence do nothing.
-static int queue_substeps(Ctx *c, int rk, int n) {
-  for (int b = 0; b < n; b++) {
-    TRY(launch_substep_gate(c));
-    EIF_HIP(hipEventRecord(c->ev_gate, c->stream));
-    EIF_HIP(hipStreamWaitEvent(c->side, c->ev_gate, 0));
-    TRY(launch_collision(c, 1, true, c->side));
-    EIF_HIP(hipEventRecord(c->ev_coll, c->side));
-    TRY(launch_inertia(c, 1, true, c->stream, true));
-    EIF_HIP(hipStreamWaitEvent(c->stream, c->ev_coll, 0));
-    TRY(launch_increment(c, rk, true, true));
+// Sub-iterations first .. first+n-1 of the stage's Crank-Nicolson loop
+// (src/main.c:37-50), queued speculatively: each kernel checks the
+// previous sub-iteration's residual (sub_open), so the ones queued past
+// convergence do nothing.  Then the status record for the host.
+static int queue_substeps(Ctx *c, int rk, int first, int n) {
+  if (first + n > RES_SLOTS) {
+    set_error("sub-iteration slots exhausted");
+    return EIF_ERR_CAPACITY;
   }
-  // the count, residual, dt and capacity flags, for the host
-  TRY(launch_status(c));
+  for (int s = first; s < first + n; s++) {
+    TRY(launch_substep_forces(c, s));
+    TRY(launch_increment(c, rk, s));
+  }
+  TRY(launch_status(c, first + n));
   EIF_HIP(hipEventRecord(c->ev_sub, c->stream));
   return EIF_OK;
 }
 
 int eif_gpu_integrate(eif_ctx *c, int substeps[3]) {
   CHECK_CTX(c);
-  // Two streams.  Main: fluid kernels, exchange, deposit, inertia(1),
-  // increments.  Side: inertia(0), the collisions and the exchange's
-  // particle sums, which read only the particles and the stage-start
-  // velocity, so they overlap the predictor, the pressure solve and
-  // inertia(1).  Events join the side work back before anything that reads
-  // its results or reuses its buffers.
+  // Two streams.  Main: fluid kernels, exchange, deposit and the
+  // sub-iterations (inertia(1) with collision(1) in one launch, then the
+  // increment).  Side: inertia(0), collision(0) and the exchange's particle
+  // sums, which read only the particles and the stage-start velocity, so
+  // they overlap the predictor and the pressure solve.  Events join the
+  // side work back before anything that reads its results or reuses its
+  // buffers.
   //
   // The predictor of stage rk+1 reads only the fluid fields, which the
   // sub-iterations of stage rk leave alone.  It is queued right behind the
@@ -727,14 +726,14 @@ int eif_gpu_integrate(eif_ctx *c, int substeps[3]) {
       std::swap(c->ux, c->ux_alt);
       std::swap(c->uy, c->uy_alt);
     }
-    const int ri = launch_inertia(c, 0, false, c->side);
+    const int ri = launch_inertia(c, 0, c->side);
     if (pred_queued) {
       std::swap(c->ux, c->ux_alt);
       std::swap(c->uy, c->uy_alt);
     }
     TRY(ri);
     EIF_HIP(hipEventRecord(c->ev_i0, c->side));
-    TRY(launch_collision(c, 0, false, c->side));
+    TRY(launch_collision(c, 0, c->side));
     if (!pred_queued) TRY(launch_predictor(c, rk));  // writes the other ux, uy buffers
     EIF_HIP(hipStreamWaitEvent(c->stream, c->ev_i0, 0));
     // same position as inertia(0) (dx = 0 after the reset); the reset
@@ -749,17 +748,17 @@ int eif_gpu_integrate(eif_ctx *c, int substeps[3]) {
     // the last stage also reduces the advective dt limit of the final
     // velocities for the next param_decide_dt
     TRY(launch_correct_pressure(c, rk, true, true, rk == 2));
-    // the sub-iteration gate clears the pair counters collision(0) used
+    // the sub-iterations read the exchange sums and collision(0)
     EIF_HIP(hipStreamWaitEvent(c->stream, c->ev_c0, 0));
     int queued = 0, done = 0;
-    int batch = c->substep_guess > 0 ? c->substep_guess : 1;
+    int batch = std::min(std::max(c->substep_guess, 1), 102);
     bool early = false;  // predictor of stage rk+1 queued
     for (;;) {
       if (early) {  // the stage's own velocity fields for inertia(1)
         std::swap(c->ux, c->ux_alt);
         std::swap(c->uy, c->uy_alt);
       }
-      const int r = queue_substeps(c, rk, batch);
+      const int r = queue_substeps(c, rk, queued, batch);
       if (early) {
         std::swap(c->ux, c->ux_alt);
         std::swap(c->uy, c->uy_alt);
@@ -872,7 +871,18 @@ int eif_gpu_profile_reset(eif_ctx *c) {
   for (auto &kv : c->prof->recs) {
     kv.second.count = 0;
     kv.second.total_ms = 0.;
+    kv.second.seen = 0;
   }
+  return EIF_OK;
+}
+
+int eif_gpu_profile_sample(eif_ctx *c, int every) {
+  CHECK_CTX(c);
+  if (every < 1) {
+    set_error("every must be at least 1");
+    return EIF_ERR_ARG;
+  }
+  c->prof->every = every;
   return EIF_OK;
 }
 

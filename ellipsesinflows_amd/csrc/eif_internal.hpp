@@ -56,6 +56,7 @@ constexpr int TILE_CAP = 48;     // boxes registered per tile
 constexpr int CONTACT_CAP = 64;  // colliding partners per particle
 constexpr int EQC_ITER_CAP = 100000;  // guard of the uncapped fixed point
 constexpr int DT_PARTS_CAP = 4096;    // workgroups of the projection with fused dt
+constexpr int RES_SLOTS = 128;        // sub-iterations of one stage queued at most
 
 struct Profiler;
 
@@ -71,7 +72,7 @@ __host__ __device__ inline double qdiv(double x, double d, double inv) {
 struct HostStatus {
   double residual;  // of the last sub-iteration run
   double dt;        // of the current step
-  int substeps;     // sub-iterations run in this stage (gate[1])
+  int substeps;     // sub-iterations run in this stage
   int flags;        // device capacity flags
 };
 
@@ -85,7 +86,6 @@ struct Ctx {
   // fluid kernels, and the events that join it back
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_i0 = nullptr, ev_c0 = nullptr;
-  hipEvent_t ev_gate = nullptr, ev_coll = nullptr;
   hipEvent_t ev_x = nullptr;    // exchange done (main), for its finalize on the side stream
   hipEvent_t ev_sub = nullptr;  // sub-iteration batch and its status record done (main)
   int device = 0;
@@ -153,19 +153,13 @@ struct Ctx {
   int *flags = nullptr;    // device error flags
   unsigned long long *ibm_stats = nullptr;  // [8] measurement counters
   unsigned long long *ibm_pp = nullptr;     // np x 4 IBM work counters, then calls
-  int *gate = nullptr;   // [0] sub-iteration open, [1] sub-iterations run
+  double *res_slots = nullptr;  // RES_SLOTS: residual of each sub-iteration of the stage
   HostStatus *hstat = nullptr;  // pinned, written by k_status
   int substep_guess = 3; // sub-iterations queued before the first check
   // collisions
-  int *pair_cnt = nullptr;  // [0] colliding pairs, [1] candidate pairs
-  int pair_cap = 0;
-  int *cand = nullptr;      // cand_cap x 2 candidate pairs
-  int cand_cap = 0;
-  int *pair_ids = nullptr;        // pair_cap x 2
-  double *pair_val = nullptr;     // pair_cap x 6 contributions
-  double *wall_val = nullptr;     // np x 2 walls x 3
-  int *wall_hit = nullptr;        // np x 2
-  int last_npairs = 0;
+  int *rows = nullptr;      // np x CONTACT_CAP colliding partners, ascending
+  int *row_cnt = nullptr;   // np
+  int *wall_hit = nullptr;  // np x 2
   // statistics accumulators (allocated on first use), reference layouts
   double *st_ux1 = nullptr, *st_ux2 = nullptr;                   // like ux
   double *st_uy1 = nullptr, *st_uy2 = nullptr, *st_phi = nullptr;  // like uy
@@ -184,17 +178,23 @@ int hip_fail(hipError_t e, const char *what, const char *file, int line);
   } while (0)
 
 // profiling helpers (host)
-void prof_begin(Ctx *c, const char *name, hipStream_t st);
+// prof_begin: whether this launch is timed (then prof_end closes it)
+bool prof_begin(Ctx *c, const char *name, hipStream_t st);
 void prof_end(Ctx *c, const char *name, hipStream_t st);
 struct ProfScope {  // events on the stream the scoped kernels run on
   Ctx *c;
   const char *n;
   hipStream_t st;
-  ProfScope(Ctx *c_, const char *n_) : c(c_), n(n_), st(c_->stream) { prof_begin(c, n, st); }
-  ProfScope(Ctx *c_, const char *n_, hipStream_t s_) : c(c_), n(n_), st(s_) {
-    prof_begin(c, n, st);
+  bool timed;
+  ProfScope(Ctx *c_, const char *n_) : c(c_), n(n_), st(c_->stream) {
+    timed = prof_begin(c, n, st);
   }
-  ~ProfScope() { prof_end(c, n, st); }
+  ProfScope(Ctx *c_, const char *n_, hipStream_t s_) : c(c_), n(n_), st(s_) {
+    timed = prof_begin(c, n, st);
+  }
+  ~ProfScope() {
+    if (timed) prof_end(c, n, st);
+  }
 };
 
 // launchers implemented in the .hip files
@@ -212,8 +212,7 @@ int launch_poisson(Ctx *c, int rk);
 int susp_init(Ctx *c);
 // defer_finalize: the slice sums are added by the next launch_increment
 // with fin_inertia (one launch fewer per sub-iteration)
-int launch_inertia(Ctx *c, int cnstep, bool gated = false, hipStream_t st = nullptr,
-                   bool defer_finalize = false);
+int launch_inertia(Ctx *c, int cnstep, hipStream_t st = nullptr, bool defer_finalize = false);
 // cached_known: the distance cache holds the exchange position (it follows
 // inertia(0) of the same stage), so the distance pass is skipped
 // tiles_clear: the tile registrations were cleared by launch_reset(clear_tiles);
@@ -223,13 +222,16 @@ int launch_exchange(Ctx *c, bool cached_known = false, bool tiles_clear = false,
 int launch_exchange_finalize(Ctx *c, hipStream_t st);
 int launch_update_momentum(Ctx *c);
 int launch_materialise_du(Ctx *c);
-int launch_collision(Ctx *c, int cnstep, bool gated = false, hipStream_t st = nullptr);
-int launch_increment(Ctx *c, int rk, bool gated = false, bool fin_inertia = false);
-int launch_substep_gate(Ctx *c);
+int launch_collision(Ctx *c, int cnstep, hipStream_t st = nullptr);
+// sub >= 0: sub-iteration `sub` of the stage, gated on the residual slots;
+// it also finalises the preceding launch_substep_forces
+int launch_increment(Ctx *c, int rk, int sub = -1);
+// inertia(1) and collision(1) of sub-iteration `sub`, gated, in one launch
+int launch_substep_forces(Ctx *c, int sub);
 int launch_update_particles(Ctx *c, hipStream_t st = nullptr);
 // clear_tiles: also clear the IBM tile registrations for the next exchange
 int launch_reset(Ctx *c, hipStream_t st = nullptr, bool clear_tiles = false);
-int launch_status(Ctx *c);
+int launch_status(Ctx *c, int queued);
 int check_flags(Ctx *c);
 int flags_error(int h);
 int launch_statistics(Ctx *c);

@@ -372,6 +372,16 @@ constexpr int IBM_THREADS = 256;
 constexpr int IBM_SLICE = 256;  // box cells per wave in the inertia / exchange kernels
 constexpr int SLICE_VALS = 5;   // per-slice partials: 3 sums, FP64 and FP32 iterations
 
+// Sub-iteration `sub` of a stage's speculatively queued Crank-Nicolson
+// loop (src/main.c:37-50) runs unless the previous one converged (residual
+// below 1e-8) or the stage has already run 101 of them.  res[s] is the
+// residual of sub-iteration s, cleared at the stage start, so a sub-iteration
+// that did not run leaves 0 and closes every later one.  res == nullptr:
+// not gated.
+__device__ inline bool sub_open(const double *res, int sub) {
+  return !res || sub == 0 || (!(res[sub - 1] < 1.e-8) && sub <= 101);
+}
+
 struct IbmArgs {
   Layout L;
   int nx, ny;
@@ -393,7 +403,8 @@ struct IbmArgs {
   const int *slice_first;  // np+1: first slice of each particle
   const int *slice_owner;  // particle of each slice
   long *slice_info;        // np x 2: box cells, cells evaluated (this call)
-  const int *gate;         // speculative sub-iteration gate (null: always run)
+  const double *res;       // sub-iteration residuals (null: not gated)
+  int sub;                 // sub-iteration index
   double *slice_part;      // per-slice partial sums (3 each)
 };
 
@@ -494,9 +505,7 @@ __device__ inline void cell_of(const BoxSet &bs, int t, int &k, int &i, int &j) 
 // src/suspensions/inertia.c:9-70.  One wave per slice of IBM_SLICE box
 // cells of one particle (its three images' boxes flattened); the last
 // slice of a particle to finish adds the slices' sums in slice order.
-__global__ __launch_bounds__(64) void k_inertia(IbmArgs a, int cnstep) {
-  if (a.gate && !a.gate[0]) return;
-  const int slice = blockIdx.x;
+__device__ __forceinline__ void inertia_slice(const IbmArgs &a, int cnstep, int slice) {
   const int n = a.slice_owner[slice];
   const int s0 = a.slice_first[n], nsl = a.slice_first[n + 1] - s0;
   double *q = a.prt + (long)n * R_N;
@@ -556,6 +565,11 @@ __global__ __launch_bounds__(64) void k_inertia(IbmArgs a, int cnstep) {
   }
 }
 
+__global__ __launch_bounds__(64) void k_inertia(IbmArgs a, int cnstep) {
+  if (!sub_open(a.res, a.sub)) return;
+  inertia_slice(a, cnstep, blockIdx.x);
+}
+
 // Per-particle totals of the sliced inertia / exchange kernels, added in
 // slice order; also records the position whose distances are now cached
 // and the work counters.  which: 0..1 inertia (cnstep), 2 exchange.
@@ -596,7 +610,7 @@ __device__ void finalize_particle(const IbmArgs &a, int n, int which) {
 }
 
 __global__ void k_ibm_finalize(IbmArgs a, int which) {
-  if (a.gate && !a.gate[0]) return;
+  if (!sub_open(a.res, a.sub)) return;
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n < a.np) finalize_particle(a, n, which);
 }
@@ -673,7 +687,7 @@ __device__ inline void register_tile(const IbmArgs &a, int tile, int box_id) {
 // already hold them; sliced like k_inertia.  Used before k_exchange when
 // the cache may be stale.
 __global__ __launch_bounds__(64) void k_ibm_dist(IbmArgs a, int with_dx) {
-  if (a.gate && !a.gate[0]) return;
+  if (!sub_open(a.res, a.sub)) return;
   const int slice = blockIdx.x;
   const int n = a.slice_owner[slice];
   const int s0 = a.slice_first[n], nsl = a.slice_first[n + 1] - s0;
@@ -975,19 +989,15 @@ __device__ void evolute_fit_fast(const EllD &e, const EllipseShape &s,
 }
 
 struct CollArgs {
-  const int *gate;  // speculative sub-iteration gate (null: always run)
-  int *cand;     // candidate pairs (n0, n1)
-  int cand_cap;
+  const double *res;  // sub-iteration residuals (null: not gated)
+  int sub;            // sub-iteration index
   int np;
   double lx, ly;
-  const double *prt;
+  double *prt;
   int cnstep;
-  int *pair_cnt;
-  int pair_cap;
-  int *pair_ids;
-  double *pair_val;
-  double *wall_val;
-  int *wall_hit;
+  int *rows;       // np x CONTACT_CAP: colliding partners of each particle, ascending
+  int *row_cnt;    // np
+  int *wall_hit;   // np x 2
   int *flags;
 };
 
@@ -995,11 +1005,6 @@ __device__ inline double harmonic(double v0, double v1) {
   const double r = 0.5 * (1. / v0 + 1. / v1);
   return 1. / r;
 }
-
-// Candidate pairs (n0 < n1) whose bounding circles (radius max(a, b)) of
-// the minimum image overlap, the early exit of collision.c:156-189; tiles
-// of 64 x 64 pairs read their particles once into LDS.
-constexpr int COLL_TILE = 64;
 
 __device__ inline double min_image_offset(double p0y, double p1y, double ly) {
   double yoffset = 0., minval = DBL_MAX;
@@ -1013,82 +1018,95 @@ __device__ inline double min_image_offset(double p0y, double p1y, double ly) {
   return yoffset;
 }
 
-__global__ __launch_bounds__(256) void k_collide_search(CollArgs a) {
-  if (a.gate && !a.gate[0]) return;
-  __shared__ double sx[2][COLL_TILE], sy[2][COLL_TILE], sr[2][COLL_TILE];
-  const int nt = (a.np + COLL_TILE - 1) / COLL_TILE;
-  int b = blockIdx.x, tb0 = 0;
-  while (b >= nt - tb0) {
-    b -= nt - tb0;
-    tb0++;
-  }
-  const int tb1 = tb0 + b;
-  for (int e = threadIdx.x; e < 2 * COLL_TILE; e += blockDim.x) {
-    const int side = e / COLL_TILE, m = e % COLL_TILE;
-    const int n = (side ? tb1 : tb0) * COLL_TILE + m;
-    if (n < a.np) {
-      const double *p = a.prt + (long)n * R_N;
-      sx[side][m] = p[R_X] + p[R_DX];
-      sy[side][m] = p[R_Y] + p[R_DY];
-      sr[side][m] = fmax(p[R_A], p[R_B]);
-    }
-  }
-  __syncthreads();
-  for (int e = threadIdx.x; e < COLL_TILE * COLL_TILE; e += blockDim.x) {
-    const int m0 = e / COLL_TILE, m1 = e % COLL_TILE;
-    const int n0 = tb0 * COLL_TILE + m0, n1 = tb1 * COLL_TILE + m1;
-    if (n1 <= n0 || n1 >= a.np) continue;
-    const double y0 = sy[0][m0];
-    const double y1 = sy[1][m1] + min_image_offset(y0, sy[1][m1], a.ly);
-    const double norm = fmax(hypot(sx[1][m1] - sx[0][m0], y1 - y0), DBL_EPSILON);
-    if (sr[0][m0] + sr[1][m1] - norm < 0.) continue;
-    const int slot = atomicAdd(&a.pair_cnt[1], 1);
-    if (slot < a.cand_cap) {
-      a.cand[2 * slot] = n0;
-      a.cand[2 * slot + 1] = n1;
-    } else {
-      atomicOr(&a.flags[0], 4);
-    }
-  }
+// The early exit of collision.c:156-189 for the pair n0 < n1: the bounding
+// circles (radius max(a, b)) of the minimum image overlap
+__device__ inline bool pair_candidate(const CollArgs &a, int n0, int n1) {
+  const double *p0 = a.prt + (long)n0 * R_N, *p1 = a.prt + (long)n1 * R_N;
+  const double x0 = p0[R_X] + p0[R_DX], y0 = p0[R_Y] + p0[R_DY];
+  const double x1 = p1[R_X] + p1[R_DX], y1r = p1[R_Y] + p1[R_DY];
+  const double y1 = y1r + min_image_offset(y0, y1r, a.ly);
+  const double norm = fmax(hypot(x1 - x0, y1 - y0), DBL_EPSILON);
+  return !(fmax(p0[R_A], p0[R_B]) + fmax(p1[R_A], p1[R_B]) - norm < 0.);
 }
 
-// collision.c:156-247 for one candidate pair per thread
-__device__ void collide_pair(const CollArgs &a, int t) {
-  const int ncand = min(a.pair_cnt[1], a.cand_cap);
-  if (t >= ncand) return;
-  const int n0 = a.cand[2 * t], n1 = a.cand[2 * t + 1];
-  const double *p0 = a.prt + (long)n0 * R_N, *p1 = a.prt + (long)n1 * R_N;
-  const double yoffset = min_image_offset(p0[R_Y] + p0[R_DY], p1[R_Y] + p1[R_DY], a.ly);
-  const EllD e0 = {p0[R_A], p0[R_B], p0[R_X] + p0[R_DX], p0[R_Y] + p0[R_DY], p0[R_AZ]};
-  const EllD e1 = {p1[R_A], p1[R_B], p1[R_X] + p1[R_DX], p1[R_Y] + p1[R_DY] + yoffset,
-                   p1[R_AZ]};
+// One contact of collision.c:156-304, by the two lanes l and l^1.
+// A pair n0 < n1 (wall < 0): the lane of side 0 fits ellipse n0 and the
+// lane of side 1 ellipse n1, each towards the other's current equivalent-
+// circle centre, exchanged every iteration (the reference evaluates both
+// fits from the previous centres too).  A wall (wall = 0: x = 0, 1: x = lx)
+// of particle n0: both lanes fit n0 towards its mirror image.  Returns
+// whether the circles overlap, with the contributions to n0 in v[0..2] and,
+// for a pair, to n1 in v[3..5]; both lanes return the same values.
+__device__ bool contact_force(const CollArgs &a, int n0, int n1, int wall, int side,
+                              double v[6]) {
+  const double wallx = wall == 0 ? 0. : a.lx;
+  const double *p0 = a.prt + (long)n0 * R_N;
+  const double *p1 = wall < 0 ? a.prt + (long)n1 * R_N : p0;
+  const double *pm = side && wall < 0 ? p1 : p0;
+  double yoffset = 0.;
+  if (wall < 0) {
+    yoffset = min_image_offset(p0[R_Y] + p0[R_DY], p1[R_Y] + p1[R_DY], a.ly);
+  } else {  // wall early exit, collision.c:259-266
+    const double x = p0[R_X] + p0[R_DX];
+    const double r = fmax(p0[R_A], p0[R_B]);
+    const double norm = fabs(wallx - x);
+    if (r - norm < 0.) return false;
+  }
+  const double ym = pm[R_Y] + pm[R_DY];
+  const EllD e = {pm[R_A], pm[R_B], pm[R_X] + pm[R_DX],
+                  side && wall < 0 ? ym + yoffset : ym, pm[R_AZ]};
   // find_equivalent_circles, collision.c:111-135 (guarded iteration count)
-  const EllipseShape s0 = make_shape(e0.a, e0.b), s1 = make_shape(e1.a, e1.b);
-  const EllipseShape32 f0 = shape32(s0), f1 = shape32(s1);
-  double cm0, sm0, cp0, sp0, cm1, sm1, cp1, sp1;
-  sincos(-e0.angle, &sm0, &cm0);
-  sincos(+e0.angle, &sp0, &cp0);
-  sincos(-e1.angle, &sm1, &cm1);
-  sincos(+e1.angle, &sp1, &cp1);
-  double tw0 = -1., tw1 = -1.;
-  double c0x = e0.x, c0y = e0.y, c1x = e1.x, c1y = e1.y, c0r = 0., c1r = 0.;
-  double ix0 = 0., iy0 = 0., ix1 = 0., iy1 = 0.;
+  const EllipseShape sh = make_shape(e.a, e.b);
+  const EllipseShape32 f = shape32(sh);
+  double cm, sm, cp, sp;
+  sincos(-e.angle, &sm, &cm);
+  sincos(+e.angle, &sp, &cp);
+  double tw = -1.;
+  double cx = e.x, cy = e.y, cr = 0., ix = 0., iy = 0.;
   for (int it = 0;; it++) {
-    double n0x, n0y, n1x, n1y;
-    evolute_fit_fast(e0, s0, f0, cm0, sm0, cp0, sp0, c1x, c1y, n0x, n0y, ix0, iy0, c0r, tw0);
-    evolute_fit_fast(e1, s1, f1, cm1, sm1, cp1, sp1, c0x, c0y, n1x, n1y, ix1, iy1, c1r, tw1);
-    const double d0x = n0x - c0x, d0y = n0y - c0y;
-    const double d1x = n1x - c1x, d1y = n1y - c1y;
-    c0x += d0x;
-    c0y += d0y;
-    c1x += d1x;
-    c1y += d1y;
-    if (fmax(fmax(fabs(d0x), fabs(d0y)), fmax(fabs(d1x), fabs(d1y))) < 1.e-8) break;
+    const double ox = __shfl_xor(cx, 1), oy = __shfl_xor(cy, 1);
+    const double tx = wall < 0 ? ox : 2. * wallx - cx, ty = wall < 0 ? oy : cy;
+    double nx, ny;
+    evolute_fit_fast(e, sh, f, cm, sm, cp, sp, tx, ty, nx, ny, ix, iy, cr, tw);
+    const double dx = nx - cx, dy = ny - cy;
+    cx += dx;
+    cy += dy;
+    const double dm = fmax(fabs(dx), fabs(dy));
+    const double dmo = __shfl_xor(dm, 1);
+    const double dmax = wall >= 0 ? dm : side ? fmax(dmo, dm) : fmax(dm, dmo);
+    if (dmax < 1.e-8) break;
     if (it >= EQC_ITER_CAP) {
-      atomicOr(&a.flags[0], 8);
+      if (side == 0) atomicOr(&a.flags[0], 8);
       break;
     }
   }
+  const double ocx = __shfl_xor(cx, 1), ocy = __shfl_xor(cy, 1), ocr = __shfl_xor(cr, 1);
+  const double oix = __shfl_xor(ix, 1), oiy = __shfl_xor(iy, 1);
+  if (wall >= 0) {  // collision.c:268-304
+    const double m = p_mass(p0), im = p_moi(p0);
+    double k;
+    {
+      const double mass = m;
+      const double r = sqrt(p0[R_A] * p0[R_B]);
+      const double reft = pow(r, 1.5);
+      k = mass * (EIF_PI * EIF_PI) / (reft * reft);
+    }
+    double nxv = wallx - cx;
+    const double norm = fabs(nxv);
+    nxv /= norm;
+    const double overlap = cr - norm;
+    if (!(overlap > 0.)) return false;
+    const double cfx = k * overlap * nxv;
+    const double cfy = 0.;
+    v[0] = 1. / m * (-cfx);
+    v[1] = 1. / m * (-cfy);
+    v[2] = 1. / im * (ix * (-cfy) - iy * (-cfx));
+    return true;
+  }
+  const double c0x = side ? ocx : cx, c0y = side ? ocy : cy, c0r = side ? ocr : cr;
+  const double c1x = side ? cx : ocx, c1y = side ? cy : ocy, c1r = side ? cr : ocr;
+  const double ix0 = side ? oix : ix, iy0 = side ? oiy : iy;
+  const double ix1 = side ? ix : oix, iy1 = side ? iy : oiy;
   const double m0 = p_mass(p0), m1 = p_mass(p1);
   const double i0 = p_moi(p0), i1 = p_moi(p1);
   double k;
@@ -1104,140 +1122,109 @@ __device__ void collide_pair(const CollArgs &a, int t) {
   nxv /= norm;
   nyv /= norm;
   const double overlap = c0r + c1r - norm;
-  if (overlap > 0.) {
-    const double cfx = k * overlap * nxv;
-    const double cfy = k * overlap * nyv;
-    const int slot = atomicAdd(&a.pair_cnt[0], 1);
-    if (slot >= a.pair_cap) {
-      atomicOr(&a.flags[0], 4);
-      return;
-    }
-    a.pair_ids[2 * slot] = n0;
-    a.pair_ids[2 * slot + 1] = n1;
-    double *v = a.pair_val + 6 * (long)slot;
-    v[0] = 1. / m0 * (-cfx);
-    v[1] = 1. / m0 * (-cfy);
-    v[2] = 1. / i0 * (ix0 * (-cfy) - iy0 * (-cfx));
-    v[3] = 1. / m1 * (+cfx);
-    v[4] = 1. / m1 * (+cfy);
-    v[5] = 1. / i1 * (ix1 * (+cfy) - iy1 * (+cfx));
-  }
+  if (!(overlap > 0.)) return false;
+  const double cfx = k * overlap * nxv;
+  const double cfy = k * overlap * nyv;
+  v[0] = 1. / m0 * (-cfx);
+  v[1] = 1. / m0 * (-cfy);
+  v[2] = 1. / i0 * (ix0 * (-cfy) - iy0 * (-cfx));
+  v[3] = 1. / m1 * (+cfx);
+  v[4] = 1. / m1 * (+cfy);
+  v[5] = 1. / i1 * (ix1 * (+cfy) - iy1 * (+cfx));
+  return true;
 }
 
-// collision.c:249-304, thread t: particle t/2, wall t%2
-__device__ void collide_wall(const CollArgs &a, int t) {
-  if (t >= 2 * a.np) return;
-  const int n = t >> 1, w = t & 1;
-  const double wallx = w == 0 ? 0. : a.lx;
-  const double *p = a.prt + (long)n * R_N;
-  a.wall_hit[t] = 0;
-  {
-    const double x = p[R_X] + p[R_DX];
-    const double r = fmax(p[R_A], p[R_B]);
-    const double norm = fabs(wallx - x);
-    if (r - norm < 0.) return;
-  }
-  const EllD e = {p[R_A], p[R_B], p[R_X] + p[R_DX], p[R_Y] + p[R_DY], p[R_AZ]};
-  const EllipseShape sh = make_shape(e.a, e.b);
-  const EllipseShape32 f = shape32(sh);
-  double cm, sm, cp, sp;
-  sincos(-e.angle, &sm, &cm);
-  sincos(+e.angle, &sp, &cp);
-  double tw = -1.;
-  double cx = e.x, cy = e.y, cr = 0., ix = 0., iy = 0.;
-  for (int it = 0;; it++) {
-    double nxc, nyc;
-    const double mirrorx = 2. * wallx - cx;
-    evolute_fit_fast(e, sh, f, cm, sm, cp, sp, mirrorx, cy, nxc, nyc, ix, iy, cr, tw);
-    const double ddx = nxc - cx, ddy = nyc - cy;
-    cx += ddx;
-    cy += ddy;
-    if (fmax(fabs(ddx), fabs(ddy)) < 1.e-8) break;
-    if (it >= EQC_ITER_CAP) {
-      atomicOr(&a.flags[0], 8);
-      break;
-    }
-  }
-  const double m = p_mass(p), im = p_moi(p);
-  double k;
-  {
-    const double mass = m;
-    const double r = sqrt(p[R_A] * p[R_B]);
-    const double reft = pow(r, 1.5);
-    k = mass * (EIF_PI * EIF_PI) / (reft * reft);
-  }
-  double nxv = wallx - cx;
-  const double norm = fabs(nxv);
-  nxv /= norm;
-  const double overlap = cr - norm;
-  if (overlap > 0.) {
-    const double cfx = k * overlap * nxv;
-    const double cfy = 0.;
-    double *v = a.wall_val + 3 * (long)t;
-    v[0] = 1. / m * (-cfx);
-    v[1] = 1. / m * (-cfy);
-    v[2] = 1. / im * (ix * (-cfy) - iy * (-cfx));
-    a.wall_hit[t] = 1;
-  }
-}
+// The collision force of particle n (collision.c:306-367), by one wave:
+// the lanes test all partners, then fit the candidate pairs and the two
+// walls in parallel, two lanes per contact (each pair is fitted by both of
+// its particles' waves, with the same operands), and the contributions are
+// added in the order of the reference's loops: partners in ascending
+// index, then wall x = 0, then wall x = lx.
+constexpr int COLL_CAND_CAP = 256;  // candidate partners per particle
 
-// candidate pairs in the first pair_blocks workgroups, particle-wall pairs
-// in the rest
-__global__ void k_collide_pairs_walls(CollArgs a, int pair_blocks) {
-  if (a.gate && !a.gate[0]) return;
-  if ((int)blockIdx.x < pair_blocks)
-    collide_pair(a, blockIdx.x * blockDim.x + threadIdx.x);
-  else
-    collide_wall(a, (blockIdx.x - pair_blocks) * blockDim.x + threadIdx.x);
-}
-
-// per particle: partners in ascending index (the order of the reference's
-// flattened pair loop), then wall x=0, then wall x=lx
-__global__ void k_collide_gather(CollArgs a) {
-  if (a.gate && !a.gate[0]) return;
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= a.np) return;
-  int partner[CONTACT_CAP];
-  int slot[CONTACT_CAP];
+__device__ void collide_particle(const CollArgs &a, int n) {
+  __shared__ int s_cand[COLL_CAND_CAP];
+  const int lane = threadIdx.x;
   int cnt = 0;
-  const int npairs = min(a.pair_cnt[0], a.pair_cap);
-  for (int s = 0; s < npairs; s++) {
-    const int n0 = a.pair_ids[2 * s], n1 = a.pair_ids[2 * s + 1];
-    if (n0 != m && n1 != m) continue;
-    if (cnt >= CONTACT_CAP) {
-      atomicOr(&a.flags[0], 16);
-      break;
+  for (int base = 0; base < a.np; base += 64) {
+    const int m = base + lane;
+    const bool c = m < a.np && m != n && pair_candidate(a, min(n, m), max(n, m));
+    const unsigned long long b = __ballot(c);
+    if (c) {
+      const int pos = cnt + __popcll(b & ((1ull << lane) - 1ull));
+      if (pos < COLL_CAND_CAP) s_cand[pos] = m;
     }
-    const int other = n0 == m ? n1 : n0;
-    int r = cnt - 1;
-    while (r >= 0 && partner[r] > other) {
-      partner[r + 1] = partner[r];
-      slot[r + 1] = slot[r];
-      r--;
-    }
-    partner[r + 1] = other;
-    slot[r + 1] = s;
-    cnt++;
+    cnt += __popcll(b);
   }
+  if (cnt > COLL_CAND_CAP) {
+    if (lane == 0) atomicOr(&a.flags[0], 4);
+    cnt = COLL_CAND_CAP;
+  }
+  __syncthreads();
   double fx = 0., fy = 0., tz = 0.;
-  for (int r = 0; r < cnt; r++) {
-    const int s = slot[r];
-    const double *v = a.pair_val + 6 * (long)s + (a.pair_ids[2 * s] == m ? 0 : 3);
-    fx += v[0];
-    fy += v[1];
-    tz += v[2];
-  }
-  for (int w = 0; w < 2; w++)
-    if (a.wall_hit[2 * m + w]) {
-      const double *v = a.wall_val + 3 * (long)(2 * m + w);
-      fx += v[0];
-      fy += v[1];
-      tz += v[2];
+  int ncont = 0;
+  int *row = a.rows + (long)n * CONTACT_CAP;
+  const int njobs = cnt + 2;  // the candidates, then the two walls
+  for (int r = 0; r < njobs; r += 32) {
+    const int k = r + (lane >> 1), side = lane & 1;
+    int m = -1;
+    bool hit = false;
+    double w0 = 0., w1 = 0., w2 = 0.;
+    if (k < njobs) {
+      const int wall = k < cnt ? -1 : k - cnt;
+      if (wall < 0) m = s_cand[k];
+      double v[6] = {0., 0., 0., 0., 0., 0.};
+      hit = contact_force(a, wall < 0 ? min(n, m) : n, wall < 0 ? max(n, m) : n, wall, side, v);
+      const bool first = wall >= 0 || n < m;  // n's own contributions
+      w0 = first ? v[0] : v[3];
+      w1 = first ? v[1] : v[4];
+      w2 = first ? v[2] : v[5];
+      if (wall >= 0 && side == 0) a.wall_hit[2 * (long)n + wall] = hit ? 1 : 0;
     }
-  double *q = const_cast<double *>(a.prt) + (long)m * R_N;
-  q[R_CFX + a.cnstep] = fx;
-  q[R_CFY + a.cnstep] = fy;
-  q[R_CTZ + a.cnstep] = tz;
+    unsigned long long hb = __ballot(hit && side == 0);
+    while (hb) {
+      const int l = __ffsll((long long)hb) - 1;
+      hb &= hb - 1ull;
+      const double u0 = __shfl(w0, l), u1 = __shfl(w1, l), u2 = __shfl(w2, l);
+      const int mm = __shfl(m, l);
+      if (mm >= 0) {  // a partner
+        if (ncont >= CONTACT_CAP) {
+          if (lane == 0) atomicOr(&a.flags[0], 16);
+          continue;
+        }
+        if (lane == 0) row[ncont] = mm;
+        ncont++;
+      }
+      fx += u0;
+      fy += u1;
+      tz += u2;
+    }
+  }
+  if (lane == 0) {
+    a.row_cnt[n] = ncont;
+    double *q = a.prt + (long)n * R_N;
+    q[R_CFX + a.cnstep] = fx;
+    q[R_CFY + a.cnstep] = fy;
+    q[R_CTZ + a.cnstep] = tz;
+  }
+}
+
+// one 64-lane workgroup per particle
+__global__ __launch_bounds__(64) void k_collide(CollArgs a) {
+  if (!sub_open(a.res, a.sub)) return;
+  collide_particle(a, blockIdx.x);
+}
+
+// One launch for a sub-iteration's inertia(1) and collision(1), which read
+// the same positions and write disjoint particle members: the first np
+// workgroups are the collisions (long, started first), the rest the IBM
+// slices.
+__global__ __launch_bounds__(64) void k_inertia_collide(IbmArgs a, CollArgs ca, int cnstep) {
+  if (!sub_open(a.res, a.sub)) return;
+  if ((int)blockIdx.x < ca.np)
+    collide_particle(ca, blockIdx.x);
+  else
+    inertia_slice(a, cnstep, blockIdx.x - ca.np);
 }
 
 __global__ void k_zero_int(int *p, int n) {
@@ -1245,25 +1232,6 @@ __global__ void k_zero_int(int *p, int n) {
   if (t < n) p[t] = 0;
 }
 
-// Gate of one speculatively queued Crank-Nicolson sub-iteration
-// (src/main.c:37-50): the loop ends after a sub-iteration whose residual
-// is below 1e-8, or after sub-iteration 101; a closed gate makes every
-// kernel of the later sub-iterations return at once.  An open gate also
-// clears what the sub-iteration accumulates into (residual, pair counts)
-// and counts the sub-iteration.
-__global__ void k_substep_gate(double *dscal, int *gate, int *pair_cnt) {
-  const int s = gate[1];
-  const bool open = s == 0 || (!(dscal[S_RESIDUAL] < 1.e-8) && !(s - 1 > 100));
-  gate[0] = open ? 1 : 0;
-  if (open) {
-    gate[1] = s + 1;
-    dscal[S_RESIDUAL] = 0.;
-    if (pair_cnt) {
-      pair_cnt[0] = 0;
-      pair_cnt[1] = 0;
-    }
-  }
-}
 
 // ---------------------------------------------------------------------
 // particle increments / update / reset
@@ -1279,9 +1247,8 @@ __global__ void k_zero_residual(double *dscal) { dscal[S_RESIDUAL] = 0.; }
 // fin: also finalises the preceding inertia(1) call (its slice sums and
 // cache records) for this particle first, saving a launch per sub-iteration
 __global__ void k_increment(double *prt, int np, const double *dscal, double ga,
-                            double Fr, double *res_out, const int *gate, IbmArgs ia,
-                            int fin) {
-  if (gate && !gate[0]) return;
+                            double Fr, double *res_out, IbmArgs ia, int fin) {
+  if (!sub_open(ia.res, ia.sub)) return;
   __shared__ double red[256];
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   double res = 0.;
@@ -1336,15 +1303,13 @@ __global__ void k_update_particles(double *prt, int np, double ly) {
 }
 
 // src/suspensions/reset.c:7-18
-// tile_cnt (ntiles) and active[0]: cleared when given, ready for the next
-// exchange's tile registrations
-__global__ void k_reset(double *prt, int np, int *gate, int *sc_key, int *tile_cnt,
+// res (RES_SLOTS sub-iteration residuals), tile_cnt (ntiles) and
+// active[0]: cleared when given, for the stage's sub-iterations and the
+// next exchange's tile registrations
+__global__ void k_reset(double *prt, int np, double *res, int *sc_key, int *tile_cnt,
                         int ntiles, int *active) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n == 0 && gate) {
-    gate[0] = 1;
-    gate[1] = 0;
-  }
+  if (res && n < RES_SLOTS) res[n] = 0.;
   if (tile_cnt) {
     if (n < ntiles) tile_cnt[n] = 0;
     if (n == 0) active[0] = 0;
@@ -1399,7 +1364,8 @@ static IbmArgs ibm_args(Ctx *c) {
   a.slice_first = c->slice_first;
   a.slice_owner = c->slice_owner;
   a.slice_info = c->slice_info;
-  a.gate = nullptr;
+  a.res = nullptr;
+  a.sub = 0;
   a.slice_part = c->slice_part;
   return a;
 }
@@ -1468,14 +1434,12 @@ int susp_init(Ctx *c) {
   EIF_HIP(hipMalloc(&c->active, (ntiles + 1) * sizeof(int)));
   EIF_HIP(hipMemset(c->active, 0, (ntiles + 1) * sizeof(int)));
   // collisions
-  c->pair_cap = c->np * 8 + 64;
-  c->cand_cap = c->np * 16 + 64;
-  EIF_HIP(hipMalloc(&c->pair_cnt, 2 * sizeof(int)));
-  EIF_HIP(hipMemset(c->pair_cnt, 0, 2 * sizeof(int)));
-  EIF_HIP(hipMalloc(&c->cand, 2 * (size_t)c->cand_cap * sizeof(int)));
-  EIF_HIP(hipMalloc(&c->pair_ids, 2 * (size_t)c->pair_cap * sizeof(int)));
-  EIF_HIP(hipMalloc(&c->pair_val, 6 * (size_t)c->pair_cap * sizeof(double)));
-  EIF_HIP(hipMalloc(&c->wall_val, 6 * (size_t)(c->np > 0 ? c->np : 1) * sizeof(double)));
+  {
+    const size_t npp = (size_t)(c->np > 0 ? c->np : 1);
+    EIF_HIP(hipMalloc(&c->rows, npp * CONTACT_CAP * sizeof(int)));
+    EIF_HIP(hipMalloc(&c->row_cnt, npp * sizeof(int)));
+    EIF_HIP(hipMemset(c->row_cnt, 0, npp * sizeof(int)));
+  }
   EIF_HIP(hipMalloc(&c->wall_hit, 2 * (size_t)(c->np > 0 ? c->np : 1) * sizeof(int)));
   EIF_HIP(hipMemset(c->wall_hit, 0, 2 * (size_t)(c->np > 0 ? c->np : 1) * sizeof(int)));
   return EIF_OK;
@@ -1483,21 +1447,21 @@ int susp_init(Ctx *c) {
 
 int launch_reset(Ctx *c, hipStream_t st, bool clear_tiles) {
   if (!st) st = c->stream;
-  // also rewinds the sub-iteration gate, so it runs without particles too
+  // also clears the sub-iteration residuals, so it runs without particles too
   const int ntiles = c->np > 0 && clear_tiles ? c->ntx * c->nty : 0;
-  const int n = std::max(std::max(c->np, ntiles), 1);
-  hipLaunchKernelGGL(k_reset, dim3((n + 255) / 256), dim3(256), 0, st, c->prt, c->np, c->gate,
-                     c->sc_key, ntiles > 0 ? c->tile_cnt : nullptr, ntiles, c->active);
+  const int n = std::max(std::max(c->np, ntiles), RES_SLOTS);
+  hipLaunchKernelGGL(k_reset, dim3((n + 255) / 256), dim3(256), 0, st, c->prt, c->np,
+                     c->res_slots, c->sc_key, ntiles > 0 ? c->tile_cnt : nullptr, ntiles,
+                     c->active);
   EIF_HIP(hipGetLastError());
   return EIF_OK;
 }
 
-int launch_inertia(Ctx *c, int cnstep, bool gated, hipStream_t st, bool defer_finalize) {
+int launch_inertia(Ctx *c, int cnstep, hipStream_t st, bool defer_finalize) {
   if (c->np == 0) return EIF_OK;
   if (!st) st = c->stream;
   ProfScope ps(c, "ibm_inertia", st);
   IbmArgs a = ibm_args(c);
-  if (gated) a.gate = c->gate;
   hipLaunchKernelGGL(k_inertia, dim3(c->nslices), dim3(64), 0, st, a, cnstep);
   if (!defer_finalize)
     hipLaunchKernelGGL(k_ibm_finalize, dim3((c->np + 63) / 64), dim3(64), 0, st, a, cnstep);
@@ -1582,70 +1546,81 @@ int launch_materialise_du(Ctx *c) {
   return launch_bc_p(c, c->duy);
 }
 
-int launch_collision(Ctx *c, int cnstep, bool gated, hipStream_t st) {
-  if (c->np == 0) return EIF_OK;
-  if (!st) st = c->stream;
-  ProfScope ps(c, "collision", st);
+static CollArgs coll_args(Ctx *c, int cnstep) {
   CollArgs a;
+  a.res = nullptr;
+  a.sub = 0;
   a.np = c->np;
   a.lx = c->lx;
   a.ly = c->ly;
   a.prt = c->prt;
   a.cnstep = cnstep;
-  a.pair_cnt = c->pair_cnt;
-  a.pair_cap = c->pair_cap;
-  a.pair_ids = c->pair_ids;
-  a.pair_val = c->pair_val;
-  a.wall_val = c->wall_val;
+  a.rows = c->rows;
+  a.row_cnt = c->row_cnt;
   a.wall_hit = c->wall_hit;
   a.flags = c->flags;
-  a.cand = c->cand;
-  a.cand_cap = c->cand_cap;
-  a.gate = gated ? c->gate : nullptr;
-  if (!gated)  // an open gate has cleared the counts
-    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, c->pair_cnt, 2);
-  const int nt = (c->np + COLL_TILE - 1) / COLL_TILE;
-  hipLaunchKernelGGL(k_collide_search, dim3(nt * (nt + 1) / 2), dim3(256), 0, st, a);
-  const int pair_blocks = (c->cand_cap + 63) / 64, wall_blocks = (2 * c->np + 63) / 64;
-  hipLaunchKernelGGL(k_collide_pairs_walls, dim3(pair_blocks + wall_blocks), dim3(64), 0,
-                     st, a, pair_blocks);
-  hipLaunchKernelGGL(k_collide_gather, dim3((c->np + 127) / 128), dim3(128), 0, st,
-                     a);
+  return a;
+}
+
+int launch_collision(Ctx *c, int cnstep, hipStream_t st) {
+  if (c->np == 0) return EIF_OK;
+  if (!st) st = c->stream;
+  ProfScope ps(c, "collision", st);
+  hipLaunchKernelGGL(k_collide, dim3(c->np), dim3(64), 0, st, coll_args(c, cnstep));
   EIF_HIP(hipGetLastError());
   return EIF_OK;
 }
 
-int launch_increment(Ctx *c, int rk, bool gated, bool fin_inertia) {
-  if (!gated)  // an open gate has cleared the residual
+int launch_substep_forces(Ctx *c, int sub) {
+  if (c->np == 0) return EIF_OK;
+  ProfScope ps(c, "ibm_substep");
+  IbmArgs a = ibm_args(c);
+  CollArgs ca = coll_args(c, 1);
+  a.res = ca.res = c->res_slots;
+  a.sub = ca.sub = sub;
+  hipLaunchKernelGGL(k_inertia_collide, dim3(c->np + c->nslices), dim3(64), 0, c->stream, a,
+                     ca, 1);
+  EIF_HIP(hipGetLastError());
+  return EIF_OK;
+}
+
+int launch_increment(Ctx *c, int rk, int sub) {
+  IbmArgs ia = ibm_args(c);
+  double *res_out = c->dscal + S_RESIDUAL;
+  if (sub < 0) {  // not gated: the residual starts from zero
     hipLaunchKernelGGL(k_zero_residual, dim3(1), dim3(1), 0, c->stream, c->dscal);
+  } else {  // the residual slot was cleared at the stage start
+    ia.res = c->res_slots;
+    ia.sub = sub;
+    res_out = c->res_slots + sub;
+  }
   if (c->np > 0)
     hipLaunchKernelGGL(k_increment, dim3((c->np + 255) / 256), dim3(256), 0, c->stream,
-                       c->prt, c->np, c->dscal, c->gamma[rk], c->cfg.Fr,
-                       c->dscal + S_RESIDUAL, gated ? c->gate : nullptr, ibm_args(c),
-                       fin_inertia ? 1 : 0);
+                       c->prt, c->np, c->dscal, c->gamma[rk], c->cfg.Fr, res_out, ia,
+                       sub >= 0 ? 1 : 0);
   EIF_HIP(hipGetLastError());
   return EIF_OK;
 }
 
-// one thread copies the sub-iteration state into pinned host memory
-__global__ void k_status(const int *gate, const int *flags, const double *dscal,
-                         HostStatus *h) {
-  h->residual = dscal[S_RESIDUAL];
+// one thread copies the state of the first `queued` sub-iterations into
+// pinned host memory: how many ran, and the residual of the last one
+__global__ void k_status(const double *res, int queued, const int *flags,
+                         const double *dscal, HostStatus *h) {
+  int done = 0;
+  double r = 0.;
+  for (int s = 0; s < queued && sub_open(res, s); s++) {
+    done = s + 1;
+    r = res[s];
+  }
+  h->residual = r;
   h->dt = dscal[S_DT];
-  h->substeps = gate[1];
+  h->substeps = done;
   h->flags = flags[0];
 }
 
-int launch_status(Ctx *c) {
-  hipLaunchKernelGGL(k_status, dim3(1), dim3(1), 0, c->stream, c->gate, c->flags, c->dscal,
-                     c->hstat);
-  EIF_HIP(hipGetLastError());
-  return EIF_OK;
-}
-
-int launch_substep_gate(Ctx *c) {
-  hipLaunchKernelGGL(k_substep_gate, dim3(1), dim3(1), 0, c->stream, c->dscal, c->gate,
-                     c->np > 0 ? c->pair_cnt : nullptr);
+int launch_status(Ctx *c, int queued) {
+  hipLaunchKernelGGL(k_status, dim3(1), dim3(1), 0, c->stream, c->res_slots, queued,
+                     c->flags, c->dscal, c->hstat);
   EIF_HIP(hipGetLastError());
   return EIF_OK;
 }

@@ -186,12 +186,16 @@ int eif_gpu_ibm_stats(eif_ctx *ctx, unsigned long long out[5], int reset);
 int eif_gpu_ibm_reference_iterations(eif_ctx *ctx, unsigned long long out[2]);
 const char *eif_gpu_last_error(void);
 
-/* Per-kernel timing with HIP events on the context's stream.  Names are
- * the kernel names listed by eif_gpu_profile_names(). */
+/* Per-kernel timing with HIP events on the stream each kernel runs on.
+ * Names are those listed by eif_gpu_profile_names() ("ibm_substep": the
+ * launch of a sub-iteration's inertia(1) and collision(1)). */
 int eif_gpu_profile_enable(eif_ctx *ctx, int enable);
 /* restrict timing to one kernel name (NULL or "" times all of them) */
 int eif_gpu_profile_select(eif_ctx *ctx, const char *name);
 int eif_gpu_profile_reset(eif_ctx *ctx);
+/* time one launch in `every` of each kernel name (default 1: all); an
+ * event pair briefly idles the stream it is recorded on */
+int eif_gpu_profile_sample(eif_ctx *ctx, int every);
 int eif_gpu_profile_query(eif_ctx *ctx, const char *name, long *count,
                           double *total_ms);
 const char *eif_gpu_profile_names(void);
