@@ -685,28 +685,30 @@ static int queue_substeps(Ctx *c, int rk, int first, int n) {
   return EIF_OK;
 }
 
+// the velocity buffers swapped with their ping-pong partners (the stage's
+// own fields while the next stage's predictor is already queued)
+static void swap_fields(Ctx *c) {
+  std::swap(c->ux, c->ux_alt);
+  std::swap(c->uy, c->uy_alt);
+}
+
 int eif_gpu_integrate(eif_ctx *c, int substeps[3]) {
   CHECK_CTX(c);
   // Two streams.  Main: fluid kernels, exchange, deposit and the
   // sub-iterations (inertia(1) with collision(1) in one launch, then the
   // increment).  Side: inertia(0), collision(0) and the exchange's particle
   // sums, which read only the particles and the stage-start velocity, so
-  // they overlap the predictor and the pressure solve.  Events join the
-  // side work back before anything that reads its results or reuses its
-  // buffers.
+  // they overlap the fluid kernels.  Events join the side work back before
+  // anything that reads its results or reuses its buffers.
   //
   // The predictor of stage rk+1 reads only the fluid fields, which the
   // sub-iterations of stage rk leave alone.  It is queued right behind the
   // first batch of sub-iterations, so the GPU runs it while the host waits
-  // for that batch's count; the particle update of stage rk and the reset
-  // of stage rk+1 then go to the side stream.
+  // for that batch's count.  (Beside the sub-iterations, on the side
+  // stream, it slows them more than it gains.)
   bool pred_queued = false;  // this stage's predictor is already queued
   for (int rk = 0; rk < 3; rk++) {
-    if (pred_queued) {
-      EIF_HIP(hipStreamWaitEvent(c->side, c->ev_sub, 0));
-      TRY(launch_update_particles(c, c->side));
-      TRY(launch_reset(c, c->side, true));
-    } else {
+    if (!pred_queued) {
       TRY(launch_reset(c, c->stream, true));
       // src/main.c:16-19: the kernels of every stage keep the boundary and
       // halo values they change up to date, so this only runs after a
@@ -719,29 +721,35 @@ int eif_gpu_integrate(eif_ctx *c, int substeps[3]) {
       }
       EIF_HIP(hipEventRecord(c->ev_fork, c->stream));
       EIF_HIP(hipStreamWaitEvent(c->side, c->ev_fork, 0));
+      TRY(launch_inertia(c, 0, c->side));  // reads ux, uy before the predictor
+      EIF_HIP(hipEventRecord(c->ev_i0, c->side));
+      TRY(launch_collision(c, 0, c->side));
+      TRY(launch_predictor(c, rk));  // writes the other ux, uy buffers
+      EIF_HIP(hipStreamWaitEvent(c->stream, c->ev_i0, 0));
+      // same position as inertia(0) (dx = 0 after the reset), whose
+      // distances are cached; the reset cleared the tile registrations
+      TRY(launch_exchange(c, true, true, true));
+      EIF_HIP(hipEventRecord(c->ev_x, c->stream));
+      EIF_HIP(hipStreamWaitEvent(c->side, c->ev_x, 0));
+      TRY(launch_exchange_finalize(c, c->side));
+    } else {
+      // behind the predictor: the previous stage's particle update and this
+      // stage's reset; the exchange computes the distances at the new
+      // positions (the reset cleared the tile registrations) and inertia(0)
+      // reuses them
+      TRY(launch_update_particles(c));
+      TRY(launch_reset(c, c->stream, true));
+      TRY(launch_exchange(c, false, true, true));
+      EIF_HIP(hipEventRecord(c->ev_x, c->stream));
+      EIF_HIP(hipStreamWaitEvent(c->side, c->ev_x, 0));
+      TRY(launch_exchange_finalize(c, c->side));
+      // the stage-start velocity is the pair the predictor read
+      swap_fields(c);
+      const int ri = launch_inertia(c, 0, c->side);
+      swap_fields(c);
+      TRY(ri);
+      TRY(launch_collision(c, 0, c->side));
     }
-    // inertia(0) reads the stage-start velocity: the buffers the predictor
-    // reads (already swapped out when the predictor is queued)
-    if (pred_queued) {
-      std::swap(c->ux, c->ux_alt);
-      std::swap(c->uy, c->uy_alt);
-    }
-    const int ri = launch_inertia(c, 0, c->side);
-    if (pred_queued) {
-      std::swap(c->ux, c->ux_alt);
-      std::swap(c->uy, c->uy_alt);
-    }
-    TRY(ri);
-    EIF_HIP(hipEventRecord(c->ev_i0, c->side));
-    TRY(launch_collision(c, 0, c->side));
-    if (!pred_queued) TRY(launch_predictor(c, rk));  // writes the other ux, uy buffers
-    EIF_HIP(hipStreamWaitEvent(c->stream, c->ev_i0, 0));
-    // same position as inertia(0) (dx = 0 after the reset); the reset
-    // cleared the tile registrations
-    TRY(launch_exchange(c, true, true, true));
-    EIF_HIP(hipEventRecord(c->ev_x, c->stream));
-    EIF_HIP(hipStreamWaitEvent(c->side, c->ev_x, 0));
-    TRY(launch_exchange_finalize(c, c->side));
     EIF_HIP(hipEventRecord(c->ev_c0, c->side));
     TRY(launch_update_momentum(c));
     TRY(launch_poisson(c, rk));
@@ -754,15 +762,9 @@ int eif_gpu_integrate(eif_ctx *c, int substeps[3]) {
     int batch = std::min(std::max(c->substep_guess, 1), 102);
     bool early = false;  // predictor of stage rk+1 queued
     for (;;) {
-      if (early) {  // the stage's own velocity fields for inertia(1)
-        std::swap(c->ux, c->ux_alt);
-        std::swap(c->uy, c->uy_alt);
-      }
+      if (early) swap_fields(c);  // the stage's own velocity for inertia(1)
       const int r = queue_substeps(c, rk, queued, batch);
-      if (early) {
-        std::swap(c->ux, c->ux_alt);
-        std::swap(c->uy, c->uy_alt);
-      }
+      if (early) swap_fields(c);
       TRY(r);
       queued += batch;
       if (!early && rk < 2) {
@@ -786,8 +788,8 @@ int eif_gpu_integrate(eif_ctx *c, int substeps[3]) {
     c->substep_guess = done;
     if (substeps) substeps[rk] = done;
     pred_queued = early;
-    if (!early) TRY(launch_update_particles(c));
   }
+  TRY(launch_update_particles(c));
   c->dt_adv_valid = c->dt_adv_parts > 0;
   return EIF_OK;
 }

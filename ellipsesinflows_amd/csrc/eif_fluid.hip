@@ -221,8 +221,9 @@ __global__ __launch_bounds__(PRED_BX) void k_predictor(PredArgs a) {
   }
 }
 
-int launch_predictor(Ctx *c, int rk) {
-  ProfScope ps(c, "predictor");
+int launch_predictor(Ctx *c, int rk, hipStream_t st) {
+  if (!st) st = c->stream;
+  ProfScope ps(c, "predictor", st);
   PredArgs a;
   a.L = c->L;
   a.ux = c->ux; a.uy = c->uy; a.p = c->p;
@@ -236,9 +237,9 @@ int launch_predictor(Ctx *c, int rk) {
   a.use_b = c->beta[rk] != 0.;
   dim3 grid((c->nx + PRED_BX - 1) / PRED_BX, (c->ny + PRED_ROWS - 1) / PRED_ROWS);
   if (c->pow2_grid)
-    hipLaunchKernelGGL(k_predictor<true>, grid, dim3(PRED_BX), 0, c->stream, a);
+    hipLaunchKernelGGL(k_predictor<true>, grid, dim3(PRED_BX), 0, st, a);
   else
-    hipLaunchKernelGGL(k_predictor<false>, grid, dim3(PRED_BX), 0, c->stream, a);
+    hipLaunchKernelGGL(k_predictor<false>, grid, dim3(PRED_BX), 0, st, a);
   EIF_HIP(hipGetLastError());
   std::swap(c->ux, c->ux_alt);
   std::swap(c->uy, c->uy_alt);

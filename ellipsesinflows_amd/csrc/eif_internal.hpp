@@ -201,7 +201,8 @@ struct ProfScope {  // events on the stream the scoped kernels run on
 int launch_bc_ux(Ctx *c, double *ux);
 int launch_bc_uy(Ctx *c, double *uy);
 int launch_bc_p(Ctx *c, double *p);
-int launch_predictor(Ctx *c, int rk);
+// swaps ux, uy with their ping-pong partners at launch
+int launch_predictor(Ctx *c, int rk, hipStream_t st = nullptr);
 // dt_fused: also leave the next decide_dt's advective limit as partial minima
 int launch_correct_pressure(Ctx *c, int rk, bool correct, bool pressure,
                             bool dt_fused = false);

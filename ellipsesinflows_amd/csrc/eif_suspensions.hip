@@ -570,13 +570,25 @@ __global__ __launch_bounds__(64) void k_inertia(IbmArgs a, int cnstep) {
   inertia_slice(a, cnstep, blockIdx.x);
 }
 
-// Per-particle totals of the sliced inertia / exchange kernels, added in
-// slice order; also records the position whose distances are now cached
-// and the work counters.  which: 0..1 inertia (cnstep), 2 exchange.
+// Per-particle totals of the sliced inertia / exchange kernels, by the 64
+// lanes of one wave: the lanes load the slices' partial sums together, and
+// they are added in slice order through lane shuffles.  Lane 0 also
+// records the position whose distances are now cached and the work
+// counters.  which: 0..1 inertia (cnstep), 2 exchange.
 __device__ void finalize_particle(const IbmArgs &a, int n, int which) {
+  const int lane = threadIdx.x & 63;
   double tot[SLICE_VALS] = {0., 0., 0., 0., 0.};
-  for (int sl = a.slice_first[n]; sl < a.slice_first[n + 1]; sl++)
-    for (int k = 0; k < SLICE_VALS; k++) tot[k] += a.slice_part[SLICE_VALS * (long)sl + k];
+  const int s1 = a.slice_first[n + 1];
+  for (int base = a.slice_first[n]; base < s1; base += 64) {
+    double v[SLICE_VALS] = {0., 0., 0., 0., 0.};
+    if (base + lane < s1)
+      for (int k = 0; k < SLICE_VALS; k++)
+        v[k] = a.slice_part[SLICE_VALS * (long)(base + lane) + k];
+    const int cnt = min(64, s1 - base);
+    for (int l = 0; l < cnt; l++)
+      for (int k = 0; k < SLICE_VALS; k++) tot[k] += __shfl(v[k], l);
+  }
+  if (lane != 0) return;
   double *q = a.prt + (long)n * R_N;
   double px = q[R_X], py = q[R_Y], paz = q[R_AZ];
   if (which < 2) {
@@ -609,9 +621,10 @@ __device__ void finalize_particle(const IbmArgs &a, int n, int which) {
   c[3] += (unsigned long long)tot[4];
 }
 
-__global__ void k_ibm_finalize(IbmArgs a, int which) {
+// one wave per particle, four per workgroup
+__global__ __launch_bounds__(256) void k_ibm_finalize(IbmArgs a, int which) {
   if (!sub_open(a.res, a.sub)) return;
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
   if (n < a.np) finalize_particle(a, n, which);
 }
 
@@ -1246,14 +1259,18 @@ __global__ void k_zero_residual(double *dscal) { dscal[S_RESIDUAL] = 0.; }
 // src/suspensions/increment.c:10-64
 // fin: also finalises the preceding inertia(1) call (its slice sums and
 // cache records) for this particle first, saving a launch per sub-iteration
-__global__ void k_increment(double *prt, int np, const double *dscal, double ga,
-                            double Fr, double *res_out, IbmArgs ia, int fin) {
+// one wave per particle (its lanes finalise the inertia sums), four per
+// workgroup; lane 0 computes the increment
+__global__ __launch_bounds__(256) void k_increment(double *prt, int np, const double *dscal,
+                                                   double ga, double Fr, double *res_out,
+                                                   IbmArgs ia, int fin) {
   if (!sub_open(ia.res, ia.sub)) return;
-  __shared__ double red[256];
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ double red[4];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int n = blockIdx.x * 4 + w;
   double res = 0.;
-  if (n < np) {
-    if (fin) finalize_particle(ia, n, 1);
+  if (n < np && fin) finalize_particle(ia, n, 1);
+  if (n < np && lane == 0) {
     double *p = prt + (long)n * R_N;
     const double dt = dscal[S_DT];
     const double den = p[R_DEN];
@@ -1276,13 +1293,10 @@ __global__ void k_increment(double *prt, int np, const double *dscal, double ga,
     p[R_DY] = 0.5 * ga * dt * (uy0 + uy1);
     p[R_DAZ] = 0.5 * ga * dt * (vz0 + vz1);
   }
-  red[threadIdx.x] = res;
+  if (lane == 0) red[w] = res;
   __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) atomic_max_nonneg(res_out, red[0]);
+  if (threadIdx.x == 0)
+    atomic_max_nonneg(res_out, fmax(fmax(red[0], red[1]), fmax(red[2], red[3])));
 }
 
 // src/suspensions/update.c:8-34
@@ -1464,7 +1478,7 @@ int launch_inertia(Ctx *c, int cnstep, hipStream_t st, bool defer_finalize) {
   IbmArgs a = ibm_args(c);
   hipLaunchKernelGGL(k_inertia, dim3(c->nslices), dim3(64), 0, st, a, cnstep);
   if (!defer_finalize)
-    hipLaunchKernelGGL(k_ibm_finalize, dim3((c->np + 63) / 64), dim3(64), 0, st, a, cnstep);
+    hipLaunchKernelGGL(k_ibm_finalize, dim3((c->np + 3) / 4), dim3(256), 0, st, a, cnstep);
   EIF_HIP(hipGetLastError());
   return EIF_OK;
 }
@@ -1492,7 +1506,7 @@ int launch_exchange(Ctx *c, bool cached_known, bool tiles_clear, bool defer_fina
 // reads them, so this may run on another stream after the exchange
 int launch_exchange_finalize(Ctx *c, hipStream_t st) {
   if (c->np == 0) return EIF_OK;
-  hipLaunchKernelGGL(k_ibm_finalize, dim3((c->np + 63) / 64), dim3(64), 0, st, ibm_args(c), 2);
+  hipLaunchKernelGGL(k_ibm_finalize, dim3((c->np + 3) / 4), dim3(256), 0, st, ibm_args(c), 2);
   EIF_HIP(hipGetLastError());
   return EIF_OK;
 }
@@ -1595,7 +1609,7 @@ int launch_increment(Ctx *c, int rk, int sub) {
     res_out = c->res_slots + sub;
   }
   if (c->np > 0)
-    hipLaunchKernelGGL(k_increment, dim3((c->np + 255) / 256), dim3(256), 0, c->stream,
+    hipLaunchKernelGGL(k_increment, dim3((c->np + 3) / 4), dim3(256), 0, c->stream,
                        c->prt, c->np, c->dscal, c->gamma[rk], c->cfg.Fr, res_out, ia,
                        sub >= 0 ? 1 : 0);
   EIF_HIP(hipGetLastError());
