@@ -522,20 +522,24 @@ __global__ __launch_bounds__(PART_LANES) void k_part_local(
   if (k >= nx) return;
   const long j0 = (long)c * L;
   const double l = coef;
-  // forward sweep stores y in place; the backward sweep reads it back
+  // the forward sweep keeps y in LDS (one column per lane) for the
+  // backward sweep, so q is read and written once
+  __shared__ double ys[L][PART_LANES];
+  const int lane = threadIdx.x;
   double prev = 0.;
 #pragma unroll 8
   for (int r = 0; r < L; r++) {
     const double d = q[(j0 + r) * nx + k];
     const double v = fwd[(long)r * nx + k] * (d - l * prev);
-    q[(j0 + r) * nx + k] = v;
+    ys[r][lane] = v;
     prev = v;
   }
   const double ylast = prev;
+  q[(j0 + L - 1) * nx + k] = prev;
   double next = prev;
 #pragma unroll 8
   for (int r = L - 2; r >= 0; r--) {
-    next = q[(j0 + r) * nx + k] - um[(long)r * nx + k] * next;
+    next = ys[r][lane] - um[(long)r * nx + k] * next;
     q[(j0 + r) * nx + k] = next;
   }
   ends[((long)c * 2 + 0) * nx + k] = next;   // y_c[0]
