@@ -252,7 +252,7 @@ int launch_predictor(Ctx *c, int rk, hipStream_t st) {
 // values of ux, uy and p refreshed by the threads that own the source
 // interior values.
 // ---------------------------------------------------------------------
-constexpr int CP_ROWS = 16;  // rows per workgroup of k_correct_pressure
+constexpr int CP_ROWS = 8;   // rows per workgroup of k_correct_pressure
 
 template <bool P2, bool DT>
 __global__ __launch_bounds__(256) void k_correct_pressure(

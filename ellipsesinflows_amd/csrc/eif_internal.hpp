@@ -55,7 +55,7 @@ constexpr int TILE_Y = 4;
 constexpr int TILE_CAP = 48;     // boxes registered per tile
 constexpr int CONTACT_CAP = 64;  // colliding partners per particle
 constexpr int EQC_ITER_CAP = 100000;  // guard of the uncapped fixed point
-constexpr int DT_PARTS_CAP = 4096;    // workgroups of the projection with fused dt
+constexpr int DT_PARTS_CAP = 16384;   // workgroups of the projection with fused dt
 constexpr int RES_SLOTS = 128;        // sub-iterations of one stage queued at most
 
 struct Profiler;

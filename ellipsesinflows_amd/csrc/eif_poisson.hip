@@ -527,7 +527,7 @@ __global__ __launch_bounds__(PART_LANES) void k_part_local(
   __shared__ double ys[L][PART_LANES];
   const int lane = threadIdx.x;
   double prev = 0.;
-#pragma unroll 8
+#pragma unroll 16
   for (int r = 0; r < L; r++) {
     const double d = q[(j0 + r) * nx + k];
     const double v = fwd[(long)r * nx + k] * (d - l * prev);
@@ -537,7 +537,7 @@ __global__ __launch_bounds__(PART_LANES) void k_part_local(
   const double ylast = prev;
   q[(j0 + L - 1) * nx + k] = prev;
   double next = prev;
-#pragma unroll 8
+#pragma unroll 16
   for (int r = L - 2; r >= 0; r--) {
     next = ys[r][lane] - um[(long)r * nx + k] * next;
     q[(j0 + r) * nx + k] = next;

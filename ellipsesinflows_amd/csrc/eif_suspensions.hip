@@ -862,92 +862,111 @@ __device__ inline double box_value(const Box &b, const double *sc, int i, int j)
   return sc[b.off + (long)(j - b.jmin) * w + (i - b.imin)];
 }
 
-__global__ __launch_bounds__(TILE_X * TILE_Y) void k_tiles(TileArgs a) {
-  __shared__ int list[TILE_CAP];
+// One 64-lane wave per active tile (lane = tile column, all TILE_Y rows),
+// so that many tiles are in flight per CU.  The tile's box registrations
+// (in registration order) are ranked in parallel into the reference's order
+// (particle index, then image; a box registered twice counts once), the
+// boxes are staged in LDS, and every cell sums its boxes' values in that
+// order.
+constexpr int TILE_THREADS = 64;
+static_assert(TILE_X == TILE_THREADS, "one lane per tile column");
+static_assert(TILE_CAP <= TILE_THREADS, "one lane per registration");
+
+__global__ __launch_bounds__(TILE_THREADS) void k_tiles(TileArgs a) {
+  __shared__ int raw[TILE_CAP];
+  __shared__ bool rep[TILE_CAP];
+  __shared__ Box sbox[TILE_CAP];
   __shared__ int nlist;
-  __shared__ double dxr[(TILE_X + 1) * TILE_Y];
-  __shared__ double dyr[TILE_X * (TILE_Y + 1)];
-  const int nact = a.active[0];
-  const int tid = threadIdx.x;
-  for (int at = blockIdx.x; at < nact; at += gridDim.x) {
-    const int tile = a.active[1 + at];
-    const int tx = tile % a.ntx, ty = tile / a.ntx;
-    const int i0 = tx * TILE_X + 1, j0 = ty * TILE_Y + 1;
-    if (tid == 0) {
-      int cnt = a.tile_cnt[tile];
-      cnt = cnt > TILE_CAP ? TILE_CAP : cnt;
-      for (int s = 0; s < cnt; s++) list[s] = a.tile_list[(long)tile * TILE_CAP + s];
-      // insertion sort (particle index, then image: the reference's order)
-      for (int s = 1; s < cnt; s++) {
-        const int v = list[s];
-        int r = s - 1;
-        while (r >= 0 && list[r] > v) {
-          list[r + 1] = list[r];
-          r--;
-        }
-        list[r + 1] = v;
+  // tiles in grid order, so neighbouring workgroups touch neighbouring
+  // memory; tiles no box reaches return at once
+  const int tile = blockIdx.x;
+  const int cnt = min(a.tile_cnt[tile], TILE_CAP);
+  if (cnt == 0) return;
+  const int lane = threadIdx.x;
+  const Layout L = a.L;
+  const int tx = tile % a.ntx, ty = tile / a.ntx;
+  const int i0 = tx * TILE_X + 1, j0 = ty * TILE_Y + 1;
+  if (lane < cnt) raw[lane] = a.tile_list[(long)tile * TILE_CAP + lane];
+  if (lane == 0) nlist = 0;
+  __syncthreads();
+  int v = 0;
+  if (lane < cnt) {
+    v = raw[lane];
+    bool dup = false;
+    for (int s = 0; s < lane; s++) dup |= raw[s] == v;
+    rep[lane] = dup;
+  }
+  __syncthreads();
+  if (lane < cnt && !rep[lane]) {
+    int pos = 0;
+    for (int s = 0; s < cnt; s++) pos += raw[s] < v && !rep[s];
+    sbox[pos] = a.boxes[v];
+    atomicMax(&nlist, pos + 1);
+  }
+  __syncthreads();
+  const int nl = nlist;
+  // column i0-1+lane of DUX and column i0+lane of DUY on rows j0-1 .. j0+3
+  // (DUY's row j0-1 = 0 is row jtot), then column i0+lane of the tile.
+  // Boxes outermost: the 13 loads of one box are in flight together, and
+  // every cell still adds its boxes in list order.
+  const int i = i0 + lane;
+  double dxw[TILE_Y], dxc[TILE_Y], dy[TILE_Y + 1];
+  int jy[TILE_Y + 1];
+#pragma unroll
+  for (int r = 0; r < TILE_Y; r++) dxw[r] = dxc[r] = 0.;
+#pragma unroll
+  for (int r = 0; r <= TILE_Y; r++) {
+    dy[r] = 0.;
+    jy[r] = j0 - 1 + r == 0 ? a.ny : j0 - 1 + r;
+  }
+  const bool iw = i - 1 >= 1 && i - 1 <= a.nx, ic = i <= a.nx;
+  for (int s = 0; s < nl; s++) {
+    const Box b = sbox[s];
+    double tw[TILE_Y], tc[TILE_Y], ty_[TILE_Y + 1];
+#pragma unroll
+    for (int r = 0; r < TILE_Y; r++) {
+      const bool rin = j0 + r <= a.ny;
+      tw[r] = rin && iw ? box_value(b, a.sc_fx, i - 1, j0 + r) : 0.;
+      tc[r] = rin && ic ? box_value(b, a.sc_fx, i, j0 + r) : 0.;
+    }
+#pragma unroll
+    for (int r = 0; r <= TILE_Y; r++)
+      ty_[r] = ic && jy[r] <= a.ny ? box_value(b, a.sc_fy, i, jy[r]) : 0.;
+#pragma unroll
+    for (int r = 0; r < TILE_Y; r++) {
+      dxw[r] += tw[r];
+      dxc[r] += tc[r];
+    }
+#pragma unroll
+    for (int r = 0; r <= TILE_Y; r++) dy[r] += ty_[r];
+  }
+  if (i > a.nx) return;
+#pragma unroll
+  for (int r = 0; r < TILE_Y; r++) {
+    const int j = j0 + r;
+    if (j > a.ny) break;
+    const double dux_w = dxw[r], dux_c = dxc[r], duy_s = dy[r], duy_c = dy[r + 1];
+    if (a.apply) {
+      // cells no box reaches get an increment of exactly zero: adding it
+      // changes no value (a -0.0 could become +0.0), so they are skipped
+      if (i >= 2 && (dux_w != 0. || dux_c != 0.)) {
+        const long k = L.ux(i, j);
+        const double u = a.ux[k] + 0.5 * (dux_w + dux_c);
+        a.ux[k] = u;
+        if (j == a.ny) a.ux[L.ux(i, 0)] = u;
+        if (j == 1) a.ux[L.ux(i, a.ny + 1)] = u;
       }
-      int m = 0;
-      for (int s = 0; s < cnt; s++)
-        if (m == 0 || list[m - 1] != list[s]) list[m++] = list[s];
-      nlist = m;
-    }
-    __syncthreads();
-    const int nl = nlist;
-    // DUX on (i0-1 .. i0+63) x (j0 .. j0+3)
-    for (int e = tid; e < (TILE_X + 1) * TILE_Y; e += blockDim.x) {
-      const int ii = i0 - 1 + e % (TILE_X + 1), jj = j0 + e / (TILE_X + 1);
-      double v = 0.;
-      if (ii >= 1 && ii <= a.nx && jj <= a.ny)
-        for (int s = 0; s < nl; s++) {
-          const Box b = a.boxes[list[s]];
-          v += box_value(b, a.sc_fx, ii, jj);
-        }
-      dxr[e] = v;
-    }
-    // DUY on (i0 .. i0+63) x (j0-1 .. j0+3), row 0 is row jtot
-    for (int e = tid; e < TILE_X * (TILE_Y + 1); e += blockDim.x) {
-      const int ii = i0 + e % TILE_X;
-      int jj = j0 - 1 + e / TILE_X;
-      if (jj == 0) jj = a.ny;
-      double v = 0.;
-      if (ii <= a.nx && jj <= a.ny)
-        for (int s = 0; s < nl; s++) {
-          const Box b = a.boxes[list[s]];
-          v += box_value(b, a.sc_fy, ii, jj);
-        }
-      dyr[e] = v;
-    }
-    __syncthreads();
-    const int li = tid % TILE_X, lj = tid / TILE_X;
-    const int i = i0 + li, j = j0 + lj;
-    if (i <= a.nx && j <= a.ny) {
-      const double dux_c = dxr[lj * (TILE_X + 1) + li + 1];
-      const double dux_w = dxr[lj * (TILE_X + 1) + li];
-      const double duy_c = dyr[(lj + 1) * TILE_X + li];
-      const double duy_s = dyr[lj * TILE_X + li];
-      if (a.apply) {
-        const Layout L = a.L;
-        if (i >= 2) {
-          const long k = L.ux(i, j);
-          const double v = a.ux[k] + 0.5 * (dux_w + dux_c);
-          a.ux[k] = v;
-          if (j == a.ny) a.ux[L.ux(i, 0)] = v;
-          if (j == 1) a.ux[L.ux(i, a.ny + 1)] = v;
-        }
-        {
-          const long k = L.p(i, j);
-          const double v = a.uy[k] + 0.5 * (duy_s + duy_c);
-          a.uy[k] = v;
-          if (j == a.ny) a.uy[L.p(i, 0)] = v;
-          if (j == 1) a.uy[L.p(i, a.ny + 1)] = v;
-        }
-      } else {
-        a.dux[a.L.p(i, j)] = dux_c;
-        a.duy[a.L.p(i, j)] = duy_c;
+      if (duy_s != 0. || duy_c != 0.) {
+        const long k = L.p(i, j);
+        const double u = a.uy[k] + 0.5 * (duy_s + duy_c);
+        a.uy[k] = u;
+        if (j == a.ny) a.uy[L.p(i, 0)] = u;
+        if (j == 1) a.uy[L.p(i, a.ny + 1)] = u;
       }
+    } else {
+      a.dux[L.p(i, j)] = dux_c;
+      a.duy[L.p(i, j)] = duy_c;
     }
-    __syncthreads();
   }
 }
 
@@ -1540,7 +1559,7 @@ int launch_update_momentum(Ctx *c) {
   if (c->np == 0) return EIF_OK;
   ProfScope ps(c, "ibm_apply");
   TileArgs t = tile_args(c, 1);
-  hipLaunchKernelGGL(k_tiles, dim3(tile_grid(c)), dim3(TILE_X * TILE_Y), 0, c->stream, t);
+  hipLaunchKernelGGL(k_tiles, dim3(tile_grid(c)), dim3(TILE_THREADS), 0, c->stream, t);
   EIF_HIP(hipGetLastError());
   return EIF_OK;
 }
@@ -1551,7 +1570,7 @@ int launch_materialise_du(Ctx *c) {
   EIF_HIP(hipMemsetAsync(c->duy, 0, bytes, c->stream));
   if (c->np > 0) {
     TileArgs t = tile_args(c, 0);
-    hipLaunchKernelGGL(k_tiles, dim3(tile_grid(c)), dim3(TILE_X * TILE_Y), 0, c->stream,
+    hipLaunchKernelGGL(k_tiles, dim3(tile_grid(c)), dim3(TILE_THREADS), 0, c->stream,
                        t);
     EIF_HIP(hipGetLastError());
   }
