@@ -87,6 +87,16 @@ __device__ inline RegMap reg_phase(int p, int logn) {
 
 __device__ inline int reg_nphase(int logn) { return (logn + 2) / 3; }
 
+// the same maps with 2^LOGR values per thread (stages taken LOGR at a time)
+template <int LOGR>
+__device__ inline RegMap reg_phase_r(int p, int logn) {
+  RegMap m;
+  m.lg = min(LOGR, logn - LOGR * p);
+  m.lh0 = LOGR * p;
+  m.lgpt = LOGR - m.lg;
+  return m;
+}
+
 // LDS address of element p: XOR-swizzles the 16-double bank group so that
 // the bit-reversed, strided and natural access patterns of the transform
 // hit at most two-way bank conflicts
@@ -146,30 +156,32 @@ __device__ void reg_fft(double (&xr)[8], double (&xi)[8], double *lre, double *l
   }
 }
 
-// The same transform with n = 2^LOGN known at compile time: phases and
-// stages unroll, so the slot -> position maps and twiddle offsets fold to
-// constants plus a per-thread base.  Butterflies, operand order and twiddle
-// values are those of reg_fft.
-template <int LOGN, int SIGN>
-__device__ inline void reg_fft_t(double (&xr)[8], double (&xi)[8], double *lre, double *lim,
-                                 const double *__restrict__ W) {
+// The same transform with n = 2^LOGN known at compile time and 2^LOGR
+// values per thread: phases and stages unroll, so the slot -> position maps
+// and twiddle offsets fold to constants plus a per-thread base.
+// Butterflies, operand order and twiddle values are those of reg_fft (only
+// the grouping of stages into register phases differs).
+template <int LOGN, int LOGR, int SIGN>
+__device__ inline void reg_fft_t(double (&xr)[1 << LOGR], double (&xi)[1 << LOGR], double *lre,
+                                 double *lim, const double *__restrict__ W) {
+  constexpr int R = 1 << LOGR;
   const int t = threadIdx.x;
-  constexpr int NP = (LOGN + 2) / 3;
+  constexpr int NP = (LOGN + LOGR - 1) / LOGR;
 #pragma unroll
   for (int p = 0; p < NP; p++) {
-    const RegMap mp = reg_phase(p, LOGN);
+    const RegMap mp = reg_phase_r<LOGR>(p, LOGN);
     if (p > 0) {
-      const RegMap mq = reg_phase(p - 1, LOGN);
+      const RegMap mq = reg_phase_r<LOGR>(p - 1, LOGN);
       __syncthreads();
 #pragma unroll
-      for (int s = 0; s < 8; s++) {
+      for (int s = 0; s < R; s++) {
         const int q = sw(mq.pos(t, s));
         lre[q] = xr[s];
         lim[q] = xi[s];
       }
       __syncthreads();
 #pragma unroll
-      for (int s = 0; s < 8; s++) {
+      for (int s = 0; s < R; s++) {
         const int q = sw(mp.pos(t, s));
         xr[s] = lre[q];
         xi[s] = lim[q];
@@ -178,12 +190,12 @@ __device__ inline void reg_fft_t(double (&xr)[8], double (&xi)[8], double *lre, 
     const int r = mp.lg;
     const int gmask = (1 << r) - 1;
 #pragma unroll
-    for (int ss = 0; ss < 3; ss++) {
+    for (int ss = 0; ss < LOGR; ss++) {
       if (ss >= r) break;
       const int half = 1 << (mp.lh0 + ss), len = half << 1;
       const double *Ws = W + 2 * (half - 1);
 #pragma unroll
-      for (int s = 0; s < 8; s++) {
+      for (int s = 0; s < R; s++) {
         const int e = s & gmask;
         if (e & (1 << ss)) continue;
         if ((e | (1 << ss)) > gmask) continue;
@@ -202,6 +214,14 @@ __device__ inline void reg_fft_t(double (&xr)[8], double (&xi)[8], double *lre, 
       }
     }
   }
+}
+
+// log2 of the values per thread of the register FFT for n = 2^LOGN: 8
+// (16 per thread measured slower at n = 2048: fewer waves hide the LDS
+// round trips less well than the saved phase gains)
+template <int LOGN>
+constexpr int dct_logr() {
+  return 3;
 }
 
 __device__ inline double cos_lookup(const double *C, int n, long m) {
@@ -367,86 +387,90 @@ __global__ __launch_bounds__(1024) void k_dct3_psi(
   }
 }
 
-// Register-FFT kernels specialised for nx = 2^LOGN (nx/8 threads); the
-// arithmetic is that of the mode-2 branches of k_rhs_dct2 / k_dct3_psi.
+// Register-FFT kernels specialised for nx = 2^LOGN (nx / 2^LOGR threads);
+// the arithmetic is that of the mode-2 branches of k_rhs_dct2 / k_dct3_psi.
 template <int LOGN, bool P2>
-__global__ __launch_bounds__(1024) void k_rhs_dct2_t(
+__global__ __launch_bounds__((1 << LOGN) >> dct_logr<LOGN>()) void k_rhs_dct2_t(
     Layout L, const double *ux, const double *uy, const double *dscal, double ga,
     double dx, double dy, const double *__restrict__ C, const double *__restrict__ W,
     double *qhat) {
   extern __shared__ double sh[];
+  constexpr int LOGR = dct_logr<LOGN>(), R = 1 << LOGR;
   constexpr int nx = 1 << LOGN;
-  constexpr int T = nx / 8;
+  constexpr int T = nx / R;
   const int j = blockIdx.x + 1;
   double *re = sh, *im = sh + nx;
   const double dt = dscal[S_DT];
   const double fac = 1. / (ga * dt);
   double *out = qhat + (long)(j - 1) * nx;
 #pragma unroll
-  for (int s = 0; s < 8; s++) {
+  for (int s = 0; s < R; s++) {
     const int m = threadIdx.x + s * T;
     const int i = m < nx / 2 ? 2 * m : 2 * (nx - 1 - m) + 1;  // Makhoul order
     re[sw(bitrev(m, LOGN))] = rhs_value<P2>(L, ux, uy, fac, dx, dy, i, j);
   }
   __syncthreads();
-  double xr[8], xi[8];
+  double xr[R], xi[R];
 #pragma unroll
-  for (int s = 0; s < 8; s++) {
-    xr[s] = re[sw(8 * threadIdx.x + s)];
+  for (int s = 0; s < R; s++) {
+    xr[s] = re[sw(R * threadIdx.x + s)];
     xi[s] = 0.;
   }
-  reg_fft_t<LOGN, -1>(xr, xi, re, im, W);
-  const RegMap ml = reg_phase(reg_nphase(LOGN) - 1, LOGN);
+  reg_fft_t<LOGN, LOGR, -1>(xr, xi, re, im, W);
+  constexpr int NP = (LOGN + LOGR - 1) / LOGR;
+  const RegMap ml = reg_phase_r<LOGR>(NP - 1, LOGN);
   __syncthreads();
 #pragma unroll
-  for (int s = 0; s < 8; s++) {
+  for (int s = 0; s < R; s++) {
     const int q = sw(ml.pos(threadIdx.x, s));
     re[q] = xr[s];
     im[q] = xi[s];
   }
   __syncthreads();
 #pragma unroll
-  for (int s = 0; s < 8; s++) {
+  for (int s = 0; s < R; s++) {
     const int k = threadIdx.x + s * T;
     out[k] = 2. * (C[k] * re[sw(k)] + C[nx - k] * im[sw(k)]);
   }
 }
 
 template <int LOGN>
-__global__ __launch_bounds__(1024) void k_dct3_psi_t(
+__global__ __launch_bounds__((1 << LOGN) >> dct_logr<LOGN>()) void k_dct3_psi_t(
     Layout L, const double *qhat, CombineArgs ca, const double *__restrict__ C,
     const double *__restrict__ W, double *psi) {
   extern __shared__ double sh[];
+  constexpr int LOGR = dct_logr<LOGN>(), R = 1 << LOGR;
   constexpr int nx = 1 << LOGN;
-  constexpr int T = nx / 8;
+  constexpr int T = nx / R;
   const int ny = L.ny;
   const int j0 = blockIdx.x;
   double *re = sh, *im = sh + nx;
   // division by 2 nx, a power of two, is the exact product with its inverse
   constexpr double inv_norm = 1. / (2. * nx);
 #pragma unroll
-  for (int s = 0; s < 8; s++) {
+  for (int s = 0; s < R; s++) {
     const int k = threadIdx.x + s * T;
     re[sw(k)] = combined(ca, qhat, nx, ny, j0, k);
   }
   __syncthreads();
-  double xr[8], xi[8];
+  double xr[R], xi[R];
 #pragma unroll
-  for (int s = 0; s < 8; s++) {
-    const int k = bitrev(8 * threadIdx.x + s, LOGN);
+  for (int s = 0; s < R; s++) {
+    const int k = bitrev(R * threadIdx.x + s, LOGN);
     const double vr = re[sw(k)];
     const double vi = k == 0 ? 0. : -re[sw(nx - k)];
     xr[s] = C[k] * vr - C[nx - k] * vi;
     xi[s] = C[k] * vi + C[nx - k] * vr;
   }
-  reg_fft_t<LOGN, +1>(xr, xi, re, im, W);
-  const RegMap ml = reg_phase(reg_nphase(LOGN) - 1, LOGN);
+  reg_fft_t<LOGN, LOGR, +1>(xr, xi, re, im, W);
+  constexpr int NP = (LOGN + LOGR - 1) / LOGR;
+  const RegMap ml = reg_phase_r<LOGR>(NP - 1, LOGN);
   __syncthreads();
 #pragma unroll
-  for (int s = 0; s < 8; s++) re[sw(ml.pos(threadIdx.x, s))] = xr[s];
+  for (int s = 0; s < R; s++) re[sw(ml.pos(threadIdx.x, s))] = xr[s];
   __syncthreads();
 #pragma unroll
-  for (int s = 0; s < 8; s++) {
+  for (int s = 0; s < R; s++) {
     const int i = threadIdx.x + s * T;
     const double v = (i & 1) ? re[sw(nx - 1 - (i >> 1))] : re[sw(i >> 1)];
     psi[L.p(i + 1, j0 + 1)] = v * inv_norm;
@@ -800,7 +824,8 @@ int launch_poisson(Ctx *c, int rk) {
   case LG:                                                                               \
     {                                                                                    \
       auto kern = c->pow2_grid ? k_rhs_dct2_t<LG, true> : k_rhs_dct2_t<LG, false>;       \
-      hipLaunchKernelGGL(kern, dim3(ny), dim3(threads), shmem, c->stream, c->L, c->ux,   \
+      hipLaunchKernelGGL(kern, dim3(ny), dim3(nx >> dct_logr<LG>()), shmem, c->stream,   \
+                         c->L, c->ux,                                                    \
                          c->uy, c->dscal, c->gamma[rk], c->dx, c->dy, c->ctab, c->twid,  \
                          c->qhat);                                                       \
     }                                                                                    \
@@ -852,7 +877,8 @@ int launch_poisson(Ctx *c, int rk) {
     ProfScope ps(c, "dct3_psi");
 #define EIF_DCT3_T(LG)                                                                  \
   case LG:                                                                              \
-    hipLaunchKernelGGL(k_dct3_psi_t<LG>, dim3(ny), dim3(threads), shmem, c->stream, c->L, \
+    hipLaunchKernelGGL(k_dct3_psi_t<LG>, dim3(ny), dim3(nx >> dct_logr<LG>()), shmem,    \
+                       c->stream, c->L,                                                 \
                        c->qhat, ca, c->ctab, c->twid, c->psi);                          \
     break;
     switch (mode == 2 ? logn : -1) {
