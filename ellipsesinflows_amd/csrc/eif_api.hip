@@ -165,7 +165,7 @@ static void free_ctx(Ctx *c) {
     delete c->prof;
   }
   if (c->side) (void)hipStreamSynchronize(c->side);
-  hipEvent_t evs[] = {c->ev_fork, c->ev_i0, c->ev_c0, c->ev_x, c->ev_sub};
+  hipEvent_t evs[] = {c->ev_fork, c->ev_i0, c->ev_c0, c->ev_x};
   for (auto ev : evs)
     if (ev) (void)hipEventDestroy(ev);
   if (c->side) (void)hipStreamDestroy(c->side);
@@ -209,7 +209,7 @@ int fluid_gpu_create(const eif_config *cfg, eif_ctx **out) {
   {
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
-    hipEvent_t *evs[] = {&c->ev_fork, &c->ev_i0, &c->ev_c0, &c->ev_x, &c->ev_sub};
+    hipEvent_t *evs[] = {&c->ev_fork, &c->ev_i0, &c->ev_c0, &c->ev_x};
     for (auto ev : evs)
       if (e == hipSuccess) e = hipEventCreateWithFlags(ev, hipEventDisableTiming);
     if (e != hipSuccess) return fail(hip_fail(e, "hipStreamCreate", __FILE__, __LINE__));
@@ -300,6 +300,7 @@ int fluid_gpu_create(const eif_config *cfg, eif_ctx **out) {
     if (e == hipSuccess) e = hipMemset(c->res_slots, 0, RES_SLOTS * sizeof(double));
     if (e == hipSuccess)
       e = hipHostMalloc((void **)&c->hstat, sizeof(HostStatus), hipHostMallocCoherent);
+    if (e == hipSuccess) memset(c->hstat, 0, sizeof(HostStatus));
     if (e == hipSuccess) e = hipMalloc(&c->ibm_stats, 8 * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMemset(c->ibm_stats, 0, 8 * sizeof(unsigned long long));
     if (e != hipSuccess) return fail(hip_fail(e, "hipMalloc", __FILE__, __LINE__));
@@ -681,7 +682,32 @@ static int queue_substeps(Ctx *c, int rk, int first, int n) {
     TRY(launch_increment(c, rk, s));
   }
   TRY(launch_status(c, first + n));
-  EIF_HIP(hipEventRecord(c->ev_sub, c->stream));
+  return EIF_OK;
+}
+
+// Waits for the status record queued last.  The host polls the pinned
+// record (an event record between the batch and the kernels queued behind
+// it would idle the stream).  Only a long wait queries the stream (a query
+// can itself cost the stream a pause), so a failed kernel still ends the
+// wait with its error.
+static int wait_status(Ctx *c, HostStatus *out) {
+  const unsigned want = c->hstat_seq;
+  for (long spin = 1;; spin++) {
+    if (__atomic_load_n(&c->hstat->seq, __ATOMIC_ACQUIRE) == want) break;
+    if (spin >= (1L << 20) && spin % (1L << 14) == 0) {
+      const hipError_t e = hipStreamQuery(c->stream);
+      if (e == hipSuccess) {
+        if (__atomic_load_n(&c->hstat->seq, __ATOMIC_ACQUIRE) == want) break;
+        set_error("sub-iteration status record missing after the stream drained");
+        return EIF_ERR_HIP;
+      }
+      if (e != hipErrorNotReady) return hip_fail(e, "hipStreamQuery", __FILE__, __LINE__);
+    }
+  }
+  out->residual = c->hstat->residual;
+  out->dt = c->hstat->dt;
+  out->substeps = c->hstat->substeps;
+  out->flags = c->hstat->flags;
   return EIF_OK;
 }
 
@@ -771,8 +797,8 @@ int eif_gpu_integrate(eif_ctx *c, int substeps[3]) {
         TRY(launch_predictor(c, rk + 1));
         early = true;
       }
-      EIF_HIP(hipEventSynchronize(c->ev_sub));
-      const HostStatus st = *c->hstat;
+      HostStatus st;
+      TRY(wait_status(c, &st));
       if (st.flags) {
         EIF_HIP(hipStreamSynchronize(c->side));
         EIF_HIP(hipStreamSynchronize(c->stream));

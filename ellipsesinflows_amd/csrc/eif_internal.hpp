@@ -74,6 +74,7 @@ struct HostStatus {
   double dt;        // of the current step
   int substeps;     // sub-iterations run in this stage
   int flags;        // device capacity flags
+  unsigned seq;     // written last (system-scope release): record number
 };
 
 struct Ctx {
@@ -87,7 +88,6 @@ struct Ctx {
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_i0 = nullptr, ev_c0 = nullptr;
   hipEvent_t ev_x = nullptr;    // exchange done (main), for its finalize on the side stream
-  hipEvent_t ev_sub = nullptr;  // sub-iteration batch and its status record done (main)
   int device = 0;
   // host copies of the grid (exact reference arithmetic)
   std::vector<double> h_xf, h_xc, h_dxf, h_dxc, h_yf, h_yc;
@@ -155,6 +155,7 @@ struct Ctx {
   unsigned long long *ibm_pp = nullptr;     // np x 4 IBM work counters, then calls
   double *res_slots = nullptr;  // RES_SLOTS: residual of each sub-iteration of the stage
   HostStatus *hstat = nullptr;  // pinned, written by k_status
+  unsigned hstat_seq = 0;       // number of the last status record requested
   int substep_guess = 3; // sub-iterations queued before the first check
   // collisions
   int *rows = nullptr;      // np x CONTACT_CAP colliding partners, ascending

@@ -469,11 +469,23 @@ __global__ __launch_bounds__((1 << LOGN) >> dct_logr<LOGN>()) void k_dct3_psi_t(
 #pragma unroll
   for (int s = 0; s < R; s++) re[sw(ml.pos(threadIdx.x, s))] = xr[s];
   __syncthreads();
+  // psi row j and its boundary values (the Neumann walls, and the periodic
+  // halo rows 0 and ny+1 copied from rows ny and 1: k_bc_p's values)
+  const int j = j0 + 1;
 #pragma unroll
   for (int s = 0; s < R; s++) {
     const int i = threadIdx.x + s * T;
     const double v = (i & 1) ? re[sw(nx - 1 - (i >> 1))] : re[sw(i >> 1)];
-    psi[L.p(i + 1, j0 + 1)] = v * inv_norm;
+    const double w = v * inv_norm;
+    psi[L.p(i + 1, j)] = w;
+    if (j == ny) psi[L.p(i + 1, 0)] = w;
+    if (j == 1) psi[L.p(i + 1, ny + 1)] = w;
+    if (i == 0 || i == nx - 1) {
+      const int iw = i == 0 ? 0 : nx + 1;
+      psi[L.p(iw, j)] = w;
+      if (j == ny) psi[L.p(iw, 0)] = w;
+      if (j == 1) psi[L.p(iw, ny + 1)] = w;
+    }
   }
 }
 
@@ -873,10 +885,12 @@ int launch_poisson(Ctx *c, int rk) {
     ca.q1 = c->th_q1;
     ca.last = c->th_last;
   }
+  bool halo_done = false;  // the register-FFT kernel writes psi's boundary values
   {
     ProfScope ps(c, "dct3_psi");
 #define EIF_DCT3_T(LG)                                                                  \
   case LG:                                                                              \
+    halo_done = true;                                                                   \
     hipLaunchKernelGGL(k_dct3_psi_t<LG>, dim3(ny), dim3(nx >> dct_logr<LG>()), shmem,    \
                        c->stream, c->L,                                                 \
                        c->qhat, ca, c->ctab, c->twid, c->psi);                          \
@@ -891,7 +905,7 @@ int launch_poisson(Ctx *c, int rk) {
 #undef EIF_DCT3_T
     EIF_HIP(hipGetLastError());
   }
-  return launch_bc_p(c, c->psi);
+  return halo_done ? EIF_OK : launch_bc_p(c, c->psi);
 }
 
 }  // namespace eif

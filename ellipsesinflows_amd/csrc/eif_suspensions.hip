@@ -1636,9 +1636,11 @@ int launch_increment(Ctx *c, int rk, int sub) {
 }
 
 // one thread copies the state of the first `queued` sub-iterations into
-// pinned host memory: how many ran, and the residual of the last one
+// pinned host memory: how many ran, and the residual of the last one.  The
+// record number is stored last with system-scope release, so a host that
+// sees it sees the rest.
 __global__ void k_status(const double *res, int queued, const int *flags,
-                         const double *dscal, HostStatus *h) {
+                         const double *dscal, HostStatus *h, unsigned seq) {
   int done = 0;
   double r = 0.;
   for (int s = 0; s < queued && sub_open(res, s); s++) {
@@ -1649,11 +1651,12 @@ __global__ void k_status(const double *res, int queued, const int *flags,
   h->dt = dscal[S_DT];
   h->substeps = done;
   h->flags = flags[0];
+  __hip_atomic_store(&h->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 int launch_status(Ctx *c, int queued) {
   hipLaunchKernelGGL(k_status, dim3(1), dim3(1), 0, c->stream, c->res_slots, queued,
-                     c->flags, c->dscal, c->hstat);
+                     c->flags, c->dscal, c->hstat, ++c->hstat_seq);
   EIF_HIP(hipGetLastError());
   return EIF_OK;
 }
