@@ -387,8 +387,13 @@ __global__ __launch_bounds__(1024) void k_dct3_psi(
   }
 }
 
-// Register-FFT kernels specialised for nx = 2^LOGN (nx / 2^LOGR threads);
-// the arithmetic is that of the mode-2 branches of k_rhs_dct2 / k_dct3_psi.
+// Register-FFT kernels specialised for nx = 2^LOGN (nx / 2^LOGR threads),
+// two rows per workgroup through one complex FFT: row a is the real and
+// row b the imaginary part.  With Z the transform and kk = (n - k) mod n,
+// row a's spectrum is (Z[k] + conj Z[kk]) / 2 and row b's
+// (Z[k] - conj Z[kk]) / 2i (the halves cancel against REDFT10's factor 2);
+// the inverse takes the two pre-twiddled spectra as Va + i Vb.  An odd last
+// row pairs with a zero row.
 template <int LOGN, bool P2>
 __global__ __launch_bounds__((1 << LOGN) >> dct_logr<LOGN>()) void k_rhs_dct2_t(
     Layout L, const double *ux, const double *uy, const double *dscal, double ga,
@@ -398,23 +403,25 @@ __global__ __launch_bounds__((1 << LOGN) >> dct_logr<LOGN>()) void k_rhs_dct2_t(
   constexpr int LOGR = dct_logr<LOGN>(), R = 1 << LOGR;
   constexpr int nx = 1 << LOGN;
   constexpr int T = nx / R;
-  const int j = blockIdx.x + 1;
+  const int ja = 2 * blockIdx.x + 1, jb = ja + 1;
+  const bool hasb = jb <= L.ny;
   double *re = sh, *im = sh + nx;
   const double dt = dscal[S_DT];
   const double fac = 1. / (ga * dt);
-  double *out = qhat + (long)(j - 1) * nx;
 #pragma unroll
   for (int s = 0; s < R; s++) {
     const int m = threadIdx.x + s * T;
     const int i = m < nx / 2 ? 2 * m : 2 * (nx - 1 - m) + 1;  // Makhoul order
-    re[sw(bitrev(m, LOGN))] = rhs_value<P2>(L, ux, uy, fac, dx, dy, i, j);
+    const int q = sw(bitrev(m, LOGN));
+    re[q] = rhs_value<P2>(L, ux, uy, fac, dx, dy, i, ja);
+    im[q] = hasb ? rhs_value<P2>(L, ux, uy, fac, dx, dy, i, jb) : 0.;
   }
   __syncthreads();
   double xr[R], xi[R];
 #pragma unroll
   for (int s = 0; s < R; s++) {
     xr[s] = re[sw(R * threadIdx.x + s)];
-    xi[s] = 0.;
+    xi[s] = im[sw(R * threadIdx.x + s)];
   }
   reg_fft_t<LOGN, LOGR, -1>(xr, xi, re, im, W);
   constexpr int NP = (LOGN + LOGR - 1) / LOGR;
@@ -427,10 +434,13 @@ __global__ __launch_bounds__((1 << LOGN) >> dct_logr<LOGN>()) void k_rhs_dct2_t(
     im[q] = xi[s];
   }
   __syncthreads();
+  double *outa = qhat + (long)(ja - 1) * nx, *outb = outa + nx;
 #pragma unroll
   for (int s = 0; s < R; s++) {
-    const int k = threadIdx.x + s * T;
-    out[k] = 2. * (C[k] * re[sw(k)] + C[nx - k] * im[sw(k)]);
+    const int k = threadIdx.x + s * T, kk = (nx - k) & (nx - 1);
+    const double zr = re[sw(k)], zi = im[sw(k)], wr = re[sw(kk)], wi = im[sw(kk)];
+    outa[k] = C[k] * (zr + wr) + C[nx - k] * (zi - wi);
+    if (hasb) outb[k] = C[k] * (zi + wi) + C[nx - k] * (wr - zr);
   }
 }
 
@@ -443,48 +453,61 @@ __global__ __launch_bounds__((1 << LOGN) >> dct_logr<LOGN>()) void k_dct3_psi_t(
   constexpr int nx = 1 << LOGN;
   constexpr int T = nx / R;
   const int ny = L.ny;
-  const int j0 = blockIdx.x;
+  const int ja0 = 2 * blockIdx.x, jb0 = ja0 + 1;
+  const bool hasb = jb0 < ny;
   double *re = sh, *im = sh + nx;
   // division by 2 nx, a power of two, is the exact product with its inverse
   constexpr double inv_norm = 1. / (2. * nx);
 #pragma unroll
   for (int s = 0; s < R; s++) {
     const int k = threadIdx.x + s * T;
-    re[sw(k)] = combined(ca, qhat, nx, ny, j0, k);
+    re[sw(k)] = combined(ca, qhat, nx, ny, ja0, k);
+    im[sw(k)] = hasb ? combined(ca, qhat, nx, ny, jb0, k) : 0.;
   }
   __syncthreads();
   double xr[R], xi[R];
 #pragma unroll
   for (int s = 0; s < R; s++) {
     const int k = bitrev(R * threadIdx.x + s, LOGN);
-    const double vr = re[sw(k)];
-    const double vi = k == 0 ? 0. : -re[sw(nx - k)];
-    xr[s] = C[k] * vr - C[nx - k] * vi;
-    xi[s] = C[k] * vi + C[nx - k] * vr;
+    const double ar = re[sw(k)], ai = k == 0 ? 0. : -re[sw(nx - k)];
+    const double br = im[sw(k)], bi = k == 0 ? 0. : -im[sw(nx - k)];
+    const double var = C[k] * ar - C[nx - k] * ai, vai = C[k] * ai + C[nx - k] * ar;
+    const double vbr = C[k] * br - C[nx - k] * bi, vbi = C[k] * bi + C[nx - k] * br;
+    xr[s] = var - vbi;
+    xi[s] = vai + vbr;
   }
   reg_fft_t<LOGN, LOGR, +1>(xr, xi, re, im, W);
   constexpr int NP = (LOGN + LOGR - 1) / LOGR;
   const RegMap ml = reg_phase_r<LOGR>(NP - 1, LOGN);
   __syncthreads();
 #pragma unroll
-  for (int s = 0; s < R; s++) re[sw(ml.pos(threadIdx.x, s))] = xr[s];
-  __syncthreads();
-  // psi row j and its boundary values (the Neumann walls, and the periodic
-  // halo rows 0 and ny+1 copied from rows ny and 1: k_bc_p's values)
-  const int j = j0 + 1;
-#pragma unroll
   for (int s = 0; s < R; s++) {
-    const int i = threadIdx.x + s * T;
-    const double v = (i & 1) ? re[sw(nx - 1 - (i >> 1))] : re[sw(i >> 1)];
-    const double w = v * inv_norm;
-    psi[L.p(i + 1, j)] = w;
-    if (j == ny) psi[L.p(i + 1, 0)] = w;
-    if (j == 1) psi[L.p(i + 1, ny + 1)] = w;
-    if (i == 0 || i == nx - 1) {
-      const int iw = i == 0 ? 0 : nx + 1;
-      psi[L.p(iw, j)] = w;
-      if (j == ny) psi[L.p(iw, 0)] = w;
-      if (j == 1) psi[L.p(iw, ny + 1)] = w;
+    const int q = sw(ml.pos(threadIdx.x, s));
+    re[q] = xr[s];
+    im[q] = xi[s];
+  }
+  __syncthreads();
+  // psi rows and their boundary values (the Neumann walls, and the periodic
+  // halo rows 0 and ny+1 copied from rows ny and 1: k_bc_p's values)
+#pragma unroll
+  for (int r = 0; r < 2; r++) {
+    if (r == 1 && !hasb) break;
+    const double *src = r == 0 ? re : im;
+    const int j = ja0 + r + 1;
+#pragma unroll
+    for (int s = 0; s < R; s++) {
+      const int i = threadIdx.x + s * T;
+      const double v = (i & 1) ? src[sw(nx - 1 - (i >> 1))] : src[sw(i >> 1)];
+      const double w = v * inv_norm;
+      psi[L.p(i + 1, j)] = w;
+      if (j == ny) psi[L.p(i + 1, 0)] = w;
+      if (j == 1) psi[L.p(i + 1, ny + 1)] = w;
+      if (i == 0 || i == nx - 1) {
+        const int iw = i == 0 ? 0 : nx + 1;
+        psi[L.p(iw, j)] = w;
+        if (j == ny) psi[L.p(iw, 0)] = w;
+        if (j == 1) psi[L.p(iw, ny + 1)] = w;
+      }
     }
   }
 }
@@ -836,7 +859,8 @@ int launch_poisson(Ctx *c, int rk) {
   case LG:                                                                               \
     {                                                                                    \
       auto kern = c->pow2_grid ? k_rhs_dct2_t<LG, true> : k_rhs_dct2_t<LG, false>;       \
-      hipLaunchKernelGGL(kern, dim3(ny), dim3(nx >> dct_logr<LG>()), shmem, c->stream,   \
+      hipLaunchKernelGGL(kern, dim3((ny + 1) / 2), dim3(nx >> dct_logr<LG>()), shmem,    \
+                         c->stream,                                                      \
                          c->L, c->ux,                                                    \
                          c->uy, c->dscal, c->gamma[rk], c->dx, c->dy, c->ctab, c->twid,  \
                          c->qhat);                                                       \
@@ -891,7 +915,8 @@ int launch_poisson(Ctx *c, int rk) {
 #define EIF_DCT3_T(LG)                                                                  \
   case LG:                                                                              \
     halo_done = true;                                                                   \
-    hipLaunchKernelGGL(k_dct3_psi_t<LG>, dim3(ny), dim3(nx >> dct_logr<LG>()), shmem,    \
+    hipLaunchKernelGGL(k_dct3_psi_t<LG>, dim3((ny + 1) / 2), dim3(nx >> dct_logr<LG>()),  \
+                       shmem,                                                           \
                        c->stream, c->L,                                                 \
                        c->qhat, ca, c->ctab, c->twid, c->psi);                          \
     break;

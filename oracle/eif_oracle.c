@@ -503,6 +503,71 @@ static void dct3_tab(int n, const double *C, double *x, double *re, double *im) 
   }
 }
 
+/* Two real rows through one complex FFT (n a power of two): row a is the
+ * real and row b the imaginary part of the Makhoul-ordered input.  With Z
+ * the transform and kk = (n - k) mod n, row a's spectrum is
+ * (Z[k] + conj Z[kk]) / 2 and row b's (Z[k] - conj Z[kk]) / 2i; the halves
+ * cancel against REDFT10's factor 2. */
+static void dct2_pair_tab(int n, const double *C, double *xa, double *xb, double *re,
+                          double *im) {
+  for (int j = 0; j < n / 2; j++) {
+    re[j] = xa[2 * j];
+    re[n - 1 - j] = xa[2 * j + 1];
+    im[j] = xb[2 * j];
+    im[n - 1 - j] = xb[2 * j + 1];
+  }
+  fft_pow2(n, C, re, im, -1);
+  for (int k = 0; k < n; k++) {
+    const int kk = (n - k) & (n - 1);
+    xa[k] = C[k] * (re[k] + re[kk]) + C[n - k] * (im[k] - im[kk]);
+    xb[k] = C[k] * (im[k] + im[kk]) + C[n - k] * (re[kk] - re[k]);
+  }
+}
+
+/* The inverse pair: the pre-twiddled spectra Va (row a) and Vb (row b) go
+ * in as Va + i Vb, and the two real rows come out as the real and the
+ * imaginary part. */
+static void dct3_pair_tab(int n, const double *C, double *xa, double *xb, double *re,
+                          double *im) {
+  for (int k = 0; k < n; k++) {
+    const double ar = xa[k], ai = k == 0 ? 0. : -xa[n - k];
+    const double br = xb[k], bi = k == 0 ? 0. : -xb[n - k];
+    const double var = C[k] * ar - C[n - k] * ai, vai = C[k] * ai + C[n - k] * ar;
+    const double vbr = C[k] * br - C[n - k] * bi, vbi = C[k] * bi + C[n - k] * br;
+    re[k] = var - vbi;
+    im[k] = vai + vbr;
+  }
+  fft_pow2(n, C, re, im, +1);
+  for (int j = 0; j < n / 2; j++) {
+    xa[2 * j] = re[j];
+    xa[2 * j + 1] = re[n - 1 - j];
+    xb[2 * j] = im[j];
+    xb[2 * j + 1] = im[n - 1 - j];
+  }
+}
+
+/* row lengths whose Poisson solve pairs rows (the device's register-FFT
+ * sizes) */
+static int pairs_rows(int n) { return is_pow2(n) && n >= 64 && n <= 8192; }
+
+int orc_dct2_pair(int n, double *xa, double *xb) {
+  if (!is_pow2(n)) return -1;
+  double *C = cos_table(n);
+  double *re = xcalloc(n, sizeof(double)), *im = xcalloc(n, sizeof(double));
+  dct2_pair_tab(n, C, xa, xb, re, im);
+  free(C); free(re); free(im);
+  return 0;
+}
+
+int orc_dct3_pair(int n, double *xa, double *xb) {
+  if (!is_pow2(n)) return -1;
+  double *C = cos_table(n);
+  double *re = xcalloc(n, sizeof(double)), *im = xcalloc(n, sizeof(double));
+  dct3_pair_tab(n, C, xa, xb, re, im);
+  free(C); free(re); free(im);
+  return 0;
+}
+
 int orc_dct2(int n, double *x) {
   double *C = cos_table(n);
   double *re = xcalloc(n, sizeof(double)), *im = xcalloc(n, sizeof(double));
@@ -577,9 +642,11 @@ int orc_compute_potential(orc_t *s, int rk) {
   const double *ux = s->f[ORC_UX], *uy = s->f[ORC_UY];
   double *qx = s->qx, *qy = s->qy, *psi = s->f[ORC_PSI];
   double *C = cos_table(nx);
+  const int paired = pairs_rows(nx);
 #pragma omp parallel
   {
   double *re = xcalloc(nx, sizeof(double)), *im = xcalloc(nx, sizeof(double));
+  double *zero = xcalloc(nx, sizeof(double));
 #pragma omp for schedule(static)
   for (int j = 1; j <= ny; j++) {
     for (int i = 1; i <= nx; i++) {
@@ -590,9 +657,21 @@ int orc_compute_potential(orc_t *s, int rk) {
       qx[(long)(j - 1) * nx + (i - 1)] =
           1. / (gamma * dt) * (+(ux_xp - ux_xm) / dx + (uy_yp - uy_ym) / dy);
     }
-    dct2_tab(nx, C, &qx[(long)(j - 1) * nx], re, im);
   }
-  free(re); free(im);
+  /* rows 2p and 2p+1 (from 0) share one FFT; an odd last row pairs with a
+   * zero row */
+#pragma omp for schedule(static)
+  for (int jp = 0; jp < (paired ? (ny + 1) / 2 : ny); jp++) {
+    if (!paired) {
+      dct2_tab(nx, C, &qx[(long)jp * nx], re, im);
+      continue;
+    }
+    double *ra = &qx[(long)(2 * jp) * nx];
+    double *rb = 2 * jp + 1 < ny ? &qx[(long)(2 * jp + 1) * nx] : zero;
+    if (rb == zero) memset(zero, 0, sizeof(double) * nx);
+    dct2_pair_tab(nx, C, ra, rb, re, im);
+  }
+  free(re); free(im); free(zero);
   }
 #pragma omp parallel for schedule(static)
   for (int i = 0; i < nx; i++)
@@ -621,14 +700,23 @@ int orc_compute_potential(orc_t *s, int rk) {
 #pragma omp parallel
   {
   double *re = xcalloc(nx, sizeof(double)), *im = xcalloc(nx, sizeof(double));
+  double *zero = xcalloc(nx, sizeof(double));
 #pragma omp for schedule(static)
-  for (int j = 1; j <= ny; j++) {
-    double *row = &qx[(long)(j - 1) * nx];
-    dct3_tab(nx, C, row, re, im);
-    for (int i = 1; i <= nx; i++) psi[IX_P(i, j)] = row[i - 1] / (2. * nx);
+  for (int jp = 0; jp < (paired ? (ny + 1) / 2 : ny); jp++) {
+    if (!paired) {
+      dct3_tab(nx, C, &qx[(long)jp * nx], re, im);
+      continue;
+    }
+    double *ra = &qx[(long)(2 * jp) * nx];
+    double *rb = 2 * jp + 1 < ny ? &qx[(long)(2 * jp + 1) * nx] : zero;
+    if (rb == zero) memset(zero, 0, sizeof(double) * nx);
+    dct3_pair_tab(nx, C, ra, rb, re, im);
   }
-  free(re); free(im);
+  free(re); free(im); free(zero);
   }
+#pragma omp parallel for schedule(static)
+  for (int j = 1; j <= ny; j++)
+    for (int i = 1; i <= nx; i++) psi[IX_P(i, j)] = qx[(long)(j - 1) * nx + i - 1] / (2. * nx);
   free(C);
   orc_update_boundaries_p(s, psi);
   return 0;

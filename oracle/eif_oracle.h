@@ -113,6 +113,10 @@ int orc_decide_loop_size(int lbound, int ubound, double grid_size,
 /* FFTW_REDFT10 / FFTW_REDFT01 (unnormalised DCT-II / DCT-III), in place */
 int orc_dct2(int n, double *x);
 int orc_dct3(int n, double *x);
+/* the same for two rows at once through one complex FFT (n a power of two;
+ * the Poisson solve's form for 64 <= n <= 8192) */
+int orc_dct2_pair(int n, double *xa, double *xb);
+int orc_dct3_pair(int n, double *xa, double *xb);
 /* periodic tridiagonal solve, bordering method of src/tdm.c:64-108 */
 int orc_tdm_periodic(int n, const double *l, const double *c, const double *u,
                      double *q);

@@ -60,6 +60,8 @@ def oracle_lib():
         lib.orc_tdm_periodic.argtypes = [ctypes.c_int, _PD, _PD, _PD, _PD]
         lib.orc_dct2.argtypes = [ctypes.c_int, _PD]
         lib.orc_dct3.argtypes = [ctypes.c_int, _PD]
+        lib.orc_dct2_pair.argtypes = [ctypes.c_int, _PD, _PD]
+        lib.orc_dct3_pair.argtypes = [ctypes.c_int, _PD, _PD]
         lib.orc_set_threads.argtypes = [ctypes.c_int]
         lib.orc_decide_loop_size.argtypes = [ctypes.c_int, ctypes.c_int, _D, _D, _D, _PI, _PI]
         _olib = lib

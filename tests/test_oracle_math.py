@@ -46,6 +46,28 @@ def test_dct_definitions_against_long_double_sums(n):
     z = y2.copy()
     lib.orc_dct3(n, _ptr(z))
     assert np.max(np.abs(z / (2 * n) - x)) < 1e-13 * np.abs(x).max() * np.log2(n)
+    if n & (n - 1):
+        return
+    # two rows through one complex FFT (the Poisson solve's form): each row
+    # matches its own sums, and swapping the rows swaps the results exactly
+    xb = rng.standard_normal(n)
+    refb2 = 2 * np.sum(xb[None, :].astype(np.longdouble) * np.cos(pi * (j[None, :] + 0.5) * k / n), axis=1)
+    refb3 = xb[0] + 2 * np.sum(xb[None, 1:].astype(np.longdouble) * np.cos(pi * j[None, 1:] * (k + 0.5) / n), axis=1)
+    tolb = 8e-16 * np.sqrt(n) * (np.abs(x).sum() + np.abs(xb).sum())
+    for fn, ra, rb in ((lib.orc_dct2_pair, ref2, refb2), (lib.orc_dct3_pair, ref3, refb3)):
+        pa, pb = x.copy(), xb.copy()
+        assert fn(n, _ptr(pa), _ptr(pb)) == 0
+        assert np.max(np.abs(pa - ra.astype(np.float64))) < tolb
+        assert np.max(np.abs(pb - rb.astype(np.float64))) < tolb
+        qa, qb = xb.copy(), x.copy()
+        fn(n, _ptr(qa), _ptr(qb))
+        assert np.max(np.abs(qa - rb.astype(np.float64))) < tolb
+        assert np.max(np.abs(qb - ra.astype(np.float64))) < tolb
+    # a zero partner: the row's result to rounding, the partner's ~0
+    pa, pb = x.copy(), np.zeros(n)
+    lib.orc_dct2_pair(n, _ptr(pa), _ptr(pb))
+    assert np.max(np.abs(pa - ref2.astype(np.float64))) < tol
+    assert np.max(np.abs(pb)) < tol
 
 
 # ------------------------------------------------------- periodic tridiagonal
