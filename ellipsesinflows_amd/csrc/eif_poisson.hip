@@ -8,9 +8,11 @@
 // reference's two MPI transposes (:94, :118) have no counterpart.
 //
 // DCT-II / DCT-III (FFTW REDFT10 / REDFT01): Makhoul's n-point complex FFT
-// (radix-2, decimation in time) in LDS for powers of two, the direct sum
-// otherwise.  Every trigonometric factor comes from the table
-// C[m] = cos(pi m/(2n)) built on the host.
+// (radix-2, decimation in time) for powers of two, the direct sum
+// otherwise.  For 64 <= n <= 8192 the FFT runs in registers and two rows
+// share one complex transform (real and imaginary part); other powers of
+// two use one row per transform in LDS.  Every trigonometric factor comes
+// from the table C[m] = cos(pi m/(2n)) built on the host.
 //
 // y-direction periodic tridiagonal solve, two modes:
 //   EIF_POISSON_THOMAS     the bordering method of src/tdm.c:64-108 with its
@@ -64,10 +66,10 @@ __device__ void lds_fft(double *re, double *im, int n, int logn,
   }
 }
 
-// Register form of the same radix-2 DIT: n/8 threads each hold 8 complex
-// values; stages are taken three at a time (groups of 8 positions closed
-// under those stages), with LDS used only to regroup between phases.  Every
-// butterfly sees the same inputs and performs the same operations as
+// Register form of the same radix-2 DIT: n/2^LOGR threads each hold 2^LOGR
+// complex values; stages are taken LOGR at a time (groups of positions
+// closed under those stages), with LDS used only to regroup between phases.
+// Every butterfly sees the same inputs and performs the same operations as
 // lds_fft, so the results are identical.
 struct RegMap {
   int lg, lh0, lgpt;  // log2 of group size, first butterfly distance, groups/thread
@@ -77,17 +79,7 @@ struct RegMap {
   }
 };
 
-__device__ inline RegMap reg_phase(int p, int logn) {
-  RegMap m;
-  m.lg = min(3, logn - 3 * p);
-  m.lh0 = 3 * p;
-  m.lgpt = 3 - m.lg;
-  return m;
-}
-
-__device__ inline int reg_nphase(int logn) { return (logn + 2) / 3; }
-
-// the same maps with 2^LOGR values per thread (stages taken LOGR at a time)
+// the position map of phase p (stages LOGR p .. LOGR p + lg - 1)
 template <int LOGR>
 __device__ inline RegMap reg_phase_r(int p, int logn) {
   RegMap m;
@@ -102,65 +94,11 @@ __device__ inline RegMap reg_phase_r(int p, int logn) {
 // hit at most two-way bank conflicts
 __device__ inline int sw(int p) { return p ^ ((p >> 4) & 15) ^ ((p >> 8) & 15); }
 
-// W: per-stage twiddles, stage len occupies entries [len/2-1, len-1):
-// W[2e] = C[4m], W[2e+1] = C[|n-4m|] with m = k n/len (the same table values
-// lds_fft reads from C)
-template <int SIGN>
-__device__ void reg_fft(double (&xr)[8], double (&xi)[8], double *lre, double *lim, int n,
-                        int logn, const double *__restrict__ W) {
-  const int t = threadIdx.x;
-  const int np = reg_nphase(logn);
-  for (int p = 0; p < np; p++) {
-    const RegMap mp = reg_phase(p, logn);
-    if (p > 0) {
-      const RegMap mq = reg_phase(p - 1, logn);
-      __syncthreads();
-#pragma unroll
-      for (int s = 0; s < 8; s++) {
-        const int q = sw(mq.pos(t, s));
-        lre[q] = xr[s];
-        lim[q] = xi[s];
-      }
-      __syncthreads();
-#pragma unroll
-      for (int s = 0; s < 8; s++) {
-        const int q = sw(mp.pos(t, s));
-        xr[s] = lre[q];
-        xi[s] = lim[q];
-      }
-    }
-    const int r = mp.lg;
-    const int gmask = (1 << r) - 1;
-    for (int ss = 0; ss < r; ss++) {
-      const int half = 1 << (mp.lh0 + ss), len = half << 1;
-      const double *Ws = W + 2 * (half - 1);
-#pragma unroll
-      for (int s = 0; s < 8; s++) {
-        const int e = s & gmask;
-        if (e & (1 << ss)) continue;
-        if ((e | (1 << ss)) > gmask) continue;
-        const int s1 = s + (1 << ss);
-        const int kk = mp.pos(t, s) & (len - 1);
-        const double wr = Ws[2 * kk];
-        const double wi = SIGN * Ws[2 * kk + 1];
-        const double r1 = xr[s1], m1 = xi[s1];
-        const double tr = wr * r1 - wi * m1;
-        const double ti = wr * m1 + wi * r1;
-        const double r0 = xr[s], m0 = xi[s];
-        xr[s1] = r0 - tr;
-        xi[s1] = m0 - ti;
-        xr[s] = r0 + tr;
-        xi[s] = m0 + ti;
-      }
-    }
-  }
-}
-
-// The same transform with n = 2^LOGN known at compile time and 2^LOGR
-// values per thread: phases and stages unroll, so the slot -> position maps
-// and twiddle offsets fold to constants plus a per-thread base.
-// Butterflies, operand order and twiddle values are those of reg_fft (only
-// the grouping of stages into register phases differs).
+// The transform with n = 2^LOGN known at compile time: phases and stages
+// unroll, so the slot -> position maps and twiddle offsets fold to constants
+// plus a per-thread base.  W holds the per-stage twiddles, stage len in
+// entries [len/2-1, len-1): W[2e] = C[4m], W[2e+1] = C[|n-4m|] with
+// m = k n/len (the table values lds_fft reads from C).
 template <int LOGN, int LOGR, int SIGN>
 __device__ inline void reg_fft_t(double (&xr)[1 << LOGR], double (&xi)[1 << LOGR], double *lre,
                                  double *lim, const double *__restrict__ W) {
@@ -231,8 +169,8 @@ __device__ inline double cos_lookup(const double *C, int n, long m) {
 
 // ---------------------------------------------------------------------
 // RHS (compute_potential.c:70-82) + DCT-II along x (:84-92), one row per
-// workgroup.  mode 2: register FFT (blockDim = nx/8), mode 1: LDS FFT,
-// mode 0: direct sum (nx not a power of two).
+// workgroup: mode 1 LDS FFT (powers of two outside the register-FFT
+// range), mode 0 direct sum (nx not a power of two).
 // ---------------------------------------------------------------------
 template <bool P2 = false>
 __device__ inline double rhs_value(const Layout &L, const double *ux, const double *uy,
@@ -252,32 +190,7 @@ __global__ __launch_bounds__(1024) void k_rhs_dct2(
   const double dt = dscal[S_DT];
   const double fac = 1. / (ga * dt);
   double *out = qhat + (long)(j - 1) * nx;
-  if (mode == 2) {
-    const int T = blockDim.x;
-    for (int m = threadIdx.x; m < nx; m += T) {
-      const int i = m < nx / 2 ? 2 * m : 2 * (nx - 1 - m) + 1;  // Makhoul order
-      re[sw(bitrev(m, logn))] = rhs_value(L, ux, uy, fac, dx, dy, i, j);
-    }
-    __syncthreads();
-    double xr[8], xi[8];
-#pragma unroll
-    for (int s = 0; s < 8; s++) {
-      xr[s] = re[sw(8 * threadIdx.x + s)];
-      xi[s] = 0.;
-    }
-    reg_fft<-1>(xr, xi, re, im, nx, logn, W);
-    const RegMap ml = reg_phase(reg_nphase(logn) - 1, logn);
-    __syncthreads();
-#pragma unroll
-    for (int s = 0; s < 8; s++) {
-      const int q = sw(ml.pos(threadIdx.x, s));
-      re[q] = xr[s];
-      im[q] = xi[s];
-    }
-    __syncthreads();
-    for (int k = threadIdx.x; k < nx; k += T)
-      out[k] = 2. * (C[k] * re[sw(k)] + C[nx - k] * im[sw(k)]);
-  } else if (mode == 1) {
+  if (mode == 1) {
     for (int i = threadIdx.x; i < nx; i += blockDim.x) {
       const int m = (i & 1) ? nx - 1 - (i >> 1) : (i >> 1);
       const int pos = bitrev(m, logn);
@@ -337,30 +250,7 @@ __global__ __launch_bounds__(1024) void k_dct3_psi(
   const int j0 = blockIdx.x;
   double *re = sh, *im = sh + nx;
   const double norm = 2. * nx;
-  if (mode == 2) {
-    const int T = blockDim.x;
-    for (int k = threadIdx.x; k < nx; k += T) re[sw(k)] = combined(ca, qhat, nx, ny, j0, k);
-    __syncthreads();
-    double xr[8], xi[8];
-#pragma unroll
-    for (int s = 0; s < 8; s++) {
-      const int k = bitrev(8 * threadIdx.x + s, logn);
-      const double vr = re[sw(k)];
-      const double vi = k == 0 ? 0. : -re[sw(nx - k)];
-      xr[s] = C[k] * vr - C[nx - k] * vi;
-      xi[s] = C[k] * vi + C[nx - k] * vr;
-    }
-    reg_fft<+1>(xr, xi, re, im, nx, logn, W);
-    const RegMap ml = reg_phase(reg_nphase(logn) - 1, logn);
-    __syncthreads();
-#pragma unroll
-    for (int s = 0; s < 8; s++) re[sw(ml.pos(threadIdx.x, s))] = xr[s];
-    __syncthreads();
-    for (int i = threadIdx.x; i < nx; i += T) {
-      const double v = (i & 1) ? re[sw(nx - 1 - (i >> 1))] : re[sw(i >> 1)];
-      psi[L.p(i + 1, j0 + 1)] = v / norm;
-    }
-  } else if (mode == 1) {
+  if (mode == 1) {
     for (int k = threadIdx.x; k < nx; k += blockDim.x) {
       const double xr = combined(ca, qhat, nx, ny, j0, k);
       const double xi = k == 0 ? 0. : -combined(ca, qhat, nx, ny, j0, nx - k);
@@ -852,7 +742,7 @@ int launch_poisson(Ctx *c, int rk) {
   const size_t shmem = 2 * (size_t)nx * sizeof(double);
   // register FFT for 64 <= nx <= 8192, LDS FFT for other powers of two
   const int mode = logn < 0 ? 0 : ((nx >= 64 && nx <= 8192) ? 2 : 1);
-  const int threads = mode == 2 ? nx / 8 : DCT_THREADS;
+  const int threads = DCT_THREADS;  // of the one-row kernels (modes 0, 1)
   {
     ProfScope ps(c, "rhs_dct2");
 #define EIF_RHS_T(LG)                                                                    \
