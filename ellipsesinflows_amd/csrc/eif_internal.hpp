@@ -138,7 +138,7 @@ struct Ctx {
   long scratch_cells = 0;
   int box_cap_x = 0, box_cap_y = 0;
   double *sc_fx = nullptr, *sc_fy = nullptr;  // per box cell forcing*dt
-  double *sc_dist = nullptr;                  // cached signed distance
+  double *sc_dist = nullptr;                  // cached tanh(BETA d), d the signed distance
   double *sc_t = nullptr;                     // foot parameter per box cell
   int *sc_key = nullptr;                      // np x 8: boxes of sc_t
   double *sc_pos = nullptr;                   // np x 3: positions of cache

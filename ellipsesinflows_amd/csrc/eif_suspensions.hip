@@ -349,11 +349,13 @@ __device__ inline double tanh_w(double x) {
   const double th = 1. - 2. / (exp(2. * fabs(x)) + 1.);
   return copysign(th, x);
 }
-__device__ inline double v_weight(double d) { return 0.5 * (1. + tanh_w(BETA * d)); }
-__device__ inline double s_weight(double d) {
-  const double th = tanh_w(BETA * d);
-  return 0.5 * BETA * (1. - th * th);
-}
+// Both weights are functions of th = tanh(BETA d); the per-cell cache keeps
+// th, so a cached cell needs no exp.
+__device__ inline double weight_th(double d) { return tanh_w(BETA * d); }
+__device__ inline double v_weight_th(double th) { return 0.5 * (1. + th); }
+__device__ inline double s_weight_th(double th) { return 0.5 * BETA * (1. - th * th); }
+__device__ inline double v_weight(double d) { return v_weight_th(weight_th(d)); }
+__device__ inline double s_weight(double d) { return s_weight_th(weight_th(d)); }
 
 // mass and moment of inertia, src/suspensions/auxiliary.c:63-75
 __device__ inline double p_mass(const double *q) {
@@ -392,7 +394,9 @@ struct IbmArgs {
   const double *dscal;
   Box *boxes;
   const long *box_off;
-  double *sc_fx, *sc_fy, *sc_dist, *sc_pos;
+  double *sc_fx, *sc_fy;
+  double *sc_dist;  // per box cell: tanh(BETA d) of the signed distance d
+  double *sc_pos;
   double *sc_t;   // per box cell: foot parameter of the last evaluation
   int *sc_key;    // per particle: boxes of that evaluation (8 ints)
   int *tile_cnt, *tile_list, *active;
@@ -540,15 +544,15 @@ __device__ __forceinline__ void inertia_slice(const IbmArgs &a, int cnstep, int 
     cell_of(bs, t, k, i, j);
     const double py_ = py + a.ly * (k - 1);
     const double x = a.xc[i], y = a.yc[j - 1];
-    double d;
+    double th;
     if (cached) {
-      d = dist[t];
+      th = dist[t];
     } else {
-      d = signed_dist_warm(e, f, 1. / a.grid_size, px, py_, cr, sr, x, y, warm, tcell[t], it32,
-                           it64);
-      dist[t] = d;
+      th = weight_th(signed_dist_warm(e, f, 1. / a.grid_size, px, py_, cr, sr, x, y, warm,
+                                      tcell[t], it32, it64));
+      dist[t] = th;
     }
-    const double w = v_weight(d);
+    const double w = v_weight_th(th);
     const double valx = w * 0.5 * (a.ux[a.L.ux(i, j)] + a.ux[a.L.ux(i + 1, j)]) * dxdy;
     const double valy = w * 0.5 * (a.uy[a.L.p(i, j)] + a.uy[a.L.p(i, j + 1)]) * dxdy;
     acc[0] += valx * inv_pm;
@@ -733,8 +737,8 @@ __global__ __launch_bounds__(64) void k_ibm_dist(IbmArgs a, int with_dx) {
       int k, i, j;
       cell_of(bs, t, k, i, j);
       const double py_ = py + a.ly * (k - 1);
-      dist[t] = signed_dist_warm(e, f, 1. / a.grid_size, px, py_, cr, sr, a.xc[i],
-                                 a.yc[j - 1], warm, tcell[t], it32, it64);
+      dist[t] = weight_th(signed_dist_warm(e, f, 1. / a.grid_size, px, py_, cr, sr, a.xc[i],
+                                           a.yc[j - 1], warm, tcell[t], it32, it64));
     }
   }
   const double c64 = wave_sum((double)it64), c32 = wave_sum((double)it32);
@@ -811,7 +815,7 @@ __global__ __launch_bounds__(64) void k_exchange(IbmArgs a, int own_counts) {
     cell_of(bs, t, k, i, j);
     const double py_ = py + a.ly * (k - 1);
     const double x = a.xc[i], y = a.yc[j - 1];
-    const double w = s_weight(dist[t]);
+    const double w = s_weight_th(dist[t]);
     const double ux_p = pux - pvz * (y - py_);
     const double uy_p = puy + pvz * (x - px);
     const double ux_f = 0.5 * (a.ux[a.L.ux(i, j)] + a.ux[a.L.ux(i + 1, j)]);
@@ -1291,6 +1295,11 @@ __global__ __launch_bounds__(256) void k_increment(double *prt, int np, const do
   if (n < np && fin) finalize_particle(ia, n, 1);
   if (n < np && lane == 0) {
     double *p = prt + (long)n * R_N;
+    if (ia.res && ia.sub == 0) {  // collision(1) at the stage-start positions
+      p[R_CFX + 1] = p[R_CFX];
+      p[R_CFY + 1] = p[R_CFY];
+      p[R_CTZ + 1] = p[R_CTZ];
+    }
     const double dt = dscal[S_DT];
     const double den = p[R_DEN];
     const double dux_prev = p[R_DUX], duy_prev = p[R_DUY], dvz_prev = p[R_DVZ];
@@ -1606,11 +1615,22 @@ int launch_collision(Ctx *c, int cnstep, hipStream_t st) {
 
 int launch_substep_forces(Ctx *c, int sub) {
   if (c->np == 0) return EIF_OK;
-  ProfScope ps(c, "ibm_substep");
   IbmArgs a = ibm_args(c);
+  a.res = c->res_slots;
+  a.sub = sub;
+  if (sub == 0) {
+    // The first sub-iteration's positions are the stage start's (dx = 0
+    // after the reset): collision(1) would repeat collision(0) operand for
+    // operand, so the increment copies collision(0)'s forces instead
+    ProfScope ps(c, "ibm_inertia");
+    hipLaunchKernelGGL(k_inertia, dim3(c->nslices), dim3(64), 0, c->stream, a, 1);
+    EIF_HIP(hipGetLastError());
+    return EIF_OK;
+  }
+  ProfScope ps(c, "ibm_substep");
   CollArgs ca = coll_args(c, 1);
-  a.res = ca.res = c->res_slots;
-  a.sub = ca.sub = sub;
+  ca.res = a.res;
+  ca.sub = sub;
   hipLaunchKernelGGL(k_inertia_collide, dim3(c->np + c->nslices), dim3(64), 0, c->stream, a,
                      ca, 1);
   EIF_HIP(hipGetLastError());
