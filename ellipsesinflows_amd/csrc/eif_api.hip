@@ -145,7 +145,7 @@ static void free_ctx(Ctx *c) {
                      &c->pt_v, &c->pt_w, &c->pt_bnd, &c->pt_red, &c->pt_lvl, &c->pt_ends, &c->pt_g, &c->dscal, &c->dred,
                      &c->prt, &c->pconst, &c->dt_part, &c->sc_fx, &c->sc_fy, &c->sc_dist, &c->sc_pos,
                      &c->sc_t, &c->res_slots, &c->st_ux1, &c->st_ux2, &c->st_uy1,
-                     &c->st_uy2, &c->st_phi, &c->slice_part};
+                     &c->st_uy2, &c->st_phi, &c->slice_part, &c->slice_part0};
   for (auto b : bufs)
     if (*b) (void)hipFree(*b);
   void *others[] = {c->th_flag, c->boxes, c->box_off, c->tile_cnt, c->tile_list,
@@ -720,12 +720,13 @@ static void swap_fields(Ctx *c) {
 
 int eif_gpu_integrate(eif_ctx *c, int substeps[3]) {
   CHECK_CTX(c);
-  // Two streams.  Main: fluid kernels, exchange, deposit and the
-  // sub-iterations (inertia(1) with collision(1) in one launch, then the
-  // increment).  Side: inertia(0), collision(0) and the exchange's particle
-  // sums, which read only the particles and the stage-start velocity, so
-  // they overlap the fluid kernels.  Events join the side work back before
-  // anything that reads its results or reuses its buffers.
+  // Two streams.  Main: fluid kernels, exchange (with inertia(0) in the
+  // same pass), deposit and the sub-iterations (inertia(1) with
+  // collision(1) in one launch, then the increment).  Side: the stage-start
+  // distances of stage 0 (beside its predictor), the exchange's particle
+  // sums and collision(0), which overlap the fluid kernels.  Events join the
+  // side work back before anything that reads its results or reuses its
+  // buffers.
   //
   // The predictor of stage rk+1 reads only the fluid fields, which the
   // sub-iterations of stage rk leave alone.  It is queued right behind the
@@ -745,37 +746,28 @@ int eif_gpu_integrate(eif_ctx *c, int substeps[3]) {
         TRY(launch_bc_p(c, c->p));
         c->bc_valid = true;
       }
+      // the distances at the stage-start positions beside the predictor
       EIF_HIP(hipEventRecord(c->ev_fork, c->stream));
       EIF_HIP(hipStreamWaitEvent(c->side, c->ev_fork, 0));
-      TRY(launch_inertia(c, 0, c->side));  // reads ux, uy before the predictor
+      TRY(launch_ibm_dist(c, c->side));
       EIF_HIP(hipEventRecord(c->ev_i0, c->side));
-      TRY(launch_collision(c, 0, c->side));
       TRY(launch_predictor(c, rk));  // writes the other ux, uy buffers
       EIF_HIP(hipStreamWaitEvent(c->stream, c->ev_i0, 0));
-      // same position as inertia(0) (dx = 0 after the reset), whose
-      // distances are cached; the reset cleared the tile registrations
-      TRY(launch_exchange(c, true, true, true));
-      EIF_HIP(hipEventRecord(c->ev_x, c->stream));
-      EIF_HIP(hipStreamWaitEvent(c->side, c->ev_x, 0));
-      TRY(launch_exchange_finalize(c, c->side));
     } else {
-      // behind the predictor: the previous stage's particle update and this
-      // stage's reset; the exchange computes the distances at the new
-      // positions (the reset cleared the tile registrations) and inertia(0)
-      // reuses them
+      // behind the predictor: the previous stage's particle update, this
+      // stage's reset and the distances at the new positions
       TRY(launch_update_particles(c));
       TRY(launch_reset(c, c->stream, true));
-      TRY(launch_exchange(c, false, true, true));
-      EIF_HIP(hipEventRecord(c->ev_x, c->stream));
-      EIF_HIP(hipStreamWaitEvent(c->side, c->ev_x, 0));
-      TRY(launch_exchange_finalize(c, c->side));
-      // the stage-start velocity is the pair the predictor read
-      swap_fields(c);
-      const int ri = launch_inertia(c, 0, c->side);
-      swap_fields(c);
-      TRY(ri);
-      TRY(launch_collision(c, 0, c->side));
+      TRY(launch_ibm_dist(c));
     }
+    // the exchange with the predicted velocity and, in the same pass,
+    // inertia(0) with the stage-start velocity: the pair the predictor read.
+    // The reset cleared the tile registrations.
+    TRY(launch_exchange_inertia0(c, c->ux_alt, c->uy_alt));
+    EIF_HIP(hipEventRecord(c->ev_x, c->stream));
+    EIF_HIP(hipStreamWaitEvent(c->side, c->ev_x, 0));
+    TRY(launch_exchange_finalize_x0(c, c->side));
+    TRY(launch_collision(c, 0, c->side));
     EIF_HIP(hipEventRecord(c->ev_c0, c->side));
     TRY(launch_update_momentum(c));
     TRY(launch_poisson(c, rk));

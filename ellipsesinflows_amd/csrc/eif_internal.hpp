@@ -146,6 +146,7 @@ struct Ctx {
   long *slice_info = nullptr;
   int *slice_owner = nullptr;
   double *slice_part = nullptr;
+  double *slice_part0 = nullptr;  // inertia(0) sums of the fused exchange
   int nslices = 0;
   int *tile_cnt = nullptr, *tile_list = nullptr;
   int ntx = 0, nty = 0;
@@ -222,6 +223,10 @@ int launch_inertia(Ctx *c, int cnstep, hipStream_t st = nullptr, bool defer_fina
 int launch_exchange(Ctx *c, bool cached_known = false, bool tiles_clear = false,
                     bool defer_finalize = false);
 int launch_exchange_finalize(Ctx *c, hipStream_t st);
+int launch_ibm_dist(Ctx *c, hipStream_t st = nullptr);
+// the exchange with inertia(0) of the stage-start velocity in the same pass
+int launch_exchange_inertia0(Ctx *c, const double *us_x, const double *us_y);
+int launch_exchange_finalize_x0(Ctx *c, hipStream_t st);
 int launch_update_momentum(Ctx *c);
 int launch_materialise_du(Ctx *c);
 int launch_collision(Ctx *c, int cnstep, hipStream_t st = nullptr);

@@ -409,6 +409,10 @@ struct IbmArgs {
   long *slice_info;        // np x 2: box cells, cells evaluated (this call)
   const double *res;       // sub-iteration residuals (null: not gated)
   int sub;                 // sub-iteration index
+  // k_exchange only: with us_x, also inertia(0) of the stage-start
+  // velocity (us_x, us_y), its slice sums into slice_part0
+  const double *us_x, *us_y;
+  double *slice_part0;
   double *slice_part;      // per-slice partial sums (3 each)
 };
 
@@ -579,7 +583,7 @@ __global__ __launch_bounds__(64) void k_inertia(IbmArgs a, int cnstep) {
 // they are added in slice order through lane shuffles.  Lane 0 also
 // records the position whose distances are now cached and the work
 // counters.  which: 0..1 inertia (cnstep), 2 exchange.
-__device__ void finalize_particle(const IbmArgs &a, int n, int which) {
+__device__ void finalize_particle(const IbmArgs &a, int n, int which, bool record = true) {
   const int lane = threadIdx.x & 63;
   double tot[SLICE_VALS] = {0., 0., 0., 0., 0.};
   const int s1 = a.slice_first[n + 1];
@@ -607,6 +611,7 @@ __device__ void finalize_particle(const IbmArgs &a, int n, int which) {
     q[R_FUY] = tot[1];
     q[R_TVZ] = tot[2];
   }
+  if (!record) return;  // the cache records and counters are another call's
   double *cw = a.sc_pos + 4 * (long)n;
   cw[0] = px;
   cw[1] = py;
@@ -623,6 +628,18 @@ __device__ void finalize_particle(const IbmArgs &a, int n, int which) {
   c[1] += (unsigned long long)a.slice_info[2 * (long)n + 1];
   c[2] += (unsigned long long)tot[3];
   c[3] += (unsigned long long)tot[4];
+}
+
+// The exchange's sums and those of the inertia(0) computed beside it (one
+// wave per particle, four per workgroup); the exchange's call records the
+// distance cache and the counters.
+__global__ __launch_bounds__(256) void k_ibm_finalize_x0(IbmArgs a) {
+  const int n = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+  if (n >= a.np) return;
+  finalize_particle(a, n, 2);
+  IbmArgs a0 = a;
+  a0.slice_part = a.slice_part0;
+  finalize_particle(a0, n, 0, false);
 }
 
 // one wave per particle, four per workgroup
@@ -807,7 +824,9 @@ __global__ __launch_bounds__(64) void k_exchange(IbmArgs a, int own_counts) {
   double *fxo = a.sc_fx + a.box_off[n];
   double *fyo = a.sc_fy + a.box_off[n];
   const double dx = a.dx, dy = a.dy;
+  const double dxdy = a.dx * a.dy;
   double acc[3] = {0., 0., 0.};  // fux, fuy, tvz
+  double ai[3] = {0., 0., 0.};   // inertia(0): iux, iuy, ivz
   const int t0 = (slice - s0) * IBM_SLICE;
   const int t1 = t0 + IBM_SLICE < total ? t0 + IBM_SLICE : total;
   for (int t = t0 + (int)threadIdx.x; t < t1; t += 64) {
@@ -815,7 +834,17 @@ __global__ __launch_bounds__(64) void k_exchange(IbmArgs a, int own_counts) {
     cell_of(bs, t, k, i, j);
     const double py_ = py + a.ly * (k - 1);
     const double x = a.xc[i], y = a.yc[j - 1];
-    const double w = s_weight_th(dist[t]);
+    const double th = dist[t];
+    if (a.us_x) {  // inertia(0), src/suspensions/inertia.c:55-60, at dx = 0
+      const double wv = v_weight_th(th);
+      const double valx = wv * 0.5 * (a.us_x[a.L.ux(i, j)] + a.us_x[a.L.ux(i + 1, j)]) * dxdy;
+      const double valy = wv * 0.5 * (a.us_y[a.L.p(i, j)] + a.us_y[a.L.p(i, j + 1)]) * dxdy;
+      ai[0] += valx * inv_pm;
+      ai[2] += -(y - py_) * valx * inv_pim;
+      ai[1] += valy * inv_pm;
+      ai[2] += +(x - px) * valy * inv_pim;
+    }
+    const double w = s_weight_th(th);
     const double ux_p = pux - pvz * (y - py_);
     const double uy_p = puy + pvz * (x - px);
     const double ux_f = 0.5 * (a.ux[a.L.ux(i, j)] + a.ux[a.L.ux(i + 1, j)]);
@@ -830,6 +859,17 @@ __global__ __launch_bounds__(64) void k_exchange(IbmArgs a, int own_counts) {
     acc[2] -= +(x - px) * (fy * dx * dy) * inv_pim;
   }
   const double s0v = wave_sum(acc[0]), s1v = wave_sum(acc[1]), s2v = wave_sum(acc[2]);
+  if (a.us_x) {
+    const double i0 = wave_sum(ai[0]), i1 = wave_sum(ai[1]), i2 = wave_sum(ai[2]);
+    if (threadIdx.x == 0) {
+      double *pp = a.slice_part0 + SLICE_VALS * (long)slice;
+      pp[0] = i0;
+      pp[1] = i1;
+      pp[2] = i2;
+      pp[3] = 0.;
+      pp[4] = 0.;
+    }
+  }
   if (threadIdx.x == 0) {
     double *pp = a.slice_part + SLICE_VALS * (long)slice;
     pp[0] = s0v;
@@ -1408,6 +1448,8 @@ static IbmArgs ibm_args(Ctx *c) {
   a.slice_info = c->slice_info;
   a.res = nullptr;
   a.sub = 0;
+  a.us_x = a.us_y = nullptr;
+  a.slice_part0 = c->slice_part0;
   a.slice_part = c->slice_part;
   return a;
 }
@@ -1452,6 +1494,7 @@ int susp_init(Ctx *c) {
   EIF_HIP(hipMalloc(&c->slice_info, 2 * (size_t)(c->np > 0 ? c->np : 1) * sizeof(long)));
   EIF_HIP(hipMemset(c->slice_info, 0, 2 * (size_t)(c->np > 0 ? c->np : 1) * sizeof(long)));
   EIF_HIP(hipMalloc(&c->slice_part, SLICE_VALS * (size_t)(c->nslices > 0 ? c->nslices : 1) * sizeof(double)));
+  EIF_HIP(hipMalloc(&c->slice_part0, SLICE_VALS * (size_t)(c->nslices > 0 ? c->nslices : 1) * sizeof(double)));
   EIF_HIP(hipMalloc(&c->ibm_pp, (4 * (size_t)c->np + 1) * sizeof(unsigned long long)));
   EIF_HIP(hipMemset(c->ibm_pp, 0, (4 * (size_t)c->np + 1) * sizeof(unsigned long long)));
   EIF_HIP(hipMalloc(&c->box_off, (c->np + 1) * sizeof(long)));
@@ -1527,6 +1570,38 @@ int launch_exchange(Ctx *c, bool cached_known, bool tiles_clear, bool defer_fina
                      cached_known ? 1 : 0);
   EIF_HIP(hipGetLastError());
   if (!defer_finalize) return launch_exchange_finalize(c, c->stream);
+  return EIF_OK;
+}
+
+// Signed distances (as tanh(BETA d)) of the box cells at the particles'
+// positions without increments, into the per-cell cache
+int launch_ibm_dist(Ctx *c, hipStream_t st) {
+  if (c->np == 0) return EIF_OK;
+  hipLaunchKernelGGL(k_ibm_dist, dim3(c->nslices), dim3(64), 0, st ? st : c->stream,
+                     ibm_args(c), 0);
+  EIF_HIP(hipGetLastError());
+  return EIF_OK;
+}
+
+// The exchange at the stage start (distances cached by launch_ibm_dist, tile
+// registrations cleared by the reset) with inertia(0) of the stage-start
+// velocity (us_x, us_y) in the same pass over the cells; the sums are added
+// by launch_exchange_finalize_x0
+int launch_exchange_inertia0(Ctx *c, const double *us_x, const double *us_y) {
+  if (c->np == 0) return EIF_OK;
+  ProfScope ps(c, "ibm_exchange");
+  IbmArgs a = ibm_args(c);
+  a.us_x = us_x;
+  a.us_y = us_y;
+  hipLaunchKernelGGL(k_exchange, dim3(c->nslices), dim3(64), 0, c->stream, a, 0);
+  EIF_HIP(hipGetLastError());
+  return EIF_OK;
+}
+
+int launch_exchange_finalize_x0(Ctx *c, hipStream_t st) {
+  if (c->np == 0) return EIF_OK;
+  hipLaunchKernelGGL(k_ibm_finalize_x0, dim3((c->np + 3) / 4), dim3(256), 0, st, ibm_args(c));
+  EIF_HIP(hipGetLastError());
   return EIF_OK;
 }
 
