@@ -151,7 +151,7 @@ static void free_ctx(Ctx *c) {
   void *others[] = {c->th_flag, c->boxes, c->box_off, c->tile_cnt, c->tile_list,
                     c->active, c->flags, c->rows, c->row_cnt, c->wall_hit,
                     c->ibm_stats, c->slice_first, c->slice_info, c->ibm_pp, c->slice_owner,
-                    c->sc_key};
+                    c->sc_key, c->heads};
   for (auto o : others)
     if (o) (void)hipFree(o);
   if (c->hscal) (void)hipHostFree(c->hscal);

@@ -147,6 +147,7 @@ struct Ctx {
   int *slice_owner = nullptr;
   double *slice_part = nullptr;
   double *slice_part0 = nullptr;  // inertia(0) sums of the fused exchange
+  void *heads = nullptr;          // np per-particle pass headers (SliceHead)
   int nslices = 0;
   int *tile_cnt = nullptr, *tile_list = nullptr;
   int ntx = 0, nty = 0;

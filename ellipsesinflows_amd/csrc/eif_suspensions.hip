@@ -384,6 +384,8 @@ __device__ inline bool sub_open(const double *res, int sub) {
   return !res || sub == 0 || (!(res[sub - 1] < 1.e-8) && sub <= 101);
 }
 
+struct SliceHead;
+
 struct IbmArgs {
   Layout L;
   int nx, ny;
@@ -413,7 +415,8 @@ struct IbmArgs {
   // velocity (us_x, us_y), its slice sums into slice_part0
   const double *us_x, *us_y;
   double *slice_part0;
-  double *slice_part;      // per-slice partial sums (3 each)
+  double *slice_part;
+  SliceHead *heads;  // per particle: the pass's position, shape, boxes, cache state      // per-slice partial sums (3 each)
 };
 
 // fixed-order tree over the 64 lanes of a wave (lane 0 receives the sum)
@@ -510,34 +513,75 @@ __device__ inline void cell_of(const BoxSet &bs, int t, int &k, int &i, int &j) 
   i = imin + r;
 }
 
+// What every slice wave of one IBM pass needs of its particle: computed
+// once per particle and pass, by the kernel that last moved the particle
+// or updated its cache (k_reset, the finalise passes, k_increment, or
+// k_heads), instead of by each of the particle's waves.  The arithmetic is
+// that the waves would do themselves.
+struct SliceHead {
+  EllipseShape e;
+  double px, py, paz;  // position of the pass
+  double cr, sr;       // cos, sin of -paz
+  double inv_pm, inv_pim;
+  long off, cap;       // the particle's per-cell scratch range
+  BoxSet bs;
+  EllipseShape32 f;
+  int cached;          // the distance cache holds this position
+  int warm;            // the foot parameters hold these boxes
+};
+
+__device__ inline void write_head(const IbmArgs &a, int n, double px, double py, double paz) {
+  const double *q = a.prt + (long)n * R_N;
+  SliceHead h;
+  h.e = make_shape(q[R_A], q[R_B]);
+  h.f = shape32(h.e);
+  h.px = px;
+  h.py = py;
+  h.paz = paz;
+  sincos(-paz, &h.sr, &h.cr);
+  h.inv_pm = 1. / p_mass(q);
+  h.inv_pim = 1. / p_moi(q);
+  h.off = a.box_off[n];
+  h.cap = a.box_off[n + 1] - h.off;
+  make_boxes(a, px, py, fmax(q[R_A], q[R_B]), h.bs);
+  const double *cp = a.sc_pos + 4 * (long)n;
+  h.cached = cp[3] != 0. && cp[0] == px && cp[1] == py && cp[2] == paz;
+  h.warm = same_boxes(a.sc_key + 8 * (long)n, h.bs);
+  a.heads[n] = h;
+}
+
+// heads at the particles' positions, with the increments when with_dx
+__global__ void k_heads(IbmArgs a, int with_dx) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= a.np) return;
+  const double *q = a.prt + (long)n * R_N;
+  if (with_dx)
+    write_head(a, n, q[R_X] + q[R_DX], q[R_Y] + q[R_DY], q[R_AZ] + q[R_DAZ]);
+  else
+    write_head(a, n, q[R_X], q[R_Y], q[R_AZ]);
+}
+
 // src/suspensions/inertia.c:9-70.  One wave per slice of IBM_SLICE box
 // cells of one particle (its three images' boxes flattened); the last
 // slice of a particle to finish adds the slices' sums in slice order.
 __device__ __forceinline__ void inertia_slice(const IbmArgs &a, int cnstep, int slice) {
   const int n = a.slice_owner[slice];
   const int s0 = a.slice_first[n], nsl = a.slice_first[n + 1] - s0;
-  double *q = a.prt + (long)n * R_N;
-  const double pa = q[R_A], pb = q[R_B];
-  const double px = q[R_X] + q[R_DX];
-  const double py = q[R_Y] + q[R_DY];
-  const double paz = q[R_AZ] + q[R_DAZ];
-  const double inv_pm = 1. / p_mass(q), inv_pim = 1. / p_moi(q);
-  const EllipseShape e = make_shape(pa, pb);
-  const EllipseShape32 f = shape32(e);
-  BoxSet bs;
-  make_boxes(a, px, py, fmax(pa, pb), bs);
+  const SliceHead &hd = a.heads[n];
+  const BoxSet bs = hd.bs;
   const int total = bs.total;
-  if (total > a.box_off[n + 1] - a.box_off[n] || total > nsl * IBM_SLICE) {
+  if (total > hd.cap || total > nsl * IBM_SLICE) {
     if (threadIdx.x == 0) atomicOr(&a.flags[0], 1);
     return;
   }
-  const double *cp = a.sc_pos + 4 * (long)n;
-  const bool cached = cp[3] != 0. && cp[0] == px && cp[1] == py && cp[2] == paz;
-  const bool warm = same_boxes(a.sc_key + 8 * (long)n, bs);
-  double *dist = a.sc_dist + a.box_off[n];
-  double *tcell = a.sc_t + a.box_off[n];
-  double cr, sr;
-  sincos(-paz, &sr, &cr);
+  const EllipseShape e = hd.e;
+  const EllipseShape32 f = hd.f;
+  const double px = hd.px, py = hd.py;
+  const double inv_pm = hd.inv_pm, inv_pim = hd.inv_pim;
+  const bool cached = hd.cached, warm = hd.warm;
+  double *dist = a.sc_dist + hd.off;
+  double *tcell = a.sc_t + hd.off;
+  const double cr = hd.cr, sr = hd.sr;
   const double dxdy = a.dx * a.dy;
   double acc[3] = {0., 0., 0.};  // iux, iuy, ivz
   int it64 = 0, it32 = 0;
@@ -640,6 +684,11 @@ __global__ __launch_bounds__(256) void k_ibm_finalize_x0(IbmArgs a) {
   IbmArgs a0 = a;
   a0.slice_part = a.slice_part0;
   finalize_particle(a0, n, 0, false);
+  // the first sub-iteration's pass, at the cached stage-start positions
+  if ((threadIdx.x & 63) == 0) {
+    const double *q = a.prt + (long)n * R_N;
+    write_head(a, n, q[R_X] + q[R_DX], q[R_Y] + q[R_DY], q[R_AZ] + q[R_DAZ]);
+  }
 }
 
 // one wave per particle, four per workgroup
@@ -720,34 +769,28 @@ __device__ inline void register_tile(const IbmArgs &a, int tile, int box_id) {
 // increments dx, dy, daz when with_dx) where the per-particle cache does not
 // already hold them; sliced like k_inertia.  Used before k_exchange when
 // the cache may be stale.
-__global__ __launch_bounds__(64) void k_ibm_dist(IbmArgs a, int with_dx) {
+__global__ __launch_bounds__(64) void k_ibm_dist(IbmArgs a) {
   if (!sub_open(a.res, a.sub)) return;
   const int slice = blockIdx.x;
   const int n = a.slice_owner[slice];
   const int s0 = a.slice_first[n], nsl = a.slice_first[n + 1] - s0;
-  const double *q = a.prt + (long)n * R_N;
-  const double pa = q[R_A], pb = q[R_B];
-  const double px = q[R_X] + (with_dx ? q[R_DX] : 0.);
-  const double py = q[R_Y] + (with_dx ? q[R_DY] : 0.);
-  const double paz = q[R_AZ] + (with_dx ? q[R_DAZ] : 0.);
-  BoxSet bs;
-  make_boxes(a, px, py, fmax(pa, pb), bs);
+  const SliceHead &hd = a.heads[n];
+  const BoxSet bs = hd.bs;
   const int total = bs.total;
-  if (total > a.box_off[n + 1] - a.box_off[n] || total > nsl * IBM_SLICE) {
+  if (total > hd.cap || total > nsl * IBM_SLICE) {
     if (threadIdx.x == 0) atomicOr(&a.flags[0], 1);
     return;
   }
-  const double *cp = a.sc_pos + 4 * (long)n;
-  const bool cached = cp[3] != 0. && cp[0] == px && cp[1] == py && cp[2] == paz;
+  const bool cached = hd.cached;
+  const double px = hd.px, py = hd.py;
   int it64 = 0, it32 = 0;
   if (!cached) {
-    const EllipseShape e = make_shape(pa, pb);
-    const EllipseShape32 f = shape32(e);
-    const bool warm = same_boxes(a.sc_key + 8 * (long)n, bs);
-    double *dist = a.sc_dist + a.box_off[n];
-    double *tcell = a.sc_t + a.box_off[n];
-    double cr, sr;
-    sincos(-paz, &sr, &cr);
+    const EllipseShape e = hd.e;
+    const EllipseShape32 f = hd.f;
+    const bool warm = hd.warm;
+    double *dist = a.sc_dist + hd.off;
+    double *tcell = a.sc_t + hd.off;
+    const double cr = hd.cr, sr = hd.sr;
     const int t0 = (slice - s0) * IBM_SLICE;
     const int t1 = t0 + IBM_SLICE < total ? t0 + IBM_SLICE : total;
     for (int t = t0 + (int)threadIdx.x; t < t1; t += 64) {
@@ -772,27 +815,27 @@ __global__ __launch_bounds__(64) void k_ibm_dist(IbmArgs a, int with_dx) {
 // publishes its boxes and registers them with the tiles they reach.
 // own_counts: no k_ibm_dist ran before, so this kernel reports no
 // distance work.
+// The heads hold the particles' positions without increments.
 __global__ __launch_bounds__(64) void k_exchange(IbmArgs a, int own_counts) {
   const int slice = blockIdx.x;
   const int n = a.slice_owner[slice];
   const int s0 = a.slice_first[n], nsl = a.slice_first[n + 1] - s0;
-  double *q = a.prt + (long)n * R_N;
+  const double *q = a.prt + (long)n * R_N;
+  const SliceHead &hd = a.heads[n];
   const double dt = a.dscal[S_DT];
   const double inv_dt = 1. / dt;
-  const double pa = q[R_A], pb = q[R_B];
-  const double inv_pm = 1. / p_mass(q), inv_pim = 1. / p_moi(q);
-  const double px = q[R_X], py = q[R_Y];
+  const double inv_pm = hd.inv_pm, inv_pim = hd.inv_pim;
+  const double px = hd.px, py = hd.py;
   const double pux = q[R_UX], puy = q[R_UY], pvz = q[R_VZ];
-  BoxSet bs;
-  make_boxes(a, px, py, fmax(pa, pb), bs);
+  const BoxSet bs = hd.bs;
   const int total = bs.total;
-  if (total > a.box_off[n + 1] - a.box_off[n] || total > nsl * IBM_SLICE) {
+  if (total > hd.cap || total > nsl * IBM_SLICE) {
     if (threadIdx.x == 0) atomicOr(&a.flags[0], 1);
     return;
   }
   if (slice == s0) {
     if (threadIdx.x == 0) {
-      long off = a.box_off[n];
+      long off = hd.off;
 #pragma unroll
       for (int k = 0; k < 3; k++) {
         const Img m = pick(bs, k);
@@ -820,9 +863,9 @@ __global__ __launch_bounds__(64) void k_exchange(IbmArgs a, int own_counts) {
         register_tile(a, (rg.ty0 + e_ / w) * a.ntx + rg.tx0 + e_ % w, n * 3 + r / 2);
     }
   }
-  const double *dist = a.sc_dist + a.box_off[n];
-  double *fxo = a.sc_fx + a.box_off[n];
-  double *fyo = a.sc_fy + a.box_off[n];
+  const double *dist = a.sc_dist + hd.off;
+  double *fxo = a.sc_fx + hd.off;
+  double *fyo = a.sc_fy + hd.off;
   const double dx = a.dx, dy = a.dy;
   const double dxdy = a.dx * a.dy;
   double acc[3] = {0., 0., 0.};  // fux, fuy, tvz
@@ -1360,6 +1403,8 @@ __global__ __launch_bounds__(256) void k_increment(double *prt, int np, const do
     p[R_DX] = 0.5 * ga * dt * (ux0 + ux1);
     p[R_DY] = 0.5 * ga * dt * (uy0 + uy1);
     p[R_DAZ] = 0.5 * ga * dt * (vz0 + vz1);
+    // the next sub-iteration's inertia pass
+    write_head(ia, n, p[R_X] + p[R_DX], p[R_Y] + p[R_DY], p[R_AZ] + p[R_DAZ]);
   }
   if (lane == 0) red[w] = res;
   __syncthreads();
@@ -1388,25 +1433,26 @@ __global__ void k_update_particles(double *prt, int np, double ly) {
 // res (RES_SLOTS sub-iteration residuals), tile_cnt (ntiles) and
 // active[0]: cleared when given, for the stage's sub-iterations and the
 // next exchange's tile registrations
-__global__ void k_reset(double *prt, int np, double *res, int *sc_key, int *tile_cnt,
-                        int ntiles, int *active) {
+// Also the heads of the stage-start pass (positions without increments).
+__global__ void k_reset(IbmArgs a, double *res, int *tile_cnt, int ntiles, int *active) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (res && n < RES_SLOTS) res[n] = 0.;
   if (tile_cnt) {
     if (n < ntiles) tile_cnt[n] = 0;
     if (n == 0) active[0] = 0;
   }
-  if (n >= np) return;
+  if (n >= a.np) return;
   // foot-point warm starts reach back no further than the stage start, so
   // that a run restarted from a save repeats the same operations
-  if (sc_key) sc_key[8 * (long)n + 7] = 0;
-  double *p = prt + (long)n * R_N;
+  a.sc_key[8 * (long)n + 7] = 0;
+  double *p = a.prt + (long)n * R_N;
   p[R_DUX] = 0.;
   p[R_DUY] = 0.;
   p[R_DVZ] = 0.;
   p[R_DX] = 0.;
   p[R_DY] = 0.;
   p[R_DAZ] = 0.;
+  write_head(a, n, p[R_X], p[R_Y], p[R_AZ]);
 }
 
 // ---------------------------------------------------------------------
@@ -1451,6 +1497,7 @@ static IbmArgs ibm_args(Ctx *c) {
   a.us_x = a.us_y = nullptr;
   a.slice_part0 = c->slice_part0;
   a.slice_part = c->slice_part;
+  a.heads = static_cast<SliceHead *>(c->heads);
   return a;
 }
 
@@ -1495,6 +1542,7 @@ int susp_init(Ctx *c) {
   EIF_HIP(hipMemset(c->slice_info, 0, 2 * (size_t)(c->np > 0 ? c->np : 1) * sizeof(long)));
   EIF_HIP(hipMalloc(&c->slice_part, SLICE_VALS * (size_t)(c->nslices > 0 ? c->nslices : 1) * sizeof(double)));
   EIF_HIP(hipMalloc(&c->slice_part0, SLICE_VALS * (size_t)(c->nslices > 0 ? c->nslices : 1) * sizeof(double)));
+  EIF_HIP(hipMalloc(&c->heads, sizeof(SliceHead) * (size_t)(c->np > 0 ? c->np : 1)));
   EIF_HIP(hipMalloc(&c->ibm_pp, (4 * (size_t)c->np + 1) * sizeof(unsigned long long)));
   EIF_HIP(hipMemset(c->ibm_pp, 0, (4 * (size_t)c->np + 1) * sizeof(unsigned long long)));
   EIF_HIP(hipMalloc(&c->box_off, (c->np + 1) * sizeof(long)));
@@ -1535,9 +1583,8 @@ int launch_reset(Ctx *c, hipStream_t st, bool clear_tiles) {
   // also clears the sub-iteration residuals, so it runs without particles too
   const int ntiles = c->np > 0 && clear_tiles ? c->ntx * c->nty : 0;
   const int n = std::max(std::max(c->np, ntiles), RES_SLOTS);
-  hipLaunchKernelGGL(k_reset, dim3((n + 255) / 256), dim3(256), 0, st, c->prt, c->np,
-                     c->res_slots, c->sc_key, ntiles > 0 ? c->tile_cnt : nullptr, ntiles,
-                     c->active);
+  hipLaunchKernelGGL(k_reset, dim3((n + 255) / 256), dim3(256), 0, st, ibm_args(c),
+                     c->res_slots, ntiles > 0 ? c->tile_cnt : nullptr, ntiles, c->active);
   EIF_HIP(hipGetLastError());
   return EIF_OK;
 }
@@ -1547,6 +1594,7 @@ int launch_inertia(Ctx *c, int cnstep, hipStream_t st, bool defer_finalize) {
   if (!st) st = c->stream;
   ProfScope ps(c, "ibm_inertia", st);
   IbmArgs a = ibm_args(c);
+  hipLaunchKernelGGL(k_heads, dim3((c->np + 255) / 256), dim3(256), 0, st, a, 1);
   hipLaunchKernelGGL(k_inertia, dim3(c->nslices), dim3(64), 0, st, a, cnstep);
   if (!defer_finalize)
     hipLaunchKernelGGL(k_ibm_finalize, dim3((c->np + 3) / 4), dim3(256), 0, st, a, cnstep);
@@ -1564,8 +1612,9 @@ int launch_exchange(Ctx *c, bool cached_known, bool tiles_clear, bool defer_fina
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, c->active, 1);
   }
   IbmArgs a = ibm_args(c);
+  hipLaunchKernelGGL(k_heads, dim3((c->np + 255) / 256), dim3(256), 0, c->stream, a, 0);
   if (!cached_known)
-    hipLaunchKernelGGL(k_ibm_dist, dim3(c->nslices), dim3(64), 0, c->stream, a, 0);
+    hipLaunchKernelGGL(k_ibm_dist, dim3(c->nslices), dim3(64), 0, c->stream, a);
   hipLaunchKernelGGL(k_exchange, dim3(c->nslices), dim3(64), 0, c->stream, a,
                      cached_known ? 1 : 0);
   EIF_HIP(hipGetLastError());
@@ -1578,7 +1627,7 @@ int launch_exchange(Ctx *c, bool cached_known, bool tiles_clear, bool defer_fina
 int launch_ibm_dist(Ctx *c, hipStream_t st) {
   if (c->np == 0) return EIF_OK;
   hipLaunchKernelGGL(k_ibm_dist, dim3(c->nslices), dim3(64), 0, st ? st : c->stream,
-                     ibm_args(c), 0);
+                     ibm_args(c));
   EIF_HIP(hipGetLastError());
   return EIF_OK;
 }
