@@ -49,10 +49,11 @@ __device__ inline double cube(double x) {
 }
 
 // src/suspensions/auxiliary.c:11-17
-__device__ inline void loop_size(int lb, int ub, double h, double r, double c,
-                                 int &mn, int &mx) {
-  mn = (int)(floor((c - r) / h) - NEXTRA);
-  mx = (int)(ceil((c + r) / h) + NEXTRA);
+// ih = 1/h, used when h is a power of two (pow2): then x * ih == x / h
+__device__ inline void loop_size(int lb, int ub, double h, double ih, bool pow2, double r,
+                                 double c, int &mn, int &mx) {
+  mn = (int)(floor(pow2 ? (c - r) * ih : (c - r) / h) - NEXTRA);
+  mx = (int)(ceil(pow2 ? (c + r) * ih : (c + r) / h) + NEXTRA);
   mn = mn < lb ? lb : mn;
   mx = mx > ub ? ub : mx;
 }
@@ -390,6 +391,8 @@ struct IbmArgs {
   Layout L;
   int nx, ny;
   double ly, dx, dy, grid_size;
+  double idx, idy;  // 1/dx, 1/dy
+  int pow2;         // dx, dy powers of two: divisions by them are products with idx, idy
   const double *xc, *yc;
   const double *ux, *uy;
   double *prt;
@@ -446,8 +449,8 @@ struct BoxSet {
 
 __device__ inline Img make_img(const IbmArgs &a, double px, double py_, double r) {
   Img m;
-  loop_size(1, a.nx, a.dx, r, px, m.imin, m.imax);
-  loop_size(1, a.ny, a.dy, r, py_ - 0., m.jmin, m.jmax);
+  loop_size(1, a.nx, a.dx, a.idx, a.pow2, r, px, m.imin, m.imax);
+  loop_size(1, a.ny, a.dy, a.idy, a.pow2, r, py_ - 0., m.jmin, m.jmax);
   const int w = m.imax - m.imin + 1, h = m.jmax - m.jmin + 1;
   m.cnt = (w > 0 && h > 0) ? w * h : 0;
   m.inv_w = w > 0 ? 1.f / (float)w : 0.f;
@@ -530,24 +533,34 @@ struct SliceHead {
   int warm;            // the foot parameters hold these boxes
 };
 
-__device__ inline void write_head(const IbmArgs &a, int n, double px, double py, double paz) {
+// full: also the parts that depend only on the particle's shape and mass
+// (written at every stage start, so records changed between calls are
+// picked up); otherwise only the position-dependent parts
+__device__ inline void write_head(const IbmArgs &a, int n, double px, double py, double paz,
+                                  bool full) {
   const double *q = a.prt + (long)n * R_N;
-  SliceHead h;
-  h.e = make_shape(q[R_A], q[R_B]);
-  h.f = shape32(h.e);
-  h.px = px;
-  h.py = py;
-  h.paz = paz;
-  sincos(-paz, &h.sr, &h.cr);
-  h.inv_pm = 1. / p_mass(q);
-  h.inv_pim = 1. / p_moi(q);
-  h.off = a.box_off[n];
-  h.cap = a.box_off[n + 1] - h.off;
-  make_boxes(a, px, py, fmax(q[R_A], q[R_B]), h.bs);
+  SliceHead *h = a.heads + n;
+  if (full) {
+    h->e = make_shape(q[R_A], q[R_B]);
+    h->f = shape32(h->e);
+    h->inv_pm = 1. / p_mass(q);
+    h->inv_pim = 1. / p_moi(q);
+    h->off = a.box_off[n];
+    h->cap = a.box_off[n + 1] - h->off;
+  }
+  h->px = px;
+  h->py = py;
+  h->paz = paz;
+  double sr, cr;
+  sincos(-paz, &sr, &cr);
+  h->cr = cr;
+  h->sr = sr;
+  BoxSet bs;
+  make_boxes(a, px, py, fmax(q[R_A], q[R_B]), bs);
+  h->bs = bs;
   const double *cp = a.sc_pos + 4 * (long)n;
-  h.cached = cp[3] != 0. && cp[0] == px && cp[1] == py && cp[2] == paz;
-  h.warm = same_boxes(a.sc_key + 8 * (long)n, h.bs);
-  a.heads[n] = h;
+  h->cached = cp[3] != 0. && cp[0] == px && cp[1] == py && cp[2] == paz;
+  h->warm = same_boxes(a.sc_key + 8 * (long)n, bs);
 }
 
 // heads at the particles' positions, with the increments when with_dx
@@ -556,9 +569,9 @@ __global__ void k_heads(IbmArgs a, int with_dx) {
   if (n >= a.np) return;
   const double *q = a.prt + (long)n * R_N;
   if (with_dx)
-    write_head(a, n, q[R_X] + q[R_DX], q[R_Y] + q[R_DY], q[R_AZ] + q[R_DAZ]);
+    write_head(a, n, q[R_X] + q[R_DX], q[R_Y] + q[R_DY], q[R_AZ] + q[R_DAZ], true);
   else
-    write_head(a, n, q[R_X], q[R_Y], q[R_AZ]);
+    write_head(a, n, q[R_X], q[R_Y], q[R_AZ], true);
 }
 
 // src/suspensions/inertia.c:9-70.  One wave per slice of IBM_SLICE box
@@ -656,16 +669,14 @@ __device__ void finalize_particle(const IbmArgs &a, int n, int which, bool recor
     q[R_TVZ] = tot[2];
   }
   if (!record) return;  // the cache records and counters are another call's
+  // the pass's header holds this position and its boxes
+  const SliceHead &hd = a.heads[n];
   double *cw = a.sc_pos + 4 * (long)n;
-  cw[0] = px;
-  cw[1] = py;
-  cw[2] = paz;
+  cw[0] = hd.px;
+  cw[1] = hd.py;
+  cw[2] = hd.paz;
   cw[3] = 1.;
-  {
-    BoxSet bs;
-    make_boxes(a, px, py, fmax(q[R_A], q[R_B]), bs);
-    store_boxes(a.sc_key + 8 * (long)n, bs);
-  }
+  store_boxes(a.sc_key + 8 * (long)n, hd.bs);
   if (n == 0) a.stats[4 * (long)a.np] += 1ull;  // IBM calls
   unsigned long long *c = a.stats + 4 * (long)n;
   c[0] += (unsigned long long)a.slice_info[2 * (long)n];
@@ -687,7 +698,7 @@ __global__ __launch_bounds__(256) void k_ibm_finalize_x0(IbmArgs a) {
   // the first sub-iteration's pass, at the cached stage-start positions
   if ((threadIdx.x & 63) == 0) {
     const double *q = a.prt + (long)n * R_N;
-    write_head(a, n, q[R_X] + q[R_DX], q[R_Y] + q[R_DY], q[R_AZ] + q[R_DAZ]);
+    write_head(a, n, q[R_X] + q[R_DX], q[R_Y] + q[R_DY], q[R_AZ] + q[R_DAZ], false);
   }
 }
 
@@ -1404,7 +1415,7 @@ __global__ __launch_bounds__(256) void k_increment(double *prt, int np, const do
     p[R_DY] = 0.5 * ga * dt * (uy0 + uy1);
     p[R_DAZ] = 0.5 * ga * dt * (vz0 + vz1);
     // the next sub-iteration's inertia pass
-    write_head(ia, n, p[R_X] + p[R_DX], p[R_Y] + p[R_DY], p[R_AZ] + p[R_DAZ]);
+    write_head(ia, n, p[R_X] + p[R_DX], p[R_Y] + p[R_DY], p[R_AZ] + p[R_DAZ], false);
   }
   if (lane == 0) red[w] = res;
   __syncthreads();
@@ -1452,7 +1463,7 @@ __global__ void k_reset(IbmArgs a, double *res, int *tile_cnt, int ntiles, int *
   p[R_DX] = 0.;
   p[R_DY] = 0.;
   p[R_DAZ] = 0.;
-  write_head(a, n, p[R_X], p[R_Y], p[R_AZ]);
+  write_head(a, n, p[R_X], p[R_Y], p[R_AZ], true);
 }
 
 // ---------------------------------------------------------------------
@@ -1467,6 +1478,9 @@ static IbmArgs ibm_args(Ctx *c) {
   a.dx = c->dx;
   a.dy = c->dy;
   a.grid_size = fmin(c->dx, c->dy);
+  a.idx = 1. / c->dx;
+  a.idy = 1. / c->dy;
+  a.pow2 = c->pow2_grid ? 1 : 0;
   a.xc = c->xc;
   a.yc = c->yc;
   a.ux = c->ux;
