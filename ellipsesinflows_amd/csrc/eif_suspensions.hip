@@ -533,11 +533,20 @@ struct SliceHead {
   int warm;            // the foot parameters hold these boxes
 };
 
+// same_boxes against the key store_boxes(k) would leave
+__device__ inline bool same_boxset(const BoxSet &k, const BoxSet &bs) {
+  return k.b0.imin == bs.b0.imin && k.b0.jmin == bs.b0.jmin && k.b1.imin == bs.b1.imin &&
+         k.b1.jmin == bs.b1.jmin && k.b2.imin == bs.b2.imin && k.b2.jmin == bs.b2.jmin &&
+         k.total == bs.total;
+}
+
 // full: also the parts that depend only on the particle's shape and mass
 // (written at every stage start, so records changed between calls are
 // picked up); otherwise only the position-dependent parts
+// prev: the header whose pass the cache records now describe (known to the
+// caller), else the records are read
 __device__ inline void write_head(const IbmArgs &a, int n, double px, double py, double paz,
-                                  bool full) {
+                                  bool full, const SliceHead *prev = nullptr) {
   const double *q = a.prt + (long)n * R_N;
   SliceHead *h = a.heads + n;
   if (full) {
@@ -558,9 +567,14 @@ __device__ inline void write_head(const IbmArgs &a, int n, double px, double py,
   BoxSet bs;
   make_boxes(a, px, py, fmax(q[R_A], q[R_B]), bs);
   h->bs = bs;
-  const double *cp = a.sc_pos + 4 * (long)n;
-  h->cached = cp[3] != 0. && cp[0] == px && cp[1] == py && cp[2] == paz;
-  h->warm = same_boxes(a.sc_key + 8 * (long)n, bs);
+  if (prev) {  // the cache records were just made from prev
+    h->cached = prev->px == px && prev->py == py && prev->paz == paz;
+    h->warm = same_boxset(prev->bs, bs);
+  } else {
+    const double *cp = a.sc_pos + 4 * (long)n;
+    h->cached = cp[3] != 0. && cp[0] == px && cp[1] == py && cp[2] == paz;
+    h->warm = same_boxes(a.sc_key + 8 * (long)n, bs);
+  }
 }
 
 // heads at the particles' positions, with the increments when with_dx
@@ -640,9 +654,10 @@ __global__ __launch_bounds__(64) void k_inertia(IbmArgs a, int cnstep) {
 // they are added in slice order through lane shuffles.  Lane 0 also
 // records the position whose distances are now cached and the work
 // counters.  which: 0..1 inertia (cnstep), 2 exchange.
-__device__ void finalize_particle(const IbmArgs &a, int n, int which, bool record = true) {
+// the particle's slice sums in slice order (every lane gets them)
+__device__ inline void slice_sums(const IbmArgs &a, int n, double tot[SLICE_VALS]) {
   const int lane = threadIdx.x & 63;
-  double tot[SLICE_VALS] = {0., 0., 0., 0., 0.};
+  for (int k = 0; k < SLICE_VALS; k++) tot[k] = 0.;
   const int s1 = a.slice_first[n + 1];
   for (int base = a.slice_first[n]; base < s1; base += 64) {
     double v[SLICE_VALS] = {0., 0., 0., 0., 0.};
@@ -653,24 +668,11 @@ __device__ void finalize_particle(const IbmArgs &a, int n, int which, bool recor
     for (int l = 0; l < cnt; l++)
       for (int k = 0; k < SLICE_VALS; k++) tot[k] += __shfl(v[k], l);
   }
-  if (lane != 0) return;
-  double *q = a.prt + (long)n * R_N;
-  double px = q[R_X], py = q[R_Y], paz = q[R_AZ];
-  if (which < 2) {
-    q[R_IUX + which] = tot[0];
-    q[R_IUY + which] = tot[1];
-    q[R_IVZ + which] = tot[2];
-    px += q[R_DX];
-    py += q[R_DY];
-    paz += q[R_DAZ];
-  } else {
-    q[R_FUX] = tot[0];
-    q[R_FUY] = tot[1];
-    q[R_TVZ] = tot[2];
-  }
-  if (!record) return;  // the cache records and counters are another call's
-  // the pass's header holds this position and its boxes
-  const SliceHead &hd = a.heads[n];
+}
+
+// the cache records of the pass with header hd, and the work counters
+__device__ inline void record_pass(const IbmArgs &a, int n, const SliceHead &hd,
+                                   const double tot[SLICE_VALS]) {
   double *cw = a.sc_pos + 4 * (long)n;
   cw[0] = hd.px;
   cw[1] = hd.py;
@@ -683,6 +685,25 @@ __device__ void finalize_particle(const IbmArgs &a, int n, int which, bool recor
   c[1] += (unsigned long long)a.slice_info[2 * (long)n + 1];
   c[2] += (unsigned long long)tot[3];
   c[3] += (unsigned long long)tot[4];
+}
+
+__device__ void finalize_particle(const IbmArgs &a, int n, int which, bool record = true) {
+  double tot[SLICE_VALS];
+  slice_sums(a, n, tot);
+  if ((threadIdx.x & 63) != 0) return;
+  double *q = a.prt + (long)n * R_N;
+  if (which < 2) {
+    q[R_IUX + which] = tot[0];
+    q[R_IUY + which] = tot[1];
+    q[R_IVZ + which] = tot[2];
+  } else {
+    q[R_FUX] = tot[0];
+    q[R_FUY] = tot[1];
+    q[R_TVZ] = tot[2];
+  }
+  // the cache records and counters (unless another call's); the pass's
+  // header holds its position and boxes
+  if (record) record_pass(a, n, a.heads[n], tot);
 }
 
 // The exchange's sums and those of the inertia(0) computed beside it (one
@@ -1386,36 +1407,63 @@ __global__ __launch_bounds__(256) void k_increment(double *prt, int np, const do
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int n = blockIdx.x * 4 + w;
   double res = 0.;
-  if (n < np && fin) finalize_particle(ia, n, 1);
+  double tot[SLICE_VALS];
+  if (n < np && fin) slice_sums(ia, n, tot);
   if (n < np && lane == 0) {
     double *p = prt + (long)n * R_N;
-    if (ia.res && ia.sub == 0) {  // collision(1) at the stage-start positions
-      p[R_CFX + 1] = p[R_CFX];
-      p[R_CFY + 1] = p[R_CFY];
-      p[R_CTZ + 1] = p[R_CTZ];
-    }
+    // the record's inputs, loaded together
     const double dt = dscal[S_DT];
     const double den = p[R_DEN];
     const double dux_prev = p[R_DUX], duy_prev = p[R_DUY], dvz_prev = p[R_DVZ];
-    const double dux = +0.5 * ga * dt * (p[R_CFX] + p[R_CFX + 1]) + dt * p[R_FUX]
-                       + (p[R_IUX + 1] - p[R_IUX]) + 1.0 * ga * dt * (1. - den) / (Fr * Fr);
-    const double duy = +0.5 * ga * dt * (p[R_CFY] + p[R_CFY + 1]) + dt * p[R_FUY]
-                       + (p[R_IUY + 1] - p[R_IUY]);
-    const double dvz = +0.5 * ga * dt * (p[R_CTZ] + p[R_CTZ + 1]) + dt * p[R_TVZ]
-                       + (p[R_IVZ + 1] - p[R_IVZ]);
+    const double cfx0 = p[R_CFX], cfy0 = p[R_CFY], ctz0 = p[R_CTZ];
+    double cfx1 = p[R_CFX + 1], cfy1 = p[R_CFY + 1], ctz1 = p[R_CTZ + 1];
+    const double fux = p[R_FUX], fuy = p[R_FUY], tvz = p[R_TVZ];
+    const double iux0 = p[R_IUX], iuy0 = p[R_IUY], ivz0 = p[R_IVZ];
+    const double ux0 = p[R_UX], uy0 = p[R_UY], vz0 = p[R_VZ];
+    const double x0 = p[R_X], y0 = p[R_Y], az0 = p[R_AZ];
+    double iux1, iuy1, ivz1;
+    SliceHead hd;
+    if (fin) {  // inertia(1) of this sub-iteration: its sums and cache records
+      hd = ia.heads[n];
+      iux1 = tot[0];
+      iuy1 = tot[1];
+      ivz1 = tot[2];
+      p[R_IUX + 1] = iux1;
+      p[R_IUY + 1] = iuy1;
+      p[R_IVZ + 1] = ivz1;
+      record_pass(ia, n, hd, tot);
+    } else {
+      iux1 = p[R_IUX + 1];
+      iuy1 = p[R_IUY + 1];
+      ivz1 = p[R_IVZ + 1];
+    }
+    if (ia.res && ia.sub == 0) {  // collision(1) at the stage-start positions
+      cfx1 = cfx0;
+      cfy1 = cfy0;
+      ctz1 = ctz0;
+      p[R_CFX + 1] = cfx1;
+      p[R_CFY + 1] = cfy1;
+      p[R_CTZ + 1] = ctz1;
+    }
+    const double dux = +0.5 * ga * dt * (cfx0 + cfx1) + dt * fux + (iux1 - iux0)
+                       + 1.0 * ga * dt * (1. - den) / (Fr * Fr);
+    const double duy = +0.5 * ga * dt * (cfy0 + cfy1) + dt * fuy + (iuy1 - iuy0);
+    const double dvz = +0.5 * ga * dt * (ctz0 + ctz1) + dt * tvz + (ivz1 - ivz0);
     p[R_DUX] = dux;
     p[R_DUY] = duy;
     p[R_DVZ] = dvz;
     res = fmax(res, fabs(dux - dux_prev));
     res = fmax(res, fabs(duy - duy_prev));
     res = fmax(res, fabs(dvz - dvz_prev));
-    const double ux0 = p[R_UX], uy0 = p[R_UY], vz0 = p[R_VZ];
-    const double ux1 = p[R_UX] + dux, uy1 = p[R_UY] + duy, vz1 = p[R_VZ] + dvz;
-    p[R_DX] = 0.5 * ga * dt * (ux0 + ux1);
-    p[R_DY] = 0.5 * ga * dt * (uy0 + uy1);
-    p[R_DAZ] = 0.5 * ga * dt * (vz0 + vz1);
+    const double ux1 = ux0 + dux, uy1 = uy0 + duy, vz1 = vz0 + dvz;
+    const double dxn = 0.5 * ga * dt * (ux0 + ux1);
+    const double dyn = 0.5 * ga * dt * (uy0 + uy1);
+    const double dazn = 0.5 * ga * dt * (vz0 + vz1);
+    p[R_DX] = dxn;
+    p[R_DY] = dyn;
+    p[R_DAZ] = dazn;
     // the next sub-iteration's inertia pass
-    write_head(ia, n, p[R_X] + p[R_DX], p[R_Y] + p[R_DY], p[R_AZ] + p[R_DAZ], false);
+    write_head(ia, n, x0 + dxn, y0 + dyn, az0 + dazn, false, fin ? &hd : nullptr);
   }
   if (lane == 0) red[w] = res;
   __syncthreads();
