@@ -298,10 +298,12 @@ __global__ __launch_bounds__((1 << LOGN) >> dct_logr<LOGN>()) void k_rhs_dct2_t(
   double *re = sh, *im = sh + nx;
   const double dt = dscal[S_DT];
   const double fac = 1. / (ga * dt);
+  // cells in memory order (coalesced loads), each stored at its Makhoul
+  // position m: i = 2m for m < nx/2, i = 2(nx-1-m)+1 above
 #pragma unroll
   for (int s = 0; s < R; s++) {
-    const int m = threadIdx.x + s * T;
-    const int i = m < nx / 2 ? 2 * m : 2 * (nx - 1 - m) + 1;  // Makhoul order
+    const int i = threadIdx.x + s * T;
+    const int m = (i & 1) ? nx - 1 - (i >> 1) : i >> 1;
     const int q = sw(bitrev(m, LOGN));
     re[q] = rhs_value<P2>(L, ux, uy, fac, dx, dy, i, ja);
     im[q] = hasb ? rhs_value<P2>(L, ux, uy, fac, dx, dy, i, jb) : 0.;
