@@ -614,19 +614,49 @@ __device__ __forceinline__ void inertia_slice(const IbmArgs &a, int cnstep, int 
   int it64 = 0, it32 = 0;
   const int t0 = (slice - s0) * IBM_SLICE;
   const int t1 = t0 + IBM_SLICE < total ? t0 + IBM_SLICE : total;
+  if (cached) {
+    // all loads of the lane's cells first (a cell past t1 loads the last
+    // cell instead and is not added), then the sums in cell order
+    if (t0 < t1) {
+      constexpr int U = IBM_SLICE / 64;
+      double th_[U], x_[U], y_[U], vx0[U], vx1[U], vy0[U], vy1[U];
+      int k_[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int t = min(t0 + (int)threadIdx.x + 64 * u, t1 - 1);
+        int k, i, j;
+        cell_of(bs, t, k, i, j);
+        k_[u] = k;
+        x_[u] = a.xc[i];
+        y_[u] = a.yc[j - 1];
+        th_[u] = dist[t];
+        vx0[u] = a.ux[a.L.ux(i, j)];
+        vx1[u] = a.ux[a.L.ux(i + 1, j)];
+        vy0[u] = a.uy[a.L.p(i, j)];
+        vy1[u] = a.uy[a.L.p(i, j + 1)];
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        if (t0 + (int)threadIdx.x + 64 * u >= t1) continue;
+        const double py_ = py + a.ly * (k_[u] - 1);
+        const double w = v_weight_th(th_[u]);
+        const double valx = w * 0.5 * (vx0[u] + vx1[u]) * dxdy;
+        const double valy = w * 0.5 * (vy0[u] + vy1[u]) * dxdy;
+        acc[0] += valx * inv_pm;
+        acc[2] += -(y_[u] - py_) * valx * inv_pim;
+        acc[1] += valy * inv_pm;
+        acc[2] += +(x_[u] - px) * valy * inv_pim;
+      }
+    }
+  } else
   for (int t = t0 + (int)threadIdx.x; t < t1; t += 64) {
     int k, i, j;
     cell_of(bs, t, k, i, j);
     const double py_ = py + a.ly * (k - 1);
     const double x = a.xc[i], y = a.yc[j - 1];
-    double th;
-    if (cached) {
-      th = dist[t];
-    } else {
-      th = weight_th(signed_dist_warm(e, f, 1. / a.grid_size, px, py_, cr, sr, x, y, warm,
-                                      tcell[t], it32, it64));
-      dist[t] = th;
-    }
+    const double th = weight_th(signed_dist_warm(e, f, 1. / a.grid_size, px, py_, cr, sr, x,
+                                                 y, warm, tcell[t], it32, it64));
+    dist[t] = th;
     const double w = v_weight_th(th);
     const double valx = w * 0.5 * (a.ux[a.L.ux(i, j)] + a.ux[a.L.ux(i + 1, j)]) * dxdy;
     const double valy = w * 0.5 * (a.uy[a.L.p(i, j)] + a.uy[a.L.p(i, j + 1)]) * dxdy;
@@ -904,34 +934,63 @@ __global__ __launch_bounds__(64) void k_exchange(IbmArgs a, int own_counts) {
   double ai[3] = {0., 0., 0.};   // inertia(0): iux, iuy, ivz
   const int t0 = (slice - s0) * IBM_SLICE;
   const int t1 = t0 + IBM_SLICE < total ? t0 + IBM_SLICE : total;
-  for (int t = t0 + (int)threadIdx.x; t < t1; t += 64) {
-    int k, i, j;
-    cell_of(bs, t, k, i, j);
-    const double py_ = py + a.ly * (k - 1);
-    const double x = a.xc[i], y = a.yc[j - 1];
-    const double th = dist[t];
-    if (a.us_x) {  // inertia(0), src/suspensions/inertia.c:55-60, at dx = 0
-      const double wv = v_weight_th(th);
-      const double valx = wv * 0.5 * (a.us_x[a.L.ux(i, j)] + a.us_x[a.L.ux(i + 1, j)]) * dxdy;
-      const double valy = wv * 0.5 * (a.us_y[a.L.p(i, j)] + a.us_y[a.L.p(i, j + 1)]) * dxdy;
-      ai[0] += valx * inv_pm;
-      ai[2] += -(y - py_) * valx * inv_pim;
-      ai[1] += valy * inv_pm;
-      ai[2] += +(x - px) * valy * inv_pim;
+  // all loads of the lane's cells first (a cell past t1 loads the last cell
+  // instead and is skipped), then the forcing and the sums in cell order
+  if (t0 < t1) {
+    constexpr int U = IBM_SLICE / 64;
+    double th_[U], x_[U], y_[U], fx0[U], fx1[U], fy0[U], fy1[U], sx0[U], sx1[U], sy0[U],
+        sy1[U];
+    int k_[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int t = min(t0 + (int)threadIdx.x + 64 * u, t1 - 1);
+      int k, i, j;
+      cell_of(bs, t, k, i, j);
+      k_[u] = k;
+      x_[u] = a.xc[i];
+      y_[u] = a.yc[j - 1];
+      th_[u] = dist[t];
+      fx0[u] = a.ux[a.L.ux(i, j)];
+      fx1[u] = a.ux[a.L.ux(i + 1, j)];
+      fy0[u] = a.uy[a.L.p(i, j)];
+      fy1[u] = a.uy[a.L.p(i, j + 1)];
+      if (a.us_x) {
+        sx0[u] = a.us_x[a.L.ux(i, j)];
+        sx1[u] = a.us_x[a.L.ux(i + 1, j)];
+        sy0[u] = a.us_y[a.L.p(i, j)];
+        sy1[u] = a.us_y[a.L.p(i, j + 1)];
+      }
     }
-    const double w = s_weight_th(th);
-    const double ux_p = pux - pvz * (y - py_);
-    const double uy_p = puy + pvz * (x - px);
-    const double ux_f = 0.5 * (a.ux[a.L.ux(i, j)] + a.ux[a.L.ux(i + 1, j)]);
-    const double uy_f = 0.5 * (a.uy[a.L.p(i, j)] + a.uy[a.L.p(i, j + 1)]);
-    const double fx = w * (ux_p - ux_f) * inv_dt;
-    const double fy = w * (uy_p - uy_f) * inv_dt;
-    fxo[t] = fx * dt;
-    fyo[t] = fy * dt;
-    acc[0] -= +(fx * dx * dy) * inv_pm;
-    acc[2] -= -(y - py_) * (fx * dx * dy) * inv_pim;
-    acc[1] -= +(fy * dx * dy) * inv_pm;
-    acc[2] -= +(x - px) * (fy * dx * dy) * inv_pim;
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int t = t0 + (int)threadIdx.x + 64 * u;
+      if (t >= t1) continue;
+      const double py_ = py + a.ly * (k_[u] - 1);
+      const double x = x_[u], y = y_[u];
+      const double th = th_[u];
+      if (a.us_x) {  // inertia(0), src/suspensions/inertia.c:55-60, at dx = 0
+        const double wv = v_weight_th(th);
+        const double valx = wv * 0.5 * (sx0[u] + sx1[u]) * dxdy;
+        const double valy = wv * 0.5 * (sy0[u] + sy1[u]) * dxdy;
+        ai[0] += valx * inv_pm;
+        ai[2] += -(y - py_) * valx * inv_pim;
+        ai[1] += valy * inv_pm;
+        ai[2] += +(x - px) * valy * inv_pim;
+      }
+      const double w = s_weight_th(th);
+      const double ux_p = pux - pvz * (y - py_);
+      const double uy_p = puy + pvz * (x - px);
+      const double ux_f = 0.5 * (fx0[u] + fx1[u]);
+      const double uy_f = 0.5 * (fy0[u] + fy1[u]);
+      const double fx = w * (ux_p - ux_f) * inv_dt;
+      const double fy = w * (uy_p - uy_f) * inv_dt;
+      fxo[t] = fx * dt;
+      fyo[t] = fy * dt;
+      acc[0] -= +(fx * dx * dy) * inv_pm;
+      acc[2] -= -(y - py_) * (fx * dx * dy) * inv_pim;
+      acc[1] -= +(fy * dx * dy) * inv_pm;
+      acc[2] -= +(x - px) * (fy * dx * dy) * inv_pim;
+    }
   }
   const double s0v = wave_sum(acc[0]), s1v = wave_sum(acc[1]), s2v = wave_sum(acc[2]);
   if (a.us_x) {
