@@ -722,11 +722,12 @@ int eif_gpu_integrate(eif_ctx *c, int substeps[3]) {
   CHECK_CTX(c);
   // Two streams.  Main: fluid kernels, exchange (with inertia(0) in the
   // same pass), deposit and the sub-iterations (inertia(1) with
-  // collision(1) in one launch, then the increment).  Side: the stage-start
-  // distances of stage 0 (beside its predictor), the exchange's particle
-  // sums and collision(0), which overlap the fluid kernels.  Events join the
-  // side work back before anything that reads its results or reuses its
-  // buffers.
+  // collision(1) in one launch, then the increment).  Side: the particle
+  // update, reset and stage-start distances beside the stage's predictor
+  // (stage 0, and later stages whose previous loop ended with its first
+  // batch), the exchange's particle sums and collision(0), which overlap the
+  // fluid kernels.  Events join the side work back before anything that
+  // reads its results or reuses its buffers.
   //
   // The predictor of stage rk+1 reads only the fluid fields, which the
   // sub-iterations of stage rk leave alone.  It is queued right behind the
@@ -734,6 +735,7 @@ int eif_gpu_integrate(eif_ctx *c, int substeps[3]) {
   // for that batch's count.  (Beside the sub-iterations, on the side
   // stream, it slows them more than it gains.)
   bool pred_queued = false;  // this stage's predictor is already queued
+  bool side_start = false;   // ... and the previous loop ended at ev_fork
   for (int rk = 0; rk < 3; rk++) {
     if (!pred_queued) {
       TRY(launch_reset(c, c->stream, true));
@@ -753,9 +755,20 @@ int eif_gpu_integrate(eif_ctx *c, int substeps[3]) {
       EIF_HIP(hipEventRecord(c->ev_i0, c->side));
       TRY(launch_predictor(c, rk));  // writes the other ux, uy buffers
       EIF_HIP(hipStreamWaitEvent(c->stream, c->ev_i0, 0));
+    } else if (side_start) {
+      // the previous stage's loop ended with its first batch, which ev_fork
+      // closes: the particle update, this stage's reset and the distances
+      // at the new positions run on the side stream beside the predictor
+      EIF_HIP(hipStreamWaitEvent(c->side, c->ev_fork, 0));
+      TRY(launch_update_particles(c, c->side));
+      TRY(launch_reset(c, c->side, true));
+      TRY(launch_ibm_dist(c, c->side));
+      EIF_HIP(hipEventRecord(c->ev_i0, c->side));
+      EIF_HIP(hipStreamWaitEvent(c->stream, c->ev_i0, 0));
     } else {
-      // behind the predictor: the previous stage's particle update, this
-      // stage's reset and the distances at the new positions
+      // behind the predictor (and the batches queued after it): the previous
+      // stage's particle update, this stage's reset and the distances at the
+      // new positions
       TRY(launch_update_particles(c));
       TRY(launch_reset(c, c->stream, true));
       TRY(launch_ibm_dist(c));
@@ -779,13 +792,16 @@ int eif_gpu_integrate(eif_ctx *c, int substeps[3]) {
     int queued = 0, done = 0;
     int batch = std::min(std::max(c->substep_guess, 1), 102);
     bool early = false;  // predictor of stage rk+1 queued
+    int batches = 0;
     for (;;) {
       if (early) swap_fields(c);  // the stage's own velocity for inertia(1)
       const int r = queue_substeps(c, rk, queued, batch);
       if (early) swap_fields(c);
       TRY(r);
       queued += batch;
+      batches++;
       if (!early && rk < 2) {
+        EIF_HIP(hipEventRecord(c->ev_fork, c->stream));  // the end of the first batch
         TRY(launch_predictor(c, rk + 1));
         early = true;
       }
@@ -806,6 +822,7 @@ int eif_gpu_integrate(eif_ctx *c, int substeps[3]) {
     c->substep_guess = done;
     if (substeps) substeps[rk] = done;
     pred_queued = early;
+    side_start = early && batches == 1;
   }
   TRY(launch_update_particles(c));
   c->dt_adv_valid = c->dt_adv_parts > 0;

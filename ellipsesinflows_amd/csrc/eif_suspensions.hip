@@ -654,12 +654,15 @@ __device__ __forceinline__ void inertia_slice(const IbmArgs &a, int cnstep, int 
     cell_of(bs, t, k, i, j);
     const double py_ = py + a.ly * (k - 1);
     const double x = a.xc[i], y = a.yc[j - 1];
+    // the velocities do not depend on the distance: loaded first
+    const double vx0 = a.ux[a.L.ux(i, j)], vx1 = a.ux[a.L.ux(i + 1, j)];
+    const double vy0 = a.uy[a.L.p(i, j)], vy1 = a.uy[a.L.p(i, j + 1)];
     const double th = weight_th(signed_dist_warm(e, f, 1. / a.grid_size, px, py_, cr, sr, x,
                                                  y, warm, tcell[t], it32, it64));
     dist[t] = th;
     const double w = v_weight_th(th);
-    const double valx = w * 0.5 * (a.ux[a.L.ux(i, j)] + a.ux[a.L.ux(i + 1, j)]) * dxdy;
-    const double valy = w * 0.5 * (a.uy[a.L.p(i, j)] + a.uy[a.L.p(i, j + 1)]) * dxdy;
+    const double valx = w * 0.5 * (vx0 + vx1) * dxdy;
+    const double valy = w * 0.5 * (vy0 + vy1) * dxdy;
     acc[0] += valx * inv_pm;
     acc[2] += -(y - py_) * valx * inv_pim;
     acc[1] += valy * inv_pm;
