@@ -139,7 +139,7 @@ const char *eif_gpu_profile_names(void) { return k_prof_names; }
 
 static void free_ctx(Ctx *c) {
   double **bufs[] = {&c->xf, &c->xc, &c->dxf, &c->dxc, &c->yf, &c->yc, &c->ux, &c->uy,
-                     &c->p, &c->psi, &c->ux_alt, &c->uy_alt, &c->srcuxa, &c->srcuya,
+                     &c->p, &c->psi, &c->ux_alt, &c->uy_alt, &c->ux_spare, &c->uy_spare, &c->srcuxa, &c->srcuya,
                      &c->dux, &c->duy, &c->qhat, &c->ctab, &c->twid, &c->th_val, &c->th_u,
                      &c->th_q1, &c->th_den, &c->th_last, &c->pt_fwd, &c->pt_u,
                      &c->pt_v, &c->pt_w, &c->pt_bnd, &c->pt_red, &c->pt_lvl, &c->pt_ends, &c->pt_g, &c->dscal, &c->dred,
@@ -280,6 +280,8 @@ int fluid_gpu_create(const eif_config *cfg, eif_ctx **out) {
       (rc = dalloc(&c->ux_alt, L.ux_len(), nullptr)) ||
       (rc = dalloc(&c->uy, L.p_len(), nullptr)) ||
       (rc = dalloc(&c->uy_alt, L.p_len(), nullptr)) ||
+      (rc = dalloc(&c->ux_spare, L.ux_len(), nullptr)) ||
+      (rc = dalloc(&c->uy_spare, L.p_len(), nullptr)) ||
       (rc = dalloc(&c->p, L.p_len(), nullptr)) ||
       (rc = dalloc(&c->psi, L.p_len(), nullptr)) ||
       (rc = dalloc(&c->dux, L.p_len(), nullptr)) ||
@@ -310,6 +312,7 @@ int fluid_gpu_create(const eif_config *cfg, eif_ctx **out) {
   // initial boundary values (src/fluid/init.c:33-85)
   if ((rc = launch_bc_ux(c, c->ux)) || (rc = launch_bc_uy(c, c->uy)) ||
       (rc = launch_bc_ux(c, c->ux_alt)) || (rc = launch_bc_uy(c, c->uy_alt)) ||
+      (rc = launch_bc_ux(c, c->ux_spare)) || (rc = launch_bc_uy(c, c->uy_spare)) ||
       (rc = launch_bc_p(c, c->p)))
     return fail(rc);
   if (hipStreamSynchronize(c->stream) != hipSuccess)
@@ -385,6 +388,7 @@ int fluid_gpu_download_field(eif_ctx *c, int which, double *host) {
   size_t n;
   TRY(field_ptr(c, which, &ptr, &n));
   if (which == EIF_FIELD_DUX || which == EIF_FIELD_DUY) TRY(launch_materialise_du(c));
+  if (which == EIF_FIELD_SRCUXA || which == EIF_FIELD_SRCUYA) TRY(launch_materialise_src(c));
   EIF_HIP(hipMemcpyAsync(host, ptr, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   EIF_HIP(hipStreamSynchronize(c->stream));
   return EIF_OK;
@@ -400,6 +404,8 @@ int fluid_gpu_upload_field(eif_ctx *c, int which, const double *host) {
   double *ptr;
   size_t n;
   TRY(field_ptr(c, which, &ptr, &n));
+  // the other source array keeps its values
+  if (which == EIF_FIELD_SRCUXA || which == EIF_FIELD_SRCUYA) TRY(launch_materialise_src(c));
   EIF_HIP(hipMemcpyAsync(ptr, host, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
   EIF_HIP(hipStreamSynchronize(c->stream));
   return EIF_OK;
@@ -753,7 +759,7 @@ int eif_gpu_integrate(eif_ctx *c, int substeps[3]) {
       EIF_HIP(hipStreamWaitEvent(c->side, c->ev_fork, 0));
       TRY(launch_ibm_dist(c, c->side));
       EIF_HIP(hipEventRecord(c->ev_i0, c->side));
-      TRY(launch_predictor(c, rk));  // writes the other ux, uy buffers
+      TRY(launch_predictor_prev(c, rk));  // writes the spare ux, uy buffers
       EIF_HIP(hipStreamWaitEvent(c->stream, c->ev_i0, 0));
     } else if (side_start) {
       // the previous stage's loop ended with its first batch, which ev_fork
@@ -802,7 +808,7 @@ int eif_gpu_integrate(eif_ctx *c, int substeps[3]) {
       batches++;
       if (!early && rk < 2) {
         EIF_HIP(hipEventRecord(c->ev_fork, c->stream));  // the end of the first batch
-        TRY(launch_predictor(c, rk + 1));
+        TRY(launch_predictor_prev(c, rk + 1));
         early = true;
       }
       HostStatus st;

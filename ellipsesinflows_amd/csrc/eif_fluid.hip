@@ -88,16 +88,27 @@ int launch_bc_p(Ctx *c, double *p) {
 
 // ---------------------------------------------------------------------
 // predictor: compute_src_ux/uy + update_ux/uy + boundary conditions,
-// fused; reads (ux, uy, p, srcA) and writes (ux_new, uy_new, srcA).
-// Each thread owns column i and marches over PRED_ROWS rows, keeping the
-// three-row window of ux/uy in registers.
+// fused.  Each thread owns column i and marches over PRED_ROWS rows,
+// keeping the three-row window of ux/uy in registers.  The explicit terms
+// A of the previous stage (the beta term) come from one of two places:
+//   PRED_STORE: srcuxa/srcuya, which this call overwrites with its own A
+//               (the reference's data flow; reads ux, uy, p, srcA and
+//               writes ux_new, uy_new, srcA);
+//   PRED_PREV:  recomputed from the previous stage's velocity, which the
+//               caller keeps (A depends on the velocity alone, so the
+//               values are the same bits); srcA is neither read nor
+//               written, which saves 2N values per stage.
+// PRED_SRC only evaluates A of (ux, uy) into srcA: srcuxa/srcuya on demand
+// after PRED_PREV stages.
 // ---------------------------------------------------------------------
 constexpr int PRED_BX = 256;
 constexpr int PRED_ROWS = 16;
+enum { PRED_STORE = 0, PRED_PREV = 1, PRED_SRC = 2 };
 
 struct PredArgs {
   Layout L;
   const double *ux, *uy, *p;
+  const double *uxo, *uyo;  // previous stage's velocity (PRED_PREV)
   double *uxn, *uyn, *sa_x, *sa_y;
   const double *dxf, *dxc;
   const double *dscal;
@@ -105,7 +116,88 @@ struct PredArgs {
   int use_b;
 };
 
+// the velocity stencil around (i, j): ux columns i-1, i, i+1 and uy
+// columns i-1, i, i+1 on rows j-1 (m), j (c), j+1 (p), as far as the
+// explicit terms read them
+struct PredWin {
+  double uxm_c, uxm_e, uxc_w, uxc_c, uxc_e, uxp_w, uxp_c, uxp_e;
+  double uym_c, uyc_w, uyc_c, uyc_e, uyp_w, uyp_c, uyp_e;
+};
+
+__device__ inline void win_start(const Layout &L, const double *ux, const double *uy, int i,
+                                 int j0, bool hx, PredWin &w) {
+  w.uxm_c = ux[L.ux(i, j0 - 1)];
+  w.uxm_e = ux[L.ux(i + 1, j0 - 1)];
+  w.uxc_w = hx ? ux[L.ux(i - 1, j0)] : 0.;
+  w.uxc_c = ux[L.ux(i, j0)];
+  w.uxc_e = ux[L.ux(i + 1, j0)];
+  w.uym_c = uy[L.p(i, j0 - 1)];
+  w.uyc_w = uy[L.p(i - 1, j0)];
+  w.uyc_c = uy[L.p(i, j0)];
+  w.uyc_e = uy[L.p(i + 1, j0)];
+}
+__device__ inline void win_row(const Layout &L, const double *ux, const double *uy, int i,
+                               int j, bool hx, PredWin &w) {
+  w.uxp_w = hx ? ux[L.ux(i - 1, j + 1)] : 0.;
+  w.uxp_c = ux[L.ux(i, j + 1)];
+  w.uxp_e = ux[L.ux(i + 1, j + 1)];
+  w.uyp_w = uy[L.p(i - 1, j + 1)];
+  w.uyp_c = uy[L.p(i, j + 1)];
+  w.uyp_e = uy[L.p(i + 1, j + 1)];
+}
+__device__ inline void win_slide(PredWin &w) {
+  w.uxm_c = w.uxc_c; w.uxm_e = w.uxc_e;
+  w.uxc_w = w.uxp_w; w.uxc_c = w.uxp_c; w.uxc_e = w.uxp_e;
+  w.uym_c = w.uyc_c;
+  w.uyc_w = w.uyp_w; w.uyc_c = w.uyp_c; w.uyc_e = w.uyp_e;
+}
+
+// grid factors of column i
+struct PredCol {
+  double dxf_im, dxf_i, dxc_i, dxc_ip, idxf_im, idxf_i, idxc_i, idxc_ip, dy, idy, rre;
+};
+
+// explicit terms of the x-momentum at face (i, j), src/fluid/update_velocity.c:28-88
 template <bool P2>
+__device__ inline double src_x(const PredWin &w, const PredCol &g) {
+  const double g_w = qdiv<P2>(-w.uxc_w + w.uxc_c, g.dxf_im, g.idxf_im);
+  const double g_e = qdiv<P2>(-w.uxc_c + w.uxc_e, g.dxf_i, g.idxf_i);
+  const double g_s = qdiv<P2>(-w.uxm_c + w.uxc_c, g.dy, g.idy);
+  const double g_n = qdiv<P2>(-w.uxc_c + w.uxp_c, g.dy, g.idy);
+  const double cw = g.dxf_im / (2. * g.dxc_i);
+  const double ce = g.dxf_i / (2. * g.dxc_i);
+  const double aw = 0.5 * w.uxc_w + 0.5 * w.uxc_c;
+  const double ae = 0.5 * w.uxc_c + 0.5 * w.uxc_e;
+  const double adv1 = -cw * aw * g_w - ce * ae * g_e;
+  const double vs = cw * w.uyc_w + ce * w.uyc_c;
+  const double vn = cw * w.uyp_w + ce * w.uyp_c;
+  const double adv2 = -0.5 * vs * g_s - 0.5 * vn * g_n;
+  const double dif1 = qdiv<P2>(g.rre * (g_e - g_w), g.dxc_i, g.idxc_i);
+  const double dif2 = qdiv<P2>(g.rre * (g_n - g_s), g.dy, g.idy);
+  return (adv1 + adv2) + (dif1 + dif2);
+}
+
+// explicit terms of the y-momentum at face (i, j), src/fluid/update_velocity.c:112-171
+template <bool P2>
+__device__ inline double src_y(const PredWin &w, const PredCol &g, double ext) {
+  const double g_w = qdiv<P2>(-w.uyc_w + w.uyc_c, g.dxc_i, g.idxc_i);
+  const double g_e = qdiv<P2>(-w.uyc_c + w.uyc_e, g.dxc_ip, g.idxc_ip);
+  const double g_s = qdiv<P2>(-w.uym_c + w.uyc_c, g.dy, g.idy);
+  const double g_n = qdiv<P2>(-w.uyc_c + w.uyp_c, g.dy, g.idy);
+  const double cw = g.dxc_i / (2. * g.dxf_i);
+  const double ce = g.dxc_ip / (2. * g.dxf_i);
+  const double uw = 0.5 * w.uxm_c + 0.5 * w.uxc_c;
+  const double ue = 0.5 * w.uxm_e + 0.5 * w.uxc_e;
+  const double adv1 = -cw * uw * g_w - ce * ue * g_e;
+  const double as = 0.5 * w.uym_c + 0.5 * w.uyc_c;
+  const double an = 0.5 * w.uyc_c + 0.5 * w.uyp_c;
+  const double adv2 = -0.5 * as * g_s - 0.5 * an * g_n;
+  const double dif1 = qdiv<P2>(g.rre * (g_e - g_w), g.dxf_i, g.idxf_i);
+  const double dif2 = qdiv<P2>(g.rre * (g_n - g_s), g.dy, g.idy);
+  return (adv1 + adv2) + (dif1 + dif2) + ext;
+}
+
+template <bool P2, int MODE>
 __global__ __launch_bounds__(PRED_BX) void k_predictor(PredArgs a) {
   const Layout L = a.L;
   const int nx = L.nx, ny = L.ny;
@@ -115,84 +207,77 @@ __global__ __launch_bounds__(PRED_BX) void k_predictor(PredArgs a) {
   const int j1 = min(j0 + PRED_ROWS - 1, ny);
   const double dt = a.dscal[S_DT];
   const double aldt = a.al * dt, bedt = a.be * dt, gadt = a.ga * dt;
-  const double dy = a.dy;
-  const double rre = 1. / a.Re;
-  // grid factors of this column
   const bool hx = i >= 2;  // ux(i) is an unknown for i = 2..nx
-  const double dxf_im = hx ? a.dxf[i - 2] : 0.;  // DXF(i-1)
-  const double dxf_i = a.dxf[i - 1];             // DXF(i)
-  const double dxc_i = a.dxc[i - 1];             // DXC(i)
-  const double dxc_ip = a.dxc[i];                // DXC(i+1)
-  const double idxf_im = hx ? 1. / dxf_im : 0., idxf_i = 1. / dxf_i;
-  const double idxc_i = 1. / dxc_i, idxc_ip = 1. / dxc_ip, idy = 1. / dy;
-  // ux window: columns i-1, i, i+1 (ux index i-1 exists for i >= 2)
-  const double *ux = a.ux, *uy = a.uy, *p = a.p;
-  auto ldux = [&](int ii, int jj) { return ux[L.ux(ii, jj)]; };
-  double uxm_c = ldux(i, j0 - 1), uxm_e = ldux(i + 1, j0 - 1);
-  double uxc_w = hx ? ldux(i - 1, j0) : 0., uxc_c = ldux(i, j0), uxc_e = ldux(i + 1, j0);
-  double uym_c = uy[L.p(i, j0 - 1)];
-  double uyc_w = uy[L.p(i - 1, j0)], uyc_c = uy[L.p(i, j0)], uyc_e = uy[L.p(i + 1, j0)];
-  double pm_c = p[L.p(i, j0 - 1)];
+  PredCol g;
+  g.dy = a.dy;
+  g.rre = 1. / a.Re;
+  g.dxf_im = hx ? a.dxf[i - 2] : 0.;  // DXF(i-1)
+  g.dxf_i = a.dxf[i - 1];             // DXF(i)
+  g.dxc_i = a.dxc[i - 1];             // DXC(i)
+  g.dxc_ip = a.dxc[i];                // DXC(i+1)
+  g.idxf_im = hx ? 1. / g.dxf_im : 0.;
+  g.idxf_i = 1. / g.dxf_i;
+  g.idxc_i = 1. / g.dxc_i;
+  g.idxc_ip = 1. / g.dxc_ip;
+  g.idy = 1. / g.dy;
+  const bool prev = MODE == PRED_PREV && a.use_b;
+  const double *p = a.p;
+  PredWin w, wo;
+  win_start(L, a.ux, a.uy, i, j0, hx, w);
+  if (prev) win_start(L, a.uxo, a.uyo, i, j0, hx, wo);
+  double pm_c = MODE == PRED_SRC ? 0. : p[L.p(i, j0 - 1)];
   for (int j = j0; j <= j1; j++) {
-    const double uxp_w = hx ? ldux(i - 1, j + 1) : 0.;
-    const double uxp_c = ldux(i, j + 1), uxp_e = ldux(i + 1, j + 1);
-    const double uyp_w = uy[L.p(i - 1, j + 1)], uyp_c = uy[L.p(i, j + 1)],
-                 uyp_e = uy[L.p(i + 1, j + 1)];
-    const double pc_w = p[L.p(i - 1, j)], pc_c = p[L.p(i, j)];
-    // ---- x-momentum at face (i, j), src/fluid/update_velocity.c:28-88
-    if (hx) {
-      const double g_w = qdiv<P2>(-uxc_w + uxc_c, dxf_im, idxf_im);
-      const double g_e = qdiv<P2>(-uxc_c + uxc_e, dxf_i, idxf_i);
-      const double g_s = qdiv<P2>(-uxm_c + uxc_c, dy, idy);
-      const double g_n = qdiv<P2>(-uxc_c + uxp_c, dy, idy);
-      const double cw = dxf_im / (2. * dxc_i);
-      const double ce = dxf_i / (2. * dxc_i);
-      const double aw = 0.5 * uxc_w + 0.5 * uxc_c;
-      const double ae = 0.5 * uxc_c + 0.5 * uxc_e;
-      const double adv1 = -cw * aw * g_w - ce * ae * g_e;
-      const double vs = cw * uyc_w + ce * uyc_c;
-      const double vn = cw * uyp_w + ce * uyp_c;
-      const double adv2 = -0.5 * vs * g_s - 0.5 * vn * g_n;
-      const double dif1 = qdiv<P2>(rre * (g_e - g_w), dxc_i, idxc_i);
-      const double dif2 = qdiv<P2>(rre * (g_n - g_s), dy, idy);
-      const double pre = qdiv<P2>(-(pc_c - pc_w), dxc_i, idxc_i);
-      const double A = (adv1 + adv2) + (dif1 + dif2);
-      const long si = L.sx(i, j);
-      const double bterm = a.use_b ? bedt * a.sa_x[si] : 0.;
-      a.sa_x[si] = A;
-      const double v = uxc_c + (aldt * A + bterm + gadt * pre);
-      a.uxn[L.ux(i, j)] = v;
-      if (j == ny) a.uxn[L.ux(i, 0)] = v;
-      if (j == 1) a.uxn[L.ux(i, ny + 1)] = v;
+    win_row(L, a.ux, a.uy, i, j, hx, w);
+    if (prev) win_row(L, a.uxo, a.uyo, i, j, hx, wo);
+    double pc_w = 0., pc_c = 0.;
+    if (MODE != PRED_SRC) {
+      pc_w = p[L.p(i - 1, j)];
+      pc_c = p[L.p(i, j)];
     }
-    // ---- y-momentum at face (i, j), src/fluid/update_velocity.c:112-171
+    // ---- x-momentum at face (i, j)
+    if (hx) {
+      const double A = src_x<P2>(w, g);
+      const long si = L.sx(i, j);
+      if (MODE == PRED_SRC) {
+        a.sa_x[si] = A;
+      } else {
+        const double pre = qdiv<P2>(-(pc_c - pc_w), g.dxc_i, g.idxc_i);
+        double bterm = 0.;
+        if (MODE == PRED_STORE) {
+          bterm = a.use_b ? bedt * a.sa_x[si] : 0.;
+          a.sa_x[si] = A;
+        } else if (prev) {
+          bterm = bedt * src_x<P2>(wo, g);
+        }
+        const double v = w.uxc_c + (aldt * A + bterm + gadt * pre);
+        a.uxn[L.ux(i, j)] = v;
+        if (j == ny) a.uxn[L.ux(i, 0)] = v;
+        if (j == 1) a.uxn[L.ux(i, ny + 1)] = v;
+      }
+    }
+    // ---- y-momentum at face (i, j)
     {
-      const double g_w = qdiv<P2>(-uyc_w + uyc_c, dxc_i, idxc_i);
-      const double g_e = qdiv<P2>(-uyc_c + uyc_e, dxc_ip, idxc_ip);
-      const double g_s = qdiv<P2>(-uym_c + uyc_c, dy, idy);
-      const double g_n = qdiv<P2>(-uyc_c + uyp_c, dy, idy);
-      const double cw = dxc_i / (2. * dxf_i);
-      const double ce = dxc_ip / (2. * dxf_i);
-      const double uw = 0.5 * uxm_c + 0.5 * uxc_c;
-      const double ue = 0.5 * uxm_e + 0.5 * uxc_e;
-      const double adv1 = -cw * uw * g_w - ce * ue * g_e;
-      const double as = 0.5 * uym_c + 0.5 * uyc_c;
-      const double an = 0.5 * uyc_c + 0.5 * uyp_c;
-      const double adv2 = -0.5 * as * g_s - 0.5 * an * g_n;
-      const double dif1 = qdiv<P2>(rre * (g_e - g_w), dxf_i, idxf_i);
-      const double dif2 = qdiv<P2>(rre * (g_n - g_s), dy, idy);
-      const double pre = qdiv<P2>(-(pc_c - pm_c), dy, idy);
-      const double A = (adv1 + adv2) + (dif1 + dif2) + a.ext;
+      const double A = src_y<P2>(w, g, a.ext);
       const long si = L.sy(i, j);
-      const double bterm = a.use_b ? bedt * a.sa_y[si] : 0.;
-      a.sa_y[si] = A;
-      const double v = uyc_c + (aldt * A + bterm + gadt * pre);
-      a.uyn[L.p(i, j)] = v;
-      if (j == ny) a.uyn[L.p(i, 0)] = v;
-      if (j == 1) a.uyn[L.p(i, ny + 1)] = v;
+      if (MODE == PRED_SRC) {
+        a.sa_y[si] = A;
+      } else {
+        const double pre = qdiv<P2>(-(pc_c - pm_c), g.dy, g.idy);
+        double bterm = 0.;
+        if (MODE == PRED_STORE) {
+          bterm = a.use_b ? bedt * a.sa_y[si] : 0.;
+          a.sa_y[si] = A;
+        } else if (prev) {
+          bterm = bedt * src_y<P2>(wo, g, a.ext);
+        }
+        const double v = w.uyc_c + (aldt * A + bterm + gadt * pre);
+        a.uyn[L.p(i, j)] = v;
+        if (j == ny) a.uyn[L.p(i, 0)] = v;
+        if (j == 1) a.uyn[L.p(i, ny + 1)] = v;
+      }
     }
     // walls of the new fields (boundary_conditions.c:19-22, :38-41)
-    if (i == 1) {
+    if (MODE != PRED_SRC && i == 1) {
       a.uxn[L.ux(1, j)] = 0.;
       a.uyn[L.p(0, j)] = a.wxm;
       if (j == 1) {
@@ -202,7 +287,7 @@ __global__ __launch_bounds__(PRED_BX) void k_predictor(PredArgs a) {
         a.uyn[L.p(0, ny + 1)] = a.wxm;
       }
     }
-    if (i == nx) {
+    if (MODE != PRED_SRC && i == nx) {
       a.uxn[L.ux(nx + 1, j)] = 0.;
       a.uyn[L.p(nx + 1, j)] = a.wxp;
       if (j == 1) {
@@ -212,22 +297,27 @@ __global__ __launch_bounds__(PRED_BX) void k_predictor(PredArgs a) {
         a.uyn[L.p(nx + 1, ny + 1)] = a.wxp;
       }
     }
-    // slide the window
-    uxm_c = uxc_c; uxm_e = uxc_e;
-    uxc_w = uxp_w; uxc_c = uxp_c; uxc_e = uxp_e;
-    uym_c = uyc_c;
-    uyc_w = uyp_w; uyc_c = uyp_c; uyc_e = uyp_e;
+    win_slide(w);
+    if (prev) win_slide(wo);
     pm_c = pc_c;
   }
 }
 
-int launch_predictor(Ctx *c, int rk, hipStream_t st) {
-  if (!st) st = c->stream;
-  ProfScope ps(c, "predictor", st);
+template <int MODE>
+static void pred_launch(Ctx *c, const PredArgs &a, hipStream_t st) {
+  dim3 grid((c->nx + PRED_BX - 1) / PRED_BX, (c->ny + PRED_ROWS - 1) / PRED_ROWS);
+  if (c->pow2_grid)
+    hipLaunchKernelGGL((k_predictor<true, MODE>), grid, dim3(PRED_BX), 0, st, a);
+  else
+    hipLaunchKernelGGL((k_predictor<false, MODE>), grid, dim3(PRED_BX), 0, st, a);
+}
+
+static PredArgs pred_args(Ctx *c, int rk) {
   PredArgs a;
   a.L = c->L;
   a.ux = c->ux; a.uy = c->uy; a.p = c->p;
-  a.uxn = c->ux_alt; a.uyn = c->uy_alt;
+  a.uxo = nullptr; a.uyo = nullptr;
+  a.uxn = nullptr; a.uyn = nullptr;
   a.sa_x = c->srcuxa; a.sa_y = c->srcuya;
   a.dxf = c->dxf; a.dxc = c->dxc;
   a.dscal = c->dscal;
@@ -235,14 +325,57 @@ int launch_predictor(Ctx *c, int rk, hipStream_t st) {
   a.al = c->alpha[rk]; a.be = c->beta[rk]; a.ga = c->gamma[rk];
   a.wxm = c->cfg.uy_wall_xm; a.wxp = c->cfg.uy_wall_xp;
   a.use_b = c->beta[rk] != 0.;
-  dim3 grid((c->nx + PRED_BX - 1) / PRED_BX, (c->ny + PRED_ROWS - 1) / PRED_ROWS);
-  if (c->pow2_grid)
-    hipLaunchKernelGGL(k_predictor<true>, grid, dim3(PRED_BX), 0, st, a);
-  else
-    hipLaunchKernelGGL(k_predictor<false>, grid, dim3(PRED_BX), 0, st, a);
+  return a;
+}
+
+// srcuxa/srcuya of the velocity they were last evaluated at, if the
+// PRED_PREV stages left them unevaluated
+int launch_materialise_src(Ctx *c) {
+  if (!c->src_lazy) return EIF_OK;
+  PredArgs a = pred_args(c, 0);
+  a.ux = c->ux_alt;  // the last stage's start velocity
+  a.uy = c->uy_alt;
+  pred_launch<PRED_SRC>(c, a, c->stream);
   EIF_HIP(hipGetLastError());
+  c->src_lazy = false;
+  return EIF_OK;
+}
+
+int launch_predictor(Ctx *c, int rk, hipStream_t st) {
+  if (!st) st = c->stream;
+  ProfScope ps(c, "predictor", st);
+  if (c->beta[rk] != 0.) {
+    const int rc = launch_materialise_src(c);
+    if (rc) return rc;
+  }
+  PredArgs a = pred_args(c, rk);
+  a.uxn = c->ux_alt; a.uyn = c->uy_alt;
+  pred_launch<PRED_STORE>(c, a, st);
+  EIF_HIP(hipGetLastError());
+  c->src_lazy = false;
   std::swap(c->ux, c->ux_alt);
   std::swap(c->uy, c->uy_alt);
+  return EIF_OK;
+}
+
+// The predictor of a stage of eif_gpu_integrate: the beta term from the
+// previous stage's velocity (ux_alt, kept since that stage's predictor),
+// the new velocity into the spare pair.  Afterwards ux holds the new
+// velocity, ux_alt this stage's start velocity and the spare the previous
+// one (free).  srcuxa/srcuya are evaluated only if read.
+int launch_predictor_prev(Ctx *c, int rk, hipStream_t st) {
+  if (!st) st = c->stream;
+  ProfScope ps(c, "predictor", st);
+  PredArgs a = pred_args(c, rk);
+  a.uxo = c->ux_alt; a.uyo = c->uy_alt;
+  a.uxn = c->ux_spare; a.uyn = c->uy_spare;
+  pred_launch<PRED_PREV>(c, a, st);
+  EIF_HIP(hipGetLastError());
+  c->src_lazy = true;
+  double *ox = c->ux_alt, *oy = c->uy_alt;
+  c->ux_alt = c->ux; c->uy_alt = c->uy;
+  c->ux = c->ux_spare; c->uy = c->uy_spare;
+  c->ux_spare = ox; c->uy_spare = oy;
   return EIF_OK;
 }
 

@@ -106,6 +106,11 @@ struct Ctx {
   // fluid fields (reference layouts); *_alt are ping-pong partners
   double *ux = nullptr, *uy = nullptr, *p = nullptr, *psi = nullptr;
   double *ux_alt = nullptr, *uy_alt = nullptr;
+  // the third velocity pair of eif_gpu_integrate's stages (see
+  // launch_predictor_prev); src_lazy: srcuxa/srcuya are due the values of
+  // A at (ux_alt, uy_alt)
+  double *ux_spare = nullptr, *uy_spare = nullptr;
+  bool src_lazy = false;
   double *srcuxa = nullptr, *srcuya = nullptr;
   double *dux = nullptr, *duy = nullptr;  // materialised only on request
   // Poisson
@@ -206,6 +211,8 @@ int launch_bc_uy(Ctx *c, double *uy);
 int launch_bc_p(Ctx *c, double *p);
 // swaps ux, uy with their ping-pong partners at launch
 int launch_predictor(Ctx *c, int rk, hipStream_t st = nullptr);
+int launch_predictor_prev(Ctx *c, int rk, hipStream_t st = nullptr);
+int launch_materialise_src(Ctx *c);
 // dt_fused: also leave the next decide_dt's advective limit as partial minima
 int launch_correct_pressure(Ctx *c, int rk, bool correct, bool pressure,
                             bool dt_fused = false);

@@ -332,6 +332,36 @@ def test_statistics_bit_exact():
     assert abs(a["phi"].sum() / 2 * (1.0 / nx) * (1.0 / ny) - area) < 0.05 * area
 
 
+def test_whole_steps_and_source_terms_bit_exact_thomas():
+    """Fluid-only Thomas steps: the step's predictors take the beta term
+    from the previous stage's velocity instead of srcuxa/srcuya, so those
+    are evaluated only when read.  After each step ux, uy, p and the source
+    arrays equal the oracle's bit for bit, and a per-function predictor
+    stage after the steps (which reads the sources) does too."""
+    nx, ny = 64, 48
+    g, o = pair(0, itot=nx, jtot=ny, ly=1.0, Re=150.0, extfrcy=0.2, uy_wall_xm=0.1)
+    ux, uy, p = cases.random_fields(nx, ny, 91, amp=0.3)
+    for s in (g, o):
+        s.upload(ux, uy, p)
+        s.update_boundaries()
+    for k in range(3):
+        dg, do = g.step(), o.step()
+        assert dg == do, k
+        a, b = g.download(), o.download()
+        for f in ("ux", "uy", "p"):
+            assert np.array_equal(a[f], b[f]), (k, f)
+        if k != 1:  # one step leaves the sources unread
+            for f in ("srcuxa", "srcuya"):
+                assert np.array_equal(g.field(f), o.field(f)), (k, f)
+    both(g, o, "update_velocity", 1)
+    a, b = g.download(), o.download()
+    for f in ("ux", "uy"):
+        assert np.array_equal(a[f], b[f]), f
+    for f in ("srcuxa", "srcuya"):
+        assert np.array_equal(g.field(f), o.field(f)), f
+    g.close()
+
+
 def test_errors_are_returned_not_raised_in_native_code():
     with pytest.raises(eif.EIFError):
         eif.Solver(itot=1, jtot=8)
