@@ -79,7 +79,10 @@ int fluid_gpu_upload(eif_ctx *ctx, const double *ux, const double *uy,
                      const double *p);
 int fluid_gpu_download(eif_ctx *ctx, double *ux, double *uy, double *p);
 
-/* Auxiliary fields for inspection, reference layouts. */
+/* Auxiliary fields for inspection, reference layouts.  After eif_gpu_step
+ * the source arrays hold the reference's values (A of the last stage's start
+ * velocity); the step itself does not store them, so the first read after a
+ * step evaluates them (one kernel). */
 #define EIF_FIELD_PSI 0    /* fluid_t.psi   (include/fluid.h:22)        */
 #define EIF_FIELD_SRCUXA 1 /* fluid_t.srcuxa (include/fluid.h:23)       */
 #define EIF_FIELD_SRCUYA 2 /* fluid_t.srcuya (include/fluid.h:24)       */
