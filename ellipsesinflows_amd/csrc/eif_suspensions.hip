@@ -183,14 +183,13 @@ __device__ inline double cos_poly(double u) {
 __device__ inline void sincos_q1(double t, double &s, double &c) {
   constexpr double PIO2_HI = 1.5707963267948966;      // nearest double to pi/2
   constexpr double PIO2_LO = 6.123233995736766e-17;   // pi/2 - PIO2_HI
-  if (t <= 0.78539816339744831) {
-    s = sin_poly(t);
-    c = cos_poly(t);
-  } else {
-    const double u = (PIO2_HI - t) + PIO2_LO;
-    s = cos_poly(u);
-    c = sin_poly(u);
-  }
+  // one pair of polynomials for both halves (no divergent branch): the
+  // argument and the roles of the results are selected instead
+  const bool low = t <= 0.78539816339744831;
+  const double u = low ? t : (PIO2_HI - t) + PIO2_LO;
+  const double ps = sin_poly(u), pc = cos_poly(u);
+  s = low ? ps : pc;
+  c = low ? pc : ps;
 }
 
 // 1/sqrt(x) from the hardware estimate and one Newton step (relative error
