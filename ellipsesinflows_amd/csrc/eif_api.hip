@@ -165,7 +165,7 @@ static void free_ctx(Ctx *c) {
     delete c->prof;
   }
   if (c->side) (void)hipStreamSynchronize(c->side);
-  hipEvent_t evs[] = {c->ev_fork, c->ev_i0, c->ev_c0, c->ev_x};
+  hipEvent_t evs[] = {c->ev_fork, c->ev_i0, c->ev_c0};
   for (auto ev : evs)
     if (ev) (void)hipEventDestroy(ev);
   if (c->side) (void)hipStreamDestroy(c->side);
@@ -209,7 +209,7 @@ int fluid_gpu_create(const eif_config *cfg, eif_ctx **out) {
   {
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
-    hipEvent_t *evs[] = {&c->ev_fork, &c->ev_i0, &c->ev_c0, &c->ev_x};
+    hipEvent_t *evs[] = {&c->ev_fork, &c->ev_i0, &c->ev_c0};
     for (auto ev : evs)
       if (e == hipSuccess) e = hipEventCreateWithFlags(ev, hipEventDisableTiming);
     if (e != hipSuccess) return fail(hip_fail(e, "hipStreamCreate", __FILE__, __LINE__));
@@ -727,13 +727,14 @@ static void swap_fields(Ctx *c) {
 int eif_gpu_integrate(eif_ctx *c, int substeps[3]) {
   CHECK_CTX(c);
   // Two streams.  Main: fluid kernels, exchange (with inertia(0) in the
-  // same pass), deposit and the sub-iterations (inertia(1) with
-  // collision(1) in one launch, then the increment).  Side: the particle
-  // update, reset and stage-start distances beside the stage's predictor
-  // (stage 0, and later stages whose previous loop ended with its first
-  // batch), the exchange's particle sums and collision(0), which overlap the
-  // fluid kernels.  Events join the side work back before anything that
-  // reads its results or reuses its buffers.
+  // same pass), deposit (with the exchange's particle sums in the same
+  // launch) and the sub-iterations (inertia(1) with collision(1) in one
+  // launch, then the increment).  Side: the particle update, reset and
+  // stage-start distances beside the stage's predictor (stage 0, and later
+  // stages whose previous loop ended with its first batch), then
+  // collision(0), which overlaps the fluid kernels.  Events join the side
+  // work back before anything that reads its results or reuses its
+  // buffers.
   //
   // The predictor of stage rk+1 reads only the fluid fields, which the
   // sub-iterations of stage rk leave alone.  It is queued right behind the
@@ -754,11 +755,14 @@ int eif_gpu_integrate(eif_ctx *c, int substeps[3]) {
         TRY(launch_bc_p(c, c->p));
         c->bc_valid = true;
       }
-      // the distances at the stage-start positions beside the predictor
+      // the distances at the stage-start positions and collision(0) beside
+      // the predictor
       EIF_HIP(hipEventRecord(c->ev_fork, c->stream));
       EIF_HIP(hipStreamWaitEvent(c->side, c->ev_fork, 0));
       TRY(launch_ibm_dist(c, c->side));
       EIF_HIP(hipEventRecord(c->ev_i0, c->side));
+      TRY(launch_collision(c, 0, c->side));
+      EIF_HIP(hipEventRecord(c->ev_c0, c->side));
       TRY(launch_predictor_prev(c, rk));  // writes the spare ux, uy buffers
       EIF_HIP(hipStreamWaitEvent(c->stream, c->ev_i0, 0));
     } else if (side_start) {
@@ -770,30 +774,33 @@ int eif_gpu_integrate(eif_ctx *c, int substeps[3]) {
       TRY(launch_reset(c, c->side, true));
       TRY(launch_ibm_dist(c, c->side));
       EIF_HIP(hipEventRecord(c->ev_i0, c->side));
+      TRY(launch_collision(c, 0, c->side));
+      EIF_HIP(hipEventRecord(c->ev_c0, c->side));
       EIF_HIP(hipStreamWaitEvent(c->stream, c->ev_i0, 0));
     } else {
       // behind the predictor (and the batches queued after it): the previous
       // stage's particle update, this stage's reset and the distances at the
-      // new positions
+      // new positions; collision(0) on the side stream
       TRY(launch_update_particles(c));
       TRY(launch_reset(c, c->stream, true));
       TRY(launch_ibm_dist(c));
+      EIF_HIP(hipEventRecord(c->ev_fork, c->stream));
+      EIF_HIP(hipStreamWaitEvent(c->side, c->ev_fork, 0));
+      TRY(launch_collision(c, 0, c->side));
+      EIF_HIP(hipEventRecord(c->ev_c0, c->side));
     }
     // the exchange with the predicted velocity and, in the same pass,
     // inertia(0) with the stage-start velocity: the pair the predictor read.
     // The reset cleared the tile registrations.
     TRY(launch_exchange_inertia0(c, c->ux_alt, c->uy_alt));
-    EIF_HIP(hipEventRecord(c->ev_x, c->stream));
-    EIF_HIP(hipStreamWaitEvent(c->side, c->ev_x, 0));
-    TRY(launch_exchange_finalize_x0(c, c->side));
-    TRY(launch_collision(c, 0, c->side));
-    EIF_HIP(hipEventRecord(c->ev_c0, c->side));
-    TRY(launch_update_momentum(c));
+    // the deposit, with the exchange's and inertia(0)'s particle sums in the
+    // same launch
+    TRY(launch_update_momentum_x0(c));
     TRY(launch_poisson(c, rk));
     // the last stage also reduces the advective dt limit of the final
     // velocities for the next param_decide_dt
     TRY(launch_correct_pressure(c, rk, true, true, rk == 2));
-    // the sub-iterations read the exchange sums and collision(0)
+    // the sub-iterations read collision(0)
     EIF_HIP(hipStreamWaitEvent(c->stream, c->ev_c0, 0));
     int queued = 0, done = 0;
     int batch = std::min(std::max(c->substep_guess, 1), 102);

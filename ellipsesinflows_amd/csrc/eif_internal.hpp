@@ -87,7 +87,6 @@ struct Ctx {
   // fluid kernels, and the events that join it back
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_i0 = nullptr, ev_c0 = nullptr;
-  hipEvent_t ev_x = nullptr;    // exchange done (main), for its finalize on the side stream
   int device = 0;
   // host copies of the grid (exact reference arithmetic)
   std::vector<double> h_xf, h_xc, h_dxf, h_dxc, h_yf, h_yc;
@@ -234,8 +233,8 @@ int launch_exchange_finalize(Ctx *c, hipStream_t st);
 int launch_ibm_dist(Ctx *c, hipStream_t st = nullptr);
 // the exchange with inertia(0) of the stage-start velocity in the same pass
 int launch_exchange_inertia0(Ctx *c, const double *us_x, const double *us_y);
-int launch_exchange_finalize_x0(Ctx *c, hipStream_t st);
 int launch_update_momentum(Ctx *c);
+int launch_update_momentum_x0(Ctx *c);
 int launch_materialise_du(Ctx *c);
 int launch_collision(Ctx *c, int cnstep, hipStream_t st = nullptr);
 // sub >= 0: sub-iteration `sub` of the stage, gated on the residual slots;

@@ -738,12 +738,10 @@ __device__ void finalize_particle(const IbmArgs &a, int n, int which, bool recor
   if (record) record_pass(a, n, a.heads[n], tot);
 }
 
-// The exchange's sums and those of the inertia(0) computed beside it (one
-// wave per particle, four per workgroup); the exchange's call records the
-// distance cache and the counters.
-__global__ __launch_bounds__(256) void k_ibm_finalize_x0(IbmArgs a) {
-  const int n = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
-  if (n >= a.np) return;
+// The exchange's sums of particle n and those of the inertia(0) computed
+// beside it, by one wave; the exchange's call records the distance cache
+// and the counters.
+__device__ inline void finalize_x0_particle(const IbmArgs &a, int n) {
   finalize_particle(a, n, 2);
   IbmArgs a0 = a;
   a0.slice_part = a.slice_part0;
@@ -1052,14 +1050,13 @@ constexpr int TILE_THREADS = 64;
 static_assert(TILE_X == TILE_THREADS, "one lane per tile column");
 static_assert(TILE_CAP <= TILE_THREADS, "one lane per registration");
 
-__global__ __launch_bounds__(TILE_THREADS) void k_tiles(TileArgs a) {
+__device__ __forceinline__ void tile_body(const TileArgs &a, int tile) {
   __shared__ int raw[TILE_CAP];
   __shared__ bool rep[TILE_CAP];
   __shared__ Box sbox[TILE_CAP];
   __shared__ int nlist;
   // tiles in grid order, so neighbouring workgroups touch neighbouring
   // memory; tiles no box reaches return at once
-  const int tile = blockIdx.x;
   const int cnt = min(a.tile_cnt[tile], TILE_CAP);
   if (cnt == 0) return;
   const int lane = threadIdx.x;
@@ -1148,6 +1145,19 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tiles(TileArgs a) {
       a.duy[L.p(i, j)] = duy_c;
     }
   }
+}
+
+__global__ __launch_bounds__(TILE_THREADS) void k_tiles(TileArgs a) { tile_body(a, blockIdx.x); }
+
+// The stage start's deposit with the exchange's per-particle sums (and
+// inertia(0)'s) in the same launch: the first np workgroups finalise one
+// particle each, the rest are the tiles.  Neither reads what the other
+// writes.
+__global__ __launch_bounds__(TILE_THREADS) void k_tiles_x0(TileArgs t, IbmArgs a) {
+  if ((int)blockIdx.x < a.np)
+    finalize_x0_particle(a, blockIdx.x);
+  else
+    tile_body(t, blockIdx.x - a.np);
 }
 
 // ---------------------------------------------------------------------
@@ -1758,7 +1768,7 @@ int launch_ibm_dist(Ctx *c, hipStream_t st) {
 // The exchange at the stage start (distances cached by launch_ibm_dist, tile
 // registrations cleared by the reset) with inertia(0) of the stage-start
 // velocity (us_x, us_y) in the same pass over the cells; the sums are added
-// by launch_exchange_finalize_x0
+// by the deposit launch (launch_update_momentum_x0)
 int launch_exchange_inertia0(Ctx *c, const double *us_x, const double *us_y) {
   if (c->np == 0) return EIF_OK;
   ProfScope ps(c, "ibm_exchange");
@@ -1766,13 +1776,6 @@ int launch_exchange_inertia0(Ctx *c, const double *us_x, const double *us_y) {
   a.us_x = us_x;
   a.us_y = us_y;
   hipLaunchKernelGGL(k_exchange, dim3(c->nslices), dim3(64), 0, c->stream, a, 0);
-  EIF_HIP(hipGetLastError());
-  return EIF_OK;
-}
-
-int launch_exchange_finalize_x0(Ctx *c, hipStream_t st) {
-  if (c->np == 0) return EIF_OK;
-  hipLaunchKernelGGL(k_ibm_finalize_x0, dim3((c->np + 3) / 4), dim3(256), 0, st, ibm_args(c));
   EIF_HIP(hipGetLastError());
   return EIF_OK;
 }
@@ -1816,6 +1819,17 @@ int launch_update_momentum(Ctx *c) {
   ProfScope ps(c, "ibm_apply");
   TileArgs t = tile_args(c, 1);
   hipLaunchKernelGGL(k_tiles, dim3(tile_grid(c)), dim3(TILE_THREADS), 0, c->stream, t);
+  EIF_HIP(hipGetLastError());
+  return EIF_OK;
+}
+
+// the stage start's deposit and the exchange's particle sums, one launch
+int launch_update_momentum_x0(Ctx *c) {
+  if (c->np == 0) return EIF_OK;
+  ProfScope ps(c, "ibm_apply");
+  TileArgs t = tile_args(c, 1);
+  hipLaunchKernelGGL(k_tiles_x0, dim3(c->np + tile_grid(c)), dim3(TILE_THREADS), 0, c->stream,
+                     t, ibm_args(c));
   EIF_HIP(hipGetLastError());
   return EIF_OK;
 }
