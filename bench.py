@@ -42,9 +42,9 @@ PEAK_HBM = 8.0e12
 # integrals, with its collision forces in the same launch (k_inertia_collide)
 DOMINANT = "ibm_substep"
 # one launch in PROFILE_EVERY is bracketed by timing events (each event pair
-# idles the stream for a few microseconds); 7 is prime to the usual 4-6
-# sub-iterations of a stage, so the timed launches rotate through them
-PROFILE_EVERY = 7
+# idles the stream for ~12 microseconds); 13 is prime to the usual 3-6 such
+# launches of a stage, so the timed launches rotate through them
+PROFILE_EVERY = 13
 
 
 def log(*a):

@@ -756,14 +756,15 @@ int eif_gpu_integrate(eif_ctx *c, int substeps[3]) {
         c->bc_valid = true;
       }
       // the distances at the stage-start positions and collision(0) beside
-      // the predictor
+      // the predictor (queued first: at a step's start the device waits
+      // for these submissions)
       EIF_HIP(hipEventRecord(c->ev_fork, c->stream));
+      TRY(launch_predictor_prev(c, rk));  // writes the spare ux, uy buffers
       EIF_HIP(hipStreamWaitEvent(c->side, c->ev_fork, 0));
       TRY(launch_ibm_dist(c, c->side));
       EIF_HIP(hipEventRecord(c->ev_i0, c->side));
       TRY(launch_collision(c, 0, c->side));
       EIF_HIP(hipEventRecord(c->ev_c0, c->side));
-      TRY(launch_predictor_prev(c, rk));  // writes the spare ux, uy buffers
       EIF_HIP(hipStreamWaitEvent(c->stream, c->ev_i0, 0));
     } else if (side_start) {
       // the previous stage's loop ended with its first batch, which ev_fork
