@@ -345,16 +345,39 @@ __global__ __launch_bounds__((1 << LOGN) >> dct_logr<LOGN>()) void k_dct3_psi_t(
   constexpr int nx = 1 << LOGN;
   constexpr int T = nx / R;
   const int ny = L.ny;
-  const int ja0 = 2 * blockIdx.x, jb0 = ja0 + 1;
+  // the two rows of this workgroup: consecutive rows, or in partition mode
+  // mirror rows r, L-1-r of one chunk, whose fix-ups read the same two spike
+  // rows v[r], v[L-1-r] and boundary values (half the table loads; the
+  // pairing changes only the rounding inside the shared transform)
+  int ja0 = 2 * blockIdx.x, jb0 = ja0 + 1;
+  if (ca.mode == 1) {
+    const int h = ca.L >> 1, cc = blockIdx.x / h, t = blockIdx.x - cc * h;
+    ja0 = cc * ca.L + t;
+    jb0 = cc * ca.L + ca.L - 1 - t;
+  }
   const bool hasb = jb0 < ny;
   double *re = sh, *im = sh + nx;
   // division by 2 nx, a power of two, is the exact product with its inverse
   constexpr double inv_norm = 1. / (2. * nx);
+  if (ca.mode == 1) {  // both rows from one load of the spike rows and boundary values
+    const int cc = ja0 / ca.L, ra = ja0 - cc * ca.L, rb = ca.L - 1 - ra;
+    const double *xlp = ca.bnd + ((long)cc * 2 + 0) * nx, *xrp = ca.bnd + ((long)cc * 2 + 1) * nx;
+    const double *va = ca.v + (long)ra * nx, *vb = ca.v + (long)rb * nx;
 #pragma unroll
-  for (int s = 0; s < R; s++) {
-    const int k = threadIdx.x + s * T;
-    re[sw(k)] = combined(ca, qhat, nx, ny, ja0, k);
-    im[sw(k)] = hasb ? combined(ca, qhat, nx, ny, jb0, k) : 0.;
+    for (int s = 0; s < R; s++) {
+      const int k = threadIdx.x + s * T;
+      const double qa = qhat[(long)ja0 * nx + k], qb = qhat[(long)jb0 * nx + k];
+      const double xl = xlp[k], xr = xrp[k], v0 = va[k], v1 = vb[k];
+      re[sw(k)] = qa - ca.coef * xl * v0 - ca.coef * xr * v1;
+      im[sw(k)] = qb - ca.coef * xl * v1 - ca.coef * xr * v0;
+    }
+  } else {
+#pragma unroll
+    for (int s = 0; s < R; s++) {
+      const int k = threadIdx.x + s * T;
+      re[sw(k)] = combined(ca, qhat, nx, ny, ja0, k);
+      im[sw(k)] = hasb ? combined(ca, qhat, nx, ny, jb0, k) : 0.;
+    }
   }
   __syncthreads();
   double xr[R], xi[R];
@@ -385,7 +408,7 @@ __global__ __launch_bounds__((1 << LOGN) >> dct_logr<LOGN>()) void k_dct3_psi_t(
   for (int r = 0; r < 2; r++) {
     if (r == 1 && !hasb) break;
     const double *src = r == 0 ? re : im;
-    const int j = ja0 + r + 1;
+    const int j = (r == 0 ? ja0 : jb0) + 1;
 #pragma unroll
     for (int s = 0; s < R; s++) {
       const int i = threadIdx.x + s * T;
