@@ -590,12 +590,16 @@ __global__ void k_heads(IbmArgs a, int with_dx) {
 // src/suspensions/inertia.c:9-70.  One wave per slice of IBM_SLICE box
 // cells of one particle (its three images' boxes flattened); the last
 // slice of a particle to finish adds the slices' sums in slice order.
-__device__ __forceinline__ void inertia_slice(const IbmArgs &a, int cnstep, int slice) {
+// open: the sub-iteration gate, evaluated by the caller; tested after this
+// slice's own loads are issued, so the gate's load is not waited for alone
+__device__ __forceinline__ void inertia_slice(const IbmArgs &a, int cnstep, int slice,
+                                              bool open = true) {
   const int n = a.slice_owner[slice];
   const int s0 = a.slice_first[n], nsl = a.slice_first[n + 1] - s0;
   const SliceHead &hd = a.heads[n];
   const BoxSet bs = hd.bs;
   const int total = bs.total;
+  if (!open) return;
   if (total > hd.cap || total > nsl * IBM_SLICE) {
     if (threadIdx.x == 0) atomicOr(&a.flags[0], 1);
     return;
@@ -677,8 +681,7 @@ __device__ __forceinline__ void inertia_slice(const IbmArgs &a, int cnstep, int 
 }
 
 __global__ __launch_bounds__(64) void k_inertia(IbmArgs a, int cnstep) {
-  if (!sub_open(a.res, a.sub)) return;
-  inertia_slice(a, cnstep, blockIdx.x);
+  inertia_slice(a, cnstep, blockIdx.x, sub_open(a.res, a.sub));
 }
 
 // Per-particle totals of the sliced inertia / exchange kernels, by the 64
@@ -687,11 +690,10 @@ __global__ __launch_bounds__(64) void k_inertia(IbmArgs a, int cnstep) {
 // records the position whose distances are now cached and the work
 // counters.  which: 0..1 inertia (cnstep), 2 exchange.
 // the particle's slice sums in slice order (every lane gets them)
-__device__ inline void slice_sums(const IbmArgs &a, int n, double tot[SLICE_VALS]) {
+__device__ inline void slice_sums(const IbmArgs &a, int s0, int s1, double tot[SLICE_VALS]) {
   const int lane = threadIdx.x & 63;
   for (int k = 0; k < SLICE_VALS; k++) tot[k] = 0.;
-  const int s1 = a.slice_first[n + 1];
-  for (int base = a.slice_first[n]; base < s1; base += 64) {
+  for (int base = s0; base < s1; base += 64) {
     double v[SLICE_VALS] = {0., 0., 0., 0., 0.};
     if (base + lane < s1)
       for (int k = 0; k < SLICE_VALS; k++)
@@ -721,7 +723,7 @@ __device__ inline void record_pass(const IbmArgs &a, int n, const SliceHead &hd,
 
 __device__ void finalize_particle(const IbmArgs &a, int n, int which, bool record = true) {
   double tot[SLICE_VALS];
-  slice_sums(a, n, tot);
+  slice_sums(a, a.slice_first[n], a.slice_first[n + 1], tot);
   if ((threadIdx.x & 63) != 0) return;
   double *q = a.prt + (long)n * R_N;
   if (which < 2) {
@@ -1442,11 +1444,12 @@ __global__ __launch_bounds__(64) void k_collide(CollArgs a) {
 // workgroups are the collisions (long, started first), the rest the IBM
 // slices.
 __global__ __launch_bounds__(64) void k_inertia_collide(IbmArgs a, CollArgs ca, int cnstep) {
-  if (!sub_open(a.res, a.sub)) return;
-  if ((int)blockIdx.x < ca.np)
-    collide_particle(ca, blockIdx.x);
-  else
-    inertia_slice(a, cnstep, blockIdx.x - ca.np);
+  const bool open = sub_open(a.res, a.sub);
+  if ((int)blockIdx.x < ca.np) {
+    if (open) collide_particle(ca, blockIdx.x);
+  } else {
+    inertia_slice(a, cnstep, blockIdx.x - ca.np, open);
+  }
 }
 
 __global__ void k_zero_int(int *p, int n) {
@@ -1473,13 +1476,21 @@ __global__ void k_zero_residual(double *dscal) { dscal[S_RESIDUAL] = 0.; }
 __global__ __launch_bounds__(256) void k_increment(double *prt, int np, const double *dscal,
                                                    double ga, double Fr, double *res_out,
                                                    IbmArgs ia, int fin) {
-  if (!sub_open(ia.res, ia.sub)) return;
+  const bool open = sub_open(ia.res, ia.sub);
   __shared__ double red[4];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int n = blockIdx.x * 4 + w;
+  // the slice range is loaded before the gate is tested (all threads of
+  // the workgroup see the same gate)
+  int s0 = 0, s1 = 0;
+  if (n < np && fin) {
+    s0 = ia.slice_first[n];
+    s1 = ia.slice_first[n + 1];
+  }
+  if (!open) return;
   double res = 0.;
   double tot[SLICE_VALS];
-  if (n < np && fin) slice_sums(ia, n, tot);
+  if (n < np && fin) slice_sums(ia, s0, s1, tot);
   if (n < np && lane == 0) {
     double *p = prt + (long)n * R_N;
     // the record's inputs, loaded together
