@@ -260,6 +260,34 @@ def test_c1_full_size_steps():
     g.close()
 
 
+def test_partition_mode_whole_steps_with_ellipses():
+    """The bench's path: partitioned y-solve (mirror-paired DCT-III), IBM,
+    collisions, the previous-velocity predictors and both streams, over
+    whole steps against the oracle (Thomas).  The partitioned solve changes
+    the rounding of psi only (conditioning floor ~1e-13), so the same bars
+    as the Thomas-mode whole steps hold."""
+    parts = cases.place_ellipses(16, 2.0, (0.03, 0.05), (0.4, 1.0), 0.01, 5, den_range=(1.0, 1.5))
+    kw = dict(itot=256, jtot=512, ly=2.0, Re=400.0, extfrcy=8.0 / 400.0)
+    g = eif.Solver(poisson_mode=1, **kw)
+    o = Oracle(**kw)
+    xc = g.grid()["xc"]
+    uy = np.zeros((514, 258))
+    uy[1:-1, 1:-1] = 4.0 * xc[None, 1:-1] * (1 - xc[None, 1:-1])
+    for s in (g, o):
+        s.upload(uy=uy)
+        s.update_boundaries()
+        s.set_particles(**parts)
+    for k in range(4):
+        dg, do = g.step(), o.step()
+        assert dg == pytest.approx(do, rel=1e-12), k
+        a, b = g.download(), o.download()
+        assert rel(a["ux"], b["ux"]) < 1e-12 and rel(a["uy"], b["uy"]) < 1e-12, k
+        assert rel_mod_mean(a["p"], b["p"]) < 1e-10, k
+    rg, ro = g.particles(), o.particles()
+    assert np.max(np.abs(rg[:, 3:6] - ro[:, 3:6])) < 1e-12
+    g.close()
+
+
 def test_contacts_through_whole_steps():
     """ellipses in contact (collisions active) over several steps."""
     parts = cases.contact_pairs(6, 1.0, 21, overlap=(0.9, 0.98))
