@@ -1442,8 +1442,10 @@ __global__ __launch_bounds__(64) void k_collide(CollArgs a) {
 // One launch for a sub-iteration's inertia(1) and collision(1), which read
 // the same positions and write disjoint particle members: the first np
 // workgroups are the collisions (long, started first), the rest the IBM
-// slices.
-__global__ __launch_bounds__(64) void k_inertia_collide(IbmArgs a, CollArgs ca, int cnstep) {
+// slices.  Bounded to 4 waves per SIMD (128 VGPRs): the collision path
+// alone needs more and spills a little, which costs less than the
+// occupancy its registers would take from the slices.
+__global__ __launch_bounds__(64, 4) void k_inertia_collide(IbmArgs a, CollArgs ca, int cnstep) {
   const bool open = sub_open(a.res, a.sub);
   if ((int)blockIdx.x < ca.np) {
     if (open) collide_particle(ca, blockIdx.x);
