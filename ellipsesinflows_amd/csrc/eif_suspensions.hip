@@ -284,41 +284,14 @@ __device__ inline void foot_fp64(const EllipseShape &e, double px, double py, do
   }
 }
 
-__device__ inline void foot_point(const EllipseShape &e, const EllipseShape32 &f,
-                                  double px, double py, double &c, double &s,
-                                  int &it32, int &it64) {
-  double t = (double)foot_fp32(f, px, py, it32);
-  foot_fp64(e, px, py, t, c, s, it64);
-}
-
 // compute_signed_dist (src/suspensions/auxiliary.c:19-47) with the foot
-// from foot_point; the unfolding of ellipse.c:62-63 maps (cos t, sin t) of
-// the folded foot to (-cos t, sin t) for x_ < 0 and (cos t, -sin t) for
-// y_ < 0
-__device__ inline double signed_dist_fast(const EllipseShape &e, const EllipseShape32 &f,
-                                          double inv_grid, double px, double py, double cr,
-                                          double sr, double x, double y, int &it32,
-                                          int &it64) {
-  const double ddx = x - px, ddy = y - py;
-  const double x_ = cr * ddx - sr * ddy;
-  const double y_ = sr * ddx + cr * ddy;
-  const double xa = x_ * e.ia, yb = y_ * e.ib;
-  const double f0 = 1. - xa * xa - yb * yb;
-  const double sign = f0 > 0. ? +1. : -1.;
-  double ct, st;
-  foot_point(e, f, fabs(x_), fabs(y_), ct, st, it32, it64);
-  const double fx = x_ < 0. ? -(e.a * ct) : e.a * ct;
-  const double fy = y_ < 0. ? -(e.b * st) : e.b * st;
-  const double ex = fx - x_, ey = fy - y_;
-  const double dist = sign * sqrt(+ex * ex + ey * ey);
-  return dist * inv_grid;
-}
-
-// signed_dist_fast with the foot iteration started from the parameter this
-// cell's foot had at the particle's previous evaluation in the same RK
-// stage (warm), which the particle has barely moved from; the root on the
-// first-quadrant arc is unique, so the start changes only the number of
-// iterations.  The new parameter is stored back.
+// from foot_fp32 / foot_fp64; the unfolding of ellipse.c:62-63 maps
+// (cos t, sin t) of the folded foot to (-cos t, sin t) for x_ < 0 and
+// (cos t, -sin t) for y_ < 0.  The foot iteration starts from the parameter this cell's foot had at the
+// particle's previous evaluation in the same RK stage (warm), which the
+// particle has barely moved from, or else from the FP32 estimate; the root
+// on the first-quadrant arc is unique, so the start changes only the number
+// of iterations.  The new parameter is stored back.
 __device__ inline double signed_dist_warm(const EllipseShape &e, const EllipseShape32 &f,
                                           double inv_grid, double px, double py, double cr,
                                           double sr, double x, double y, bool warm,
@@ -354,8 +327,6 @@ __device__ inline double tanh_w(double x) {
 __device__ inline double weight_th(double d) { return tanh_w(BETA * d); }
 __device__ inline double v_weight_th(double th) { return 0.5 * (1. + th); }
 __device__ inline double s_weight_th(double th) { return 0.5 * BETA * (1. - th * th); }
-__device__ inline double v_weight(double d) { return v_weight_th(weight_th(d)); }
-__device__ inline double s_weight(double d) { return s_weight_th(weight_th(d)); }
 
 // mass and moment of inertia, src/suspensions/auxiliary.c:63-75
 __device__ inline double p_mass(const double *q) {
