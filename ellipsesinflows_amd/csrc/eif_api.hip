@@ -72,6 +72,29 @@ bool prof_begin(Ctx *c, const char *name, hipStream_t st) {
   return hipEventRecord(r.ev[r.used].first, st) == hipSuccess;
 }
 
+// For a scope that is one kernel launch: the event pair that the launch
+// itself stamps (hipExtLaunchKernel), so the elapsed time is the kernel's
+// and no marker packet sits between it and its neighbours.  False when
+// this launch is not sampled.
+bool prof_kernel_events(Ctx *c, const char *name, hipEvent_t *start, hipEvent_t *stop) {
+  if (!c->prof || !c->prof->on) return false;
+  if (!c->prof->only.empty() && c->prof->only != name) return false;
+  auto &r = c->prof->recs[name];
+  if (r.seen++ % c->prof->every != 0) return false;
+  // pairs already stamped are read back before the store is reused (the
+  // pair handed out below is stamped only by the caller's launch)
+  if (r.used >= 4096 && prof_flush(c) != EIF_OK) return false;
+  if (r.used == r.ev.size()) {
+    hipEvent_t a, b;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return false;
+    r.ev.push_back({a, b});
+  }
+  *start = r.ev[r.used].first;
+  *stop = r.ev[r.used].second;
+  r.used++;
+  return true;
+}
+
 void prof_end(Ctx *c, const char *name, hipStream_t st) {
   auto &r = c->prof->recs[name];
   if (r.used >= r.ev.size()) return;

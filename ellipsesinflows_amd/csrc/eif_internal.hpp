@@ -188,6 +188,7 @@ int hip_fail(hipError_t e, const char *what, const char *file, int line);
 // prof_begin: whether this launch is timed (then prof_end closes it)
 bool prof_begin(Ctx *c, const char *name, hipStream_t st);
 void prof_end(Ctx *c, const char *name, hipStream_t st);
+bool prof_kernel_events(Ctx *c, const char *name, hipEvent_t *start, hipEvent_t *stop);
 struct ProfScope {  // events on the stream the scoped kernels run on
   Ctx *c;
   const char *n;

@@ -29,6 +29,7 @@
 //     one thread per particle gathers its partners' forces in the
 //     reference's pair order.
 #include "eif_internal.hpp"
+#include <hip/hip_ext.h>
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
@@ -1872,12 +1873,16 @@ int launch_substep_forces(Ctx *c, int sub) {
     EIF_HIP(hipGetLastError());
     return EIF_OK;
   }
-  ProfScope ps(c, "ibm_substep");
   CollArgs ca = coll_args(c, 1);
   ca.res = a.res;
   ca.sub = sub;
-  hipLaunchKernelGGL(k_inertia_collide, dim3(c->np + c->nslices), dim3(64), 0, c->stream, a,
-                     ca, 1);
+  hipEvent_t e0, e1;
+  if (prof_kernel_events(c, "ibm_substep", &e0, &e1))  // a sampled launch stamps its events
+    hipExtLaunchKernelGGL(k_inertia_collide, dim3(c->np + c->nslices), dim3(64), 0, c->stream,
+                          e0, e1, 0, a, ca, 1);
+  else
+    hipLaunchKernelGGL(k_inertia_collide, dim3(c->np + c->nslices), dim3(64), 0, c->stream, a,
+                       ca, 1);
   EIF_HIP(hipGetLastError());
   return EIF_OK;
 }
