@@ -41,9 +41,9 @@ PEAK_HBM = 8.0e12
 # the dominant launch: a Crank-Nicolson sub-iteration's IBM inertia
 # integrals, with its collision forces in the same launch (k_inertia_collide)
 DOMINANT = "ibm_substep"
-# one launch in PROFILE_EVERY is bracketed by timing events (each event pair
-# idles the stream for ~12 microseconds); 13 is prime to the usual 3-6 such
-# launches of a stage, so the timed launches rotate through them
+# one launch in PROFILE_EVERY is timed by HIP events that the launch itself
+# stamps on the kernel's stream; 13 is prime to the usual 3-6 such launches
+# of a stage, so the timed launches rotate through them
 PROFILE_EVERY = 13
 
 
