@@ -390,6 +390,35 @@ def test_whole_steps_and_source_terms_bit_exact_thomas():
     g.close()
 
 
+def test_live_timing_changes_no_result():
+    """The profiling scopes (recorded events around multi-kernel scopes,
+    launch-stamped events for the sampled sub-iteration launch) time the
+    step without changing it: identical fields and particles with and
+    without them, and the sampled counts add up."""
+    parts = cases.contact_pairs(6, 1.0, 23)
+    outs = []
+    for prof in (False, True):
+        g = eif.Solver(itot=128, jtot=128, ly=1.0, Re=50.0, extfrcy=0.5, poisson_mode=1)
+        g.set_particles(**parts)
+        if prof:
+            g.profile(True)
+            g.profile_sample(2)
+        for _ in range(4):
+            g.step()
+        if prof:
+            g.profile(False)
+            n_sub, ms_sub = g.profile_query("ibm_substep")
+            n_pred, ms_pred = g.profile_query("predictor")
+            assert n_sub > 0 and ms_sub > 0.
+            assert n_pred > 0 and ms_pred > 0.
+            assert ms_sub / n_sub < 1e3  # milliseconds per launch, not a stale stamp
+        outs.append((g.download(), g.particles()))
+        g.close()
+    for k in ("ux", "uy", "p"):
+        assert np.array_equal(outs[0][0][k], outs[1][0][k]), k
+    assert np.array_equal(outs[0][1], outs[1][1])
+
+
 def test_errors_are_returned_not_raised_in_native_code():
     with pytest.raises(eif.EIFError):
         eif.Solver(itot=1, jtot=8)
