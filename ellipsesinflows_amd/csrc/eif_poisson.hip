@@ -530,6 +530,29 @@ __global__ __launch_bounds__(PART_LANES) void k_part_local(
 // PART_KG wavenumbers with 64 threads each.
 constexpr int PART_KG = 4;
 
+// the 2x2 system of one chunk-DFT mode (cos/sin of its angle ct, st) for
+// one wavenumber; gauge: the singular mode 0 of wavenumber 0
+__device__ inline void mode_solve(double lv0, double lvL, bool gauge, double ct, double st,
+                                  double a0r, double a0i, double a1r, double a1i, double &x0r,
+                                  double &x0i, double &x1r, double &x1i) {
+  if (gauge) {
+    const double f = 1. / (2. * ((1. + lvL) - lv0));
+    x0r = (a0r - a1r) * f;
+    x0i = (a0i - a1i) * f;
+    x1r = -x0r;
+    x1i = -x0i;
+  } else {
+    const double det = 1. + 2. * lvL * ct + lvL * lvL - lv0 * lv0;
+    // adj(M) = [[1 + lvL e^{i th}, -lv0], [-lv0, 1 + lvL e^{-i th}]]
+    const double p0r = 1. + lvL * ct, p0i = lvL * st;   // 1 + lvL e^{i th}
+    const double p1r = 1. + lvL * ct, p1i = -lvL * st;  // 1 + lvL e^{-i th}
+    x0r = (p0r * a0r - p0i * a0i - lv0 * a1r) / det;
+    x0i = (p0r * a0i + p0i * a0r - lv0 * a1i) / det;
+    x1r = (-lv0 * a0r + p1r * a1r - p1i * a1i) / det;
+    x1i = (-lv0 * a0i + p1r * a1i + p1i * a1r) / det;
+  }
+}
+
 __global__ __launch_bounds__(64 * PART_KG) void k_part_reduced(
     int nx, int P, const double *__restrict__ ends, const double *__restrict__ lv,
     const double *__restrict__ twc, const double *__restrict__ tws,
@@ -567,24 +590,9 @@ __global__ __launch_bounds__(64 * PART_KG) void k_part_reduced(
         t += m;
         if (t >= P) t -= P;
       }
-      const double ct = tc[m], st = ts[m];
       double x0r, x0i, x1r, x1i;
-      if (gauge && m == 0) {
-        const double f = 1. / (2. * ((1. + lvL) - lv0));
-        x0r = (a0r - a1r) * f;
-        x0i = (a0i - a1i) * f;
-        x1r = -x0r;
-        x1i = -x0i;
-      } else {
-        const double det = 1. + 2. * lvL * ct + lvL * lvL - lv0 * lv0;
-        // adj(M) = [[1 + lvL e^{i th}, -lv0], [-lv0, 1 + lvL e^{-i th}]]
-        const double p0r = 1. + lvL * ct, p0i = lvL * st;   // 1 + lvL e^{i th}
-        const double p1r = 1. + lvL * ct, p1i = -lvL * st;  // 1 + lvL e^{-i th}
-        x0r = (p0r * a0r - p0i * a0i - lv0 * a1r) / det;
-        x0i = (p0r * a0i + p0i * a0r - lv0 * a1i) / det;
-        x1r = (-lv0 * a0r + p1r * a1r - p1i * a1i) / det;
-        x1i = (-lv0 * a0i + p1r * a1i + p1i * a1r) / det;
-      }
+      mode_solve(lv0, lvL, gauge && m == 0, tc[m], ts[m], a0r, a0i, a1r, a1i, x0r, x0i, x1r,
+                 x1i);
       u0r[m] = x0r; u0i[m] = x0i; u1r[m] = x1r; u1i[m] = x1i;
     }
   }
@@ -603,6 +611,97 @@ __global__ __launch_bounds__(64 * PART_KG) void k_part_reduced(
       // u_c = (beta_c, alpha_{c+1}): chunk c+1's left value, chunk c's right
       bnd[((long)((c + 1) % P) * 2 + 0) * nx + k] = b0 * invP;
       bnd[((long)c * 2 + 1) * nx + k] = b1 * invP;
+    }
+  }
+}
+
+// The same reduced system with both chunk DFTs as FP64 matrix products on
+// the matrix cores (v_mfma_f64_16x16x4f64), for P a multiple of 16 up to
+// 64: a workgroup takes 16 wavenumbers and P/16 waves.  Forward: wave w
+// forms the modes m in [16w, 16w+16) of a0 = C r0 - i S r0 and a1 (C, S:
+// cos and sin of 2 pi m c / P); the 2x2 solves run on the accumulators
+// (every lane holds the four values of its (m, k)); inverse: wave w forms
+// the chunks c in [16w, 16w+16) of b = C x_r - S x_i.  Lane layout of the
+// 16x16x4 product: lane l holds A[l%16][l/16], B[l/16][l%16], and
+// D[4i + l/16][l%16] in element i.  The products are summed in another
+// order than the lane loop of k_part_reduced (rounding only).
+typedef double pr_d4 __attribute__((ext_vector_type(4)));
+constexpr int PRM_K = 16;
+
+__global__ __launch_bounds__(256) void k_part_reduced_mfma(
+    int nx, int P, const double *__restrict__ ends, const double *__restrict__ lv,
+    const double *__restrict__ twc, const double *__restrict__ tws,
+    double *__restrict__ bnd) {
+  extern __shared__ double sh[];
+  double *tc = sh, *ts = tc + P;
+  double *r0 = ts + P, *r1 = r0 + P * PRM_K;  // [c][kk]
+  double *xs = r1 + P * PRM_K;                 // [4][m][kk]: x0r, x0i, x1r, x1i
+  const int k0 = blockIdx.x * PRM_K;
+  for (int e = threadIdx.x; e < P; e += blockDim.x) {
+    tc[e] = twc[e];
+    ts[e] = tws[e];
+  }
+  for (int e = threadIdx.x; e < P * PRM_K; e += blockDim.x) {
+    const int c = e / PRM_K, kk = e - c * PRM_K, k = k0 + kk;
+    const bool on = k < nx;
+    r0[e] = on ? ends[((long)c * 2 + 1) * nx + k] : 0.;             // y_c[L-1]
+    r1[e] = on ? ends[((long)((c + 1) % P) * 2 + 0) * nx + k] : 0.;  // y_{c+1}[0]
+  }
+  __syncthreads();
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int lo = l & 15, hi = l >> 4;
+  {
+    const int m0 = 16 * w;
+    pr_d4 a0r = {0., 0., 0., 0.}, a0i = a0r, a1r = a0r, a1i = a0r;
+    for (int c0 = 0; c0 < P; c0 += 4) {
+      const int t = ((m0 + lo) * (c0 + hi)) % P;
+      const double cw = tc[t], sw = -ts[t];
+      const double v0 = r0[(c0 + hi) * PRM_K + lo], v1 = r1[(c0 + hi) * PRM_K + lo];
+      a0r = __builtin_amdgcn_mfma_f64_16x16x4f64(cw, v0, a0r, 0, 0, 0);
+      a0i = __builtin_amdgcn_mfma_f64_16x16x4f64(sw, v0, a0i, 0, 0, 0);
+      a1r = __builtin_amdgcn_mfma_f64_16x16x4f64(cw, v1, a1r, 0, 0, 0);
+      a1i = __builtin_amdgcn_mfma_f64_16x16x4f64(sw, v1, a1i, 0, 0, 0);
+    }
+    const int k = k0 + lo;
+    const bool on = k < nx;
+    const double lv0 = on ? lv[k] : 0., lvL = on ? lv[nx + k] : 0.;
+    const bool gauge = on && lv[2 * (long)nx + k] != 0.;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int m = m0 + 4 * i + hi;
+      double x0r, x0i, x1r, x1i;
+      mode_solve(lv0, lvL, gauge && m == 0, tc[m], ts[m], a0r[i], a0i[i], a1r[i], a1i[i], x0r,
+                 x0i, x1r, x1i);
+      const int e = m * PRM_K + lo;
+      xs[e] = x0r;
+      xs[P * PRM_K + e] = x0i;
+      xs[2 * P * PRM_K + e] = x1r;
+      xs[3 * P * PRM_K + e] = x1i;
+    }
+  }
+  __syncthreads();
+  {
+    const int cb = 16 * w;
+    pr_d4 b0 = {0., 0., 0., 0.}, b1 = b0;
+    for (int m0 = 0; m0 < P; m0 += 4) {
+      const int t = ((cb + lo) * (m0 + hi)) % P;
+      const double cw = tc[t], sw = -ts[t];
+      const int e = (m0 + hi) * PRM_K + lo;
+      b0 = __builtin_amdgcn_mfma_f64_16x16x4f64(cw, xs[e], b0, 0, 0, 0);
+      b0 = __builtin_amdgcn_mfma_f64_16x16x4f64(sw, xs[P * PRM_K + e], b0, 0, 0, 0);
+      b1 = __builtin_amdgcn_mfma_f64_16x16x4f64(cw, xs[2 * P * PRM_K + e], b1, 0, 0, 0);
+      b1 = __builtin_amdgcn_mfma_f64_16x16x4f64(sw, xs[3 * P * PRM_K + e], b1, 0, 0, 0);
+    }
+    const double invP = 1. / P;
+    const int k = k0 + lo;
+    if (k < nx) {
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        // u_c = (beta_c, alpha_{c+1}): chunk c+1's left value, chunk c's right
+        const int c = cb + 4 * i + hi;
+        bnd[((long)((c + 1) % P) * 2 + 0) * nx + k] = b0[i] * invP;
+        bnd[((long)c * 2 + 1) * nx + k] = b1[i] * invP;
+      }
     }
   }
 }
@@ -805,9 +904,14 @@ int launch_poisson(Ctx *c, int rk) {
       case 16: hipLaunchKernelGGL(k_part_local<16>, g1, dim3(PART_LANES), 0, c->stream, nx, c->qhat, c->pt_fwd, c->pt_u, l, c->pt_ends); break;
       default: hipLaunchKernelGGL(k_part_local<8>, g1, dim3(PART_LANES), 0, c->stream, nx, c->qhat, c->pt_fwd, c->pt_u, l, c->pt_ends); break;
     }
-    hipLaunchKernelGGL(k_part_reduced, dim3((nx + PART_KG - 1) / PART_KG), dim3(64 * PART_KG),
-                       (size_t)(PART_KG * 6 + 2) * P * sizeof(double), c->stream, nx, P, c->pt_ends,
-                       c->pt_lvl, c->pt_red, c->pt_w, c->pt_bnd);
+    if (P % 16 == 0 && P <= 64)  // the chunk DFTs on the matrix cores
+      hipLaunchKernelGGL(k_part_reduced_mfma, dim3((nx + PRM_K - 1) / PRM_K), dim3(4 * P),
+                         (size_t)(2 + 6 * PRM_K) * P * sizeof(double), c->stream, nx, P,
+                         c->pt_ends, c->pt_lvl, c->pt_red, c->pt_w, c->pt_bnd);
+    else
+      hipLaunchKernelGGL(k_part_reduced, dim3((nx + PART_KG - 1) / PART_KG), dim3(64 * PART_KG),
+                         (size_t)(PART_KG * 6 + 2) * P * sizeof(double), c->stream, nx, P,
+                         c->pt_ends, c->pt_lvl, c->pt_red, c->pt_w, c->pt_bnd);
     EIF_HIP(hipGetLastError());
     ca.mode = 1;
     ca.L = Lc;
