@@ -69,11 +69,24 @@ def test_fluid_functions_bit_exact_thomas(nx, ny, ly):
     g.close()
 
 
-@pytest.mark.parametrize("nx,ny,ly", [(64, 96, 1.0), (256, 512, 2.0)])
-def test_fluid_partition_mode_within_tolerance(nx, ny, ly):
+@pytest.mark.parametrize("nx,ny,ly,tol,tol_p", [(64, 96, 1.0, 1e-12, 1e-10),
+                                                (256, 512, 2.0, 1e-12, 1e-10),
+                                                (64, 1024, 2.0, 1e-12, 1e-10),
+                                                (64, 3072, 3.0, 1e-11, 1e-9),
+                                                (128, 4096, 2.0, 3e-11, 1e-9)])
+def test_fluid_partition_mode_within_tolerance(nx, ny, ly, tol, tol_p):
     """The partitioned y-solve changes rounding only; psi agrees modulo its
     mean to the conditioning floor of the periodic system (~cond * eps),
-    velocities to 1e-12."""
+    velocities to tol, psi and p (less their means) to tol_p.  Chunk counts
+    P = 3, 8 (lane-loop reduced system)
+    and 16, 48, 64 (the FP64-MFMA reduced system).  These random fields
+    carry O(1) divergence, so each projection is large; with cond ~
+    (ny/pi)^2 (1e6-2e6 at ny = 3072-4096) the velocities then differ from
+    the reference's Thomas order by up to ~1e-11 (4.5e-12 and 1.2e-11
+    measured, the same with either reduced-system kernel) and p by up to
+    ~2e-10 (1.7e-10 measured at ny = 3072).  The bench's
+    physical flow at 2048x4096 stays far below: see
+    test_c2_full_size_partition_mode_steps."""
     g, o = pair(1, itot=nx, jtot=ny, ly=ly, Re=120.0, extfrcy=0.3)
     ux, uy, p = cases.random_fields(nx, ny, 3)
     for s in (g, o):
@@ -83,12 +96,12 @@ def test_fluid_partition_mode_within_tolerance(nx, ny, ly):
     for rk in range(3):
         both(g, o, "update_velocity", rk)
         both(g, o, "compute_potential", rk)
-        assert rel_mod_mean(g.field("psi"), o.field("psi")) < 1e-10
+        assert rel_mod_mean(g.field("psi"), o.field("psi")) < tol_p
         both(g, o, "correct_velocity", rk)
         both(g, o, "update_pressure")
         a, b = g.download(), o.download()
-        assert rel(a["ux"], b["ux"]) < 1e-12 and rel(a["uy"], b["uy"]) < 1e-12
-        assert rel_mod_mean(a["p"], b["p"]) < 1e-10
+        assert rel(a["ux"], b["ux"]) < tol and rel(a["uy"], b["uy"]) < tol
+        assert rel_mod_mean(a["p"], b["p"]) < tol_p
     dmax, _ = g.divergence()
     assert dmax < 1e-9
     g.close()
@@ -285,6 +298,44 @@ def test_partition_mode_whole_steps_with_ellipses():
         assert rel_mod_mean(a["p"], b["p"]) < 1e-10, k
     rg, ro = g.particles(), o.particles()
     assert np.max(np.abs(rg[:, 3:6] - ro[:, 3:6])) < 1e-12
+    g.close()
+
+
+def test_c2_full_size_partition_mode_steps():
+    """The bench's own workload at its real shape (2048x4096, ly=2, Re=1000,
+    1024 ellipses from the bench's seed, laminar profile) in the bench's
+    mode (partitioned y-solve, FP64-MFMA reduced system): two whole steps
+    against the oracle (Thomas, 16 host threads).  Velocities stay within
+    1e-12 (3e-14 and 1e-14 measured).  The pressure (less its mean) after
+    the first step from p = 0 is that step's psi alone, whose partitioned
+    rounding is ~cond * eps against the reference's Thomas order: 1.3e-12
+    measured, 4.6e-13 after the second step; bar 3e-12."""
+    from ellipsesinflows_amd import initparticles
+    nx, ny, ly, re = 2048, 4096, 2.0, 1000.0
+    parts = initparticles.random_ellipses(1024, ly, (0.008, 0.012), (0.4, 1.0), 0.0, 2,
+                                          den_range=(1.0, 1.5))
+    kw = dict(itot=nx, jtot=ny, ly=ly, Re=re, extfrcy=8.0 / re)
+    g = eif.Solver(poisson_mode=1, **kw)
+    o = Oracle(threads=16, **kw)
+    xc = g.grid()["xc"]
+    uy = np.zeros((ny + 2, nx + 2))
+    uy[1:-1, 1:-1] = 4.0 * xc[None, 1:-1] * (1 - xc[None, 1:-1])
+    keys = ("den", "a", "b", "x", "y", "az", "ux", "uy", "vz")
+    for s in (g, o):
+        s.upload(uy=uy)
+        s.update_boundaries()
+        s.set_particles(**{k: parts[k] for k in keys})
+    for k in range(2):
+        dg, do = g.step(), o.step()
+        assert dg == pytest.approx(do, rel=1e-12), k
+        a, b = g.download(), o.download()
+        eu = max(rel(a["ux"], b["ux"]), rel(a["uy"], b["uy"]))
+        ep = rel_mod_mean(a["p"], b["p"])
+        print("step", k, "velocity", eu, "pressure", ep)
+        assert eu < 1e-12, k
+        assert ep < 3e-12, k
+    rg, ro = g.particles(), o.particles()
+    assert np.max(np.abs(rg[:, 3:6] - ro[:, 3:6]) / np.maximum(np.abs(ro[:, 3:6]), 1e-300)) < 1e-9
     g.close()
 
 
