@@ -441,29 +441,83 @@ __global__ __launch_bounds__(64) void k_thomas(int nx, int ny, double *q,
   if (k >= nx) return;
   const int n1 = ny - 1;  // size of the bordered (n-1) system
   const double l = coef;  // l = u = 1/dy/dy everywhere
-  // forward sweep (tdm.c:40-53) on rows 0..n-2
+  // forward sweep (tdm.c:40-53) on rows 0..n-2.  The sweeps are one
+  // dependent chain per wavenumber: the rows' loads are issued a batch of
+  // TH_U rows ahead of the arithmetic (same operations, same order).
+  constexpr int TH_U = 16;
   double prev = q[k] / cdiag[k];
   q[k] = prev;
-  for (int i = 1; i <= n1 - 1; i++) {
-    const long o = (long)i * nx + k;
-    const double qi = q[o];
-    double v;
-    if (flag[o]) {
-      v = 0.;
-    } else {
-      v = rv[o] * (qi - l * prev);
+  {
+    const int iend = n1 - 1;  // last row of the forward sweep
+    double qa[TH_U], ra[TH_U];
+    bool fa[TH_U];
+    auto load = [&](int i0, double *qv, double *rw, bool *fl) {
+#pragma unroll
+      for (int u = 0; u < TH_U; u++) {
+        const int ii = i0 + u;
+        if (ii <= iend) {
+          const long o = (long)ii * nx + k;
+          qv[u] = q[o];
+          rw[u] = rv[o];
+          fl[u] = flag[o] != 0;
+        }
+      }
+    };
+    load(1, qa, ra, fa);
+    for (int i = 1; i <= iend; i += TH_U) {
+      double qb[TH_U], rb[TH_U];
+      bool fb[TH_U];
+      if (i + TH_U <= iend) load(i + TH_U, qb, rb, fb);  // later rows: not yet written
+#pragma unroll
+      for (int u = 0; u < TH_U; u++) {
+        const int ii = i + u;
+        if (ii > iend) break;
+        const double v = fa[u] ? 0. : ra[u] * (qa[u] - l * prev);
+        q[(long)ii * nx + k] = v;
+        prev = v;
+      }
+#pragma unroll
+      for (int u = 0; u < TH_U; u++) {
+        qa[u] = qb[u];
+        ra[u] = rb[u];
+        fa[u] = fb[u];
+      }
     }
-    q[o] = v;
-    prev = v;
   }
-  // backward sweep (:55-57)
+  // backward sweep (:55-57), loads a batch ahead in the same way
   const double q_last_fwd = prev;  // q0[n-2]
   double next = prev;
-  for (int i = n1 - 2; i >= 0; i--) {
-    const long o = (long)i * nx + k;
-    const double v = q[o] - um[o] * next;
-    q[o] = v;
-    next = v;
+  {
+    double qa[TH_U], ua[TH_U];
+    auto load = [&](int i0, double *qv, double *uw) {
+#pragma unroll
+      for (int u = 0; u < TH_U; u++) {
+        const int ii = i0 - u;
+        if (ii >= 0) {
+          const long o = (long)ii * nx + k;
+          qv[u] = q[o];
+          uw[u] = um[o];
+        }
+      }
+    };
+    load(n1 - 2, qa, ua);
+    for (int i = n1 - 2; i >= 0; i -= TH_U) {
+      double qb[TH_U], ub[TH_U];
+      if (i - TH_U >= 0) load(i - TH_U, qb, ub);  // earlier rows: not yet written
+#pragma unroll
+      for (int u = 0; u < TH_U; u++) {
+        const int ii = i - u;
+        if (ii < 0) break;
+        const double v = qa[u] - ua[u] * next;
+        q[(long)ii * nx + k] = v;
+        next = v;
+      }
+#pragma unroll
+      for (int u = 0; u < TH_U; u++) {
+        qa[u] = qb[u];
+        ua[u] = ub[u];
+      }
+    }
   }
   // bottom solution (tdm.c:99-102)
   const double q0_0 = next;
