@@ -662,6 +662,14 @@ __global__ __launch_bounds__(64) void k_inertia(IbmArgs a, int cnstep) {
 // records the position whose distances are now cached and the work
 // counters.  which: 0..1 inertia (cnstep), 2 exchange.
 // the particle's slice sums in slice order (every lane gets them)
+// lane l's x, l the same on every lane of the wave
+__device__ inline double lane_value(double x, int l) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)b, l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 __device__ inline void slice_sums(const IbmArgs &a, int s0, int s1, double tot[SLICE_VALS]) {
   const int lane = threadIdx.x & 63;
   for (int k = 0; k < SLICE_VALS; k++) tot[k] = 0.;
@@ -671,8 +679,10 @@ __device__ inline void slice_sums(const IbmArgs &a, int s0, int s1, double tot[S
       for (int k = 0; k < SLICE_VALS; k++)
         v[k] = a.slice_part[SLICE_VALS * (long)(base + lane) + k];
     const int cnt = min(64, s1 - base);
+    // the slice index is the same on every lane: lane l's values are read
+    // as scalars (no LDS round trip), added in the same order
     for (int l = 0; l < cnt; l++)
-      for (int k = 0; k < SLICE_VALS; k++) tot[k] += __shfl(v[k], l);
+      for (int k = 0; k < SLICE_VALS; k++) tot[k] += lane_value(v[k], l);
   }
 }
 
@@ -1454,33 +1464,45 @@ __global__ __launch_bounds__(256) void k_increment(double *prt, int np, const do
   __shared__ double red[4];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int n = blockIdx.x * 4 + w;
-  // the slice range is loaded before the gate is tested (all threads of
-  // the workgroup see the same gate)
+  // the slice range and lane 0's record inputs are loaded before the gate
+  // is tested (all threads of the workgroup see the same gate), so that
+  // their latencies overlap
   int s0 = 0, s1 = 0;
   if (n < np && fin) {
     s0 = ia.slice_first[n];
     s1 = ia.slice_first[n + 1];
+  }
+  double dt = 0., den = 0., dux_prev = 0., duy_prev = 0., dvz_prev = 0.;
+  double cfx0 = 0., cfy0 = 0., ctz0 = 0., cfx1 = 0., cfy1 = 0., ctz1 = 0.;
+  double fux = 0., fuy = 0., tvz = 0., iux0 = 0., iuy0 = 0., ivz0 = 0.;
+  double ux0 = 0., uy0 = 0., vz0 = 0., x0 = 0., y0 = 0., az0 = 0.;
+  double iux1 = 0., iuy1 = 0., ivz1 = 0.;
+  SliceHead hd{};
+  double *p = prt + (long)n * R_N;
+  if (n < np && lane == 0) {
+    dt = dscal[S_DT];
+    den = p[R_DEN];
+    dux_prev = p[R_DUX]; duy_prev = p[R_DUY]; dvz_prev = p[R_DVZ];
+    cfx0 = p[R_CFX]; cfy0 = p[R_CFY]; ctz0 = p[R_CTZ];
+    cfx1 = p[R_CFX + 1]; cfy1 = p[R_CFY + 1]; ctz1 = p[R_CTZ + 1];
+    fux = p[R_FUX]; fuy = p[R_FUY]; tvz = p[R_TVZ];
+    iux0 = p[R_IUX]; iuy0 = p[R_IUY]; ivz0 = p[R_IVZ];
+    ux0 = p[R_UX]; uy0 = p[R_UY]; vz0 = p[R_VZ];
+    x0 = p[R_X]; y0 = p[R_Y]; az0 = p[R_AZ];
+    if (fin) {
+      hd = ia.heads[n];
+    } else {
+      iux1 = p[R_IUX + 1];
+      iuy1 = p[R_IUY + 1];
+      ivz1 = p[R_IVZ + 1];
+    }
   }
   if (!open) return;
   double res = 0.;
   double tot[SLICE_VALS];
   if (n < np && fin) slice_sums(ia, s0, s1, tot);
   if (n < np && lane == 0) {
-    double *p = prt + (long)n * R_N;
-    // the record's inputs, loaded together
-    const double dt = dscal[S_DT];
-    const double den = p[R_DEN];
-    const double dux_prev = p[R_DUX], duy_prev = p[R_DUY], dvz_prev = p[R_DVZ];
-    const double cfx0 = p[R_CFX], cfy0 = p[R_CFY], ctz0 = p[R_CTZ];
-    double cfx1 = p[R_CFX + 1], cfy1 = p[R_CFY + 1], ctz1 = p[R_CTZ + 1];
-    const double fux = p[R_FUX], fuy = p[R_FUY], tvz = p[R_TVZ];
-    const double iux0 = p[R_IUX], iuy0 = p[R_IUY], ivz0 = p[R_IVZ];
-    const double ux0 = p[R_UX], uy0 = p[R_UY], vz0 = p[R_VZ];
-    const double x0 = p[R_X], y0 = p[R_Y], az0 = p[R_AZ];
-    double iux1, iuy1, ivz1;
-    SliceHead hd;
     if (fin) {  // inertia(1) of this sub-iteration: its sums and cache records
-      hd = ia.heads[n];
       iux1 = tot[0];
       iuy1 = tot[1];
       ivz1 = tot[2];
@@ -1488,10 +1510,6 @@ __global__ __launch_bounds__(256) void k_increment(double *prt, int np, const do
       p[R_IUY + 1] = iuy1;
       p[R_IVZ + 1] = ivz1;
       record_pass(ia, n, hd, tot);
-    } else {
-      iux1 = p[R_IUX + 1];
-      iuy1 = p[R_IUY + 1];
-      ivz1 = p[R_IVZ + 1];
     }
     if (ia.res && ia.sub == 0) {  // collision(1) at the stage-start positions
       cfx1 = cfx0;
