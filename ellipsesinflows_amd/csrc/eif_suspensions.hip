@@ -516,8 +516,10 @@ __device__ inline bool same_boxset(const BoxSet &k, const BoxSet &bs) {
 // picked up); otherwise only the position-dependent parts
 // prev: the header whose pass the cache records now describe (known to the
 // caller), else the records are read
+// rad: max(a, b) of the particle, when the caller has it (< 0: read here)
 __device__ inline void write_head(const IbmArgs &a, int n, double px, double py, double paz,
-                                  bool full, const SliceHead *prev = nullptr) {
+                                  bool full, const SliceHead *prev = nullptr,
+                                  double rad = -1.) {
   const double *q = a.prt + (long)n * R_N;
   SliceHead *h = a.heads + n;
   if (full) {
@@ -536,7 +538,7 @@ __device__ inline void write_head(const IbmArgs &a, int n, double px, double py,
   h->cr = cr;
   h->sr = sr;
   BoxSet bs;
-  make_boxes(a, px, py, fmax(q[R_A], q[R_B]), bs);
+  make_boxes(a, px, py, rad >= 0. ? rad : fmax(q[R_A], q[R_B]), bs);
   h->bs = bs;
   if (prev) {  // the cache records were just made from prev
     h->cached = prev->px == px && prev->py == py && prev->paz == paz;
@@ -1475,7 +1477,7 @@ __global__ __launch_bounds__(256) void k_increment(double *prt, int np, const do
   double dt = 0., den = 0., dux_prev = 0., duy_prev = 0., dvz_prev = 0.;
   double cfx0 = 0., cfy0 = 0., ctz0 = 0., cfx1 = 0., cfy1 = 0., ctz1 = 0.;
   double fux = 0., fuy = 0., tvz = 0., iux0 = 0., iuy0 = 0., ivz0 = 0.;
-  double ux0 = 0., uy0 = 0., vz0 = 0., x0 = 0., y0 = 0., az0 = 0.;
+  double ux0 = 0., uy0 = 0., vz0 = 0., x0 = 0., y0 = 0., az0 = 0., rad = 0.;
   double iux1 = 0., iuy1 = 0., ivz1 = 0.;
   SliceHead hd{};
   double *p = prt + (long)n * R_N;
@@ -1489,6 +1491,7 @@ __global__ __launch_bounds__(256) void k_increment(double *prt, int np, const do
     iux0 = p[R_IUX]; iuy0 = p[R_IUY]; ivz0 = p[R_IVZ];
     ux0 = p[R_UX]; uy0 = p[R_UY]; vz0 = p[R_VZ];
     x0 = p[R_X]; y0 = p[R_Y]; az0 = p[R_AZ];
+    rad = fmax(p[R_A], p[R_B]);
     if (fin) {
       hd = ia.heads[n];
     } else {
@@ -1537,7 +1540,7 @@ __global__ __launch_bounds__(256) void k_increment(double *prt, int np, const do
     p[R_DY] = dyn;
     p[R_DAZ] = dazn;
     // the next sub-iteration's inertia pass
-    write_head(ia, n, x0 + dxn, y0 + dyn, az0 + dazn, false, fin ? &hd : nullptr);
+    write_head(ia, n, x0 + dxn, y0 + dyn, az0 + dazn, false, fin ? &hd : nullptr, rad);
   }
   if (lane == 0) red[w] = res;
   __syncthreads();
