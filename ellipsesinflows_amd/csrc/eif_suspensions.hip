@@ -393,10 +393,35 @@ struct IbmArgs {
   SliceHead *heads;  // per particle: the pass's position, shape, boxes, cache state      // per-slice partial sums (3 each)
 };
 
-// fixed-order tree over the 64 lanes of a wave (lane 0 receives the sum)
+// x of lane i + n of the same row of 16 lanes (DPP row_shl:n, CTRL = 0x100 + n);
+// lanes whose source leaves the row get 0
+template <int CTRL>
+__device__ inline double row_shl_f64(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// fixed-order tree over the 64 lanes of a wave (lane 0 receives the sum):
+// v[i] += v[i + off] for off = 32, 16, 8, 4, 2, 1; the last four levels stay
+// inside lanes 0-15 and move through DPP instead of LDS permutes
 __device__ inline double wave_sum(double v) {
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  v += __shfl_down(v, 32, 64);
+  v += __shfl_down(v, 16, 64);
+  v += row_shl_f64<0x108>(v);
+  v += row_shl_f64<0x104>(v);
+  v += row_shl_f64<0x102>(v);
+  v += row_shl_f64<0x101>(v);
   return v;
+}
+
+// lane l's x, l the same on every lane of the wave
+__device__ inline double lane_value(double x, int l) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)b, l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
 // A slice's partial sums (lane 0); k_ibm_finalize adds them per particle in
@@ -664,14 +689,6 @@ __global__ __launch_bounds__(64) void k_inertia(IbmArgs a, int cnstep) {
 // records the position whose distances are now cached and the work
 // counters.  which: 0..1 inertia (cnstep), 2 exchange.
 // the particle's slice sums in slice order (every lane gets them)
-// lane l's x, l the same on every lane of the wave
-__device__ inline double lane_value(double x, int l) {
-  const long long b = __double_as_longlong(x);
-  const int lo = __builtin_amdgcn_readlane((int)b, l);
-  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-}
-
 __device__ inline void slice_sums(const IbmArgs &a, int s0, int s1, double tot[SLICE_VALS]) {
   const int lane = threadIdx.x & 63;
   for (int k = 0; k < SLICE_VALS; k++) tot[k] = 0.;
@@ -1393,8 +1410,8 @@ __device__ void collide_particle(const CollArgs &a, int n) {
     while (hb) {
       const int l = __ffsll((long long)hb) - 1;
       hb &= hb - 1ull;
-      const double u0 = __shfl(w0, l), u1 = __shfl(w1, l), u2 = __shfl(w2, l);
-      const int mm = __shfl(m, l);
+      const double u0 = lane_value(w0, l), u1 = lane_value(w1, l), u2 = lane_value(w2, l);
+      const int mm = __builtin_amdgcn_readlane(m, l);
       if (mm >= 0) {  // a partner
         if (ncont >= CONTACT_CAP) {
           if (lane == 0) atomicOr(&a.flags[0], 16);
