@@ -99,17 +99,49 @@ __device__ inline int sw(int p) { return p ^ ((p >> 4) & 15) ^ ((p >> 8) & 15); 
 // plus a per-thread base.  W holds the per-stage twiddles, stage len in
 // entries [len/2-1, len-1): W[2e] = C[4m], W[2e+1] = C[|n-4m|] with
 // m = k n/len (the table values lds_fft reads from C).
+// the twiddles of phase p's butterflies (entries of butterflies that do not
+// exist are left unset and never read)
+template <int LOGN, int LOGR>
+__device__ inline void phase_twiddles(int p, int t, const double *__restrict__ W,
+                                      double (&twr)[LOGR][1 << LOGR],
+                                      double (&twi)[LOGR][1 << LOGR]) {
+  constexpr int R = 1 << LOGR;
+  const RegMap mp = reg_phase_r<LOGR>(p, LOGN);
+  const int r = mp.lg;
+  const int gmask = (1 << r) - 1;
+#pragma unroll
+  for (int ss = 0; ss < LOGR; ss++) {
+    if (ss >= r) break;
+    const int half = 1 << (mp.lh0 + ss), len = half << 1;
+    const double *Ws = W + 2 * (half - 1);
+#pragma unroll
+    for (int s = 0; s < R; s++) {
+      const int e = s & gmask;
+      if (e & (1 << ss)) continue;
+      if ((e | (1 << ss)) > gmask) continue;
+      const int kk = mp.pos(t, s) & (len - 1);
+      twr[ss][s] = Ws[2 * kk];
+      twi[ss][s] = Ws[2 * kk + 1];
+    }
+  }
+}
+
 template <int LOGN, int LOGR, int SIGN>
 __device__ inline void reg_fft_t(double (&xr)[1 << LOGR], double (&xi)[1 << LOGR], double *lre,
                                  double *lim, const double *__restrict__ W) {
   constexpr int R = 1 << LOGR;
   const int t = threadIdx.x;
   constexpr int NP = (LOGN + LOGR - 1) / LOGR;
+  // each phase's twiddles are loaded before the LDS exchange that precedes
+  // the phase, so their latency overlaps the exchange
+  double twr[LOGR][R], twi[LOGR][R];
+  phase_twiddles<LOGN, LOGR>(0, t, W, twr, twi);
 #pragma unroll
   for (int p = 0; p < NP; p++) {
     const RegMap mp = reg_phase_r<LOGR>(p, LOGN);
     if (p > 0) {
       const RegMap mq = reg_phase_r<LOGR>(p - 1, LOGN);
+      phase_twiddles<LOGN, LOGR>(p, t, W, twr, twi);
       __syncthreads();
 #pragma unroll
       for (int s = 0; s < R; s++) {
@@ -130,17 +162,14 @@ __device__ inline void reg_fft_t(double (&xr)[1 << LOGR], double (&xi)[1 << LOGR
 #pragma unroll
     for (int ss = 0; ss < LOGR; ss++) {
       if (ss >= r) break;
-      const int half = 1 << (mp.lh0 + ss), len = half << 1;
-      const double *Ws = W + 2 * (half - 1);
 #pragma unroll
       for (int s = 0; s < R; s++) {
         const int e = s & gmask;
         if (e & (1 << ss)) continue;
         if ((e | (1 << ss)) > gmask) continue;
         const int s1 = s + (1 << ss);
-        const int kk = mp.pos(t, s) & (len - 1);
-        const double wr = Ws[2 * kk];
-        const double wi = SIGN * Ws[2 * kk + 1];
+        const double wr = twr[ss][s];
+        const double wi = SIGN * twi[ss][s];
         const double r1 = xr[s1], m1 = xi[s1];
         const double tr = wr * r1 - wi * m1;
         const double ti = wr * m1 + wi * r1;
