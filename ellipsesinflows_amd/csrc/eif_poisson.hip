@@ -328,14 +328,18 @@ __global__ __launch_bounds__((1 << LOGN) >> dct_logr<LOGN>()) void k_rhs_dct2_t(
   const double dt = dscal[S_DT];
   const double fac = 1. / (ga * dt);
   // cells in memory order (coalesced loads), each stored at its Makhoul
-  // position m: i = 2m for m < nx/2, i = 2(nx-1-m)+1 above
+  // position m: i = 2m for m < nx/2, i = 2(nx-1-m)+1 above.  Without a
+  // second row, row ja is evaluated again and discarded, so that no load
+  // sits behind a branch and all of them issue together.
+  const int jbl = hasb ? jb : ja;
 #pragma unroll
   for (int s = 0; s < R; s++) {
     const int i = threadIdx.x + s * T;
     const int m = (i & 1) ? nx - 1 - (i >> 1) : i >> 1;
     const int q = sw(bitrev(m, LOGN));
+    const double vb = rhs_value<P2>(L, ux, uy, fac, dx, dy, i, jbl);
     re[q] = rhs_value<P2>(L, ux, uy, fac, dx, dy, i, ja);
-    im[q] = hasb ? rhs_value<P2>(L, ux, uy, fac, dx, dy, i, jb) : 0.;
+    im[q] = hasb ? vb : 0.;
   }
   __syncthreads();
   double xr[R], xi[R];
