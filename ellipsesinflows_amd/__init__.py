@@ -40,6 +40,7 @@ EXPORTED = (
     "fluid_gpu_update_boundaries_ux", "fluid_gpu_update_boundaries_uy",
     "fluid_gpu_update_boundaries_p", "fluid_gpu_decide_dt", "fluid_gpu_set_dt",
     "fluid_gpu_get_dt", "fluid_gpu_update_velocity", "poisson_gpu_compute_potential",
+    "poisson_gpu_partition_layout",
     "fluid_gpu_correct_velocity", "fluid_gpu_update_pressure", "fluid_gpu_divergence",
     "fluid_gpu_momentum", "fluid_gpu_energy", "suspensions_gpu_create",
     "suspensions_gpu_n_particles", "suspensions_gpu_download", "suspensions_gpu_upload",
@@ -58,6 +59,15 @@ EXPORTED = (
 
 class EIFError(RuntimeError):
     pass
+
+
+def partition_layout(jtot):
+    """(chunk rows, chunks) of the partitioned y-solve for jtot, or (0, 0)
+    where the Thomas sweep is used instead (host only, no device)."""
+    lib = load_library()
+    L, P = ctypes.c_int(), ctypes.c_int()
+    lib.poisson_gpu_partition_layout(int(jtot), ctypes.byref(L), ctypes.byref(P))
+    return L.value, P.value
 
 
 class _Config(ctypes.Structure):

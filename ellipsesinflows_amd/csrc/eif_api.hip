@@ -158,6 +158,10 @@ static void fields_changed(Ctx *c) {
 extern "C" {
 
 const char *eif_gpu_last_error(void) { return g_err.c_str(); }
+
+int poisson_gpu_partition_layout(int jtot, int *chunk, int *nchunks) {
+  return partition_layout(jtot, chunk, nchunks);
+}
 const char *eif_gpu_profile_names(void) { return k_prof_names; }
 
 static void free_ctx(Ctx *c) {

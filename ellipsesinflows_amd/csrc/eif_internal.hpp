@@ -219,6 +219,9 @@ int launch_correct_pressure(Ctx *c, int rk, bool correct, bool pressure,
 int launch_decide_dt(Ctx *c);
 int launch_diagnostics(Ctx *c, int which, double *out);
 int poisson_init(Ctx *c);
+// chunk length and count of the partitioned y-solve for jtot = ny (0 when
+// the line is solved by the Thomas sweep instead); host only
+int partition_layout(int ny, int *chunk, int *nchunks);
 int launch_poisson(Ctx *c, int rk);
 int susp_init(Ctx *c);
 // defer_finalize: the slice sums are added by the next launch_increment

@@ -498,8 +498,8 @@ __global__ __launch_bounds__(64) void k_thomas(int nx, int ny, double *q,
     };
     load(1, qa, ra, fa);
     for (int i = 1; i <= iend; i += TH_U) {
-      double qb[TH_U], rb[TH_U];
-      bool fb[TH_U];
+      double qb[TH_U] = {}, rb[TH_U] = {};
+      bool fb[TH_U] = {};
       if (i + TH_U <= iend) load(i + TH_U, qb, rb, fb);  // later rows: not yet written
 #pragma unroll
       for (int u = 0; u < TH_U; u++) {
@@ -535,7 +535,7 @@ __global__ __launch_bounds__(64) void k_thomas(int nx, int ny, double *q,
     };
     load(n1 - 2, qa, ua);
     for (int i = n1 - 2; i >= 0; i -= TH_U) {
-      double qb[TH_U], ub[TH_U];
+      double qb[TH_U] = {}, ub[TH_U] = {};
       if (i - TH_U >= 0) load(i - TH_U, qb, ub);  // earlier rows: not yet written
 #pragma unroll
       for (int u = 0; u < TH_U; u++) {
@@ -793,10 +793,22 @@ __global__ __launch_bounds__(256) void k_part_reduced_mfma(
   }
 }
 
+// The lane-loop reduced system (k_part_reduced) keeps (6 PART_KG + 2) P
+// doubles in LDS; P is capped so that fits the 64 KiB a workgroup gets
+// without an opt-in.  Longer lines use the Thomas solve.
+constexpr int PART_P_MAX = 65536 / ((6 * PART_KG + 2) * (int)sizeof(double));
+
 static int part_chunk(int ny) {
   for (int L = 64; L >= 8; L >>= 1)
-    if (ny % L == 0 && ny / L >= 2 && ny / L <= 1024) return L;
+    if (ny % L == 0 && ny / L >= 2 && ny / L <= PART_P_MAX) return L;
   return 0;
+}
+
+int partition_layout(int ny, int *chunk, int *nchunks) {
+  const int L = ny >= 3 ? part_chunk(ny) : 0;
+  if (chunk) *chunk = L;
+  if (nchunks) *nchunks = L ? ny / L : 0;
+  return L ? 1 : 0;
 }
 
 static int part_init(Ctx *c) {

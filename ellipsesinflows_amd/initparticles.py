@@ -66,6 +66,30 @@ def random_ellipses(n, ly, a_range, ba_range, gap, seed, den_range=(1.0, 1.0), l
                 vz=zeros.copy())
 
 
+ENSEMBLE_GRIDS = ((256, 512), (512, 1024), (1024, 2048))
+ENSEMBLE_COUNTS = (8, 32, 128, 256)
+
+
+def ensemble_cases(n_cases=32, seed=0, ly=2.0):
+    """Seeded draws of an ensemble of independent cases: per case a grid
+    from ENSEMBLE_GRIDS (lx = 1 walls, ly periodic), Re ~ log-U[100, 1000]
+    with the Poiseuille forcing extfrcy = 8/Re, a particle count from
+    ENSEMBLE_COUNTS, a ~ U[.01, .04], b/a ~ U[.4, 1], orientation
+    ~ U[0, pi), density 1, rejection-placed centres.  Returns a list of
+    (config, particles) pairs; config holds the eif_config fields."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n_cases):
+        nx, ny = ENSEMBLE_GRIDS[rng.integers(len(ENSEMBLE_GRIDS))]
+        re = float(np.exp(rng.uniform(np.log(100.0), np.log(1000.0))))
+        npart = int(ENSEMBLE_COUNTS[rng.integers(len(ENSEMBLE_COUNTS))])
+        parts = random_ellipses(npart, ly, (0.01, 0.04), (0.4, 1.0), 0.0,
+                                int(rng.integers(1 << 31)))
+        cfg = dict(itot=int(nx), jtot=int(ny), ly=float(ly), Re=re, extfrcy=8.0 / re)
+        out.append((cfg, parts))
+    return out
+
+
 def area_fraction(parts, ly, lx=1.0):
     return float(np.sum(np.pi * np.asarray(parts["a"]) * np.asarray(parts["b"])) / (lx * ly))
 

@@ -106,6 +106,13 @@ int fluid_gpu_get_dt(eif_ctx *ctx, double *dt_out);
 int fluid_gpu_update_velocity(eif_ctx *ctx, int rkstep);
 /* fluid_compute_potential, src/fluid/compute_potential.c:48-138 */
 int poisson_gpu_compute_potential(eif_ctx *ctx, int rkstep);
+/* Layout of the partitioned y-solve (EIF_POISSON_PARTITION) for jtot
+ * (no counterpart in the reference, whose y-solve is the sequential
+ * src/tdm.c:64-111 after a transpose): chunk rows per chunk and nchunks
+ * chunks; returns 1, or 0 (both set to 0) when the partitioned solve does
+ * not apply and the context falls back to the Thomas sweep.  Host only:
+ * needs no device. */
+int poisson_gpu_partition_layout(int jtot, int *chunk, int *nchunks);
 /* fluid_correct_velocity, src/fluid/correct_velocity.c:6-42 */
 int fluid_gpu_correct_velocity(eif_ctx *ctx, int rkstep);
 /* fluid_update_pressure, src/fluid/update_pressure.c:7-24 */

@@ -24,12 +24,24 @@
  *                                 steady states, polynomial exactness,
  *                                 discrete energy conservation of advection)
  *   inertia / exchange sums    -> reference weights (oracle/_ref) summed in
- *                                 numpy on small boxes; rigid-body invariants
- *   increment / update         -> closed forms (neutrally buoyant particle
- *                                 at rest, periodic wrap)
- * The whole-stage coupling order (src/main.c:13-55) has no independent
- * check besides these pieces and invariants (mass/divergence, momentum
- * budget); FFTW and the reference's fluid sources are not buildable here.
+ *                                 numpy on small boxes: the force and torque
+ *                                 of a translating and rotating body in
+ *                                 fluid at rest, spin-up by the enclosed
+ *                                 fluid, momentum conservation of exchange
+ *                                 + increment
+ *   increment / update         -> free fall under buoyancy plus a constant
+ *                                 contact force integrated exactly over a
+ *                                 step; periodic wrap; face-average split
+ *                                 and conservation of the field update
+ *   update_pressure            -> p_new = p_old + psi; gauge invariance of
+ *                                 a whole step
+ *   integrate (src/main.c)     -> restated call for call in the tests and
+ *                                 compared bit for bit (contacts, residuals
+ *                                 crossing 1e-8, the 102-sub-iteration cap);
+ *                                 the reference's validation cases (Feng
+ *                                 1994 migration, Aidun 1998 rotation)
+ *                                 against the literature data it ships
+ * FFTW and the reference's fluid sources are not buildable here.
  */
 #include <stdio.h>
 #include <stdlib.h>

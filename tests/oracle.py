@@ -240,6 +240,14 @@ class Oracle:
         self.lib.orc_step(self.h)
         return self.lib.orc_get_dt(self.h)
 
+    def last_substeps(self):
+        sub = (ctypes.c_int * 3)()
+        self.lib.orc_last_substeps(self.h, sub)
+        return list(sub)
+
+    def time(self):
+        return self.lib.orc_get_time(self.h)
+
     def statistics_collect(self):
         if not hasattr(self, "_stats"):
             self._stats = {k: np.zeros(self.shapes["ux" if k[:2] == "ux" else "uy"])

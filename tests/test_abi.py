@@ -45,3 +45,18 @@ def test_create_without_device_reports_nodevice():
         eif.Solver(itot=16, jtot=16)
     lib = eif.load_library()
     assert b"HIP device" in lib.eif_gpu_last_error()
+
+
+def test_partition_layout_fits_the_reduced_system_in_lds():
+    """The partitioned y-solve's reduced system keeps 26 P doubles per
+    workgroup in LDS (k_part_reduced): the chunk count stays within 64 KiB,
+    and lines it cannot split that way report (0, 0) -- the context then
+    uses the Thomas sweep -- instead of failing at launch."""
+    assert eif.partition_layout(4096) == (64, 64)
+    assert eif.partition_layout(1024) == (64, 16)
+    assert eif.partition_layout(12) == (0, 0)
+    for jtot in (8 * 789, 8 * 1024, 64 * 315, 64 * 316, 6000, 4097, 3, 16):
+        L, P = eif.partition_layout(jtot)
+        if L:
+            assert L * P == jtot and 2 <= P and 26 * P * 8 <= 65536
+    assert eif.partition_layout(8 * 789) == (0, 0)
