@@ -49,6 +49,8 @@ EXPORTED = (
     "suspensions_gpu_compute_collision_force", "suspensions_gpu_increment_particles",
     "suspensions_gpu_update_particles", "suspensions_gpu_colliding_pairs",
     "eif_gpu_integrate", "eif_gpu_step", "eif_gpu_synchronize", "eif_gpu_last_error",
+    "eif_gpu_run_steps", "eif_gpu_graph_enable", "eif_gpu_graph_stats",
+    "eif_gpu_graph_set_subiterations",
     "eif_gpu_ibm_stats", "eif_gpu_ibm_reference_iterations", "statistics_gpu_collect", "statistics_gpu_reset",
     "statistics_gpu_download",
     "eif_gpu_profile_enable", "eif_gpu_profile_select", "eif_gpu_profile_reset",
@@ -309,6 +311,28 @@ class Solver:
         dt = _D()
         self._check(self.lib.eif_gpu_step(self.h, ctypes.byref(dt)))
         return dt.value
+
+    def run_steps(self, nsteps):
+        """nsteps time steps; returns (dts, substeps[nsteps, 3])."""
+        dts = np.zeros(max(int(nsteps), 1))
+        sub = np.zeros((max(int(nsteps), 1), 3), dtype=np.int32)
+        self._check(self.lib.eif_gpu_run_steps(self.h, int(nsteps), _ptr(dts),
+                                               sub.ctypes.data_as(_PI)))
+        return dts[:nsteps], sub[:nsteps]
+
+    def graph(self, enable=True):
+        self._check(self.lib.eif_gpu_graph_enable(self.h, int(bool(enable))))
+
+    def graph_set_subiterations(self, k):
+        arr = (ctypes.c_int * 3)(*[int(v) for v in k])
+        self._check(self.lib.eif_gpu_graph_set_subiterations(self.h, arr))
+
+    def graph_stats(self):
+        """(graph launches, resumed stops, captures, K per stage)."""
+        out = (ctypes.c_long * 4)()
+        self._check(self.lib.eif_gpu_graph_stats(self.h, out))
+        k = int(out[3])
+        return int(out[0]), int(out[1]), int(out[2]), (k // 10000, k // 100 % 100, k % 100)
 
     def ibm_stats(self, reset=False):
         """(cells visited, cells evaluated, FP64 iterations, IBM calls, FP32 iterations)."""

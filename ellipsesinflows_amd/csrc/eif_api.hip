@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstring>
 #include <algorithm>
+#include <memory>
 
 namespace eif {
 
@@ -183,6 +184,11 @@ static void free_ctx(Ctx *c) {
     if (o) (void)hipFree(o);
   if (c->hscal) (void)hipHostFree(c->hscal);
   if (c->hstat) (void)hipHostFree(c->hstat);
+  if (c->gstat) (void)hipHostFree(c->gstat);
+  graphs_free(c);
+  void *gbufs[] = {c->halt, c->g_counts, c->g_resid, c->g_seq};
+  for (auto b : gbufs)
+    if (b) (void)hipFree(b);
   if (c->prof) {
     for (auto &kv : c->prof->recs)
       for (auto &e : kv.second.ev) {
@@ -330,6 +336,17 @@ int fluid_gpu_create(const eif_config *cfg, eif_ctx **out) {
     if (e == hipSuccess)
       e = hipHostMalloc((void **)&c->hstat, sizeof(HostStatus), hipHostMallocCoherent);
     if (e == hipSuccess) memset(c->hstat, 0, sizeof(HostStatus));
+    if (e == hipSuccess) e = hipMalloc(&c->halt, sizeof(int));
+    if (e == hipSuccess) e = hipMemset(c->halt, 0, sizeof(int));
+    if (e == hipSuccess) e = hipMalloc(&c->g_counts, 4 * sizeof(int));
+    if (e == hipSuccess) e = hipMemset(c->g_counts, 0, 4 * sizeof(int));
+    if (e == hipSuccess) e = hipMalloc(&c->g_resid, 4 * sizeof(double));
+    if (e == hipSuccess) e = hipMemset(c->g_resid, 0, 4 * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&c->g_seq, sizeof(unsigned));
+    if (e == hipSuccess) e = hipMemset(c->g_seq, 0, sizeof(unsigned));
+    if (e == hipSuccess)
+      e = hipHostMalloc((void **)&c->gstat, 2 * sizeof(GraphStatus), hipHostMallocCoherent);
+    if (e == hipSuccess) memset(c->gstat, 0, 2 * sizeof(GraphStatus));
     if (e == hipSuccess) e = hipMalloc(&c->ibm_stats, 8 * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMemset(c->ibm_stats, 0, 8 * sizeof(unsigned long long));
     if (e != hipSuccess) return fail(hip_fail(e, "hipMalloc", __FILE__, __LINE__));
@@ -535,6 +552,7 @@ int fluid_gpu_energy(eif_ctx *c, double q[2]) {
   return launch_diagnostics(c, 2, q);
 }
 
+
 int statistics_gpu_collect(eif_ctx *c) {
   CHECK_CTX(c);
   return launch_statistics(c);
@@ -700,6 +718,10 @@ int suspensions_gpu_colliding_pairs(eif_ctx *c, int capacity, int *pairs, int *n
   return EIF_OK;
 }
 
+}  // extern "C"
+
+namespace eif {
+
 // integrate(), src/main.c:13-55
 // Sub-iterations first .. first+n-1 of the stage's Crank-Nicolson loop
 // (src/main.c:37-50), queued speculatively: each kernel checks the
@@ -723,20 +745,23 @@ static int queue_substeps(Ctx *c, int rk, int first, int n) {
 // it would idle the stream).  Only a long wait queries the stream (a query
 // can itself cost the stream a pause), so a failed kernel still ends the
 // wait with its error.
-static int wait_status(Ctx *c, HostStatus *out) {
-  const unsigned want = c->hstat_seq;
+static int wait_seq(Ctx *c, const unsigned *seq, unsigned want, const char *what) {
   for (long spin = 1;; spin++) {
-    if (__atomic_load_n(&c->hstat->seq, __ATOMIC_ACQUIRE) == want) break;
+    if (__atomic_load_n(seq, __ATOMIC_ACQUIRE) == want) return EIF_OK;
     if (spin >= (1L << 20) && spin % (1L << 14) == 0) {
       const hipError_t e = hipStreamQuery(c->stream);
       if (e == hipSuccess) {
-        if (__atomic_load_n(&c->hstat->seq, __ATOMIC_ACQUIRE) == want) break;
-        set_error("sub-iteration status record missing after the stream drained");
+        if (__atomic_load_n(seq, __ATOMIC_ACQUIRE) == want) return EIF_OK;
+        set_error(std::string(what) + " record missing after the stream drained");
         return EIF_ERR_HIP;
       }
       if (e != hipErrorNotReady) return hip_fail(e, "hipStreamQuery", __FILE__, __LINE__);
     }
   }
+}
+
+static int wait_status(Ctx *c, HostStatus *out) {
+  TRY(wait_seq(c, &c->hstat->seq, c->hstat_seq, "sub-iteration status"));
   out->residual = c->hstat->residual;
   out->dt = c->hstat->dt;
   out->substeps = c->hstat->substeps;
@@ -751,8 +776,11 @@ static void swap_fields(Ctx *c) {
   std::swap(c->uy, c->uy_alt);
 }
 
-int eif_gpu_integrate(eif_ctx *c, int substeps[3]) {
-  CHECK_CTX(c);
+// The stage start up to the Crank-Nicolson loop (src/main.c:15-36).
+// pred_queued: the stage's predictor was queued behind the previous
+// stage's sub-iterations; side_start: ... and the previous loop ended at
+// ev_fork, so the particle update, reset and distances may run beside it.
+static int stage_start(Ctx *c, int rk, bool pred_queued, bool side_start) {
   // Two streams.  Main: fluid kernels, exchange (with inertia(0) in the
   // same pass), deposit (with the exchange's particle sums in the same
   // launch) and the sub-iterations (inertia(1) with collision(1) in one
@@ -762,76 +790,85 @@ int eif_gpu_integrate(eif_ctx *c, int substeps[3]) {
   // collision(0), which overlaps the fluid kernels.  Events join the side
   // work back before anything that reads its results or reuses its
   // buffers.
-  //
-  // The predictor of stage rk+1 reads only the fluid fields, which the
-  // sub-iterations of stage rk leave alone.  It is queued right behind the
-  // first batch of sub-iterations, so the GPU runs it while the host waits
-  // for that batch's count.  (Beside the sub-iterations, on the side
-  // stream, it slows them more than it gains.)
+  if (!pred_queued) {
+    TRY(launch_reset(c, c->stream, true));
+    // src/main.c:16-19: the kernels of every stage keep the boundary and
+    // halo values they change up to date, so this only runs after a
+    // host-side change of the fields
+    if (!c->bc_valid) {
+      TRY(launch_bc_ux(c, c->ux));
+      TRY(launch_bc_uy(c, c->uy));
+      TRY(launch_bc_p(c, c->p));
+      c->bc_valid = true;
+    }
+    // the distances at the stage-start positions and collision(0) beside
+    // the predictor (queued first: at a step's start the device waits
+    // for these submissions)
+    EIF_HIP(hipEventRecord(c->ev_fork, c->stream));
+    TRY(launch_predictor_prev(c, rk));  // writes the spare ux, uy buffers
+    EIF_HIP(hipStreamWaitEvent(c->side, c->ev_fork, 0));
+    TRY(launch_ibm_dist(c, c->side));
+    EIF_HIP(hipEventRecord(c->ev_i0, c->side));
+    TRY(launch_collision(c, 0, c->side));
+    EIF_HIP(hipEventRecord(c->ev_c0, c->side));
+    EIF_HIP(hipStreamWaitEvent(c->stream, c->ev_i0, 0));
+  } else if (side_start) {
+    // the previous stage's loop ended with its first batch, which ev_fork
+    // closes: the particle update, this stage's reset and the distances
+    // at the new positions run on the side stream beside the predictor
+    EIF_HIP(hipStreamWaitEvent(c->side, c->ev_fork, 0));
+    TRY(launch_update_particles(c, c->side));
+    TRY(launch_reset(c, c->side, true));
+    TRY(launch_ibm_dist(c, c->side));
+    EIF_HIP(hipEventRecord(c->ev_i0, c->side));
+    TRY(launch_collision(c, 0, c->side));
+    EIF_HIP(hipEventRecord(c->ev_c0, c->side));
+    EIF_HIP(hipStreamWaitEvent(c->stream, c->ev_i0, 0));
+  } else {
+    // behind the predictor (and the batches queued after it): the previous
+    // stage's particle update, this stage's reset and the distances at the
+    // new positions; collision(0) on the side stream
+    TRY(launch_update_particles(c));
+    TRY(launch_reset(c, c->stream, true));
+    TRY(launch_ibm_dist(c));
+    EIF_HIP(hipEventRecord(c->ev_fork, c->stream));
+    EIF_HIP(hipStreamWaitEvent(c->side, c->ev_fork, 0));
+    TRY(launch_collision(c, 0, c->side));
+    EIF_HIP(hipEventRecord(c->ev_c0, c->side));
+  }
+  // the exchange with the predicted velocity and, in the same pass,
+  // inertia(0) with the stage-start velocity: the pair the predictor read.
+  // The reset cleared the tile registrations.
+  TRY(launch_exchange_inertia0(c, c->ux_alt, c->uy_alt));
+  // the deposit, with the exchange's and inertia(0)'s particle sums in the
+  // same launch
+  TRY(launch_update_momentum_x0(c));
+  TRY(launch_poisson(c, rk));
+  // the last stage also reduces the advective dt limit of the final
+  // velocities for the next param_decide_dt
+  TRY(launch_correct_pressure(c, rk, true, true, rk == 2));
+  // the sub-iterations read collision(0)
+  EIF_HIP(hipStreamWaitEvent(c->stream, c->ev_c0, 0));
+  return EIF_OK;
+}
+
+// integrate() from stage rk0; sub0 > 0 resumes that stage's loop at
+// sub-iteration sub0 (its start and first sub0 sub-iterations done, the
+// next stage's predictor not queued).
+//
+// The predictor of stage rk+1 reads only the fluid fields, which the
+// sub-iterations of stage rk leave alone.  It is queued right behind the
+// first batch of sub-iterations, so the GPU runs it while the host waits
+// for that batch's count.  (Beside the sub-iterations, on the side stream,
+// it slows them more than it gains.)
+static int integrate_from(Ctx *c, int rk0, int sub0, int substeps[3]) {
   bool pred_queued = false;  // this stage's predictor is already queued
   bool side_start = false;   // ... and the previous loop ended at ev_fork
-  for (int rk = 0; rk < 3; rk++) {
-    if (!pred_queued) {
-      TRY(launch_reset(c, c->stream, true));
-      // src/main.c:16-19: the kernels of every stage keep the boundary and
-      // halo values they change up to date, so this only runs after a
-      // host-side change of the fields
-      if (!c->bc_valid) {
-        TRY(launch_bc_ux(c, c->ux));
-        TRY(launch_bc_uy(c, c->uy));
-        TRY(launch_bc_p(c, c->p));
-        c->bc_valid = true;
-      }
-      // the distances at the stage-start positions and collision(0) beside
-      // the predictor (queued first: at a step's start the device waits
-      // for these submissions)
-      EIF_HIP(hipEventRecord(c->ev_fork, c->stream));
-      TRY(launch_predictor_prev(c, rk));  // writes the spare ux, uy buffers
-      EIF_HIP(hipStreamWaitEvent(c->side, c->ev_fork, 0));
-      TRY(launch_ibm_dist(c, c->side));
-      EIF_HIP(hipEventRecord(c->ev_i0, c->side));
-      TRY(launch_collision(c, 0, c->side));
-      EIF_HIP(hipEventRecord(c->ev_c0, c->side));
-      EIF_HIP(hipStreamWaitEvent(c->stream, c->ev_i0, 0));
-    } else if (side_start) {
-      // the previous stage's loop ended with its first batch, which ev_fork
-      // closes: the particle update, this stage's reset and the distances
-      // at the new positions run on the side stream beside the predictor
-      EIF_HIP(hipStreamWaitEvent(c->side, c->ev_fork, 0));
-      TRY(launch_update_particles(c, c->side));
-      TRY(launch_reset(c, c->side, true));
-      TRY(launch_ibm_dist(c, c->side));
-      EIF_HIP(hipEventRecord(c->ev_i0, c->side));
-      TRY(launch_collision(c, 0, c->side));
-      EIF_HIP(hipEventRecord(c->ev_c0, c->side));
-      EIF_HIP(hipStreamWaitEvent(c->stream, c->ev_i0, 0));
-    } else {
-      // behind the predictor (and the batches queued after it): the previous
-      // stage's particle update, this stage's reset and the distances at the
-      // new positions; collision(0) on the side stream
-      TRY(launch_update_particles(c));
-      TRY(launch_reset(c, c->stream, true));
-      TRY(launch_ibm_dist(c));
-      EIF_HIP(hipEventRecord(c->ev_fork, c->stream));
-      EIF_HIP(hipStreamWaitEvent(c->side, c->ev_fork, 0));
-      TRY(launch_collision(c, 0, c->side));
-      EIF_HIP(hipEventRecord(c->ev_c0, c->side));
-    }
-    // the exchange with the predicted velocity and, in the same pass,
-    // inertia(0) with the stage-start velocity: the pair the predictor read.
-    // The reset cleared the tile registrations.
-    TRY(launch_exchange_inertia0(c, c->ux_alt, c->uy_alt));
-    // the deposit, with the exchange's and inertia(0)'s particle sums in the
-    // same launch
-    TRY(launch_update_momentum_x0(c));
-    TRY(launch_poisson(c, rk));
-    // the last stage also reduces the advective dt limit of the final
-    // velocities for the next param_decide_dt
-    TRY(launch_correct_pressure(c, rk, true, true, rk == 2));
-    // the sub-iterations read collision(0)
-    EIF_HIP(hipStreamWaitEvent(c->stream, c->ev_c0, 0));
-    int queued = 0, done = 0;
-    int batch = std::min(std::max(c->substep_guess, 1), 102);
+  for (int rk = rk0; rk < 3; rk++) {
+    const bool resume = rk == rk0 && sub0 > 0;
+    if (!resume) TRY(stage_start(c, rk, pred_queued, side_start));
+    int queued = resume ? sub0 : 0, done = queued;
+    int batch = std::min(std::max(c->substep_guess - queued, 1), 102);
     bool early = false;  // predictor of stage rk+1 queued
     int batches = 0;
     for (;;) {
@@ -863,11 +900,159 @@ int eif_gpu_integrate(eif_ctx *c, int substeps[3]) {
     c->substep_guess = done;
     if (substeps) substeps[rk] = done;
     pred_queued = early;
-    side_start = early && batches == 1;
+    side_start = early && batches == 1 && !resume;
   }
   TRY(launch_update_particles(c));
   c->dt_adv_valid = c->dt_adv_parts > 0;
   return EIF_OK;
+}
+
+// ---------------------------------------------------------------------
+// Whole-step graphs.  One time step (decide_dt + the three stages) is
+// captured with K[rk] sub-iterations queued per stage; a gate after them
+// (k_stage_gate) stops the rest of the step, through the device flag every
+// state-changing kernel tests, if a loop needs more.  The host then resumes
+// that stage eagerly (integrate_from) and captures a graph with more
+// sub-iterations.  A step maps its three velocity pairs onto themselves
+// (three predictor rotations), so one graph per starting assignment serves
+// every later step.
+// ---------------------------------------------------------------------
+namespace {
+struct PtrState {
+  double *p[6];
+  bool src_lazy;
+};
+
+PtrState save_ptrs(const Ctx *c) {
+  return PtrState{{c->ux, c->uy, c->ux_alt, c->uy_alt, c->ux_spare, c->uy_spare}, c->src_lazy};
+}
+
+void load_ptrs(Ctx *c, const PtrState &s) {
+  c->ux = s.p[0]; c->uy = s.p[1]; c->ux_alt = s.p[2]; c->uy_alt = s.p[3];
+  c->ux_spare = s.p[4]; c->uy_spare = s.p[5];
+  c->src_lazy = s.src_lazy;
+}
+
+bool same_ptrs(const PtrState &a, const PtrState &b) {
+  for (int k = 0; k < 6; k++)
+    if (a.p[k] != b.p[k]) return false;
+  return true;
+}
+}  // namespace
+
+struct GraphEntry {
+  int K[3];
+  PtrState start;
+  PtrState at_gate[3];  // host pointer state right after each stage's gate
+  int dt_parts;
+  hipGraphExec_t exec = nullptr;
+};
+
+struct GraphCache {
+  std::vector<std::unique_ptr<GraphEntry>> entries;  // stable addresses
+  int K[3] = {0, 0, 0};  // sub-iterations captured per stage
+  int low_run[3] = {0, 0, 0};  // consecutive steps that needed 2+ fewer
+  long launches = 0, resumes = 0, captures = 0;
+};
+
+void graphs_free(Ctx *c) {
+  if (!c->graphs) return;
+  for (auto &e : c->graphs->entries)
+    if (e->exec) (void)hipGraphExecDestroy(e->exec);
+  delete c->graphs;
+  c->graphs = nullptr;
+}
+
+static int capture_step(Ctx *c, const int K[3], GraphEntry *out) {
+  // a graph step follows a step whose last projection left the fused dt
+  // partials
+  c->dt_adv_valid = c->dt_adv_parts > 0;
+  const PtrState start = save_ptrs(c);
+  out->start = start;
+  for (int k = 0; k < 3; k++) out->K[k] = K[k];
+  const bool prof_was = c->prof->on;
+  c->prof->on = false;
+  hipGraph_t graph = nullptr;
+  EIF_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  int rc = EIF_OK;
+  do {
+    if ((rc = launch_decide_dt(c))) break;
+    out->dt_parts = 0;
+    for (int rk = 0; rk < 3 && !rc; rk++) {
+      // one batch per stage: the next predictor follows it, so stages 1 and
+      // 2 start on the side stream (stage_start's side_start form)
+      if ((rc = stage_start(c, rk, rk > 0, rk > 0))) break;
+      for (int s = 0; s < K[rk] && !rc; s++) {
+        rc = launch_substep_forces(c, s);
+        if (!rc) rc = launch_increment(c, rk, s);
+      }
+      if (rc || (rc = launch_stage_gate(c, rk, K[rk]))) break;
+      out->at_gate[rk] = save_ptrs(c);
+      if (rk < 2) {
+        if (hipEventRecord(c->ev_fork, c->stream) != hipSuccess) {
+          rc = hip_fail(hipErrorUnknown, "hipEventRecord", __FILE__, __LINE__);
+          break;
+        }
+        rc = launch_predictor_prev(c, rk + 1);
+      }
+    }
+    if (rc) break;
+    if ((rc = launch_update_particles(c))) break;
+    rc = launch_graph_status(c);
+  } while (0);
+  const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+  c->prof->on = prof_was;
+  const PtrState end = save_ptrs(c);
+  load_ptrs(c, start);
+  if (rc) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture", __FILE__, __LINE__);
+  if (!same_ptrs(start, end)) {
+    (void)hipGraphDestroy(graph);
+    set_error("graph step: the velocity buffers do not return to their assignment");
+    return EIF_ERR_STATE;
+  }
+  out->dt_parts = c->dt_adv_parts;
+  const hipError_t ei = hipGraphInstantiate(&out->exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (ei != hipSuccess) return hip_fail(ei, "hipGraphInstantiate", __FILE__, __LINE__);
+  c->graphs->captures++;
+  return EIF_OK;
+}
+
+static int graph_for(Ctx *c, GraphEntry **out) {
+  GraphCache *g = c->graphs;
+  const PtrState now = save_ptrs(c);
+  for (auto &e : g->entries)
+    if (same_ptrs(e->start, now) && e->K[0] == g->K[0] && e->K[1] == g->K[1] &&
+        e->K[2] == g->K[2] && e->dt_parts == c->dt_adv_parts) {
+      *out = e.get();
+      return EIF_OK;
+    }
+  std::unique_ptr<GraphEntry> e(new GraphEntry());
+  TRY(capture_step(c, g->K, e.get()));
+  g->entries.push_back(std::move(e));
+  *out = g->entries.back().get();
+  return EIF_OK;
+}
+
+// the record of the graph step numbered seq
+static int graph_wait(Ctx *c, unsigned seq, GraphStatus *out) {
+  GraphStatus *slot = c->gstat + (seq & 1);
+  TRY(wait_seq(c, &slot->seq, seq, "graph step status"));
+  *out = *slot;
+  return EIF_OK;
+}
+
+}  // namespace eif
+
+extern "C" {
+
+int eif_gpu_integrate(eif_ctx *c, int substeps[3]) {
+  CHECK_CTX(c);
+  return integrate_from(c, 0, 0, substeps);
 }
 
 int eif_gpu_step(eif_ctx *c, double *dt_out) {
@@ -876,6 +1061,165 @@ int eif_gpu_step(eif_ctx *c, double *dt_out) {
   TRY(eif_gpu_integrate(c, nullptr));
   // dt is read back with the first sub-iteration count of the step
   if (dt_out) *dt_out = c->hscal[S_DT];
+  return EIF_OK;
+}
+
+int eif_gpu_graph_enable(eif_ctx *c, int enable) {
+  CHECK_CTX(c);
+  EIF_HIP(hipStreamSynchronize(c->stream));
+  graphs_free(c);
+  c->graph_on = enable != 0;
+  if (c->graph_on) c->graphs = new GraphCache();
+  return EIF_OK;
+}
+
+int eif_gpu_graph_set_subiterations(eif_ctx *c, const int K[3]) {
+  CHECK_CTX(c);
+  if (!c->graphs) {
+    set_error("graph mode is off");
+    return EIF_ERR_STATE;
+  }
+  for (int r = 0; r < 3; r++)
+    if (K[r] < 1 || K[r] > 102) {
+      set_error("sub-iterations per stage must be within 1..102");
+      return EIF_ERR_ARG;
+    }
+  for (int r = 0; r < 3; r++) {
+    c->graphs->K[r] = K[r];
+    c->graphs->low_run[r] = 0;
+  }
+  return EIF_OK;
+}
+
+int eif_gpu_graph_stats(eif_ctx *c, long out[4]) {
+  CHECK_CTX(c);
+  GraphCache *g = c->graphs;
+  out[0] = g ? g->launches : 0;
+  out[1] = g ? g->resumes : 0;
+  out[2] = g ? g->captures : 0;
+  out[3] = g ? (long)g->K[0] * 10000 + g->K[1] * 100 + g->K[2] : 0;
+  return EIF_OK;
+}
+
+int eif_gpu_run_steps(eif_ctx *c, int nsteps, double *dts, int *substeps) {
+  CHECK_CTX(c);
+  if (nsteps < 0) {
+    set_error("nsteps must be non-negative");
+    return EIF_ERR_ARG;
+  }
+  int n = 0;
+  auto eager = [&](int k) -> int {
+    int sub[3];
+    TRY(launch_decide_dt(c));
+    TRY(integrate_from(c, 0, 0, sub));
+    if (dts) dts[k] = c->hscal[S_DT];
+    if (substeps)
+      for (int r = 0; r < 3; r++) substeps[3 * k + r] = sub[r];
+    return EIF_OK;
+  };
+  if (!c->graph_on) {
+    for (; n < nsteps; n++) TRY(eager(n));
+    return EIF_OK;
+  }
+  GraphCache *g = c->graphs;
+  // K changes rarely; between runs (every graph done) the cache is trimmed
+  if (g->entries.size() >= 32) {
+    EIF_HIP(hipStreamSynchronize(c->stream));
+    for (auto &e : g->entries)
+      if (e->exec) (void)hipGraphExecDestroy(e->exec);
+    g->entries.clear();
+  }
+  // a graph step starts from current boundary values and the fused dt
+  // partials of a previous step; an eager step provides both (and the
+  // first sub-iteration counts)
+  if (n < nsteps && (!c->bc_valid || !c->dt_adv_valid || g->K[0] == 0)) {
+    int sub[3];
+    TRY(launch_decide_dt(c));
+    TRY(integrate_from(c, 0, 0, sub));
+    if (dts) dts[n] = c->hscal[S_DT];
+    if (substeps)
+      for (int r = 0; r < 3; r++) substeps[3 * n + r] = sub[r];
+    for (int r = 0; r < 3; r++)
+      if (g->K[r] == 0) g->K[r] = std::min(sub[r] + 1, 102);
+    n++;
+  }
+  // graph launches run one step ahead of the host's reading of their
+  // records; a stopped step stops the one behind it too (the flag stays
+  // set), which is then launched again
+  GraphEntry *prev = nullptr;
+  unsigned prev_seq = 0;
+  unsigned seq = 0;
+  {
+    unsigned s0 = 0;
+    EIF_HIP(hipMemcpy(&s0, c->g_seq, sizeof(unsigned), hipMemcpyDeviceToHost));
+    seq = s0;
+  }
+  int k_prev = -1;
+  while (n < nsteps || prev) {
+    GraphEntry *cur = nullptr;
+    unsigned cur_seq = 0;
+    if (n < nsteps) {
+      TRY(graph_for(c, &cur));
+      EIF_HIP(hipGraphLaunch(cur->exec, c->stream));
+      cur_seq = ++seq;
+      g->launches++;
+    }
+    if (prev) {
+      GraphStatus st;
+      TRY(graph_wait(c, prev_seq, &st));
+      if (st.flags) {
+        EIF_HIP(hipStreamSynchronize(c->stream));
+        EIF_HIP(hipMemsetAsync(c->flags, 0, sizeof(int), c->stream));
+        EIF_HIP(hipStreamSynchronize(c->stream));
+        return flags_error(st.flags);
+      }
+      if (st.halt) {
+        // the step behind it did nothing; resume the stopped stage
+        const int rk = st.halt - 1;
+        EIF_HIP(hipStreamSynchronize(c->stream));
+        load_ptrs(c, prev->at_gate[rk]);
+        EIF_HIP(hipMemsetAsync(c->halt, 0, sizeof(int), c->stream));
+        int sub[3] = {st.counts[0], st.counts[1], st.counts[2]};
+        c->substep_guess = prev->K[rk] + 1;
+        TRY(integrate_from(c, rk, prev->K[rk], sub));
+        g->resumes++;
+        if (dts) dts[k_prev] = c->hscal[S_DT];
+        if (substeps)
+          for (int r = 0; r < 3; r++) substeps[3 * k_prev + r] = sub[r];
+        g->K[rk] = std::min(sub[rk] + 1, 102);
+        g->low_run[rk] = 0;
+        // the step launched behind it is launched again
+        if (cur) {
+          g->launches--;
+          // its record (a stop) is consumed here
+          GraphStatus skip;
+          TRY(graph_wait(c, cur_seq, &skip));
+          cur = nullptr;
+        }
+        prev = nullptr;
+        continue;
+      }
+      if (dts) dts[k_prev] = st.dt;
+      if (substeps)
+        for (int r = 0; r < 3; r++) substeps[3 * k_prev + r] = st.counts[r];
+      c->hscal[S_DT] = st.dt;
+      // fewer sub-iterations captured when 32 steps in a row needed 2+ fewer
+      for (int r = 0; r < 3; r++) {
+        if (st.counts[r] <= g->K[r] - 2)
+          g->low_run[r]++;
+        else
+          g->low_run[r] = 0;
+        if (g->low_run[r] >= 32) {
+          g->K[r]--;
+          g->low_run[r] = 0;
+        }
+      }
+    }
+    prev = cur;
+    prev_seq = cur_seq;
+    if (cur) k_prev = n++;
+  }
+  c->dt_adv_valid = c->dt_adv_parts > 0;
   return EIF_OK;
 }
 

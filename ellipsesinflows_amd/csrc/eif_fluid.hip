@@ -114,6 +114,7 @@ struct PredArgs {
   const double *dscal;
   double dy, Re, ext, al, be, ga, wxm, wxp;
   int use_b;
+  const int *halt;
 };
 
 // the velocity stencil around (i, j): ux columns i-1, i, i+1 and uy
@@ -199,6 +200,7 @@ __device__ inline double src_y(const PredWin &w, const PredCol &g, double ext) {
 
 template <bool P2, int MODE>
 __global__ __launch_bounds__(PRED_BX) void k_predictor(PredArgs a) {
+  if (halted(a.halt)) return;
   const Layout L = a.L;
   const int nx = L.nx, ny = L.ny;
   const int i = blockIdx.x * PRED_BX + threadIdx.x + 1;  // 1..nx
@@ -325,6 +327,7 @@ static PredArgs pred_args(Ctx *c, int rk) {
   a.al = c->alpha[rk]; a.be = c->beta[rk]; a.ga = c->gamma[rk];
   a.wxm = c->cfg.uy_wall_xm; a.wxp = c->cfg.uy_wall_xp;
   a.use_b = c->beta[rk] != 0.;
+  a.halt = c->halt;
   return a;
 }
 
@@ -391,7 +394,8 @@ template <bool P2, bool DT>
 __global__ __launch_bounds__(256) void k_correct_pressure(
     Layout L, double *ux, double *uy, double *p, const double *psi, const double *dxc,
     const double *dscal, double ga, double dy, int do_correct, int do_pressure,
-    double *dt_part) {
+    double *dt_part, const int *halt) {
+  if (halted(halt)) return;
   const int nx = L.nx, ny = L.ny;
   const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
   const double dt = dscal[S_DT];
@@ -458,7 +462,7 @@ int launch_correct_pressure(Ctx *c, int rk, bool correct, bool pressure, bool dt
                            : (dt ? k_correct_pressure<false, true> : k_correct_pressure<false, false>);
   hipLaunchKernelGGL(kern, grid, dim3(256), 0, c->stream, c->L, c->ux, c->uy, c->p, c->psi,
                      c->dxc, c->dscal, c->gamma[rk], c->dy, (int)correct, (int)pressure,
-                     c->dt_part);
+                     c->dt_part, c->halt);
   EIF_HIP(hipGetLastError());
   c->dt_adv_parts = dt ? (int)(grid.x * grid.y) : 0;
   return EIF_OK;
@@ -512,7 +516,9 @@ __global__ __launch_bounds__(256) void k_dt_adv(Layout L, const double *ux,
 // from the projection kernel of the previous step's last stage
 __global__ void k_dt_final(const double *dred, const double *part, const double *prt, int np,
                            double sf0, double sf2, double dt_dif_x,
-                           double dt_dif_y, double delta, double *dscal, int nparts) {
+                           double dt_dif_y, double delta, double *dscal, int nparts,
+                           const int *halt) {
+  if (halted(halt)) return;
   __shared__ double red[256];
   __shared__ double adv;
   if (nparts > 0) {
@@ -576,7 +582,7 @@ int launch_decide_dt(Ctx *c) {
   const double delta = fmin(c->dx, c->dy);
   hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(256), 0, c->stream, c->dred, c->dt_part, c->prt,
                      c->np, c->cfg.safefactors[0], c->cfg.safefactors[2], dt_dif_x,
-                     dt_dif_y, delta, c->dscal, nparts);
+                     dt_dif_y, delta, c->dscal, nparts, c->halt);
   EIF_HIP(hipGetLastError());
   return EIF_OK;
 }

@@ -60,6 +60,13 @@ constexpr int RES_SLOTS = 128;        // sub-iterations of one stage queued at m
 
 struct Profiler;
 
+// A graph-captured step stops early when a stage's Crank-Nicolson loop
+// needs more sub-iterations than were captured: every kernel that changes
+// state across the step tests this device flag first (null outside graph
+// steps' contexts never happens: the flag always exists, and is zero
+// except between such a stop and the host's resumption).
+__device__ inline bool halted(const int *h) { return h && *h != 0; }
+
 // x / d; with POW2 the divisor is known to be a power of two and the
 // division is the exact product with inv = 1/d (identical result)
 template <bool POW2>
@@ -76,6 +83,18 @@ struct HostStatus {
   int flags;        // device capacity flags
   unsigned seq;     // written last (system-scope release): record number
 };
+
+// a graph step's record for the host (eif_gpu_run_steps)
+struct GraphStatus {
+  double dt;
+  double resid[3];  // last residual of each stage's loop
+  int counts[3];    // sub-iterations each stage ran (within the captured ones)
+  int halt;         // 0, or the stage + 1 whose loop needed more
+  int flags;        // device capacity flags
+  unsigned seq;     // written last (system-scope release)
+};
+
+struct GraphCache;
 
 struct Ctx {
   eif_config cfg;
@@ -131,6 +150,15 @@ struct Ctx {
   double *dscal = nullptr;
   double *hscal = nullptr;  // pinned
   double *dred = nullptr;   // reduction partials
+  int *halt = nullptr;      // graph-step stop flag (see halted())
+  // graph steps: per-stage counts and residuals, the step counter, the
+  // pinned two-slot record, and the captured graphs
+  int *g_counts = nullptr;
+  double *g_resid = nullptr;
+  unsigned *g_seq = nullptr;
+  GraphStatus *gstat = nullptr;
+  GraphCache *graphs = nullptr;
+  bool graph_on = false;
   double *dt_part = nullptr;  // DT_PARTS_CAP partial minima of the fused dt
   int nred = 0;
   // particles
@@ -250,6 +278,9 @@ int launch_update_particles(Ctx *c, hipStream_t st = nullptr);
 // clear_tiles: also clear the IBM tile registrations for the next exchange
 int launch_reset(Ctx *c, hipStream_t st = nullptr, bool clear_tiles = false);
 int launch_status(Ctx *c, int queued);
+int launch_stage_gate(Ctx *c, int stage, int K);
+int launch_graph_status(Ctx *c);
+void graphs_free(Ctx *c);
 int check_flags(Ctx *c);
 int flags_error(int h);
 int launch_statistics(Ctx *c);

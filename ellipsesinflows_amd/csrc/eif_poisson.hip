@@ -256,6 +256,7 @@ struct CombineArgs {
   const double *v;      // [L][nx] spike A_L^-1 e_0 (the other is its mirror)
   const double *bnd;    // [P][2][nx] chunk-boundary neighbour values
   double coef;          // l (= u) of the matrix
+  const int *halt;      // graph-step stop flag: psi is left alone
 };
 
 __device__ inline double combined(const CombineArgs &a, const double *qhat,
@@ -274,6 +275,7 @@ __device__ inline double combined(const CombineArgs &a, const double *qhat,
 __global__ __launch_bounds__(1024) void k_dct3_psi(
     Layout L, const double *qhat, CombineArgs ca, const double *__restrict__ C,
     const double *__restrict__ W, int logn, int mode, double *psi) {
+  if (halted(ca.halt)) return;
   extern __shared__ double sh[];
   const int nx = L.nx, ny = L.ny;
   const int j0 = blockIdx.x;
@@ -373,6 +375,7 @@ template <int LOGN>
 __global__ __launch_bounds__((1 << LOGN) >> dct_logr<LOGN>()) void k_dct3_psi_t(
     Layout L, const double *qhat, CombineArgs ca, const double *__restrict__ C,
     const double *__restrict__ W, double *psi) {
+  if (halted(ca.halt)) return;
   extern __shared__ double sh[];
   constexpr int LOGR = dct_logr<LOGN>(), R = 1 << LOGR;
   constexpr int nx = 1 << LOGN;
@@ -993,6 +996,7 @@ int launch_poisson(Ctx *c, int rk) {
   const double l = 1. / c->dy / c->dy;
   CombineArgs ca{};
   ca.coef = l;
+  ca.halt = c->halt;
   if (c->cfg.poisson_mode == EIF_POISSON_PARTITION && c->part_L) {
     ProfScope ps(c, "ysolve_partition");
     const int Lc = c->part_L, P = c->part_P;

@@ -390,6 +390,7 @@ struct IbmArgs {
   const double *us_x, *us_y;
   double *slice_part0;
   double *slice_part;
+  const int *halt;   // graph-step stop flag
   SliceHead *heads;  // per particle: the pass's position, shape, boxes, cache state      // per-slice partial sums (3 each)
 };
 
@@ -577,6 +578,7 @@ __device__ inline void write_head(const IbmArgs &a, int n, double px, double py,
 
 // heads at the particles' positions, with the increments when with_dx
 __global__ void k_heads(IbmArgs a, int with_dx) {
+  if (halted(a.halt)) return;
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= a.np) return;
   const double *q = a.prt + (long)n * R_N;
@@ -680,6 +682,7 @@ __device__ __forceinline__ void inertia_slice(const IbmArgs &a, int cnstep, int 
 }
 
 __global__ __launch_bounds__(64) void k_inertia(IbmArgs a, int cnstep) {
+  if (halted(a.halt)) return;
   inertia_slice(a, cnstep, blockIdx.x, sub_open(a.res, a.sub));
 }
 
@@ -758,6 +761,7 @@ __device__ inline void finalize_x0_particle(const IbmArgs &a, int n) {
 
 // one wave per particle, four per workgroup
 __global__ __launch_bounds__(256) void k_ibm_finalize(IbmArgs a, int which) {
+  if (halted(a.halt)) return;
   if (!sub_open(a.res, a.sub)) return;
   const int n = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
   if (n < a.np) finalize_particle(a, n, which);
@@ -835,6 +839,7 @@ __device__ inline void register_tile(const IbmArgs &a, int tile, int box_id) {
 // already hold them; sliced like k_inertia.  Used before k_exchange when
 // the cache may be stale.
 __global__ __launch_bounds__(64) void k_ibm_dist(IbmArgs a) {
+  if (halted(a.halt)) return;
   if (!sub_open(a.res, a.sub)) return;
   const int slice = blockIdx.x;
   const int n = a.slice_owner[slice];
@@ -882,6 +887,7 @@ __global__ __launch_bounds__(64) void k_ibm_dist(IbmArgs a) {
 // distance work.
 // The heads hold the particles' positions without increments.
 __global__ __launch_bounds__(64) void k_exchange(IbmArgs a, int own_counts) {
+  if (halted(a.halt)) return;
   const int slice = blockIdx.x;
   const int n = a.slice_owner[slice];
   const int s0 = a.slice_first[n], nsl = a.slice_first[n + 1] - s0;
@@ -1035,6 +1041,7 @@ struct TileArgs {
   double *ux, *uy;
   double *dux, *duy;  // materialisation target (or null)
   int apply;
+  const int *halt;
 };
 
 __device__ inline double box_value(const Box &b, const double *sc, int i, int j) {
@@ -1150,13 +1157,17 @@ __device__ __forceinline__ void tile_body(const TileArgs &a, int tile) {
   }
 }
 
-__global__ __launch_bounds__(TILE_THREADS) void k_tiles(TileArgs a) { tile_body(a, blockIdx.x); }
+__global__ __launch_bounds__(TILE_THREADS) void k_tiles(TileArgs a) {
+  if (halted(a.halt)) return;
+  tile_body(a, blockIdx.x);
+}
 
 // The stage start's deposit with the exchange's per-particle sums (and
 // inertia(0)'s) in the same launch: the first np workgroups finalise one
 // particle each, the rest are the tiles.  Neither reads what the other
 // writes.
 __global__ __launch_bounds__(TILE_THREADS) void k_tiles_x0(TileArgs t, IbmArgs a) {
+  if (halted(a.halt)) return;
   if ((int)blockIdx.x < a.np)
     finalize_x0_particle(a, blockIdx.x);
   else
@@ -1224,6 +1235,7 @@ struct CollArgs {
   int *row_cnt;    // np
   int *wall_hit;   // np x 2
   int *flags;
+  const int *halt;
 };
 
 __device__ inline double harmonic(double v0, double v1) {
@@ -1436,6 +1448,7 @@ __device__ void collide_particle(const CollArgs &a, int n) {
 
 // one 64-lane workgroup per particle
 __global__ __launch_bounds__(64) void k_collide(CollArgs a) {
+  if (halted(a.halt)) return;
   if (!sub_open(a.res, a.sub)) return;
   collide_particle(a, blockIdx.x);
 }
@@ -1447,6 +1460,7 @@ __global__ __launch_bounds__(64) void k_collide(CollArgs a) {
 // alone needs more and spills a little, which costs less than the
 // occupancy its registers would take from the slices.
 __global__ __launch_bounds__(64, 4) void k_inertia_collide(IbmArgs a, CollArgs ca, int cnstep) {
+  if (halted(a.halt)) return;
   const bool open = sub_open(a.res, a.sub);
   if ((int)blockIdx.x < ca.np) {
     if (open) collide_particle(ca, blockIdx.x);
@@ -1479,6 +1493,7 @@ __global__ void k_zero_residual(double *dscal) { dscal[S_RESIDUAL] = 0.; }
 __global__ __launch_bounds__(256) void k_increment(double *prt, int np, const double *dscal,
                                                    double ga, double Fr, double *res_out,
                                                    IbmArgs ia, int fin) {
+  if (halted(ia.halt)) return;
   const bool open = sub_open(ia.res, ia.sub);
   __shared__ double red[4];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -1566,7 +1581,8 @@ __global__ __launch_bounds__(256) void k_increment(double *prt, int np, const do
 }
 
 // src/suspensions/update.c:8-34
-__global__ void k_update_particles(double *prt, int np, double ly) {
+__global__ void k_update_particles(double *prt, int np, double ly, const int *halt) {
+  if (halted(halt)) return;
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= np) return;
   double *p = prt + (long)n * R_N;
@@ -1588,6 +1604,7 @@ __global__ void k_update_particles(double *prt, int np, double ly) {
 // next exchange's tile registrations
 // Also the heads of the stage-start pass (positions without increments).
 __global__ void k_reset(IbmArgs a, double *res, int *tile_cnt, int ntiles, int *active) {
+  if (halted(a.halt)) return;
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (res && n < RES_SLOTS) res[n] = 0.;
   if (tile_cnt) {
@@ -1654,6 +1671,7 @@ static IbmArgs ibm_args(Ctx *c) {
   a.slice_part0 = c->slice_part0;
   a.slice_part = c->slice_part;
   a.heads = static_cast<SliceHead *>(c->heads);
+  a.halt = c->halt;
   return a;
 }
 
@@ -1829,6 +1847,7 @@ static TileArgs tile_args(Ctx *c, int apply) {
   t.dux = c->dux;
   t.duy = c->duy;
   t.apply = apply;
+  t.halt = c->halt;
   return t;
 }
 
@@ -1885,6 +1904,7 @@ static CollArgs coll_args(Ctx *c, int cnstep) {
   a.row_cnt = c->row_cnt;
   a.wall_hit = c->wall_hit;
   a.flags = c->flags;
+  a.halt = c->halt;
   return a;
 }
 
@@ -1969,10 +1989,61 @@ int launch_status(Ctx *c, int queued) {
   return EIF_OK;
 }
 
+// Graph steps: the end of a stage's captured sub-iterations.  Records how
+// many of the first K ran and the last residual; if the loop would go on
+// (sub-iteration K is open), stops the rest of the step (halt = stage + 1)
+// for the host to resume.
+__global__ void k_stage_gate(const double *res, int K, int stage, int *counts, double *resid,
+                             int *halt) {
+  if (*halt) return;
+  int done = 0;
+  double r = 0.;
+  for (int s = 0; s < K && sub_open(res, s); s++) {
+    done = s + 1;
+    r = res[s];
+  }
+  counts[stage] = done;
+  resid[stage] = r;
+  if (sub_open(res, K)) *halt = stage + 1;
+}
+
+int launch_stage_gate(Ctx *c, int stage, int K) {
+  hipLaunchKernelGGL(k_stage_gate, dim3(1), dim3(1), 0, c->stream, c->res_slots, K, stage,
+                     c->g_counts, c->g_resid, c->halt);
+  EIF_HIP(hipGetLastError());
+  return EIF_OK;
+}
+
+// Graph steps: the step's record for the host (two slots, by the parity of
+// the device's step counter), the record number stored last with
+// system-scope release.  Not gated: a stopped step reports its stop.
+__global__ void k_graph_status(const int *halt, const int *counts, const double *resid,
+                               const double *dscal, const int *flags, unsigned *seq,
+                               GraphStatus *h) {
+  const unsigned s = *seq + 1;
+  *seq = s;
+  GraphStatus *o = h + (s & 1);
+  o->dt = dscal[S_DT];
+  for (int k = 0; k < 3; k++) {
+    o->counts[k] = counts[k];
+    o->resid[k] = resid[k];
+  }
+  o->halt = *halt;
+  o->flags = flags[0];
+  __hip_atomic_store(&o->seq, s, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+int launch_graph_status(Ctx *c) {
+  hipLaunchKernelGGL(k_graph_status, dim3(1), dim3(1), 0, c->stream, c->halt, c->g_counts,
+                     c->g_resid, c->dscal, c->flags, c->g_seq, c->gstat);
+  EIF_HIP(hipGetLastError());
+  return EIF_OK;
+}
+
 int launch_update_particles(Ctx *c, hipStream_t st) {
   if (c->np == 0) return EIF_OK;
   hipLaunchKernelGGL(k_update_particles, dim3((c->np + 255) / 256), dim3(256), 0,
-                     st ? st : c->stream, c->prt, c->np, c->ly);
+                     st ? st : c->stream, c->prt, c->np, c->ly, c->halt);
   EIF_HIP(hipGetLastError());
   return EIF_OK;
 }

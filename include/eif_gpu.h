@@ -179,6 +179,27 @@ int eif_gpu_integrate(eif_ctx *ctx, int substeps[3]);
  * be NULL) receives the dt used. */
 int eif_gpu_step(eif_ctx *ctx, double *dt_out);
 
+/* nsteps time steps of src/main.c:71-76 (decide_dt + integrate each);
+ * dts (may be NULL) receives each step's dt, substeps (may be NULL) the
+ * sub-iteration counts, 3 per step.  Synchronising at the end.  With the
+ * graph mode on, each step is one launch of a captured hipGraph: the host
+ * reads one record per step, one step behind the device, instead of one
+ * per batch of sub-iterations. */
+int eif_gpu_run_steps(eif_ctx *ctx, int nsteps, double *dts, int *substeps);
+/* Whole-step hipGraph mode for eif_gpu_run_steps (no counterpart in the
+ * reference).  The graph of a step holds K sub-iterations per stage, K
+ * following the counts the loops need; a stage whose loop needs more stops
+ * the rest of the captured step on the device, and the host resumes it
+ * where it stopped, so results equal the eager path's bit for bit.  A step
+ * is captured again only when K or the buffer assignment changes. */
+int eif_gpu_graph_enable(eif_ctx *ctx, int enable);
+/* Sets K of the three stages (1..102) for the next graph steps; the
+ * counts the loops need adjust it again afterwards. */
+int eif_gpu_graph_set_subiterations(eif_ctx *ctx, const int K[3]);
+/* out: graph launches, resumed stops, captures, and the current K of the
+ * three stages as K0*10000 + K1*100 + K2 */
+int eif_gpu_graph_stats(eif_ctx *ctx, long out[4]);
+
 int eif_gpu_synchronize(eif_ctx *ctx);
 
 /* IBM work counters since the last reset: out[0] box cells visited
