@@ -42,7 +42,8 @@ EXPORTED = (
     "fluid_gpu_get_dt", "fluid_gpu_update_velocity", "poisson_gpu_compute_potential",
     "poisson_gpu_partition_layout",
     "fluid_gpu_correct_velocity", "fluid_gpu_update_pressure", "fluid_gpu_divergence",
-    "fluid_gpu_momentum", "fluid_gpu_energy", "suspensions_gpu_create",
+    "fluid_gpu_momentum", "fluid_gpu_energy", "fluid_gpu_diagnostics_noalloc",
+    "suspensions_gpu_create",
     "suspensions_gpu_n_particles", "suspensions_gpu_download", "suspensions_gpu_upload",
     "suspensions_gpu_reset_particle_increments", "suspensions_gpu_compute_inertia",
     "suspensions_gpu_exchange_momentum", "suspensions_gpu_update_momentum_fleid",
@@ -248,6 +249,12 @@ class Solver:
         o = np.zeros(2)
         self._check(self.lib.fluid_gpu_energy(self.h, _ptr(o)))
         return o
+
+    def diagnostics_noalloc(self, which, out_dev_ptr):
+        """Queue divergence (0), momentum (1) or energy (2) into 2 doubles of
+        device memory at out_dev_ptr (an integer address); asynchronous."""
+        self._check(self.lib.fluid_gpu_diagnostics_noalloc(self.h, int(which),
+                                                           _VP(int(out_dev_ptr))))
 
     def set_particles(self, den, a, b, x, y, az, ux=None, uy=None, vz=None):
         arrs = [np.ascontiguousarray(v, dtype=np.float64) for v in (den, a, b, x, y, az)]

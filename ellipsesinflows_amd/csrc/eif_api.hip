@@ -352,6 +352,7 @@ int fluid_gpu_create(const eif_config *cfg, eif_ctx **out) {
     if (e != hipSuccess) return fail(hip_fail(e, "hipMalloc", __FILE__, __LINE__));
   }
   if ((rc = poisson_init(c))) return fail(rc);
+  if ((rc = diag_scratch_address(c->device, &c->diag_scratch))) return fail(rc);
   c->prof = new Profiler();
   // initial boundary values (src/fluid/init.c:33-85)
   if ((rc = launch_bc_ux(c, c->ux)) || (rc = launch_bc_uy(c, c->uy)) ||
@@ -552,6 +553,14 @@ int fluid_gpu_energy(eif_ctx *c, double q[2]) {
   return launch_diagnostics(c, 2, q);
 }
 
+int fluid_gpu_diagnostics_noalloc(eif_ctx *c, int which, double *out_dev) {
+  CHECK_CTX(c);
+  if (which < 0 || which > 2 || !out_dev) {
+    set_error("diagnostics: which must be 0, 1 or 2 and out_dev non-null");
+    return EIF_ERR_ARG;
+  }
+  return launch_diagnostics_noalloc(c, which, out_dev);
+}
 
 int statistics_gpu_collect(eif_ctx *c) {
   CHECK_CTX(c);

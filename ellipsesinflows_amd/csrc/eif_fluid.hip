@@ -12,6 +12,7 @@
 //   time-step size       src/param/decide_dt.c:11-81
 //   diagnostics          src/logging.c:40-172
 #include "eif_internal.hpp"
+#include <mutex>
 
 namespace eif {
 
@@ -648,6 +649,53 @@ __global__ void k_diag_final(const double *part, int nb, int which, double *out)
   }
   out[0] = a0;
   out[1] = a1;
+}
+
+// Partial sums of the no-allocation diagnostics (fluid_gpu_diagnostics_noalloc):
+// a file-scope device array instead of context memory, so that call
+// allocates nothing.  Each device holds its own copy; host code uses only
+// the per-device address from hipGetSymbolAddress, never this name.
+static __device__ __align__(256) double scratch[2 * DIAG_SCRATCH_BLOCKS];
+
+int diag_scratch_address(int device, double **out) {
+  static std::mutex mtx;
+  static std::map<int, double *> addr;
+  std::lock_guard<std::mutex> lock(mtx);
+  auto it = addr.find(device);
+  if (it != addr.end()) {
+    *out = it->second;
+    return EIF_OK;
+  }
+  int cur = -1;
+  EIF_HIP(hipGetDevice(&cur));
+  if (cur != device) {
+    set_error("diagnostics scratch: the context's device is not current");
+    return EIF_ERR_STATE;
+  }
+  void *ptr = nullptr;
+  EIF_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  addr[device] = static_cast<double *>(ptr);
+  *out = static_cast<double *>(ptr);
+  return EIF_OK;
+}
+
+int launch_diagnostics_noalloc(Ctx *c, int which, double *out_dev) {
+  const int nb = c->nred;
+  if (nb > DIAG_SCRATCH_BLOCKS) {
+    set_error("diagnostics: more partial sums than the static scratch holds");
+    return EIF_ERR_CAPACITY;
+  }
+  if (!c->diag_scratch) {
+    set_error("diagnostics scratch address not resolved");
+    return EIF_ERR_STATE;
+  }
+  // every partial is written before it is read: nothing to clear
+  hipLaunchKernelGGL(k_diag, dim3(nb), dim3(256), 0, c->stream, c->L, c->ux, c->uy,
+                     c->dxf, c->dxc, c->dy, which, c->diag_scratch);
+  hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(64), 0, c->stream, c->diag_scratch, nb,
+                     which, out_dev);
+  EIF_HIP(hipGetLastError());
+  return EIF_OK;
 }
 
 int launch_diagnostics(Ctx *c, int which, double *out) {

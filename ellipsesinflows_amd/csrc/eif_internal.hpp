@@ -57,6 +57,7 @@ constexpr int CONTACT_CAP = 64;  // colliding partners per particle
 constexpr int EQC_ITER_CAP = 100000;  // guard of the uncapped fixed point
 constexpr int DT_PARTS_CAP = 16384;   // workgroups of the projection with fused dt
 constexpr int RES_SLOTS = 128;        // sub-iterations of one stage queued at most
+constexpr int DIAG_SCRATCH_BLOCKS = 4096;  // partial-sum pairs of the static scratch
 
 struct Profiler;
 
@@ -159,6 +160,7 @@ struct Ctx {
   GraphStatus *gstat = nullptr;
   GraphCache *graphs = nullptr;
   bool graph_on = false;
+  double *diag_scratch = nullptr;  // this device's static diagnostics scratch
   double *dt_part = nullptr;  // DT_PARTS_CAP partial minima of the fused dt
   int nred = 0;
   // particles
@@ -246,6 +248,9 @@ int launch_correct_pressure(Ctx *c, int rk, bool correct, bool pressure,
                             bool dt_fused = false);
 int launch_decide_dt(Ctx *c);
 int launch_diagnostics(Ctx *c, int which, double *out);
+// the same sums into caller device memory, partials in the static scratch
+int launch_diagnostics_noalloc(Ctx *c, int which, double *out_dev);
+int diag_scratch_address(int device, double **out);
 int poisson_init(Ctx *c);
 // chunk length and count of the partitioned y-solve for jtot = ny (0 when
 // the line is solved by the Thomas sweep instead); host only

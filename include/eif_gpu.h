@@ -123,6 +123,16 @@ int fluid_gpu_update_pressure(eif_ctx *ctx);
 int fluid_gpu_divergence(eif_ctx *ctx, double *divmax, double *divsum);
 int fluid_gpu_momentum(eif_ctx *ctx, double moms[2]);
 int fluid_gpu_energy(eif_ctx *ctx, double quantities[2]);
+/* No counterpart in the reference: the three diagnostics above for callers
+ * that cannot allocate or synchronise (code under HIP graph capture, tight
+ * serving loops).  which = 0 divergence (max, sum), 1 momentum, 2 energy;
+ * the two results go to out_dev (device memory, 2 doubles), asynchronously
+ * on the context's stream, equal bit for bit to the synchronising calls.
+ * The partial sums live in one static device array per GPU (4096 pairs,
+ * 64 KiB), not in memory the call allocates: the call allocates nothing,
+ * and calls that overlap on one GPU (other contexts, other threads) race
+ * on that array. */
+int fluid_gpu_diagnostics_noalloc(eif_ctx *ctx, int which, double *out_dev);
 
 /* statistics_collect, src/statistics/collect.c:85-95: adds ux, ux^2, uy,
  * uy^2 and the particle indicator phi (collect.c:48-83) to accumulators
