@@ -6,7 +6,10 @@ ellipses a~U[.008,.012], b/a~U[.4,1], az~U[0,pi), density~U[1,1.5], seeded
 rejection placement; initial fluid = laminar profile uy = 4x(1-x).  A step
 is the reference's main-loop body (src/main.c:71-76): param_decide_dt plus
 integrate() with its three RK stages and Crank-Nicolson sub-iterations.
-Inputs are resident on the GPU before the timed region.
+Inputs are resident on the GPU before the timed region.  By default the
+timed steps run as captured whole-step hipGraphs (--mode graph); the
+dominant kernel's launch time and the fluid kernels' HBM fractions come
+from HIP events in an eager segment of the same steps right after it.
 
 Prints one JSON line (other output goes to stderr).
 """
@@ -92,9 +95,20 @@ def traffic_from_profiles(kernel):
         return None
 
 
-def cpu_baseline(parts, xc):
-    """The oracle (C restatement, OpenMP) timed on the host cores for ONE step
-    of the same workload."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(parts, xc, warm=1, timed=4):
+    """The oracle (C restatement, OpenMP) timed on the host cores: `warm`
+    untimed steps, then `timed` steps of the same workload."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle import Oracle  # test infrastructure; cpu_baseline leg only
     o = Oracle(NX, NY, ly=LY, Re=RE, extfrcy=8.0 / RE)
@@ -103,14 +117,80 @@ def cpu_baseline(parts, xc):
     o.upload(ux, uy, p)
     o.update_boundaries()
     o.set_particles(**{k: parts[k] for k in ("den", "a", "b", "x", "y", "az", "ux", "uy", "vz")})
+    for _ in range(warm):
+        o.step()
     t0 = time.perf_counter()
-    o.step()
+    for _ in range(timed):
+        o.step()
     el = time.perf_counter() - t0
     o.close()
-    return {"value": 1.0 / el, "unit": "timesteps/s", "cores": int(cores), "kind": "port",
-            "sample": "1 full time step (decide_dt + 3 RK stages with CN sub-iterations) of the "
-                      "same 2048x4096 + 1024-ellipse workload from the same initial state, "
-                      "oracle/eif_oracle.c with OpenMP over %d threads, %.1f s" % (cores, el)}
+    return {"value": timed / el, "unit": "timesteps/s", "cores": int(cores), "kind": "port",
+            "cpu": cpu_model(),
+            "sample": "%d warm-up + %d timed full time steps (decide_dt + 3 RK stages with CN "
+                      "sub-iterations) of the same 2048x4096 + 1024-ellipse workload from the "
+                      "same initial state, oracle/eif_oracle.c with OpenMP over %d threads, "
+                      "%.1f s timed" % (warm, timed, cores, el)}
+
+
+# algorithmic HBM bytes per launch of the fluid kernels, in units of
+# N = nx*ny values of 8 B (SURVEY-style compulsory traffic): the predictor
+# reads ux, uy, p and (stages 1-2) the kept previous velocity, and writes
+# the new velocity: 5N at stage 0, 7N after, 19/3 N on average
+FLUID_BYTES_N = {"predictor": 19.0 / 3.0, "rhs_dct2": 3.0, "ysolve_partition": 2.0,
+                 "thomas": 2.0, "dct3_psi": 2.0, "correct_pressure": 7.0}
+
+
+ENSEMBLE_SEED = 3
+
+
+def _ensemble_solvers(cases):
+    out = []
+    for cfg, parts in cases:
+        g = eif.Solver(poisson_mode=eif.POISSON_PARTITION, **cfg)
+        nx, ny = cfg["itot"], cfg["jtot"]
+        xc = g.grid()["xc"]
+        uy = np.zeros((ny + 2, nx + 2))
+        uy[1:-1, 1:-1] = 4.0 * xc[None, 1:-1] * (1 - xc[None, 1:-1])
+        g.upload(uy=uy)
+        g.update_boundaries()
+        g.set_particles(**{k: parts[k] for k in ("den", "a", "b", "x", "y", "az", "ux", "uy",
+                                                 "vz")})
+        out.append(g)
+    return out
+
+
+def ensemble_bench(nsteps, warm=3):
+    """The 32-case ensemble (grids 256x512..1024x2048, Re ~ logU[100, 1000],
+    8-256 ellipses): nsteps steps of every case through the plain path one
+    case after the other, then the same from the same state through the
+    concurrent path; case-steps/s of each and their ratio."""
+    cases = initparticles.ensemble_cases(32, seed=ENSEMBLE_SEED)
+    res = {}
+    for concurrent in (False, True):
+        solvers = _ensemble_solvers(cases)
+        eif.ensemble_run_steps(solvers, warm, concurrent=concurrent)
+        for g in solvers:
+            g.synchronize()
+        t0 = time.perf_counter()
+        eif.ensemble_run_steps(solvers, nsteps, concurrent=concurrent)
+        for g in solvers:
+            g.synchronize()
+        el = time.perf_counter() - t0
+        res["concurrent" if concurrent else "sequential"] = len(cases) * nsteps / el
+        for g in solvers:
+            g.close()
+    grids = {}
+    for cfg, parts in cases:
+        key = "%dx%d" % (cfg["itot"], cfg["jtot"])
+        grids[key] = grids.get(key, 0) + 1
+    return {"workload": "32 independent cases, seed %d: grids %s, 8-256 ellipses each, "
+                        "Re~logU[100,1000]" % (ENSEMBLE_SEED, grids),
+            "steps_timed_per_case": nsteps,
+            "case_steps_per_s_sequential": res["sequential"],
+            "case_steps_per_s_batched": res["concurrent"],
+            "cases_per_s_sequential_1000_steps": res["sequential"] / 1000.0,
+            "cases_per_s_batched_1000_steps": res["concurrent"] / 1000.0,
+            "batched_over_sequential": res["concurrent"] / res["sequential"]}
 
 
 def main():
@@ -120,6 +200,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--poisson", choices=("partition", "thomas"), default="partition")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--ensemble-steps", type=int, default=20,
+                    help="steps per case of the 32-case ensemble comparison (0: skip)")
+    ap.add_argument("--mode", choices=("graph", "eager"), default="graph",
+                    help="timed steps as captured whole-step graphs or eager launches")
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
                     help="before the workload, copy a buffer of this size once on the GPU "
                          "(a kernel of known traffic for scaling FETCH_SIZE / WRITE_SIZE)")
@@ -149,25 +233,49 @@ def main():
     g.update_boundaries()
     g.set_particles(**{k: parts[k] for k in ("den", "a", "b", "x", "y", "az", "ux", "uy", "vz")})
     log("area fraction %.3f" % initparticles.area_fraction(parts, LY))
-    for _ in range(args.warmup):
-        g.step()
+    if args.mode == "graph":
+        g.graph(True)
+    g.run_steps(args.warmup)
     g.synchronize()
     cells0 = box_cells(g.particles())
     g.ibm_stats(reset=True)
+    t0 = time.perf_counter()
+    g.run_steps(args.steps)
+    g.synchronize()
+    el = time.perf_counter() - t0
+    visited, evaluated, it64, calls, it32 = g.ibm_stats()
+    graph_stats = g.graph_stats() if args.mode == "graph" else None
+    # kernel times: HIP events in an eager segment right after the timed
+    # steps (a captured graph carries no host-recorded events): the dominant
+    # kernel by launch-stamped events on its own stream, one launch in 13;
+    # then the fluid kernels, each between events on the main stream
+    seg = max(4, min(args.steps, 12))
+    g.graph(False)
     g.profile_select(DOMINANT)
     g.profile_sample(PROFILE_EVERY)
     g.profile(True)
     g.profile_reset()
-    t0 = time.perf_counter()
-    dts = []
-    for _ in range(args.steps):
-        dts.append(g.step())
+    g.run_steps(seg)
     g.synchronize()
-    el = time.perf_counter() - t0
     g.profile(False)
     ncall, kms = g.profile_query(DOMINANT)
-    visited, evaluated, it64, calls, it32 = g.ibm_stats()
     cells1 = box_cells(g.particles())
+    fluid = {}
+    g.profile_select(None)
+    g.profile_sample(1)
+    g.profile(True)
+    g.profile_reset()
+    g.run_steps(4)
+    g.synchronize()
+    g.profile(False)
+    for name, nvals in FLUID_BYTES_N.items():
+        cnt, tms = g.profile_query(name)
+        if cnt:
+            avg = tms * 1e-3 / cnt
+            nbytes = nvals * NX * NY * 8.0
+            fluid[name] = {"avg_launch_ms": 1e3 * avg, "algorithmic_bytes": nbytes,
+                           "achieved_GBps": nbytes / avg / 1e9,
+                           "frac_hbm": nbytes / avg / PEAK_HBM}
     # the reference's own FP64 iteration count on the current boxes (measured
     # outside the timed region), for the algorithmic operation count
     ref_cells, ref_iters = g.ibm_reference_iterations()
@@ -183,6 +291,8 @@ def main():
     # no timed launch (every stage's loop closed at its first sub-iteration):
     # no figure rather than a division by zero
     achieved = cells * (FLOPS_CELL + FLOPS_ITER * mean_iters) / avg_s if avg_s > 0 else None
+    # FP64 vector ALU: MI355X's FP64 vector and matrix peaks are both 78.6
+    # TFLOP/s; the contract's bound field names "mfma" for that ceiling
     roofline = {"bound": "mfma", "achieved": achieved / 1e12 if achieved else None,
                 "peak": PEAK_FP64 / 1e12,
                 "unit": "TFLOP/s", "frac": achieved / PEAK_FP64 if achieved else None,
@@ -191,7 +301,8 @@ def main():
                           "vector ALU; its collision forces in the same launch are not counted)",
                 "avg_launch_ms": kms / max(ncall, 1), "launches_timed": ncall,
                 "algorithmic_flops_per_cell": FLOPS_CELL + FLOPS_ITER * mean_iters,
-                "cells_per_launch": cells}
+                "cells_per_launch": cells, "compute_unit": "FP64 vector ALU",
+                "timing": "eager segment of %d steps after the timed steps" % seg}
     out = {
         "metric": METRIC, "value": steps_per_s, "unit": "timesteps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
@@ -209,7 +320,14 @@ def main():
                 "cell_evals_per_s": visited / el, "calls_per_step": calls / args.steps},
         "final_max_divergence": dmax,
         "roofline": roofline,
+        "fluid_kernels_hbm": fluid,
+        "step_mode": args.mode,
     }
+    if graph_stats:
+        out["graph"] = {"launches": graph_stats[0], "resumed_stops": graph_stats[1],
+                        "captures": graph_stats[2], "subiterations_captured": graph_stats[3]}
+    if args.ensemble_steps > 0:
+        out["ensemble"] = ensemble_bench(args.ensemble_steps)
     if not args.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline(parts, xc)

@@ -51,7 +51,7 @@ EXPORTED = (
     "suspensions_gpu_update_particles", "suspensions_gpu_colliding_pairs",
     "eif_gpu_integrate", "eif_gpu_step", "eif_gpu_synchronize", "eif_gpu_last_error",
     "eif_gpu_run_steps", "eif_gpu_graph_enable", "eif_gpu_graph_stats",
-    "eif_gpu_graph_set_subiterations",
+    "eif_gpu_graph_set_subiterations", "eif_gpu_ensemble_run_steps",
     "eif_gpu_ibm_stats", "eif_gpu_ibm_reference_iterations", "statistics_gpu_collect", "statistics_gpu_reset",
     "statistics_gpu_download",
     "eif_gpu_profile_enable", "eif_gpu_profile_select", "eif_gpu_profile_reset",
@@ -71,6 +71,17 @@ def partition_layout(jtot):
     L, P = ctypes.c_int(), ctypes.c_int()
     lib.poisson_gpu_partition_layout(int(jtot), ctypes.byref(L), ctypes.byref(P))
     return L.value, P.value
+
+
+def ensemble_run_steps(solvers, nsteps, concurrent=True):
+    """Advance independent Solvers by nsteps each: concurrently (each step
+    of every case one graph launch on its own stream, all launched before
+    any is waited for) or one after the other (the plain path)."""
+    lib = load_library()
+    arr = (_VP * max(len(solvers), 1))(*[s.h for s in solvers])
+    rc = lib.eif_gpu_ensemble_run_steps(arr, len(solvers), int(nsteps), int(bool(concurrent)))
+    if rc != 0:
+        raise EIFError("eif_gpu error %d: %s" % (rc, lib.eif_gpu_last_error().decode()))
 
 
 class _Config(ctypes.Structure):

@@ -962,6 +962,7 @@ struct GraphCache {
   int K[3] = {0, 0, 0};  // sub-iterations captured per stage
   int low_run[3] = {0, 0, 0};  // consecutive steps that needed 2+ fewer
   long launches = 0, resumes = 0, captures = 0;
+  unsigned seq = 0;  // number of the last graph step launched (device counter)
 };
 
 void graphs_free(Ctx *c) {
@@ -1055,6 +1056,86 @@ static int graph_wait(Ctx *c, unsigned seq, GraphStatus *out) {
   return EIF_OK;
 }
 
+// decide_dt + integrate, eagerly
+static int eager_step(Ctx *c, int sub[3]) {
+  TRY(launch_decide_dt(c));
+  return integrate_from(c, 0, 0, sub);
+}
+
+// Before graph steps: the cache trimmed (between runs every graph is done),
+// and, when the fields were changed from the host or no counts are known
+// yet, one eager step (*ran) that leaves current boundary values, the fused
+// dt partials and the first sub-iteration counts.
+static int graph_prepare(Ctx *c, bool *ran, int sub[3]) {
+  GraphCache *g = c->graphs;
+  *ran = false;
+  if (g->entries.size() >= 32) {
+    EIF_HIP(hipStreamSynchronize(c->stream));
+    for (auto &e : g->entries)
+      if (e->exec) (void)hipGraphExecDestroy(e->exec);
+    g->entries.clear();
+  }
+  if (!c->bc_valid || !c->dt_adv_valid || g->K[0] == 0) {
+    TRY(eager_step(c, sub));
+    for (int r = 0; r < 3; r++)
+      if (g->K[r] == 0) g->K[r] = sub[r];
+    *ran = true;
+  }
+  return EIF_OK;
+}
+
+// one graph step queued on the context's stream
+static int graph_submit(Ctx *c, GraphEntry **e, unsigned *seq) {
+  TRY(graph_for(c, e));
+  EIF_HIP(hipGraphLaunch((*e)->exec, c->stream));
+  *seq = ++c->graphs->seq;
+  c->graphs->launches++;
+  return EIF_OK;
+}
+
+// A graph step's record: capacity flags become the error; a stop is
+// resumed eagerly where it stopped (nothing may be queued behind it that
+// changes state), and K follows the counts.  dt, sub: the step's.
+static int graph_finish(Ctx *c, GraphEntry *e, const GraphStatus &st, double *dt, int sub[3]) {
+  GraphCache *g = c->graphs;
+  if (st.flags) {
+    EIF_HIP(hipStreamSynchronize(c->stream));
+    EIF_HIP(hipMemsetAsync(c->flags, 0, sizeof(int), c->stream));
+    EIF_HIP(hipStreamSynchronize(c->stream));
+    return flags_error(st.flags);
+  }
+  for (int r = 0; r < 3; r++) sub[r] = st.counts[r];
+  if (st.halt) {
+    const int rk = st.halt - 1;
+    EIF_HIP(hipStreamSynchronize(c->stream));
+    load_ptrs(c, e->at_gate[rk]);
+    EIF_HIP(hipMemsetAsync(c->halt, 0, sizeof(int), c->stream));
+    c->substep_guess = e->K[rk] + 1;
+    TRY(integrate_from(c, rk, e->K[rk], sub));
+    g->resumes++;
+    *dt = c->hscal[S_DT];
+    g->K[rk] = sub[rk];
+    g->low_run[rk] = 0;
+    return EIF_OK;
+  }
+  *dt = st.dt;
+  c->hscal[S_DT] = st.dt;
+  // K follows the counts: a stage captured with one sub-iteration too many
+  // for 16 steps in a row loses it (a stop costs the host one resumption;
+  // an idle sub-iteration costs two launches every step)
+  for (int r = 0; r < 3; r++) {
+    if (st.counts[r] < g->K[r])
+      g->low_run[r]++;
+    else
+      g->low_run[r] = 0;
+    if (g->low_run[r] >= 16) {
+      g->K[r]--;
+      g->low_run[r] = 0;
+    }
+  }
+  return EIF_OK;
+}
+
 }  // namespace eif
 
 extern "C" {
@@ -1078,7 +1159,10 @@ int eif_gpu_graph_enable(eif_ctx *c, int enable) {
   EIF_HIP(hipStreamSynchronize(c->stream));
   graphs_free(c);
   c->graph_on = enable != 0;
-  if (c->graph_on) c->graphs = new GraphCache();
+  if (c->graph_on) {
+    c->graphs = new GraphCache();
+    EIF_HIP(hipMemcpy(&c->graphs->seq, c->g_seq, sizeof(unsigned), hipMemcpyDeviceToHost));
+  }
   return EIF_OK;
 }
 
@@ -1117,111 +1201,54 @@ int eif_gpu_run_steps(eif_ctx *c, int nsteps, double *dts, int *substeps) {
     return EIF_ERR_ARG;
   }
   int n = 0;
-  auto eager = [&](int k) -> int {
-    int sub[3];
-    TRY(launch_decide_dt(c));
-    TRY(integrate_from(c, 0, 0, sub));
-    if (dts) dts[k] = c->hscal[S_DT];
+  auto record = [&](int k, double dt, const int sub[3]) {
+    if (dts) dts[k] = dt;
     if (substeps)
       for (int r = 0; r < 3; r++) substeps[3 * k + r] = sub[r];
-    return EIF_OK;
   };
   if (!c->graph_on) {
-    for (; n < nsteps; n++) TRY(eager(n));
+    for (; n < nsteps; n++) {
+      int sub[3];
+      TRY(eager_step(c, sub));
+      record(n, c->hscal[S_DT], sub);
+    }
     return EIF_OK;
   }
-  GraphCache *g = c->graphs;
-  // K changes rarely; between runs (every graph done) the cache is trimmed
-  if (g->entries.size() >= 32) {
-    EIF_HIP(hipStreamSynchronize(c->stream));
-    for (auto &e : g->entries)
-      if (e->exec) (void)hipGraphExecDestroy(e->exec);
-    g->entries.clear();
-  }
-  // a graph step starts from current boundary values and the fused dt
-  // partials of a previous step; an eager step provides both (and the
-  // first sub-iteration counts)
-  if (n < nsteps && (!c->bc_valid || !c->dt_adv_valid || g->K[0] == 0)) {
+  if (n < nsteps) {
+    bool ran = false;
     int sub[3];
-    TRY(launch_decide_dt(c));
-    TRY(integrate_from(c, 0, 0, sub));
-    if (dts) dts[n] = c->hscal[S_DT];
-    if (substeps)
-      for (int r = 0; r < 3; r++) substeps[3 * n + r] = sub[r];
-    for (int r = 0; r < 3; r++)
-      if (g->K[r] == 0) g->K[r] = std::min(sub[r] + 1, 102);
-    n++;
+    TRY(graph_prepare(c, &ran, sub));
+    if (ran) record(n++, c->hscal[S_DT], sub);
   }
   // graph launches run one step ahead of the host's reading of their
   // records; a stopped step stops the one behind it too (the flag stays
   // set), which is then launched again
   GraphEntry *prev = nullptr;
   unsigned prev_seq = 0;
-  unsigned seq = 0;
-  {
-    unsigned s0 = 0;
-    EIF_HIP(hipMemcpy(&s0, c->g_seq, sizeof(unsigned), hipMemcpyDeviceToHost));
-    seq = s0;
-  }
   int k_prev = -1;
   while (n < nsteps || prev) {
     GraphEntry *cur = nullptr;
     unsigned cur_seq = 0;
-    if (n < nsteps) {
-      TRY(graph_for(c, &cur));
-      EIF_HIP(hipGraphLaunch(cur->exec, c->stream));
-      cur_seq = ++seq;
-      g->launches++;
-    }
+    if (n < nsteps) TRY(graph_submit(c, &cur, &cur_seq));
     if (prev) {
       GraphStatus st;
       TRY(graph_wait(c, prev_seq, &st));
-      if (st.flags) {
+      if (st.halt && cur) {
+        // the step behind it did nothing: its record (a stop) is consumed,
+        // and it is launched again after the resumption
         EIF_HIP(hipStreamSynchronize(c->stream));
-        EIF_HIP(hipMemsetAsync(c->flags, 0, sizeof(int), c->stream));
-        EIF_HIP(hipStreamSynchronize(c->stream));
-        return flags_error(st.flags);
+        GraphStatus skip;
+        TRY(graph_wait(c, cur_seq, &skip));
+        c->graphs->launches--;
+        cur = nullptr;
       }
+      double dt;
+      int sub[3];
+      TRY(graph_finish(c, prev, st, &dt, sub));
+      record(k_prev, dt, sub);
       if (st.halt) {
-        // the step behind it did nothing; resume the stopped stage
-        const int rk = st.halt - 1;
-        EIF_HIP(hipStreamSynchronize(c->stream));
-        load_ptrs(c, prev->at_gate[rk]);
-        EIF_HIP(hipMemsetAsync(c->halt, 0, sizeof(int), c->stream));
-        int sub[3] = {st.counts[0], st.counts[1], st.counts[2]};
-        c->substep_guess = prev->K[rk] + 1;
-        TRY(integrate_from(c, rk, prev->K[rk], sub));
-        g->resumes++;
-        if (dts) dts[k_prev] = c->hscal[S_DT];
-        if (substeps)
-          for (int r = 0; r < 3; r++) substeps[3 * k_prev + r] = sub[r];
-        g->K[rk] = std::min(sub[rk] + 1, 102);
-        g->low_run[rk] = 0;
-        // the step launched behind it is launched again
-        if (cur) {
-          g->launches--;
-          // its record (a stop) is consumed here
-          GraphStatus skip;
-          TRY(graph_wait(c, cur_seq, &skip));
-          cur = nullptr;
-        }
         prev = nullptr;
         continue;
-      }
-      if (dts) dts[k_prev] = st.dt;
-      if (substeps)
-        for (int r = 0; r < 3; r++) substeps[3 * k_prev + r] = st.counts[r];
-      c->hscal[S_DT] = st.dt;
-      // fewer sub-iterations captured when 32 steps in a row needed 2+ fewer
-      for (int r = 0; r < 3; r++) {
-        if (st.counts[r] <= g->K[r] - 2)
-          g->low_run[r]++;
-        else
-          g->low_run[r] = 0;
-        if (g->low_run[r] >= 32) {
-          g->K[r]--;
-          g->low_run[r] = 0;
-        }
       }
     }
     prev = cur;
@@ -1229,6 +1256,53 @@ int eif_gpu_run_steps(eif_ctx *c, int nsteps, double *dts, int *substeps) {
     if (cur) k_prev = n++;
   }
   c->dt_adv_valid = c->dt_adv_parts > 0;
+  return EIF_OK;
+}
+
+int eif_gpu_ensemble_run_steps(eif_ctx **ctxs, int ncases, int nsteps, int concurrent) {
+  if (!ctxs || ncases < 0 || nsteps < 0) {
+    set_error("invalid ensemble arguments");
+    return EIF_ERR_ARG;
+  }
+  for (int k = 0; k < ncases; k++) CHECK_CTX(ctxs[k]);
+  if (!concurrent) {
+    for (int k = 0; k < ncases; k++) TRY(eif_gpu_run_steps(ctxs[k], nsteps, nullptr, nullptr));
+    return EIF_OK;
+  }
+  for (int k = 0; k < ncases; k++)
+    if (!ctxs[k]->graph_on) TRY(eif_gpu_graph_enable(ctxs[k], 1));
+  std::vector<int> done(ncases, 0);
+  for (int k = 0; k < ncases; k++) {
+    bool ran = false;
+    int sub[3];
+    if (nsteps > 0) TRY(graph_prepare(ctxs[k], &ran, sub));
+    done[k] = ran ? 1 : 0;
+  }
+  // step by step: every context's next graph step is launched on its own
+  // stream before the host waits for any of them
+  std::vector<GraphEntry *> ent(ncases, nullptr);
+  std::vector<unsigned> seq(ncases, 0);
+  for (;;) {
+    bool any = false;
+    for (int k = 0; k < ncases; k++) {
+      ent[k] = nullptr;
+      if (done[k] >= nsteps) continue;
+      TRY(graph_submit(ctxs[k], &ent[k], &seq[k]));
+      any = true;
+    }
+    if (!any) break;
+    for (int k = 0; k < ncases; k++) {
+      if (!ent[k]) continue;
+      GraphStatus st;
+      TRY(graph_wait(ctxs[k], seq[k], &st));
+      double dt;
+      int sub[3];
+      TRY(graph_finish(ctxs[k], ent[k], st, &dt, sub));
+      done[k]++;
+    }
+  }
+  for (int k = 0; k < ncases; k++)
+    ctxs[k]->dt_adv_valid = ctxs[k]->dt_adv_parts > 0;
   return EIF_OK;
 }
 

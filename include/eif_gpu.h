@@ -210,6 +210,16 @@ int eif_gpu_graph_set_subiterations(eif_ctx *ctx, const int K[3]);
  * three stages as K0*10000 + K1*100 + K2 */
 int eif_gpu_graph_stats(eif_ctx *ctx, long out[4]);
 
+/* Ensembles of independent cases (no counterpart in the reference, which
+ * runs one case per process): advances ncases contexts, of any grid sizes
+ * and particle counts, by nsteps steps each.  concurrent = 0: one context
+ * after the other through eif_gpu_run_steps (the plain path run
+ * sequentially); 1: step by step across the ensemble, each context's step
+ * one captured graph on its own stream, all of a step's graphs launched
+ * before the host waits for any (graph mode is switched on for them).
+ * Results equal each context's plain run bit for bit.  Synchronising. */
+int eif_gpu_ensemble_run_steps(eif_ctx **ctxs, int ncases, int nsteps, int concurrent);
+
 int eif_gpu_synchronize(eif_ctx *ctx);
 
 /* IBM work counters since the last reset: out[0] box cells visited
