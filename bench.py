@@ -193,6 +193,83 @@ def ensemble_bench(nsteps, warm=3):
             "batched_over_sequential": res["concurrent"] / res["sequential"]}
 
 
+def adjoint_bench(nsteps=200, every=20):
+    """The differentiation workload (256x512, ly 2, Re 200, extfrcy 8/Re, 16
+    ellipses a .04 b .02 placed at least .05 apart, laminar start): nsteps
+    forward steps (decide_dt + integrate) timed, then the gradient of the
+    time-averaged wall shear + drag over the same steps (dt held at that
+    schedule) by checkpoints every `every` steps: a forward pass keeping the
+    checkpoints, then per segment in reverse the recorded recomputation and
+    the stage adjoints (eif_gpu_stage_adjoint)."""
+    rng = np.random.default_rng(4)
+    parts = initparticles.random_ellipses(16, 2.0, (0.04, 0.04), (0.5, 0.5), 0.05, 4)
+    cfg = dict(itot=256, jtot=512, ly=2.0, Re=200.0, extfrcy=8.0 / 200.0)
+    g = eif.Solver(**cfg)
+    xc = g.grid()["xc"]
+    uy = np.zeros((514, 258))
+    uy[1:-1, 1:-1] = 4.0 * xc[None, 1:-1] * (1 - xc[None, 1:-1])
+    g.upload(uy=uy)
+    g.update_boundaries()
+    g.set_particles(**{k: parts[k] for k in ("den", "a", "b", "x", "y", "az", "ux", "uy", "vz")})
+    f0, r0 = g.download(), g.particles()
+    g.synchronize()
+    t0 = time.perf_counter()
+    dts = [g.step() for _ in range(nsteps)]
+    g.synchronize()
+    t_fwd = time.perf_counter() - t0
+    mass = r0[:, 0] * (3.1415926535897932 * r0[:, 1] * r0[:, 2])
+    w = 1.0 / nsteps
+    t0 = time.perf_counter()
+    # forward pass keeping the checkpoints
+    g.upload(f0["ux"], f0["uy"], f0["p"])
+    g.set_particle_records(r0)
+    ckpt = []
+    J = 0.0
+    for k, dt in enumerate(dts):
+        if k % every == 0:
+            f = g.download()
+            ckpt.append((f["ux"], f["uy"], f["p"], g.particles()))
+        g.step_fixed_dt(dt)
+        J += w * (g.wall_shear() - float(np.sum(mass * g.particles()[:, 16])))
+    # reverse, segment by segment
+    g.tape_begin()
+    pb = np.zeros((len(r0), 6))
+    first = True
+    for seg in reversed(range(len(ckpt))):
+        ux, uyc, p, rec = ckpt[seg]
+        g.upload(ux, uyc, p)
+        g.set_particle_records(rec)
+        carry = None if first else (g.adjoint_get(g.ADJ_UX), g.adjoint_get(g.ADJ_UY),
+                                    g.adjoint_get(g.ADJ_P), g.adjoint_get(g.ADJ_PARTICLES),
+                                    g.adjoint_get(g.ADJ_RE))
+        g.tape_begin()
+        seg_dts = dts[seg * every:(seg + 1) * every]
+        for dt in seg_dts:
+            g.step_fixed_dt(dt, record=True)
+        if carry is not None:
+            for which, v in zip((g.ADJ_UX, g.ADJ_UY, g.ADJ_P, g.ADJ_PARTICLES, g.ADJ_RE), carry):
+                g.adjoint_add(which, v)
+        first = False
+        fseed = np.zeros((len(r0), 3))
+        fseed[:, 1] = -mass * w
+        for _ in seg_dts:
+            g.adjoint_add_wall_shear(w)
+            g.adjoint_add(g.ADJ_FORCE, fseed)
+            for _ in range(3):
+                g.stage_adjoint()
+    pb = g.adjoint_get(g.ADJ_PARTICLES)
+    g.synchronize()
+    t_adj = time.perf_counter() - t0
+    g.close()
+    return {"workload": "256x512 grid, ly=2, Re=200, 16 ellipses a=.04 b=.02 gap .05; J = "
+                        "time-averaged wall shear + drag over %d steps, checkpoints every %d"
+                        % (nsteps, every),
+            "forward_ms_per_step": 1e3 * t_fwd / nsteps,
+            "gradient_ms_per_step": 1e3 * t_adj / nsteps,
+            "adjoint_over_forward": t_adj / t_fwd,
+            "objective": J, "grad_norm_positions": float(np.linalg.norm(pb[:, :2]))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -200,6 +277,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--poisson", choices=("partition", "thomas"), default="partition")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="skip the adjoint/forward cost measurement")
     ap.add_argument("--ensemble-steps", type=int, default=20,
                     help="steps per case of the 32-case ensemble comparison (0: skip)")
     ap.add_argument("--mode", choices=("graph", "eager"), default="graph",
@@ -328,6 +407,8 @@ def main():
                         "captures": graph_stats[2], "subiterations_captured": graph_stats[3]}
     if args.ensemble_steps > 0:
         out["ensemble"] = ensemble_bench(args.ensemble_steps)
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_bench()
     if not args.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline(parts, xc)

@@ -52,6 +52,9 @@ EXPORTED = (
     "eif_gpu_integrate", "eif_gpu_step", "eif_gpu_synchronize", "eif_gpu_last_error",
     "eif_gpu_run_steps", "eif_gpu_graph_enable", "eif_gpu_graph_stats",
     "eif_gpu_graph_set_subiterations", "eif_gpu_ensemble_run_steps",
+    "eif_gpu_tape_begin", "eif_gpu_step_fixed_dt", "eif_gpu_stage_adjoint",
+    "eif_gpu_adjoint_get", "eif_gpu_adjoint_add", "fluid_gpu_wall_shear",
+    "eif_gpu_adjoint_add_wall_shear",
     "eif_gpu_ibm_stats", "eif_gpu_ibm_reference_iterations", "statistics_gpu_collect", "statistics_gpu_reset",
     "statistics_gpu_download",
     "eif_gpu_profile_enable", "eif_gpu_profile_select", "eif_gpu_profile_reset",
@@ -337,6 +340,46 @@ class Solver:
         self._check(self.lib.eif_gpu_run_steps(self.h, int(nsteps), _ptr(dts),
                                                sub.ctypes.data_as(_PI)))
         return dts[:nsteps], sub[:nsteps]
+
+    # -- adjoint of one RK stage ---------------------------------------------
+    ADJ_UX, ADJ_UY, ADJ_P, ADJ_PARTICLES, ADJ_FORCE, ADJ_RE = range(6)
+
+    def wall_shear(self):
+        out = _D()
+        self._check(self.lib.fluid_gpu_wall_shear(self.h, ctypes.byref(out)))
+        return out.value
+
+    def tape_begin(self):
+        self._check(self.lib.eif_gpu_tape_begin(self.h))
+
+    def step_fixed_dt(self, dt, record=False):
+        self._check(self.lib.eif_gpu_step_fixed_dt(self.h, _D(dt), int(bool(record))))
+
+    def stage_adjoint(self):
+        self._check(self.lib.eif_gpu_stage_adjoint(self.h))
+
+    def _adj_shape(self, which):
+        if which == self.ADJ_UX:
+            return self.shapes["ux"]
+        if which in (self.ADJ_UY, self.ADJ_P):
+            return self.shapes["uy"]
+        if which == self.ADJ_PARTICLES:
+            return (self.n_particles, 6)
+        if which == self.ADJ_FORCE:
+            return (self.n_particles, 3)
+        return (1,)
+
+    def adjoint_add_wall_shear(self, weight):
+        self._check(self.lib.eif_gpu_adjoint_add_wall_shear(self.h, _D(weight)))
+
+    def adjoint_get(self, which):
+        out = np.zeros(self._adj_shape(which))
+        self._check(self.lib.eif_gpu_adjoint_get(self.h, int(which), _ptr(out)))
+        return out
+
+    def adjoint_add(self, which, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64).reshape(self._adj_shape(which))
+        self._check(self.lib.eif_gpu_adjoint_add(self.h, int(which), _ptr(a)))
 
     def graph(self, enable=True):
         self._check(self.lib.eif_gpu_graph_enable(self.h, int(bool(enable))))

@@ -18,7 +18,7 @@ EXE = os.path.join(HERE, "eif_gpu_main")
 HOSTLIB = os.path.join(HERE, "libeif_host.so")
 
 HIP_SOURCES = ["eif_api.hip", "eif_fluid.hip", "eif_poisson.hip", "eif_suspensions.hip",
-               "eif_statistics.hip"]
+               "eif_statistics.hip", "eif_adjoint.hip"]
 HOST_SOURCES = ["param.c", "npyio.c", "save.c", "logging.c"]
 HOST_CFLAGS = ["-O2", "-std=c99", "-Wall", "-Wextra", "-Werror"]
 

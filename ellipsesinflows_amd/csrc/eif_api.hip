@@ -186,6 +186,7 @@ static void free_ctx(Ctx *c) {
   if (c->hstat) (void)hipHostFree(c->hstat);
   if (c->gstat) (void)hipHostFree(c->gstat);
   graphs_free(c);
+  tape_free(c);
   void *gbufs[] = {c->halt, c->g_counts, c->g_resid, c->g_seq};
   for (auto b : gbufs)
     if (b) (void)hipFree(b);
@@ -742,6 +743,7 @@ static int queue_substeps(Ctx *c, int rk, int first, int n) {
     return EIF_ERR_CAPACITY;
   }
   for (int s = first; s < first + n; s++) {
+    if (c->recording) TRY(rec_sub(c, s));
     TRY(launch_substep_forces(c, s));
     TRY(launch_increment(c, rk, s));
   }
@@ -799,6 +801,7 @@ static int stage_start(Ctx *c, int rk, bool pred_queued, bool side_start) {
   // collision(0), which overlaps the fluid kernels.  Events join the side
   // work back before anything that reads its results or reuses its
   // buffers.
+  if (c->recording) TRY(rec_stage_begin(c, rk));
   if (!pred_queued) {
     TRY(launch_reset(c, c->stream, true));
     // src/main.c:16-19: the kernels of every stage keep the boundary and
@@ -810,11 +813,13 @@ static int stage_start(Ctx *c, int rk, bool pred_queued, bool side_start) {
       TRY(launch_bc_p(c, c->p));
       c->bc_valid = true;
     }
+    if (c->recording) TRY(rec_fields(c, 0, c->ux, c->uy));  // u^k
     // the distances at the stage-start positions and collision(0) beside
     // the predictor (queued first: at a step's start the device waits
     // for these submissions)
     EIF_HIP(hipEventRecord(c->ev_fork, c->stream));
     TRY(launch_predictor_prev(c, rk));  // writes the spare ux, uy buffers
+    if (c->recording) TRY(rec_fields(c, 1, c->ux, c->uy));  // u*
     EIF_HIP(hipStreamWaitEvent(c->side, c->ev_fork, 0));
     TRY(launch_ibm_dist(c, c->side));
     EIF_HIP(hipEventRecord(c->ev_i0, c->side));
@@ -822,6 +827,10 @@ static int stage_start(Ctx *c, int rk, bool pred_queued, bool side_start) {
     EIF_HIP(hipEventRecord(c->ev_c0, c->side));
     EIF_HIP(hipStreamWaitEvent(c->stream, c->ev_i0, 0));
   } else if (side_start) {
+    if (c->recording) {  // the stage-start velocity and the queued predictor's
+      TRY(rec_fields(c, 0, c->ux_alt, c->uy_alt));
+      TRY(rec_fields(c, 1, c->ux, c->uy));
+    }
     // the previous stage's loop ended with its first batch, which ev_fork
     // closes: the particle update, this stage's reset and the distances
     // at the new positions run on the side stream beside the predictor
@@ -834,6 +843,10 @@ static int stage_start(Ctx *c, int rk, bool pred_queued, bool side_start) {
     EIF_HIP(hipEventRecord(c->ev_c0, c->side));
     EIF_HIP(hipStreamWaitEvent(c->stream, c->ev_i0, 0));
   } else {
+    if (c->recording) {
+      TRY(rec_fields(c, 0, c->ux_alt, c->uy_alt));
+      TRY(rec_fields(c, 1, c->ux, c->uy));
+    }
     // behind the predictor (and the batches queued after it): the previous
     // stage's particle update, this stage's reset and the distances at the
     // new positions; collision(0) on the side stream
@@ -856,8 +869,10 @@ static int stage_start(Ctx *c, int rk, bool pred_queued, bool side_start) {
   // the last stage also reduces the advective dt limit of the final
   // velocities for the next param_decide_dt
   TRY(launch_correct_pressure(c, rk, true, true, rk == 2));
+  if (c->recording) TRY(rec_fields(c, 2, c->ux, c->uy));  // u^{k+1}
   // the sub-iterations read collision(0)
   EIF_HIP(hipStreamWaitEvent(c->stream, c->ev_c0, 0));
+  if (c->recording) TRY(rec_p0(c));  // the stage-start particles (increments reset)
   return EIF_OK;
 }
 
@@ -908,6 +923,7 @@ static int integrate_from(Ctx *c, int rk0, int sub0, int substeps[3]) {
     }
     c->substep_guess = done;
     if (substeps) substeps[rk] = done;
+    if (c->recording) TRY(rec_stage_end(c, done));
     pred_queued = early;
     side_start = early && batches == 1 && !resume;
   }
@@ -1152,6 +1168,71 @@ int eif_gpu_step(eif_ctx *c, double *dt_out) {
   // dt is read back with the first sub-iteration count of the step
   if (dt_out) *dt_out = c->hscal[S_DT];
   return EIF_OK;
+}
+
+int eif_gpu_tape_begin(eif_ctx *c) {
+  CHECK_CTX(c);
+  return tape_begin(c);
+}
+
+int eif_gpu_step_fixed_dt(eif_ctx *c, double dt, int record) {
+  CHECK_CTX(c);
+  if (!(dt > 0.) || !std::isfinite(dt)) {
+    set_error("dt must be positive and finite");
+    return EIF_ERR_ARG;
+  }
+  if (record && !c->tape) {
+    set_error("no tape: call eif_gpu_tape_begin first");
+    return EIF_ERR_STATE;
+  }
+  // integrate() itself at the given dt; recorded: with the stage records
+  // copied as it goes (rec_*)
+  c->hscal[S_DT] = dt;
+  EIF_HIP(hipMemcpyAsync(c->dscal + S_DT, c->hscal + S_DT, sizeof(double),
+                         hipMemcpyHostToDevice, c->stream));
+  c->recording = record != 0;
+  const int rc = integrate_from(c, 0, 0, nullptr);
+  c->recording = false;
+  TRY(rc);
+  if (record) TRY(rec_check_contacts(c));
+  return EIF_OK;
+}
+
+int fluid_gpu_wall_shear(eif_ctx *c, double *out) {
+  CHECK_CTX(c);
+  if (!out) {
+    set_error("null output");
+    return EIF_ERR_ARG;
+  }
+  return launch_wall_shear(c, out);
+}
+
+int eif_gpu_stage_adjoint(eif_ctx *c) {
+  CHECK_CTX(c);
+  return stage_adjoint(c);
+}
+
+int eif_gpu_adjoint_get(eif_ctx *c, int which, double *host) {
+  CHECK_CTX(c);
+  if (!host) {
+    set_error("null host buffer");
+    return EIF_ERR_ARG;
+  }
+  return adjoint_access(c, which, host, false);
+}
+
+int eif_gpu_adjoint_add_wall_shear(eif_ctx *c, double weight) {
+  CHECK_CTX(c);
+  return adjoint_add_wall_shear(c, weight);
+}
+
+int eif_gpu_adjoint_add(eif_ctx *c, int which, const double *host) {
+  CHECK_CTX(c);
+  if (!host) {
+    set_error("null host buffer");
+    return EIF_ERR_ARG;
+  }
+  return adjoint_access(c, which, const_cast<double *>(host), true);
 }
 
 int eif_gpu_graph_enable(eif_ctx *c, int enable) {

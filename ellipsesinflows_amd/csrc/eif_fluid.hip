@@ -698,6 +698,35 @@ int launch_diagnostics_noalloc(Ctx *c, int which, double *out_dev) {
   return EIF_OK;
 }
 
+// wall shear stress summed over both walls:
+// (1/Re) sum_j dy [(UY(1,j) - uy_wall_xm) / DXC(1) + (UY(nx,j) - uy_wall_xp) / DXC(nx+1)]
+__global__ __launch_bounds__(256) void k_wall_shear(Layout L, const double *uy,
+                                                    const double *dxc, double dy, double rre,
+                                                    double wxm, double wxp, double *out) {
+  double a = 0.;
+  for (int j = threadIdx.x + 1; j <= L.ny; j += blockDim.x)
+    a += dy * ((uy[L.p(1, j)] - wxm) / dxc[0] + (uy[L.p(L.nx, j)] - wxp) / dxc[L.nx]);
+  __shared__ double red[256];
+  red[threadIdx.x] = a;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = rre * red[0];
+}
+
+int launch_wall_shear(Ctx *c, double *out) {
+  hipLaunchKernelGGL(k_wall_shear, dim3(1), dim3(256), 0, c->stream, c->L, c->uy, c->dxc, c->dy,
+                     1. / c->cfg.Re, c->cfg.uy_wall_xm, c->cfg.uy_wall_xp, c->dscal + S_DIVMAX);
+  EIF_HIP(hipGetLastError());
+  EIF_HIP(hipMemcpyAsync(c->hscal + S_DIVMAX, c->dscal + S_DIVMAX, sizeof(double),
+                         hipMemcpyDeviceToHost, c->stream));
+  EIF_HIP(hipStreamSynchronize(c->stream));
+  out[0] = c->hscal[S_DIVMAX];
+  return EIF_OK;
+}
+
 int launch_diagnostics(Ctx *c, int which, double *out) {
   const int nb = c->nred;
   hipLaunchKernelGGL(k_diag, dim3(nb), dim3(256), 0, c->stream, c->L, c->ux, c->uy,

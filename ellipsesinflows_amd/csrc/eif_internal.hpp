@@ -96,6 +96,7 @@ struct GraphStatus {
 };
 
 struct GraphCache;
+struct AdjTape;
 
 struct Ctx {
   eif_config cfg;
@@ -160,6 +161,8 @@ struct Ctx {
   GraphStatus *gstat = nullptr;
   GraphCache *graphs = nullptr;
   bool graph_on = false;
+  AdjTape *tape = nullptr;  // adjoint records and cotangents
+  bool recording = false;   // eif_gpu_integrate copies the stage records
   double *diag_scratch = nullptr;  // this device's static diagnostics scratch
   double *dt_part = nullptr;  // DT_PARTS_CAP partial minima of the fused dt
   int nred = 0;
@@ -248,6 +251,7 @@ int launch_correct_pressure(Ctx *c, int rk, bool correct, bool pressure,
                             bool dt_fused = false);
 int launch_decide_dt(Ctx *c);
 int launch_diagnostics(Ctx *c, int which, double *out);
+int launch_wall_shear(Ctx *c, double *out);
 // the same sums into caller device memory, partials in the static scratch
 int launch_diagnostics_noalloc(Ctx *c, int which, double *out_dev);
 int diag_scratch_address(int device, double **out);
@@ -255,7 +259,8 @@ int poisson_init(Ctx *c);
 // chunk length and count of the partitioned y-solve for jtot = ny (0 when
 // the line is solved by the Thomas sweep instead); host only
 int partition_layout(int ny, int *chunk, int *nchunks);
-int launch_poisson(Ctx *c, int rk);
+// rhs_add (cell field, P layout): solve for div u / (gamma dt) + rhs_add / (gamma dt)^2
+int launch_poisson(Ctx *c, int rk, const double *rhs_add = nullptr);
 int susp_init(Ctx *c);
 // defer_finalize: the slice sums are added by the next launch_increment
 // with fin_inertia (one launch fewer per sub-iteration)
@@ -286,6 +291,19 @@ int launch_status(Ctx *c, int queued);
 int launch_stage_gate(Ctx *c, int stage, int K);
 int launch_graph_status(Ctx *c);
 void graphs_free(Ctx *c);
+// adjoint (eif_adjoint.hip)
+int tape_begin(Ctx *c);
+void tape_free(Ctx *c);
+// stage records taken by integrate_from while c->recording
+int rec_stage_begin(Ctx *c, int rk);
+int rec_fields(Ctx *c, int which, const double *ux, const double *uy);  // 0 u^k, 1 u*, 2 u^{k+1}
+int rec_p0(Ctx *c);
+int rec_sub(Ctx *c, int s);
+int rec_stage_end(Ctx *c, int S);
+int rec_check_contacts(Ctx *c);
+int stage_adjoint(Ctx *c);
+int adjoint_access(Ctx *c, int which, double *host, bool add);
+int adjoint_add_wall_shear(Ctx *c, double weight);
 int check_flags(Ctx *c);
 int flags_error(int h);
 int launch_statistics(Ctx *c);

@@ -208,10 +208,23 @@ __device__ inline double rhs_value(const Layout &L, const double *ux, const doub
                 + qdiv<P2>(uy[L.p(i + 1, j + 1)] - uy[L.p(i + 1, j)], dy, 1. / dy));
 }
 
+// the right-hand side with an additive cell field (the adjoint projection's
+// pressure cotangent): fac * (div u + fac * add(i, j)); add == null is the
+// forward right-hand side, operation for operation
+template <bool P2 = false>
+__device__ inline double rhs_value_add(const Layout &L, const double *ux, const double *uy,
+                                       const double *add, double fac, double dx, double dy,
+                                       int i, int j) {
+  if (!add) return rhs_value<P2>(L, ux, uy, fac, dx, dy, i, j);
+  const double div = qdiv<P2>(ux[L.ux(i + 2, j)] - ux[L.ux(i + 1, j)], dx, 1. / dx)
+                     + qdiv<P2>(uy[L.p(i + 1, j + 1)] - uy[L.p(i + 1, j)], dy, 1. / dy);
+  return fac * (div + fac * add[L.p(i + 1, j)]);
+}
+
 __global__ __launch_bounds__(1024) void k_rhs_dct2(
     Layout L, const double *ux, const double *uy, const double *dscal, double ga,
     double dx, double dy, const double *__restrict__ C, const double *__restrict__ W,
-    int logn, int mode, double *qhat) {
+    int logn, int mode, double *qhat, const double *add) {
   extern __shared__ double sh[];
   const int nx = L.nx;
   const int j = blockIdx.x + 1;
@@ -223,7 +236,7 @@ __global__ __launch_bounds__(1024) void k_rhs_dct2(
     for (int i = threadIdx.x; i < nx; i += blockDim.x) {
       const int m = (i & 1) ? nx - 1 - (i >> 1) : (i >> 1);
       const int pos = bitrev(m, logn);
-      re[pos] = rhs_value(L, ux, uy, fac, dx, dy, i, j);
+      re[pos] = rhs_value_add(L, ux, uy, add, fac, dx, dy, i, j);
       im[pos] = 0.;
     }
     __syncthreads();
@@ -231,7 +244,8 @@ __global__ __launch_bounds__(1024) void k_rhs_dct2(
     for (int k = threadIdx.x; k < nx; k += blockDim.x)
       out[k] = 2. * (C[k] * re[k] + C[nx - k] * im[k]);
   } else {
-    for (int i = threadIdx.x; i < nx; i += blockDim.x) re[i] = rhs_value(L, ux, uy, fac, dx, dy, i, j);
+    for (int i = threadIdx.x; i < nx; i += blockDim.x)
+      re[i] = rhs_value_add(L, ux, uy, add, fac, dx, dy, i, j);
     __syncthreads();
     for (int k = threadIdx.x; k < nx; k += blockDim.x) {
       double acc = 0.;
@@ -315,11 +329,11 @@ __global__ __launch_bounds__(1024) void k_dct3_psi(
 // (Z[k] - conj Z[kk]) / 2i (the halves cancel against REDFT10's factor 2);
 // the inverse takes the two pre-twiddled spectra as Va + i Vb.  An odd last
 // row pairs with a zero row.
-template <int LOGN, bool P2>
+template <int LOGN, bool P2, bool ADD = false>
 __global__ __launch_bounds__((1 << LOGN) >> dct_logr<LOGN>()) void k_rhs_dct2_t(
     Layout L, const double *ux, const double *uy, const double *dscal, double ga,
     double dx, double dy, const double *__restrict__ C, const double *__restrict__ W,
-    double *qhat) {
+    double *qhat, const double *add = nullptr) {
   extern __shared__ double sh[];
   constexpr int LOGR = dct_logr<LOGN>(), R = 1 << LOGR;
   constexpr int nx = 1 << LOGN;
@@ -339,8 +353,10 @@ __global__ __launch_bounds__((1 << LOGN) >> dct_logr<LOGN>()) void k_rhs_dct2_t(
     const int i = threadIdx.x + s * T;
     const int m = (i & 1) ? nx - 1 - (i >> 1) : i >> 1;
     const int q = sw(bitrev(m, LOGN));
-    const double vb = rhs_value<P2>(L, ux, uy, fac, dx, dy, i, jbl);
-    re[q] = rhs_value<P2>(L, ux, uy, fac, dx, dy, i, ja);
+    const double vb = ADD ? rhs_value_add<P2>(L, ux, uy, add, fac, dx, dy, i, jbl)
+                          : rhs_value<P2>(L, ux, uy, fac, dx, dy, i, jbl);
+    re[q] = ADD ? rhs_value_add<P2>(L, ux, uy, add, fac, dx, dy, i, ja)
+                : rhs_value<P2>(L, ux, uy, fac, dx, dy, i, ja);
     im[q] = hasb ? vb : 0.;
   }
   __syncthreads();
@@ -962,7 +978,7 @@ static int ilog2_exact(int n) {
   return l;
 }
 
-int launch_poisson(Ctx *c, int rk) {
+int launch_poisson(Ctx *c, int rk, const double *rhs_add) {
   const int nx = c->nx, ny = c->ny;
   const int logn = ilog2_exact(nx);
   const size_t shmem = 2 * (size_t)nx * sizeof(double);
@@ -974,12 +990,14 @@ int launch_poisson(Ctx *c, int rk) {
 #define EIF_RHS_T(LG)                                                                    \
   case LG:                                                                               \
     {                                                                                    \
-      auto kern = c->pow2_grid ? k_rhs_dct2_t<LG, true> : k_rhs_dct2_t<LG, false>;       \
+      auto kern = rhs_add ? (c->pow2_grid ? k_rhs_dct2_t<LG, true, true>                 \
+                                          : k_rhs_dct2_t<LG, false, true>)               \
+                          : (c->pow2_grid ? k_rhs_dct2_t<LG, true> : k_rhs_dct2_t<LG, false>); \
       hipLaunchKernelGGL(kern, dim3((ny + 1) / 2), dim3(nx >> dct_logr<LG>()), shmem,    \
                          c->stream,                                                      \
                          c->L, c->ux,                                                    \
                          c->uy, c->dscal, c->gamma[rk], c->dx, c->dy, c->ctab, c->twid,  \
-                         c->qhat);                                                       \
+                         c->qhat, rhs_add);                                              \
     }                                                                                    \
     break;
     switch (mode == 2 ? logn : -1) {
@@ -988,7 +1006,7 @@ int launch_poisson(Ctx *c, int rk) {
       default:
         hipLaunchKernelGGL(k_rhs_dct2, dim3(ny), dim3(threads), shmem, c->stream, c->L,
                            c->ux, c->uy, c->dscal, c->gamma[rk], c->dx, c->dy, c->ctab,
-                           c->twid, logn, mode, c->qhat);
+                           c->twid, logn, mode, c->qhat, rhs_add);
     }
 #undef EIF_RHS_T
     EIF_HIP(hipGetLastError());

@@ -220,6 +220,52 @@ int eif_gpu_graph_stats(eif_ctx *ctx, long out[4]);
  * Results equal each context's plain run bit for bit.  Synchronising. */
 int eif_gpu_ensemble_run_steps(eif_ctx **ctxs, int ncases, int nsteps, int concurrent);
 
+/* Adjoint of one RK stage (no counterpart in the reference).  The stage
+ * is integrate()'s body (src/main.c:15-53) with dt fixed; its adjoint
+ * covers the transposed advection/diffusion stencils
+ * (src/fluid/update_velocity.c:9-237), the projection and pressure update
+ * (compute_potential.c:48-138, correct_velocity.c:6-42,
+ * update_pressure.c:7-24), the IBM weights, forcing and inertia integrals
+ * (src/suspensions/auxiliary.c:19-59, exchange.c:13-88, inertia.c:9-70,
+ * update.c:36-59) and the rigid-body update with its Crank-Nicolson
+ * sub-iterations as run (increment.c:10-72, update.c:8-34).  Collisions
+ * are forward-only: a recorded step that meets a contact returns
+ * EIF_ERR_STATE.
+ *
+ * eif_gpu_tape_begin clears the records and zeroes every cotangent.
+ * eif_gpu_step_fixed_dt runs one time step with the given dt (no
+ * decide_dt): record == 0 through eif_gpu_integrate; record != 0 through
+ * the per-function calls in the reference's order, pushing its three
+ * stages.  eif_gpu_stage_adjoint replaces the cotangents of the last
+ * recorded stage's outputs by those of its inputs and pops it.  Cotangent
+ * access (host arrays; get copies, add accumulates):
+ *   EIF_ADJ_UX, EIF_ADJ_UY, EIF_ADJ_P  fields in the reference layouts
+ *                                      (meaningful on the unknowns; halo
+ *                                      rows and wall values are zero)
+ *   EIF_ADJ_PARTICLES  n x 6: x, y, az, ux, uy, vz of each particle
+ *   EIF_ADJ_FORCE      n x 3: cotangents of fux, fuy, tvz of the exchange
+ *                      of the next stage reversed (consumed by it)
+ *   EIF_ADJ_RE         1: cotangent of Re (through the diffusion terms) */
+#define EIF_ADJ_UX 0
+#define EIF_ADJ_UY 1
+#define EIF_ADJ_P 2
+#define EIF_ADJ_PARTICLES 3
+#define EIF_ADJ_FORCE 4
+#define EIF_ADJ_RE 5
+/* The wall shear stress of the adjoint runs' objective (no counterpart in
+ * the reference), summed over both walls:
+ * (1/Re) sum_j dy [(UY(1,j) - uy_wall_xm)/DXC(1) + (UY(itot,j) - uy_wall_xp)/DXC(itot+1)].
+ * Synchronising. */
+int fluid_gpu_wall_shear(eif_ctx *ctx, double *out);
+int eif_gpu_tape_begin(eif_ctx *ctx);
+int eif_gpu_step_fixed_dt(eif_ctx *ctx, double dt, int record);
+int eif_gpu_stage_adjoint(eif_ctx *ctx);
+int eif_gpu_adjoint_get(eif_ctx *ctx, int which, double *host);
+int eif_gpu_adjoint_add(eif_ctx *ctx, int which, const double *host);
+/* adds weight * d(fluid_gpu_wall_shear)/d(uy) to the uy cotangent on the
+ * device */
+int eif_gpu_adjoint_add_wall_shear(eif_ctx *ctx, double weight);
+
 int eif_gpu_synchronize(eif_ctx *ctx);
 
 /* IBM work counters since the last reset: out[0] box cells visited
