@@ -277,6 +277,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--poisson", choices=("partition", "thomas"), default="partition")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the built-in launch shapes")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the adjoint/forward cost measurement")
     ap.add_argument("--ensemble-steps", type=int, default=20,
@@ -312,6 +314,9 @@ def main():
     g.update_boundaries()
     g.set_particles(**{k: parts[k] for k in ("den", "a", "b", "x", "y", "az", "ux", "uy", "vz")})
     log("area fraction %.3f" % initparticles.area_fraction(parts, LY))
+    tuning = g.autotune() if not args.no_autotune else None
+    if tuning:
+        log("autotune: " + tuning)
     if args.mode == "graph":
         g.graph(True)
     g.run_steps(args.warmup)
@@ -401,6 +406,7 @@ def main():
         "roofline": roofline,
         "fluid_kernels_hbm": fluid,
         "step_mode": args.mode,
+        "autotune": tuning,
     }
     if graph_stats:
         out["graph"] = {"launches": graph_stats[0], "resumed_stops": graph_stats[1],

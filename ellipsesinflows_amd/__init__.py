@@ -54,7 +54,7 @@ EXPORTED = (
     "eif_gpu_graph_set_subiterations", "eif_gpu_ensemble_run_steps",
     "eif_gpu_tape_begin", "eif_gpu_step_fixed_dt", "eif_gpu_stage_adjoint",
     "eif_gpu_adjoint_get", "eif_gpu_adjoint_add", "fluid_gpu_wall_shear",
-    "eif_gpu_adjoint_add_wall_shear",
+    "eif_gpu_adjoint_add_wall_shear", "eif_gpu_autotune",
     "eif_gpu_ibm_stats", "eif_gpu_ibm_reference_iterations", "statistics_gpu_collect", "statistics_gpu_reset",
     "statistics_gpu_download",
     "eif_gpu_profile_enable", "eif_gpu_profile_select", "eif_gpu_profile_reset",
@@ -343,6 +343,13 @@ class Solver:
 
     # -- adjoint of one RK stage ---------------------------------------------
     ADJ_UX, ADJ_UY, ADJ_P, ADJ_PARTICLES, ADJ_FORCE, ADJ_RE = range(6)
+
+    def autotune(self, allow_ysolve_switch=False):
+        """Time the candidate launch shapes and keep the fastest; returns
+        the report string."""
+        buf = ctypes.create_string_buffer(1024)
+        self._check(self.lib.eif_gpu_autotune(self.h, int(bool(allow_ysolve_switch)), buf, 1024))
+        return buf.value.decode()
 
     def wall_shear(self):
         out = _D()

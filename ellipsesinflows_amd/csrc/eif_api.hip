@@ -1152,6 +1152,26 @@ static int graph_finish(Ctx *c, GraphEntry *e, const GraphStatus &st, double *dt
   return EIF_OK;
 }
 
+// mean time of `reps` launches queued by f, by events around them
+template <typename Fn>
+static int time_launches(Ctx *c, Fn f, int reps, double *ms) {
+  hipEvent_t a, b;
+  EIF_HIP(hipEventCreate(&a));
+  EIF_HIP(hipEventCreate(&b));
+  int rc = f();  // first launch untimed
+  if (!rc) rc = hipEventRecord(a, c->stream) == hipSuccess ? EIF_OK : EIF_ERR_HIP;
+  for (int k = 0; k < reps && !rc; k++) rc = f();
+  if (!rc) rc = hipEventRecord(b, c->stream) == hipSuccess ? EIF_OK : EIF_ERR_HIP;
+  float el = 0.f;
+  if (!rc && hipEventSynchronize(b) == hipSuccess && hipEventElapsedTime(&el, a, b) == hipSuccess)
+    *ms = el / reps;
+  else if (!rc)
+    rc = hip_fail(hipErrorUnknown, "timing events", __FILE__, __LINE__);
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return rc;
+}
+
 }  // namespace eif
 
 extern "C" {
@@ -1233,6 +1253,108 @@ int eif_gpu_adjoint_add(eif_ctx *c, int which, const double *host) {
     return EIF_ERR_ARG;
   }
   return adjoint_access(c, which, const_cast<double *>(host), true);
+}
+
+int eif_gpu_autotune(eif_ctx *c, int flags, char *report, int report_len) {
+  CHECK_CTX(c);
+  EIF_HIP(hipStreamSynchronize(c->stream));
+  const bool prof_was = c->prof->on;
+  c->prof->on = false;
+  const int parts_was = c->dt_adv_parts;
+  const bool dtv_was = c->dt_adv_valid;
+  std::string rep;
+  char buf[160];
+  int rc = EIF_OK;
+  // predictor rows per workgroup (same operations per cell for each)
+  {
+    double best = 1e30;
+    int pick = 0;
+    rep += "predictor_rows:";
+    for (int rows : {8, 16, 32}) {
+      c->pred_rows = rows;
+      double ms = 0.;
+      if ((rc = time_launches(c, [&] { return launch_predictor_probe(c); }, 10, &ms))) break;
+      snprintf(buf, sizeof(buf), " %d=%.4fms", rows, ms);
+      rep += buf;
+      if (ms < best) {
+        best = ms;
+        pick = rows;
+      }
+    }
+    c->pred_rows = pick;
+    snprintf(buf, sizeof(buf), " -> %d; ", pick);
+    rep += buf;
+  }
+  // projection + pressure rows per workgroup, on copies
+  double *ptmp = nullptr;
+  if (!rc && hipMalloc(&ptmp, c->L.p_len() * sizeof(double)) != hipSuccess)
+    rc = hip_fail(hipErrorOutOfMemory, "hipMalloc", __FILE__, __LINE__);
+  if (!rc) {
+    double best = 1e30;
+    int pick = 0;
+    rep += "projection_rows:";
+    for (int rows : {4, 8, 16}) {
+      c->cp_rows = rows;
+      double ms = 0.;
+      if ((rc = time_launches(c, [&] {
+             return launch_correct_pressure_on(c, 1, true, true, true, c->ux_spare, c->uy_spare,
+                                               ptmp);
+           }, 10, &ms)))
+        break;
+      snprintf(buf, sizeof(buf), " %d=%.4fms", rows, ms);
+      rep += buf;
+      if (ms < best) {
+        best = ms;
+        pick = rows;
+      }
+    }
+    c->cp_rows = pick;
+    snprintf(buf, sizeof(buf), " -> %d; ", pick);
+    rep += buf;
+  }
+  if (ptmp) (void)hipFree(ptmp);
+  // the y-solve: chunk length of the partitioned solve (and, with flags &
+  // 1, the Thomas sweep too); Thomas-mode contexts stay bit-exact unless
+  // flags allow the switch
+  if (!rc && (c->cfg.poisson_mode == EIF_POISSON_PARTITION || (flags & 1))) {
+    const int mode_was = c->cfg.poisson_mode, L_was = c->part_L;
+    double best = 1e30;
+    int pick_mode = mode_was, pick_L = L_was;
+    rep += "ysolve:";
+    std::vector<std::pair<int, int>> cands;
+    for (int L : {16, 32, 64})
+      if (c->ny % L == 0 && c->ny / L >= 2 && c->ny / L <= 315) cands.push_back({1, L});
+    if (flags & 1) cands.push_back({0, 0});
+    for (auto &cd : cands) {
+      c->cfg.poisson_mode = cd.first;
+      if (cd.first == 1 && (rc = partition_setup(c, cd.second))) break;
+      double ms = 0.;
+      if ((rc = time_launches(c, [&] { return launch_poisson(c, 1); }, 10, &ms))) break;
+      snprintf(buf, sizeof(buf), " %s=%.4fms", cd.first ? ("partition L=" + std::to_string(cd.second)).c_str() : "thomas", ms);
+      rep += buf;
+      if (ms < best) {
+        best = ms;
+        pick_mode = cd.first;
+        pick_L = cd.second;
+      }
+    }
+    if (!rc) {
+      c->cfg.poisson_mode = pick_mode;
+      if (pick_mode == 1 && pick_L != c->part_L) rc = partition_setup(c, pick_L);
+      snprintf(buf, sizeof(buf), " -> %s %d", pick_mode ? "partition" : "thomas", pick_L);
+      rep += buf;
+    } else {
+      c->cfg.poisson_mode = mode_was;
+    }
+  }
+  c->dt_adv_parts = parts_was;
+  c->dt_adv_valid = dtv_was;
+  c->prof->on = prof_was;
+  if (report && report_len > 0) {
+    strncpy(report, rep.c_str(), (size_t)report_len - 1);
+    report[report_len - 1] = 0;
+  }
+  return rc;
 }
 
 int eif_gpu_graph_enable(eif_ctx *c, int enable) {

@@ -103,7 +103,7 @@ int launch_bc_p(Ctx *c, double *p) {
 // after PRED_PREV stages.
 // ---------------------------------------------------------------------
 constexpr int PRED_BX = 256;
-constexpr int PRED_ROWS = 16;
+constexpr int PRED_ROWS = 16;  // default rows per workgroup (the tuner may pick 8 or 32)
 enum { PRED_STORE = 0, PRED_PREV = 1, PRED_SRC = 2 };
 
 struct PredArgs {
@@ -199,15 +199,15 @@ __device__ inline double src_y(const PredWin &w, const PredCol &g, double ext) {
   return (adv1 + adv2) + (dif1 + dif2) + ext;
 }
 
-template <bool P2, int MODE>
+template <bool P2, int MODE, int ROWS = PRED_ROWS>
 __global__ __launch_bounds__(PRED_BX) void k_predictor(PredArgs a) {
   if (halted(a.halt)) return;
   const Layout L = a.L;
   const int nx = L.nx, ny = L.ny;
   const int i = blockIdx.x * PRED_BX + threadIdx.x + 1;  // 1..nx
-  const int j0 = blockIdx.y * PRED_ROWS + 1;
+  const int j0 = blockIdx.y * ROWS + 1;
   if (i > nx || j0 > ny) return;
-  const int j1 = min(j0 + PRED_ROWS - 1, ny);
+  const int j1 = min(j0 + ROWS - 1, ny);
   const double dt = a.dscal[S_DT];
   const double aldt = a.al * dt, bedt = a.be * dt, gadt = a.ga * dt;
   const bool hx = i >= 2;  // ux(i) is an unknown for i = 2..nx
@@ -306,14 +306,26 @@ __global__ __launch_bounds__(PRED_BX) void k_predictor(PredArgs a) {
   }
 }
 
+template <int MODE, int ROWS>
+static void pred_launch_rows(Ctx *c, const PredArgs &a, hipStream_t st) {
+  dim3 grid((c->nx + PRED_BX - 1) / PRED_BX, (c->ny + ROWS - 1) / ROWS);
+  if (c->pow2_grid)
+    hipLaunchKernelGGL((k_predictor<true, MODE, ROWS>), grid, dim3(PRED_BX), 0, st, a);
+  else
+    hipLaunchKernelGGL((k_predictor<false, MODE, ROWS>), grid, dim3(PRED_BX), 0, st, a);
+}
+
+// rows per workgroup from the context (tuned, or the default); every
+// choice performs the same operations per cell
 template <int MODE>
 static void pred_launch(Ctx *c, const PredArgs &a, hipStream_t st) {
-  dim3 grid((c->nx + PRED_BX - 1) / PRED_BX, (c->ny + PRED_ROWS - 1) / PRED_ROWS);
-  if (c->pow2_grid)
-    hipLaunchKernelGGL((k_predictor<true, MODE>), grid, dim3(PRED_BX), 0, st, a);
-  else
-    hipLaunchKernelGGL((k_predictor<false, MODE>), grid, dim3(PRED_BX), 0, st, a);
+  switch (c->pred_rows) {
+    case 8: pred_launch_rows<MODE, 8>(c, a, st); break;
+    case 32: pred_launch_rows<MODE, 32>(c, a, st); break;
+    default: pred_launch_rows<MODE, PRED_ROWS>(c, a, st); break;
+  }
 }
+
 
 static PredArgs pred_args(Ctx *c, int rk) {
   PredArgs a;
@@ -330,6 +342,19 @@ static PredArgs pred_args(Ctx *c, int rk) {
   a.use_b = c->beta[rk] != 0.;
   a.halt = c->halt;
   return a;
+}
+
+// one predictor of a stage > 0 (the beta term from the kept pair) into the
+// spare pair, pointers untouched: the tuner's timing launch
+int launch_predictor_probe(Ctx *c) {
+  PredArgs a = pred_args(c, 1);
+  a.uxo = c->ux_alt;
+  a.uyo = c->uy_alt;
+  a.uxn = c->ux_spare;
+  a.uyn = c->uy_spare;
+  pred_launch<PRED_PREV>(c, a, c->stream);
+  EIF_HIP(hipGetLastError());
+  return EIF_OK;
 }
 
 // srcuxa/srcuya of the velocity they were last evaluated at, if the
@@ -389,9 +414,9 @@ int launch_predictor_prev(Ctx *c, int rk, hipStream_t st) {
 // values of ux, uy and p refreshed by the threads that own the source
 // interior values.
 // ---------------------------------------------------------------------
-constexpr int CP_ROWS = 8;   // rows per workgroup of k_correct_pressure
+constexpr int CP_ROWS = 8;   // default rows per workgroup of k_correct_pressure
 
-template <bool P2, bool DT>
+template <bool P2, bool DT, int ROWS = CP_ROWS>
 __global__ __launch_bounds__(256) void k_correct_pressure(
     Layout L, double *ux, double *uy, double *p, const double *psi, const double *dxc,
     const double *dscal, double ga, double dy, int do_correct, int do_pressure,
@@ -403,7 +428,7 @@ __global__ __launch_bounds__(256) void k_correct_pressure(
   // with DT: the advective time-step limit of decide_dt.c:26-48 over the
   // corrected velocities, as one partial minimum per workgroup
   double m = 5.e-2;
-  for (int j = blockIdx.y * CP_ROWS + 1; j <= ny && j <= (int)(blockIdx.y + 1) * CP_ROWS; j++) {
+  for (int j = blockIdx.y * ROWS + 1; j <= ny && j <= (int)(blockIdx.y + 1) * ROWS; j++) {
     if (i > nx) break;
     const double ps_c = psi[L.p(i, j)];
     if (do_correct) {
@@ -455,18 +480,35 @@ __global__ __launch_bounds__(256) void k_correct_pressure(
   }
 }
 
-int launch_correct_pressure(Ctx *c, int rk, bool correct, bool pressure, bool dt_fused) {
-  ProfScope ps(c, "correct_pressure");
-  dim3 grid((c->nx + 255) / 256, (c->ny + CP_ROWS - 1) / CP_ROWS);
+template <int ROWS>
+static int cp_launch(Ctx *c, int rk, bool correct, bool pressure, bool dt_fused, double *ux,
+                     double *uy, double *p) {
+  dim3 grid((c->nx + 255) / 256, (c->ny + ROWS - 1) / ROWS);
   const bool dt = dt_fused && correct && (long)grid.x * grid.y <= DT_PARTS_CAP;
-  auto kern = c->pow2_grid ? (dt ? k_correct_pressure<true, true> : k_correct_pressure<true, false>)
-                           : (dt ? k_correct_pressure<false, true> : k_correct_pressure<false, false>);
-  hipLaunchKernelGGL(kern, grid, dim3(256), 0, c->stream, c->L, c->ux, c->uy, c->p, c->psi,
+  auto kern = c->pow2_grid
+                  ? (dt ? k_correct_pressure<true, true, ROWS> : k_correct_pressure<true, false, ROWS>)
+                  : (dt ? k_correct_pressure<false, true, ROWS> : k_correct_pressure<false, false, ROWS>);
+  hipLaunchKernelGGL(kern, grid, dim3(256), 0, c->stream, c->L, ux, uy, p, c->psi,
                      c->dxc, c->dscal, c->gamma[rk], c->dy, (int)correct, (int)pressure,
                      c->dt_part, c->halt);
   EIF_HIP(hipGetLastError());
   c->dt_adv_parts = dt ? (int)(grid.x * grid.y) : 0;
   return EIF_OK;
+}
+
+// on explicit buffers (the tuner's timing launch uses copies)
+int launch_correct_pressure_on(Ctx *c, int rk, bool correct, bool pressure, bool dt_fused,
+                               double *ux, double *uy, double *p) {
+  ProfScope ps(c, "correct_pressure");
+  switch (c->cp_rows) {
+    case 4: return cp_launch<4>(c, rk, correct, pressure, dt_fused, ux, uy, p);
+    case 16: return cp_launch<16>(c, rk, correct, pressure, dt_fused, ux, uy, p);
+    default: return cp_launch<CP_ROWS>(c, rk, correct, pressure, dt_fused, ux, uy, p);
+  }
+}
+
+int launch_correct_pressure(Ctx *c, int rk, bool correct, bool pressure, bool dt_fused) {
+  return launch_correct_pressure_on(c, rk, correct, pressure, dt_fused, c->ux, c->uy, c->p);
 }
 
 // ---------------------------------------------------------------------

@@ -145,6 +145,8 @@ struct Ctx {
   unsigned char *th_flag = nullptr;  // [ny-1][nx] zero-pivot flags
   // partition solver tables
   int part_L = 0, part_P = 0;
+  // tuned launch shapes (eif_gpu_autotune); 0 is the built-in default
+  int pred_rows = 0, cp_rows = 0;
   double *pt_fwd = nullptr, *pt_u = nullptr, *pt_v = nullptr, *pt_w = nullptr;
   double *pt_bnd = nullptr, *pt_red = nullptr, *pt_lvl = nullptr;
   double *pt_ends = nullptr, *pt_g = nullptr;
@@ -249,6 +251,12 @@ int launch_materialise_src(Ctx *c);
 // dt_fused: also leave the next decide_dt's advective limit as partial minima
 int launch_correct_pressure(Ctx *c, int rk, bool correct, bool pressure,
                             bool dt_fused = false);
+int launch_correct_pressure_on(Ctx *c, int rk, bool correct, bool pressure, bool dt_fused,
+                               double *ux, double *uy, double *p);
+int launch_predictor_probe(Ctx *c);
+// the partitioned y-solve's tables for chunks of L rows (0: the default
+// choice); returns EIF_ERR_ARG when L does not split jtot within the limits
+int partition_setup(Ctx *c, int L);
 int launch_decide_dt(Ctx *c);
 int launch_diagnostics(Ctx *c, int which, double *out);
 int launch_wall_shear(Ctx *c, double *out);

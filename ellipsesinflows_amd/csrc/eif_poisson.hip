@@ -823,6 +823,8 @@ static int part_chunk(int ny) {
   return 0;
 }
 
+int partition_setup(Ctx *c, int L);
+
 int partition_layout(int ny, int *chunk, int *nchunks) {
   const int L = ny >= 3 ? part_chunk(ny) : 0;
   if (chunk) *chunk = L;
@@ -830,9 +832,20 @@ int partition_layout(int ny, int *chunk, int *nchunks) {
   return L ? 1 : 0;
 }
 
-static int part_init(Ctx *c) {
+static void part_free(Ctx *c) {
+  double **bufs[] = {&c->pt_fwd, &c->pt_u, &c->pt_v, &c->pt_red, &c->pt_w, &c->pt_lvl,
+                     &c->pt_bnd, &c->pt_ends};
+  for (auto b : bufs)
+    if (*b) {
+      (void)hipFree(*b);
+      *b = nullptr;
+    }
+  c->part_L = c->part_P = 0;
+}
+
+static int part_init(Ctx *c, int Lreq = 0) {
   const int nx = c->nx, ny = c->ny;
-  const int L = part_chunk(ny);
+  const int L = Lreq ? Lreq : part_chunk(ny);
   if (!L) return EIF_OK;  // partition mode unavailable for this jtot
   const int P = ny / L;
   c->part_L = L;
@@ -888,6 +901,17 @@ static int part_init(Ctx *c) {
   EIF_HIP(hipMalloc(&c->pt_bnd, (size_t)P * 2 * nx * sizeof(double)));
   EIF_HIP(hipMalloc(&c->pt_ends, (size_t)P * 2 * nx * sizeof(double)));
   return EIF_OK;
+}
+
+int partition_setup(Ctx *c, int L) {
+  if (L && !(L >= 8 && L <= 64 && (L & (L - 1)) == 0 && c->ny % L == 0 && c->ny / L >= 2 &&
+             c->ny / L <= PART_P_MAX)) {
+    set_error("partition chunk does not split jtot within the limits");
+    return EIF_ERR_ARG;
+  }
+  EIF_HIP(hipStreamSynchronize(c->stream));
+  part_free(c);
+  return part_init(c, L);
 }
 
 int poisson_init(Ctx *c) {

@@ -210,6 +210,16 @@ int eif_gpu_graph_set_subiterations(eif_ctx *ctx, const int K[3]);
  * three stages as K0*10000 + K1*100 + K2 */
 int eif_gpu_graph_stats(eif_ctx *ctx, long out[4]);
 
+/* Autotuner (no counterpart in the reference): times the candidate launch
+ * shapes on this context's grid and keeps the fastest: rows per workgroup
+ * of the predictor (8, 16, 32) and of the projection (4, 8, 16), which
+ * change no result, and the chunk length of the partitioned y-solve (16,
+ * 32, 64 rows where they divide jtot), which changes rounding only;
+ * flags & 1 also lets it switch between the Thomas sweep and the
+ * partitioned solve.  report (may be NULL) receives the timings and the
+ * choices.  Synchronising; leaves psi holding its last timing solve. */
+int eif_gpu_autotune(eif_ctx *ctx, int flags, char *report, int report_len);
+
 /* Ensembles of independent cases (no counterpart in the reference, which
  * runs one case per process): advances ncases contexts, of any grid sizes
  * and particle counts, by nsteps steps each.  concurrent = 0: one context
