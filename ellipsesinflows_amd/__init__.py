@@ -79,10 +79,11 @@ def partition_layout(jtot):
 def ensemble_run_steps(solvers, nsteps, concurrent=True):
     """Advance independent Solvers by nsteps each: concurrently (each step
     of every case one graph launch on its own stream, all launched before
-    any is waited for) or one after the other (the plain path)."""
+    any is waited for; concurrent=2: one launch of all cases' step graphs
+    nested in one graph) or one after the other (the plain path)."""
     lib = load_library()
     arr = (_VP * max(len(solvers), 1))(*[s.h for s in solvers])
-    rc = lib.eif_gpu_ensemble_run_steps(arr, len(solvers), int(nsteps), int(bool(concurrent)))
+    rc = lib.eif_gpu_ensemble_run_steps(arr, len(solvers), int(nsteps), int(concurrent))
     if rc != 0:
         raise EIFError("eif_gpu error %d: %s" % (rc, lib.eif_gpu_last_error().decode()))
 

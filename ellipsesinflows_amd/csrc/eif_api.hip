@@ -971,6 +971,7 @@ struct GraphEntry {
   PtrState at_gate[3];  // host pointer state right after each stage's gate
   int dt_parts;
   hipGraphExec_t exec = nullptr;
+  hipGraph_t graph = nullptr;  // kept: an ensemble step nests it as a child graph
 };
 
 struct GraphCache {
@@ -983,8 +984,10 @@ struct GraphCache {
 
 void graphs_free(Ctx *c) {
   if (!c->graphs) return;
-  for (auto &e : c->graphs->entries)
+  for (auto &e : c->graphs->entries) {
     if (e->exec) (void)hipGraphExecDestroy(e->exec);
+    if (e->graph) (void)hipGraphDestroy(e->graph);
+  }
   delete c->graphs;
   c->graphs = nullptr;
 }
@@ -1042,7 +1045,7 @@ static int capture_step(Ctx *c, const int K[3], GraphEntry *out) {
   }
   out->dt_parts = c->dt_adv_parts;
   const hipError_t ei = hipGraphInstantiate(&out->exec, graph, nullptr, nullptr, 0);
-  (void)hipGraphDestroy(graph);
+  out->graph = graph;
   if (ei != hipSuccess) return hip_fail(ei, "hipGraphInstantiate", __FILE__, __LINE__);
   c->graphs->captures++;
   return EIF_OK;
@@ -1087,8 +1090,10 @@ static int graph_prepare(Ctx *c, bool *ran, int sub[3]) {
   *ran = false;
   if (g->entries.size() >= 32) {
     EIF_HIP(hipStreamSynchronize(c->stream));
-    for (auto &e : g->entries)
+    for (auto &e : g->entries) {
       if (e->exec) (void)hipGraphExecDestroy(e->exec);
+      if (e->graph) (void)hipGraphDestroy(e->graph);
+    }
     g->entries.clear();
   }
   if (!c->bc_valid || !c->dt_adv_valid || g->K[0] == 0) {
@@ -1462,6 +1467,99 @@ int eif_gpu_run_steps(eif_ctx *c, int nsteps, double *dts, int *substeps) {
   return EIF_OK;
 }
 
+// Every case's step graph nested as a child of one graph: one launch per
+// ensemble step.  The children are independent; a stop in one case is
+// resumed on that case's own stream, and the merged graph is rebuilt only
+// when some case's captured graph changes.
+static int ensemble_merged(eif_ctx **ctxs, int ncases, int nsteps) {
+  std::vector<int> done(ncases, 0);
+  for (int k = 0; k < ncases; k++) {
+    bool ran = false;
+    int sub[3];
+    if (nsteps > 0) TRY(graph_prepare(ctxs[k], &ran, sub));
+    done[k] = ran ? 1 : 0;
+  }
+  // the merged launch runs on one stream: every case's own queued work
+  // (the eager steps above) is finished first
+  for (int k = 0; k < ncases; k++) EIF_HIP(hipStreamSynchronize(ctxs[k]->stream));
+  hipGraph_t parent = nullptr;
+  hipGraphExec_t pexec = nullptr;
+  std::vector<GraphEntry *> ent(ncases, nullptr), built(ncases, nullptr);
+  auto release = [&]() {
+    if (pexec) (void)hipGraphExecDestroy(pexec);
+    if (parent) (void)hipGraphDestroy(parent);
+    pexec = nullptr;
+    parent = nullptr;
+  };
+  int rc = EIF_OK;
+  for (;;) {
+    // cases still running; all launch together, so they share the count
+    bool any = false, changed = false;
+    for (int k = 0; k < ncases && !rc; k++) {
+      ent[k] = nullptr;
+      if (done[k] >= nsteps) {
+        changed = changed || built[k] != nullptr;
+        continue;
+      }
+      rc = graph_for(ctxs[k], &ent[k]);
+      any = true;
+      changed = changed || ent[k] != built[k];
+    }
+    if (rc || !any) break;
+    if (changed || !pexec) {
+      release();
+      if (hipGraphCreate(&parent, 0) != hipSuccess) {
+        rc = hip_fail(hipErrorUnknown, "hipGraphCreate", __FILE__, __LINE__);
+        break;
+      }
+      for (int k = 0; k < ncases && !rc; k++) {
+        built[k] = ent[k];
+        if (!ent[k]) continue;
+        hipGraphNode_t node;
+        const hipError_t e = hipGraphAddChildGraphNode(&node, parent, nullptr, 0, ent[k]->graph);
+        if (e != hipSuccess) rc = hip_fail(e, "hipGraphAddChildGraphNode", __FILE__, __LINE__);
+      }
+      if (!rc) {
+        const hipError_t e = hipGraphInstantiate(&pexec, parent, nullptr, nullptr, 0);
+        if (e != hipSuccess) rc = hip_fail(e, "hipGraphInstantiate", __FILE__, __LINE__);
+      }
+      if (rc) break;
+    }
+    // one launch for the whole ensemble step, on the first running case's stream
+    int first = 0;
+    while (!ent[first]) first++;
+    if (hipGraphLaunch(pexec, ctxs[first]->stream) != hipSuccess) {
+      rc = hip_fail(hipErrorUnknown, "hipGraphLaunch", __FILE__, __LINE__);
+      break;
+    }
+    // every case's record is written by the end of the merged launch
+    if (hipStreamSynchronize(ctxs[first]->stream) != hipSuccess) {
+      rc = hip_fail(hipErrorUnknown, "hipStreamSynchronize", __FILE__, __LINE__);
+      break;
+    }
+    for (int k = 0; k < ncases && !rc; k++) {
+      if (!ent[k]) continue;
+      Ctx *c = ctxs[k];
+      c->graphs->launches++;
+      const unsigned seq = ++c->graphs->seq;
+      GraphStatus st;
+      if ((rc = graph_wait(c, seq, &st))) break;
+      double dt;
+      int sub[3];
+      if ((rc = graph_finish(c, ent[k], st, &dt, sub))) break;
+      // a resumption ran on this case's stream, ahead of the next merged launch
+      if (st.halt && hipStreamSynchronize(c->stream) != hipSuccess)
+        rc = hip_fail(hipErrorUnknown, "hipStreamSynchronize", __FILE__, __LINE__);
+      done[k]++;
+    }
+    if (rc) break;
+  }
+  for (int k = 0; k < ncases; k++) (void)hipStreamSynchronize(ctxs[k]->stream);
+  release();
+  for (int k = 0; k < ncases; k++) ctxs[k]->dt_adv_valid = ctxs[k]->dt_adv_parts > 0;
+  return rc;
+}
+
 int eif_gpu_ensemble_run_steps(eif_ctx **ctxs, int ncases, int nsteps, int concurrent) {
   if (!ctxs || ncases < 0 || nsteps < 0) {
     set_error("invalid ensemble arguments");
@@ -1474,6 +1572,7 @@ int eif_gpu_ensemble_run_steps(eif_ctx **ctxs, int ncases, int nsteps, int concu
   }
   for (int k = 0; k < ncases; k++)
     if (!ctxs[k]->graph_on) TRY(eif_gpu_graph_enable(ctxs[k], 1));
+  if (concurrent == 2) return ensemble_merged(ctxs, ncases, nsteps);
   std::vector<int> done(ncases, 0);
   for (int k = 0; k < ncases; k++) {
     bool ran = false;

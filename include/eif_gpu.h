@@ -226,8 +226,10 @@ int eif_gpu_autotune(eif_ctx *ctx, int flags, char *report, int report_len);
  * after the other through eif_gpu_run_steps (the plain path run
  * sequentially); 1: step by step across the ensemble, each context's step
  * one captured graph on its own stream, all of a step's graphs launched
- * before the host waits for any (graph mode is switched on for them).
- * Results equal each context's plain run bit for bit.  Synchronising. */
+ * before the host waits for any (graph mode is switched on for them); 2:
+ * as 1, with every context's step graph nested in one graph, so that an
+ * ensemble step is a single launch.  Results equal each context's plain
+ * run bit for bit.  Synchronising. */
 int eif_gpu_ensemble_run_steps(eif_ctx **ctxs, int ncases, int nsteps, int concurrent);
 
 /* Adjoint of one RK stage (no counterpart in the reference).  The stage

@@ -51,11 +51,16 @@ def test_concurrent_ensemble_equals_each_case_alone():
         assert np.array_equal(g.particles(), alone[k][1]), k
         launches, resumes, captures, kk = g.graph_stats()
         assert launches >= steps - 1
-    # the sequential form gives the same again
+    # the sequential form, and one merged graph launch per ensemble step,
+    # give the same again
     for g in solvers:
         g.close()
-    solvers = [_make(cfg, parts) for cfg, parts in sel]
-    eif.ensemble_run_steps(solvers, steps, concurrent=False)
-    for k, g in enumerate(solvers):
-        assert np.array_equal(g.particles(), alone[k][1]), k
-        g.close()
+    for mode in (0, 2):
+        solvers = [_make(cfg, parts) for cfg, parts in sel]
+        eif.ensemble_run_steps(solvers, steps, concurrent=mode)
+        for k, g in enumerate(solvers):
+            f = g.download()
+            for name in ("ux", "uy", "p"):
+                assert np.array_equal(f[name], alone[k][0][name]), (mode, k, name)
+            assert np.array_equal(g.particles(), alone[k][1]), (mode, k)
+            g.close()
