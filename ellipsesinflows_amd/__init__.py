@@ -76,11 +76,15 @@ def partition_layout(jtot):
     return L.value, P.value
 
 
+ENSEMBLE_SEQUENTIAL, ENSEMBLE_STREAMS, ENSEMBLE_ONE_GRAPH, ENSEMBLE_BATCHED = 0, 1, 2, 3
+
+
 def ensemble_run_steps(solvers, nsteps, concurrent=True):
-    """Advance independent Solvers by nsteps each: concurrently (each step
-    of every case one graph launch on its own stream, all launched before
-    any is waited for; concurrent=2: one launch of all cases' step graphs
-    nested in one graph) or one after the other (the plain path)."""
+    """Advance independent Solvers by nsteps each: one after the other
+    through the plain path (0/False), each step of every case one graph
+    launch on its own stream (1/True), all cases' step graphs nested in one
+    graph (2), or batched launches, each kernel of a step launched once for
+    all cases (3)."""
     lib = load_library()
     arr = (_VP * max(len(solvers), 1))(*[s.h for s in solvers])
     rc = lib.eif_gpu_ensemble_run_steps(arr, len(solvers), int(nsteps), int(concurrent))

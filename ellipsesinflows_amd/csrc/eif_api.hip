@@ -7,8 +7,11 @@
 #include <cstring>
 #include <algorithm>
 #include <memory>
+#include <mutex>
 
 namespace eif {
+
+thread_local Recorder *g_rec = nullptr;
 
 static thread_local std::string g_err;
 
@@ -990,7 +993,10 @@ struct GraphCache {
   unsigned seq = 0;  // number of the last graph step launched (device counter)
 };
 
+void batch_forget(Ctx *c);
+
 void graphs_free(Ctx *c) {
+  batch_forget(c);
   if (!c->graphs) return;
   for (auto &e : c->graphs->entries) {
     if (e->exec) (void)hipGraphExecDestroy(e->exec);
@@ -998,6 +1004,31 @@ void graphs_free(Ctx *c) {
   }
   delete c->graphs;
   c->graphs = nullptr;
+}
+
+// The kernels of one graph step queued on the context's streams: decide_dt,
+// the three stages with K[rk] sub-iterations each and a gate after them, the
+// particle update and the step's record.  at_gate: the host pointer state
+// right after each stage's gate.
+static int queue_graph_step(Ctx *c, const int K[3], PtrState at_gate[3]) {
+  TRY(launch_decide_dt(c));
+  for (int rk = 0; rk < 3; rk++) {
+    // one batch per stage: the next predictor follows it, so stages 1 and
+    // 2 start on the side stream (stage_start's side_start form)
+    TRY(stage_start(c, rk, rk > 0, rk > 0));
+    for (int s = 0; s < K[rk]; s++) {
+      TRY(launch_substep_forces(c, s));
+      TRY(launch_increment(c, rk, s));
+    }
+    TRY(launch_stage_gate(c, rk, K[rk]));
+    at_gate[rk] = save_ptrs(c);
+    if (rk < 2) {
+      EIF_HIP(hipEventRecord(c->ev_fork, c->stream));
+      TRY(launch_predictor_prev(c, rk + 1));
+    }
+  }
+  TRY(launch_update_particles(c));
+  return launch_graph_status(c);
 }
 
 static int capture_step(Ctx *c, const int K[3], GraphEntry *out) {
@@ -1011,32 +1042,8 @@ static int capture_step(Ctx *c, const int K[3], GraphEntry *out) {
   c->prof->on = false;
   hipGraph_t graph = nullptr;
   EIF_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-  int rc = EIF_OK;
-  do {
-    if ((rc = launch_decide_dt(c))) break;
-    out->dt_parts = 0;
-    for (int rk = 0; rk < 3 && !rc; rk++) {
-      // one batch per stage: the next predictor follows it, so stages 1 and
-      // 2 start on the side stream (stage_start's side_start form)
-      if ((rc = stage_start(c, rk, rk > 0, rk > 0))) break;
-      for (int s = 0; s < K[rk] && !rc; s++) {
-        rc = launch_substep_forces(c, s);
-        if (!rc) rc = launch_increment(c, rk, s);
-      }
-      if (rc || (rc = launch_stage_gate(c, rk, K[rk]))) break;
-      out->at_gate[rk] = save_ptrs(c);
-      if (rk < 2) {
-        if (hipEventRecord(c->ev_fork, c->stream) != hipSuccess) {
-          rc = hip_fail(hipErrorUnknown, "hipEventRecord", __FILE__, __LINE__);
-          break;
-        }
-        rc = launch_predictor_prev(c, rk + 1);
-      }
-    }
-    if (rc) break;
-    if ((rc = launch_update_particles(c))) break;
-    rc = launch_graph_status(c);
-  } while (0);
+  out->dt_parts = 0;
+  const int rc = queue_graph_step(c, K, out->at_gate);
   const hipError_t e = hipStreamEndCapture(c->stream, &graph);
   c->prof->on = prof_was;
   const PtrState end = save_ptrs(c);
@@ -1163,6 +1170,299 @@ static int graph_finish(Ctx *c, GraphEntry *e, const GraphStatus &st, double *dt
     }
   }
   return EIF_OK;
+}
+
+// ---------------------------------------------------------------------
+// Batched ensemble steps.  Every case's graph step is recorded launch by
+// launch (go<B> with a Recorder set: the batched kernel, the case's grid,
+// block and LDS bytes, and its arguments), then replayed as batched
+// launches: at each position of the cases' launch sequences, the cases whose
+// launch there runs the same kernel body with the same block and LDS size
+// share one launch (k_batch), each reading its own arguments from a
+// descriptor array.  Each case's launches keep their order, cases are
+// independent, and a kernel body does per case exactly what its own launch
+// does, so every case's results are those of its own graph step.  The
+// positions are captured on one stream as one graph per ensemble step.
+// ---------------------------------------------------------------------
+struct BatchGraph {
+  std::vector<Ctx *> cases;
+  std::vector<GraphEntry> entries;  // per case: K, start and gate pointer states, dt parts
+  int K[3] = {0, 0, 0};
+  char *arena = nullptr;            // the descriptor arrays of every launch
+  hipStream_t stream = nullptr;     // where the batched step runs
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int nodes = 0;                    // batched launches per ensemble step
+};
+
+static void batch_free(BatchGraph *b) {
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->arena) (void)hipFree(b->arena);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  b->arena = nullptr;
+  b->stream = nullptr;
+}
+
+// the batched steps built so far, kept between calls (a context's graph
+// state reset drops every batch it belongs to)
+static std::mutex g_batch_mu;
+static std::vector<std::unique_ptr<BatchGraph>> g_batches;
+constexpr size_t BATCH_CACHE = 16;
+
+void batch_forget(Ctx *c) {
+  std::lock_guard<std::mutex> lk(g_batch_mu);
+  for (size_t k = 0; k < g_batches.size();) {
+    BatchGraph *b = g_batches[k].get();
+    if (std::find(b->cases.begin(), b->cases.end(), c) != b->cases.end()) {
+      batch_free(b);
+      g_batches.erase(g_batches.begin() + k);
+    } else {
+      k++;
+    }
+  }
+}
+
+// one case's graph step as recorded launches.  The step is queued inside a
+// stream capture that receives nothing but the streams' event joins: a
+// launch that went past the recorder would show there as a node.
+static int record_case(Ctx *c, const int K[3], GraphEntry *ge, std::vector<LaunchRec> *out) {
+  c->dt_adv_valid = c->dt_adv_parts > 0;
+  const PtrState start = save_ptrs(c);
+  ge->start = start;
+  for (int k = 0; k < 3; k++) ge->K[k] = K[k];
+  const bool prof_was = c->prof->on;
+  c->prof->on = false;
+  Recorder rec;
+  EIF_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  g_rec = &rec;
+  const int rc = queue_graph_step(c, K, ge->at_gate);
+  g_rec = nullptr;
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(c->stream, &g);
+  c->prof->on = prof_was;
+  const PtrState end = save_ptrs(c);
+  load_ptrs(c, start);
+  size_t nn = 0;
+  hipError_t eg = g ? hipGraphGetNodes(g, nullptr, &nn) : hipSuccess;
+  std::vector<hipGraphNode_t> nodes(nn);
+  if (eg == hipSuccess && nn) eg = hipGraphGetNodes(g, nodes.data(), &nn);
+  int stray = 0;
+  for (size_t k = 0; k < nn && eg == hipSuccess; k++) {
+    hipGraphNodeType t;
+    if ((eg = hipGraphNodeGetType(nodes[k], &t)) != hipSuccess) break;
+    if (t == hipGraphNodeTypeKernel || t == hipGraphNodeTypeMemset ||
+        t == hipGraphNodeTypeMemcpy)
+      stray++;
+  }
+  if (g) (void)hipGraphDestroy(g);
+  TRY(rc);
+  if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture", __FILE__, __LINE__);
+  if (eg != hipSuccess) return hip_fail(eg, "hipGraphGetNodes", __FILE__, __LINE__);
+  if (stray) {
+    set_error("batched step: " + std::to_string(stray) + " operation(s) bypassed the recorder");
+    return EIF_ERR_STATE;
+  }
+  if (!same_ptrs(start, end)) {
+    set_error("batched step: the velocity buffers do not return to their assignment");
+    return EIF_ERR_STATE;
+  }
+  ge->dt_parts = c->dt_adv_parts;
+  *out = std::move(rec.recs);
+  return EIF_OK;
+}
+
+static size_t round_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+// records every case's step with K sub-iterations per stage and captures
+// the batched replay on stream bs
+static int batch_build(const std::vector<Ctx *> &cs, const int K[3], BatchGraph *b) {
+  const int n = (int)cs.size();
+  if (n < 1 || n > BATCH_MAX) {
+    set_error("batched step: 1 to 64 cases per batch");
+    return EIF_ERR_ARG;
+  }
+  b->cases = cs;
+  b->entries.assign(n, GraphEntry());
+  for (int r = 0; r < 3; r++) b->K[r] = K[r];
+  std::vector<std::vector<LaunchRec>> recs(n);
+  size_t maxlen = 0;
+  for (int k = 0; k < n; k++) {
+    TRY(record_case(cs[k], K, &b->entries[k], &recs[k]));
+    maxlen = std::max(maxlen, recs[k].size());
+  }
+  struct Node {
+    const void *fn;
+    dim3 block;
+    size_t shm, off;
+    BatchMap map;
+    unsigned total;
+  };
+  std::vector<Node> nodes;
+  std::vector<char> host;
+  std::vector<char> used(n);
+  for (size_t pos = 0; pos < maxlen; pos++) {
+    std::fill(used.begin(), used.end(), 0);
+    for (int k = 0; k < n; k++) {
+      if (used[k] || pos >= recs[k].size()) continue;
+      const LaunchRec &r0 = recs[k][pos];
+      std::vector<int> grp;
+      for (int j = k; j < n; j++) {
+        if (used[j] || pos >= recs[j].size()) continue;
+        const LaunchRec &r = recs[j][pos];
+        if (r.fn == r0.fn && r.block.x == r0.block.x && r.block.y == r0.block.y &&
+            r.block.z == r0.block.z && r.shm == r0.shm && r.pack.size() == r0.pack.size()) {
+          grp.push_back(j);
+          used[j] = 1;
+        }
+      }
+      Node nd;
+      std::memset(&nd.map, 0, sizeof nd.map);
+      nd.fn = r0.fn;
+      nd.block = r0.block;
+      nd.shm = r0.shm;
+      const size_t stride = round_up(r0.pack.size(), 16);
+      nd.off = round_up(host.size(), 256);
+      host.resize(nd.off + stride * grp.size());
+      nd.map.n = (int)grp.size();
+      nd.map.stride = (unsigned)stride;
+      unsigned long long total = 0;
+      for (size_t g = 0; g < grp.size(); g++) {
+        const LaunchRec &r = recs[grp[g]][pos];
+        std::memcpy(host.data() + nd.off + g * stride, r.pack.data(), r.pack.size());
+        nd.map.start[g] = (unsigned)total;
+        nd.map.gx[g] = r.grid.x;
+        nd.map.gy[g] = r.grid.y;
+        nd.map.gz[g] = r.grid.z;
+        total += (unsigned long long)r.grid.x * r.grid.y * r.grid.z;
+      }
+      if (total >= (1ull << 31)) {
+        set_error("batched step: too many workgroups in one launch");
+        return EIF_ERR_CAPACITY;
+      }
+      nd.map.start[grp.size()] = (unsigned)total;
+      nd.total = (unsigned)total;
+      if (total > 0) nodes.push_back(nd);
+    }
+  }
+  EIF_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+  const hipStream_t bs = b->stream;
+  EIF_HIP(hipMalloc(&b->arena, std::max<size_t>(host.size(), 256)));
+  EIF_HIP(hipMemcpy(b->arena, host.data(), host.size(), hipMemcpyHostToDevice));
+  EIF_HIP(hipStreamBeginCapture(bs, hipStreamCaptureModeThreadLocal));
+  hipError_t el = hipSuccess;
+  for (size_t k = 0; k < nodes.size() && el == hipSuccess; k++) {
+    Node &nd = nodes[k];
+    const char *packs = b->arena + nd.off;
+    void *args[] = {(void *)&packs, (void *)&nd.map};
+    el = hipLaunchKernel(nd.fn, dim3(nd.total), nd.block, args, nd.shm, bs);
+  }
+  const hipError_t e = hipStreamEndCapture(bs, &b->graph);
+  if (el != hipSuccess) return hip_fail(el, "hipLaunchKernel", __FILE__, __LINE__);
+  if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture", __FILE__, __LINE__);
+  EIF_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  b->nodes = (int)nodes.size();
+  return EIF_OK;
+}
+
+// whether b replays the cases' next step: the same cases, K and states
+static bool batch_matches(const BatchGraph &b, const std::vector<Ctx *> &cs, const int K[3]) {
+  if (b.cases != cs) return false;
+  for (int r = 0; r < 3; r++)
+    if (b.K[r] != K[r]) return false;
+  for (size_t k = 0; k < cs.size(); k++)
+    if (!same_ptrs(b.entries[k].start, save_ptrs(cs[k])) ||
+        b.entries[k].dt_parts != cs[k]->dt_adv_parts)
+      return false;
+  return true;
+}
+
+// nsteps graph steps of every case, as batched ensemble steps of up to
+// BATCH_MAX cases each (chunks run on their own streams); *nodes_out:
+// batched launches per ensemble step of the last chunk
+static int ensemble_batched(eif_ctx **ctxs, int ncases, int nsteps, int *nodes_out) {
+  std::vector<int> done(ncases, 0);
+  for (int k = 0; k < ncases; k++) {
+    bool ran = false;
+    int sub[3];
+    if (nsteps > 0) TRY(graph_prepare(ctxs[k], &ran, sub));
+    done[k] = ran ? 1 : 0;
+  }
+  // the batched launches run on their own streams, after each case's queued work
+  for (int k = 0; k < ncases; k++) EIF_HIP(hipStreamSynchronize(ctxs[k]->stream));
+  std::lock_guard<std::mutex> lk(g_batch_mu);
+  int rc = EIF_OK;
+  while (!rc) {
+    std::vector<Ctx *> run;
+    int K[3] = {0, 0, 0};
+    for (int k = 0; k < ncases; k++)
+      if (done[k] < nsteps) {
+        run.push_back(ctxs[k]);
+        for (int r = 0; r < 3; r++) K[r] = std::max(K[r], ctxs[k]->graphs->K[r]);
+      }
+    if (run.empty()) break;
+    std::vector<BatchGraph *> step;
+    for (size_t k0 = 0; k0 < run.size() && !rc; k0 += BATCH_MAX) {
+      const std::vector<Ctx *> cs(run.begin() + k0,
+                                  run.begin() + std::min(run.size(), k0 + BATCH_MAX));
+      BatchGraph *b = nullptr;
+      for (auto &x : g_batches)
+        if (batch_matches(*x, cs, K)) b = x.get();
+      if (!b) {
+        // every cached step not launched in this step is idle (each step
+        // is waited for before the next)
+        for (size_t j = 0; j < g_batches.size() && g_batches.size() >= BATCH_CACHE;) {
+          if (std::find(step.begin(), step.end(), g_batches[j].get()) == step.end()) {
+            batch_free(g_batches[j].get());
+            g_batches.erase(g_batches.begin() + j);
+          } else {
+            j++;
+          }
+        }
+        g_batches.emplace_back(new BatchGraph());
+        b = g_batches.back().get();
+        rc = batch_build(cs, K, b);
+        if (rc) {
+          batch_free(b);
+          g_batches.pop_back();
+          break;
+        }
+        for (Ctx *c : cs) c->graphs->captures++;
+      }
+      if (hipGraphLaunch(b->exec, b->stream) != hipSuccess)
+        rc = hip_fail(hipErrorUnknown, "hipGraphLaunch", __FILE__, __LINE__);
+      else
+        step.push_back(b);
+      if (nodes_out) *nodes_out = b->nodes;
+    }
+    for (BatchGraph *b : step)
+      if (!rc && hipStreamSynchronize(b->stream) != hipSuccess)
+        rc = hip_fail(hipErrorUnknown, "hipStreamSynchronize", __FILE__, __LINE__);
+    for (BatchGraph *b : step) {
+      for (size_t j = 0; j < b->cases.size() && !rc; j++) {
+        Ctx *c = b->cases[j];
+        c->graphs->launches++;
+        const unsigned seq = ++c->graphs->seq;
+        GraphStatus st;
+        if ((rc = graph_wait(c, seq, &st))) break;
+        double dt;
+        int sub[3];
+        if ((rc = graph_finish(c, &b->entries[j], st, &dt, sub))) break;
+        // a resumption ran on the case's own stream, ahead of the next batch
+        if (st.halt && hipStreamSynchronize(c->stream) != hipSuccess)
+          rc = hip_fail(hipErrorUnknown, "hipStreamSynchronize", __FILE__, __LINE__);
+        const int k = (int)(std::find(ctxs, ctxs + ncases, c) - ctxs);
+        done[k]++;
+      }
+    }
+  }
+  for (auto &x : g_batches)
+    if (x->stream) (void)hipStreamSynchronize(x->stream);
+  for (int k = 0; k < ncases; k++) ctxs[k]->dt_adv_valid = ctxs[k]->dt_adv_parts > 0;
+  return rc;
 }
 
 // mean time of `reps` launches queued by f, by events around them
@@ -1574,6 +1874,16 @@ int eif_gpu_ensemble_run_steps(eif_ctx **ctxs, int ncases, int nsteps, int concu
     return EIF_ERR_ARG;
   }
   for (int k = 0; k < ncases; k++) CHECK_CTX(ctxs[k]);
+  for (int k = 0; k < ncases; k++)
+    for (int j = 0; j < k; j++)
+      if (ctxs[j] == ctxs[k]) {
+        set_error("a context appears twice in the ensemble");
+        return EIF_ERR_ARG;
+      }
+  if (concurrent < 0 || concurrent > 3) {
+    set_error("concurrent must be 0 (sequential), 1 (streams), 2 (one graph) or 3 (batched)");
+    return EIF_ERR_ARG;
+  }
   if (!concurrent) {
     for (int k = 0; k < ncases; k++) TRY(eif_gpu_run_steps(ctxs[k], nsteps, nullptr, nullptr));
     return EIF_OK;
@@ -1581,6 +1891,7 @@ int eif_gpu_ensemble_run_steps(eif_ctx **ctxs, int ncases, int nsteps, int concu
   for (int k = 0; k < ncases; k++)
     if (!ctxs[k]->graph_on) TRY(eif_gpu_graph_enable(ctxs[k], 1));
   if (concurrent == 2) return ensemble_merged(ctxs, ncases, nsteps);
+  if (concurrent == 3) return ensemble_batched(ctxs, ncases, nsteps, nullptr);
   std::vector<int> done(ncases, 0);
   for (int k = 0; k < ncases; k++) {
     bool ran = false;

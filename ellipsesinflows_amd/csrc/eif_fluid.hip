@@ -20,9 +20,11 @@ namespace eif {
 // boundary conditions (one process: the y-halo exchange is a periodic
 // copy of rows ny -> 0 and 1 -> ny+1, src/parallel/halo.c:7-31)
 // ---------------------------------------------------------------------
-__global__ void k_bc_ux(Layout L, double *ux) {
+struct BcUxB {
+  static constexpr int T = 1024, M = 0;
+  static __device__ __forceinline__ void run(const VB &vb, Layout L, double *ux) {
   const int nx = L.nx, ny = L.ny;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = vb.bx * blockDim.x + threadIdx.x;
   if (t >= 2 && t <= nx) {  // halo rows, i = 2..nx
     ux[L.ux(t, 0)] = ux[L.ux(t, ny)];
     ux[L.ux(t, ny + 1)] = ux[L.ux(t, 1)];
@@ -32,10 +34,16 @@ __global__ void k_bc_ux(Layout L, double *ux) {
     ux[L.ux(nx + 1, t)] = 0.;
   }
 }
+};
+__global__ void k_bc_ux(Layout L, double *ux) {
+  BcUxB::run(VB::here(), L, ux);
+}
 
-__global__ void k_bc_uy(Layout L, double *uy, double wxm, double wxp) {
+struct BcUyB {
+  static constexpr int T = 1024, M = 0;
+  static __device__ __forceinline__ void run(const VB &vb, Layout L, double *uy, double wxm, double wxp) {
   const int nx = L.nx, ny = L.ny;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = vb.bx * blockDim.x + threadIdx.x;
   if (t >= 1 && t <= nx) {
     uy[L.p(t, 0)] = uy[L.p(t, ny)];
     uy[L.p(t, ny + 1)] = uy[L.p(t, 1)];
@@ -45,12 +53,18 @@ __global__ void k_bc_uy(Layout L, double *uy, double wxm, double wxp) {
     uy[L.p(nx + 1, t)] = wxp;
   }
 }
+};
+__global__ void k_bc_uy(Layout L, double *uy, double wxm, double wxp) {
+  BcUyB::run(VB::here(), L, uy, wxm, wxp);
+}
 
 // Neumann walls after the halo copy; the wall values of the halo rows
 // read the interior rows they were copied from, so one pass suffices.
-__global__ void k_bc_p(Layout L, double *p) {
+struct BcPB {
+  static constexpr int T = 1024, M = 0;
+  static __device__ __forceinline__ void run(const VB &vb, Layout L, double *p) {
   const int nx = L.nx, ny = L.ny;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = vb.bx * blockDim.x + threadIdx.x;
   if (t >= 1 && t <= nx) {
     p[L.p(t, 0)] = p[L.p(t, ny)];
     p[L.p(t, ny + 1)] = p[L.p(t, 1)];
@@ -61,6 +75,10 @@ __global__ void k_bc_p(Layout L, double *p) {
     p[L.p(nx + 1, t)] = p[L.p(nx, src)];
   }
 }
+};
+__global__ void k_bc_p(Layout L, double *p) {
+  BcPB::run(VB::here(), L, p);
+}
 
 static int bc_grid(const Ctx *c) {
   const int n = (c->nx + 2 > c->ny + 2 ? c->nx + 2 : c->ny + 2);
@@ -69,20 +87,20 @@ static int bc_grid(const Ctx *c) {
 
 int launch_bc_ux(Ctx *c, double *ux) {
   ProfScope ps(c, "bc");
-  hipLaunchKernelGGL(k_bc_ux, dim3(bc_grid(c)), dim3(256), 0, c->stream, c->L, ux);
+  go<BcUxB>(k_bc_ux, dim3(bc_grid(c)), dim3(256), 0, c->stream, c->L, ux);
   EIF_HIP(hipGetLastError());
   return EIF_OK;
 }
 int launch_bc_uy(Ctx *c, double *uy) {
   ProfScope ps(c, "bc");
-  hipLaunchKernelGGL(k_bc_uy, dim3(bc_grid(c)), dim3(256), 0, c->stream, c->L, uy,
+  go<BcUyB>(k_bc_uy, dim3(bc_grid(c)), dim3(256), 0, c->stream, c->L, uy,
                      c->cfg.uy_wall_xm, c->cfg.uy_wall_xp);
   EIF_HIP(hipGetLastError());
   return EIF_OK;
 }
 int launch_bc_p(Ctx *c, double *p) {
   ProfScope ps(c, "bc");
-  hipLaunchKernelGGL(k_bc_p, dim3(bc_grid(c)), dim3(256), 0, c->stream, c->L, p);
+  go<BcPB>(k_bc_p, dim3(bc_grid(c)), dim3(256), 0, c->stream, c->L, p);
   EIF_HIP(hipGetLastError());
   return EIF_OK;
 }
@@ -200,12 +218,14 @@ __device__ inline double src_y(const PredWin &w, const PredCol &g, double ext) {
 }
 
 template <bool P2, int MODE, int ROWS = PRED_ROWS>
-__global__ __launch_bounds__(PRED_BX) void k_predictor(PredArgs a) {
+struct PredictorB {
+  static constexpr int T = PRED_BX, M = 0;
+  static __device__ __forceinline__ void run(const VB &vb, PredArgs a) {
   if (halted(a.halt)) return;
   const Layout L = a.L;
   const int nx = L.nx, ny = L.ny;
-  const int i = blockIdx.x * PRED_BX + threadIdx.x + 1;  // 1..nx
-  const int j0 = blockIdx.y * ROWS + 1;
+  const int i = vb.bx * PRED_BX + threadIdx.x + 1;  // 1..nx
+  const int j0 = vb.by * ROWS + 1;
   if (i > nx || j0 > ny) return;
   const int j1 = min(j0 + ROWS - 1, ny);
   const double dt = a.dscal[S_DT];
@@ -305,14 +325,19 @@ __global__ __launch_bounds__(PRED_BX) void k_predictor(PredArgs a) {
     pm_c = pc_c;
   }
 }
+};
+template <bool P2, int MODE, int ROWS = PRED_ROWS>
+__global__ __launch_bounds__(PRED_BX) void k_predictor(PredArgs a) {
+  PredictorB<P2, MODE, ROWS>::run(VB::here(), a);
+}
 
 template <int MODE, int ROWS>
 static void pred_launch_rows(Ctx *c, const PredArgs &a, hipStream_t st) {
   dim3 grid((c->nx + PRED_BX - 1) / PRED_BX, (c->ny + ROWS - 1) / ROWS);
   if (c->pow2_grid)
-    hipLaunchKernelGGL((k_predictor<true, MODE, ROWS>), grid, dim3(PRED_BX), 0, st, a);
+    go<PredictorB<true, MODE, ROWS>>(k_predictor<true, MODE, ROWS>, grid, dim3(PRED_BX), 0, st, a);
   else
-    hipLaunchKernelGGL((k_predictor<false, MODE, ROWS>), grid, dim3(PRED_BX), 0, st, a);
+    go<PredictorB<false, MODE, ROWS>>(k_predictor<false, MODE, ROWS>, grid, dim3(PRED_BX), 0, st, a);
 }
 
 // rows per workgroup from the context (tuned, or the default); every
@@ -417,18 +442,20 @@ int launch_predictor_prev(Ctx *c, int rk, hipStream_t st) {
 constexpr int CP_ROWS = 8;   // default rows per workgroup of k_correct_pressure
 
 template <bool P2, bool DT, int ROWS = CP_ROWS>
-__global__ __launch_bounds__(256) void k_correct_pressure(
+struct CorrectPressureB {
+  static constexpr int T = 256, M = 0;
+  static __device__ __forceinline__ void run(const VB &vb, 
     Layout L, double *ux, double *uy, double *p, const double *psi, const double *dxc,
     const double *dscal, double ga, double dy, int do_correct, int do_pressure,
     double *dt_part, const int *halt) {
   if (halted(halt)) return;
   const int nx = L.nx, ny = L.ny;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int i = vb.bx * blockDim.x + threadIdx.x + 1;
   const double dt = dscal[S_DT];
   // with DT: the advective time-step limit of decide_dt.c:26-48 over the
   // corrected velocities, as one partial minimum per workgroup
   double m = 5.e-2;
-  for (int j = blockIdx.y * ROWS + 1; j <= ny && j <= (int)(blockIdx.y + 1) * ROWS; j++) {
+  for (int j = vb.by * ROWS + 1; j <= ny && j <= (int)(vb.by + 1) * ROWS; j++) {
     if (i > nx) break;
     const double ps_c = psi[L.p(i, j)];
     if (do_correct) {
@@ -476,21 +503,41 @@ __global__ __launch_bounds__(256) void k_correct_pressure(
       if ((int)threadIdx.x < s) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + s]);
       __syncthreads();
     }
-    if (threadIdx.x == 0) dt_part[blockIdx.y * gridDim.x + blockIdx.x] = red[0];
+    if (threadIdx.x == 0) dt_part[vb.by * vb.gx + vb.bx] = red[0];
   }
 }
+};
+template <bool P2, bool DT, int ROWS = CP_ROWS>
+__global__ __launch_bounds__(256) void k_correct_pressure(
+    Layout L, double *ux, double *uy, double *p, const double *psi, const double *dxc,
+    const double *dscal, double ga, double dy, int do_correct, int do_pressure,
+    double *dt_part, const int *halt) {
+  CorrectPressureB<P2, DT, ROWS>::run(VB::here(), L, ux, uy, p, psi, dxc, dscal, ga, dy, do_correct, do_pressure, dt_part, halt);
+}
+
+// a projection kernel and its body, as one type for the launcher below
+template <bool P2, bool DT, int ROWS>
+struct CpTag {
+  using type = CorrectPressureB<P2, DT, ROWS>;
+  static constexpr auto kern = &k_correct_pressure<P2, DT, ROWS>;
+};
 
 template <int ROWS>
 static int cp_launch(Ctx *c, int rk, bool correct, bool pressure, bool dt_fused, double *ux,
                      double *uy, double *p) {
   dim3 grid((c->nx + 255) / 256, (c->ny + ROWS - 1) / ROWS);
   const bool dt = dt_fused && correct && (long)grid.x * grid.y <= DT_PARTS_CAP;
-  auto kern = c->pow2_grid
-                  ? (dt ? k_correct_pressure<true, true, ROWS> : k_correct_pressure<true, false, ROWS>)
-                  : (dt ? k_correct_pressure<false, true, ROWS> : k_correct_pressure<false, false, ROWS>);
-  hipLaunchKernelGGL(kern, grid, dim3(256), 0, c->stream, c->L, ux, uy, p, c->psi,
-                     c->dxc, c->dscal, c->gamma[rk], c->dy, (int)correct, (int)pressure,
-                     c->dt_part, c->halt);
+  auto run = [&](auto kern_tag) {
+    using B = typename decltype(kern_tag)::type;
+    go<B>(decltype(kern_tag)::kern, grid, dim3(256), 0, c->stream, c->L, ux, uy, p, c->psi,
+          c->dxc, c->dscal, c->gamma[rk], c->dy, (int)correct, (int)pressure, c->dt_part,
+          c->halt);
+  };
+  if (c->pow2_grid) {
+    if (dt) run(CpTag<true, true, ROWS>{}); else run(CpTag<true, false, ROWS>{});
+  } else {
+    if (dt) run(CpTag<false, true, ROWS>{}); else run(CpTag<false, false, ROWS>{});
+  }
   EIF_HIP(hipGetLastError());
   c->dt_adv_parts = dt ? (int)(grid.x * grid.y) : 0;
   return EIF_OK;
@@ -525,9 +572,17 @@ __device__ inline void atomic_max_pos(double *addr, double v) {
             (unsigned long long)__double_as_longlong(v));
 }
 
-__global__ void k_dt_init(double *dred) { dred[0] = 5.e-2; }
+struct DtInitB {
+  static constexpr int T = 1024, M = 0;
+  static __device__ __forceinline__ void run(const VB &vb, double *dred) { dred[0] = 5.e-2; }
+};
+__global__ void k_dt_init(double *dred) {
+  DtInitB::run(VB::here(), dred);
+}
 
-__global__ __launch_bounds__(256) void k_dt_adv(Layout L, const double *ux,
+struct DtAdvB {
+  static constexpr int T = 256, M = 0;
+  static __device__ __forceinline__ void run(const VB &vb, Layout L, const double *ux,
                                                 const double *uy,
                                                 const double *dxc, double dy,
                                                 double *dred) {
@@ -535,8 +590,8 @@ __global__ __launch_bounds__(256) void k_dt_adv(Layout L, const double *ux,
   const double small = 1.e-8;
   double m = 5.e-2;
   const long n = (long)nx * ny;
-  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n;
-       t += (long)gridDim.x * blockDim.x) {
+  for (long t = vb.bx * (long)blockDim.x + threadIdx.x; t < n;
+       t += (long)vb.gx * blockDim.x) {
     const int i = (int)(t % nx) + 1, j = (int)(t / nx) + 1;
     if (i >= 2) {
       const double vel = fmax(fabs(ux[L.ux(i, j)]), small);
@@ -554,10 +609,19 @@ __global__ __launch_bounds__(256) void k_dt_adv(Layout L, const double *ux,
   }
   if (threadIdx.x == 0) atomic_min_pos(dred, red[0]);
 }
+};
+__global__ __launch_bounds__(256) void k_dt_adv(Layout L, const double *ux,
+                                                const double *uy,
+                                                const double *dxc, double dy,
+                                                double *dred) {
+  DtAdvB::run(VB::here(), L, ux, uy, dxc, dy, dred);
+}
 
 // nparts > 0: the advective limit comes as partial minima part[0..nparts)
 // from the projection kernel of the previous step's last stage
-__global__ void k_dt_final(const double *dred, const double *part, const double *prt, int np,
+struct DtFinalB {
+  static constexpr int T = 1024, M = 0;
+  static __device__ __forceinline__ void run(const VB &vb, const double *dred, const double *part, const double *prt, int np,
                            double sf0, double sf2, double dt_dif_x,
                            double dt_dif_y, double delta, double *dscal, int nparts,
                            const int *halt) {
@@ -602,17 +666,24 @@ __global__ void k_dt_final(const double *dred, const double *part, const double 
     dscal[S_DT] = fmin(fmin(dt_adv, fmin(dt_dif_x, dt_dif_y)), dtp);
   }
 }
+};
+__global__ void k_dt_final(const double *dred, const double *part, const double *prt, int np,
+                           double sf0, double sf2, double dt_dif_x,
+                           double dt_dif_y, double delta, double *dscal, int nparts,
+                           const int *halt) {
+  DtFinalB::run(VB::here(), dred, part, prt, np, sf0, sf2, dt_dif_x, dt_dif_y, delta, dscal, nparts, halt);
+}
 
 int launch_decide_dt(Ctx *c) {
   ProfScope ps(c, "decide_dt");
   const int nparts = c->dt_adv_valid ? c->dt_adv_parts : 0;
   c->dt_adv_valid = false;
   if (nparts == 0) {
-    hipLaunchKernelGGL(k_dt_init, dim3(1), dim3(1), 0, c->stream, c->dred);
+    go<DtInitB>(k_dt_init, dim3(1), dim3(1), 0, c->stream, c->dred);
     const long n = (long)c->nx * c->ny;
     int blocks = (int)((n + 255) / 256);
     if (blocks > 1024) blocks = 1024;
-    hipLaunchKernelGGL(k_dt_adv, dim3(blocks), dim3(256), 0, c->stream, c->L, c->ux,
+    go<DtAdvB>(k_dt_adv, dim3(blocks), dim3(256), 0, c->stream, c->L, c->ux,
                        c->uy, c->dxc, c->dy, c->dred);
   }
   // diffusive limits are grid constants (decide_dt.c:50-62)
@@ -623,7 +694,7 @@ int launch_decide_dt(Ctx *c) {
   dt_dif_x *= c->cfg.safefactors[1];
   dt_dif_y *= c->cfg.safefactors[1];
   const double delta = fmin(c->dx, c->dy);
-  hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(256), 0, c->stream, c->dred, c->dt_part, c->prt,
+  go<DtFinalB>(k_dt_final, dim3(1), dim3(256), 0, c->stream, c->dred, c->dt_part, c->prt,
                      c->np, c->cfg.safefactors[0], c->cfg.safefactors[2], dt_dif_x,
                      dt_dif_y, delta, c->dscal, nparts, c->halt);
   EIF_HIP(hipGetLastError());

@@ -10,6 +10,7 @@
 #include <vector>
 #include <map>
 #include "eif_gpu.h"
+#include "eif_batch.hpp"
 
 // the reference's value of pi (include/common.h:7)
 #define EIF_PI 3.1415926535897932

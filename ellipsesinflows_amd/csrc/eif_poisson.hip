@@ -221,13 +221,15 @@ __device__ inline double rhs_value_add(const Layout &L, const double *ux, const 
   return fac * (div + fac * add[L.p(i + 1, j)]);
 }
 
-__global__ __launch_bounds__(1024) void k_rhs_dct2(
+struct RhsDct2B {
+  static constexpr int T = 1024, M = 0;
+  static __device__ __forceinline__ void run(const VB &vb, 
     Layout L, const double *ux, const double *uy, const double *dscal, double ga,
     double dx, double dy, const double *__restrict__ C, const double *__restrict__ W,
     int logn, int mode, double *qhat, const double *add) {
   extern __shared__ double sh[];
   const int nx = L.nx;
-  const int j = blockIdx.x + 1;
+  const int j = vb.bx + 1;
   double *re = sh, *im = sh + nx;
   const double dt = dscal[S_DT];
   const double fac = 1. / (ga * dt);
@@ -254,6 +256,13 @@ __global__ __launch_bounds__(1024) void k_rhs_dct2(
       out[k] = acc;
     }
   }
+}
+};
+__global__ __launch_bounds__(1024) void k_rhs_dct2(
+    Layout L, const double *ux, const double *uy, const double *dscal, double ga,
+    double dx, double dy, const double *__restrict__ C, const double *__restrict__ W,
+    int logn, int mode, double *qhat, const double *add) {
+  RhsDct2B::run(VB::here(), L, ux, uy, dscal, ga, dx, dy, C, W, logn, mode, qhat, add);
 }
 
 // ---------------------------------------------------------------------
@@ -286,13 +295,15 @@ __device__ inline double combined(const CombineArgs &a, const double *qhat,
   return q - a.coef * xl * a.v[(long)r * nx + k] - a.coef * xr * a.v[(long)(a.L - 1 - r) * nx + k];
 }
 
-__global__ __launch_bounds__(1024) void k_dct3_psi(
+struct Dct3PsiB {
+  static constexpr int T = 1024, M = 0;
+  static __device__ __forceinline__ void run(const VB &vb, 
     Layout L, const double *qhat, CombineArgs ca, const double *__restrict__ C,
     const double *__restrict__ W, int logn, int mode, double *psi) {
   if (halted(ca.halt)) return;
   extern __shared__ double sh[];
   const int nx = L.nx, ny = L.ny;
-  const int j0 = blockIdx.x;
+  const int j0 = vb.bx;
   double *re = sh, *im = sh + nx;
   const double norm = 2. * nx;
   if (mode == 1) {
@@ -321,6 +332,12 @@ __global__ __launch_bounds__(1024) void k_dct3_psi(
     }
   }
 }
+};
+__global__ __launch_bounds__(1024) void k_dct3_psi(
+    Layout L, const double *qhat, CombineArgs ca, const double *__restrict__ C,
+    const double *__restrict__ W, int logn, int mode, double *psi) {
+  Dct3PsiB::run(VB::here(), L, qhat, ca, C, W, logn, mode, psi);
+}
 
 // Register-FFT kernels specialised for nx = 2^LOGN (nx / 2^LOGR threads),
 // two rows per workgroup through one complex FFT: row a is the real and
@@ -330,7 +347,9 @@ __global__ __launch_bounds__(1024) void k_dct3_psi(
 // the inverse takes the two pre-twiddled spectra as Va + i Vb.  An odd last
 // row pairs with a zero row.
 template <int LOGN, bool P2, bool ADD = false>
-__global__ __launch_bounds__((1 << LOGN) >> dct_logr<LOGN>()) void k_rhs_dct2_t(
+struct RhsDct2TB {
+  static constexpr int T = (1 << LOGN) >> dct_logr<LOGN>(), M = 0;
+  static __device__ __forceinline__ void run(const VB &vb, 
     Layout L, const double *ux, const double *uy, const double *dscal, double ga,
     double dx, double dy, const double *__restrict__ C, const double *__restrict__ W,
     double *qhat, const double *add = nullptr) {
@@ -338,7 +357,7 @@ __global__ __launch_bounds__((1 << LOGN) >> dct_logr<LOGN>()) void k_rhs_dct2_t(
   constexpr int LOGR = dct_logr<LOGN>(), R = 1 << LOGR;
   constexpr int nx = 1 << LOGN;
   constexpr int T = nx / R;
-  const int ja = 2 * blockIdx.x + 1, jb = ja + 1;
+  const int ja = 2 * vb.bx + 1, jb = ja + 1;
   const bool hasb = jb <= L.ny;
   double *re = sh, *im = sh + nx;
   const double dt = dscal[S_DT];
@@ -386,9 +405,19 @@ __global__ __launch_bounds__((1 << LOGN) >> dct_logr<LOGN>()) void k_rhs_dct2_t(
     if (hasb) outb[k] = C[k] * (zi + wi) + C[nx - k] * (wr - zr);
   }
 }
+};
+template <int LOGN, bool P2, bool ADD = false>
+__global__ __launch_bounds__((1 << LOGN) >> dct_logr<LOGN>()) void k_rhs_dct2_t(
+    Layout L, const double *ux, const double *uy, const double *dscal, double ga,
+    double dx, double dy, const double *__restrict__ C, const double *__restrict__ W,
+    double *qhat, const double *add = nullptr) {
+  RhsDct2TB<LOGN, P2, ADD>::run(VB::here(), L, ux, uy, dscal, ga, dx, dy, C, W, qhat, add);
+}
 
 template <int LOGN>
-__global__ __launch_bounds__((1 << LOGN) >> dct_logr<LOGN>()) void k_dct3_psi_t(
+struct Dct3PsiTB {
+  static constexpr int T = (1 << LOGN) >> dct_logr<LOGN>(), M = 0;
+  static __device__ __forceinline__ void run(const VB &vb, 
     Layout L, const double *qhat, CombineArgs ca, const double *__restrict__ C,
     const double *__restrict__ W, double *psi) {
   if (halted(ca.halt)) return;
@@ -401,9 +430,9 @@ __global__ __launch_bounds__((1 << LOGN) >> dct_logr<LOGN>()) void k_dct3_psi_t(
   // mirror rows r, L-1-r of one chunk, whose fix-ups read the same two spike
   // rows v[r], v[L-1-r] and boundary values (half the table loads; the
   // pairing changes only the rounding inside the shared transform)
-  int ja0 = 2 * blockIdx.x, jb0 = ja0 + 1;
+  int ja0 = 2 * vb.bx, jb0 = ja0 + 1;
   if (ca.mode == 1) {
-    const int h = ca.L >> 1, cc = blockIdx.x / h, t = blockIdx.x - cc * h;
+    const int h = ca.L >> 1, cc = vb.bx / h, t = vb.bx - cc * h;
     ja0 = cc * ca.L + t;
     jb0 = cc * ca.L + ca.L - 1 - t;
   }
@@ -478,18 +507,27 @@ __global__ __launch_bounds__((1 << LOGN) >> dct_logr<LOGN>()) void k_dct3_psi_t(
     }
   }
 }
+};
+template <int LOGN>
+__global__ __launch_bounds__((1 << LOGN) >> dct_logr<LOGN>()) void k_dct3_psi_t(
+    Layout L, const double *qhat, CombineArgs ca, const double *__restrict__ C,
+    const double *__restrict__ W, double *psi) {
+  Dct3PsiTB<LOGN>::run(VB::here(), L, qhat, ca, C, W, psi);
+}
 
 // ---------------------------------------------------------------------
 // Thomas mode: one lane per wavenumber k, sweeps along j
 // ---------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_thomas(int nx, int ny, double *q,
+struct ThomasB {
+  static constexpr int T = 64, M = 0;
+  static __device__ __forceinline__ void run(const VB &vb, int nx, int ny, double *q,
                                                const double *__restrict__ rv,
                                                const double *__restrict__ um,
                                                const unsigned char *__restrict__ flag,
                                                const double *__restrict__ cdiag,
                                                const double *__restrict__ den,
                                                double coef, double *last) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = vb.bx * blockDim.x + threadIdx.x;
   if (k >= nx) return;
   const int n1 = ny - 1;  // size of the bordered (n-1) system
   const double l = coef;  // l = u = 1/dy/dy everywhere
@@ -578,6 +616,16 @@ __global__ __launch_bounds__(64) void k_thomas(int nx, int ny, double *q,
   const double xl = fabs(d) < DBL_EPSILON ? 0. : num / d;
   last[k] = xl;
 }
+};
+__global__ __launch_bounds__(64) void k_thomas(int nx, int ny, double *q,
+                                               const double *__restrict__ rv,
+                                               const double *__restrict__ um,
+                                               const unsigned char *__restrict__ flag,
+                                               const double *__restrict__ cdiag,
+                                               const double *__restrict__ den,
+                                               double coef, double *last) {
+  ThomasB::run(VB::here(), nx, ny, q, rv, um, flag, cdiag, den, coef, last);
+}
 
 // ---------------------------------------------------------------------
 // Partition mode (spike method).  Chunk c holds rows [cL, cL+L).  With the
@@ -594,11 +642,13 @@ __global__ __launch_bounds__(64) void k_thomas(int nx, int ny, double *q,
 constexpr int PART_LANES = 64;
 
 template <int L>
-__global__ __launch_bounds__(PART_LANES) void k_part_local(
+struct PartLocalB {
+  static constexpr int T = PART_LANES, M = 0;
+  static __device__ __forceinline__ void run(const VB &vb, 
     int nx, double *q, const double *__restrict__ fwd,
     const double *__restrict__ um, double coef, double *ends) {
-  const int k = blockIdx.x * PART_LANES + threadIdx.x;
-  const int c = blockIdx.y;
+  const int k = vb.bx * PART_LANES + threadIdx.x;
+  const int c = vb.by;
   if (k >= nx) return;
   const long j0 = (long)c * L;
   const double l = coef;
@@ -624,6 +674,13 @@ __global__ __launch_bounds__(PART_LANES) void k_part_local(
   }
   ends[((long)c * 2 + 0) * nx + k] = next;   // y_c[0]
   ends[((long)c * 2 + 1) * nx + k] = ylast;  // y_c[L-1]
+}
+};
+template <int L>
+__global__ __launch_bounds__(PART_LANES) void k_part_local(
+    int nx, double *q, const double *__restrict__ fwd,
+    const double *__restrict__ um, double coef, double *ends) {
+  PartLocalB<L>::run(VB::here(), nx, q, fwd, um, coef, ends);
 }
 
 // Reduced system: the blocks are the same for every chunk, so the periodic
@@ -659,13 +716,15 @@ __device__ inline void mode_solve(double lv0, double lvL, bool gauge, double ct,
   }
 }
 
-__global__ __launch_bounds__(64 * PART_KG) void k_part_reduced(
+struct PartReducedB {
+  static constexpr int T = 64 * PART_KG, M = 0;
+  static __device__ __forceinline__ void run(const VB &vb, 
     int nx, int P, const double *__restrict__ ends, const double *__restrict__ lv,
     const double *__restrict__ twc, const double *__restrict__ tws,
     double *__restrict__ bnd) {
   extern __shared__ double sh[];
   const int kk = threadIdx.x / 64, lane = threadIdx.x % 64;
-  const int k = blockIdx.x * PART_KG + kk;
+  const int k = vb.bx * PART_KG + kk;
   // chunk-DFT twiddles staged in LDS: the sums below read them P times per
   // lane
   double *tc = sh + (long)PART_KG * 6 * P, *ts = tc + P;
@@ -720,6 +779,13 @@ __global__ __launch_bounds__(64 * PART_KG) void k_part_reduced(
     }
   }
 }
+};
+__global__ __launch_bounds__(64 * PART_KG) void k_part_reduced(
+    int nx, int P, const double *__restrict__ ends, const double *__restrict__ lv,
+    const double *__restrict__ twc, const double *__restrict__ tws,
+    double *__restrict__ bnd) {
+  PartReducedB::run(VB::here(), nx, P, ends, lv, twc, tws, bnd);
+}
 
 // The same reduced system with both chunk DFTs as FP64 matrix products on
 // the matrix cores (v_mfma_f64_16x16x4f64), for P a multiple of 16 up to
@@ -734,7 +800,9 @@ __global__ __launch_bounds__(64 * PART_KG) void k_part_reduced(
 typedef double pr_d4 __attribute__((ext_vector_type(4)));
 constexpr int PRM_K = 16;
 
-__global__ __launch_bounds__(256) void k_part_reduced_mfma(
+struct PartReducedMfmaB {
+  static constexpr int T = 256, M = 0;
+  static __device__ __forceinline__ void run(const VB &vb, 
     int nx, int P, const double *__restrict__ ends, const double *__restrict__ lv,
     const double *__restrict__ twc, const double *__restrict__ tws,
     double *__restrict__ bnd) {
@@ -742,7 +810,7 @@ __global__ __launch_bounds__(256) void k_part_reduced_mfma(
   double *tc = sh, *ts = tc + P;
   double *r0 = ts + P, *r1 = r0 + P * PRM_K;  // [c][kk]
   double *xs = r1 + P * PRM_K;                 // [4][m][kk]: x0r, x0i, x1r, x1i
-  const int k0 = blockIdx.x * PRM_K;
+  const int k0 = vb.bx * PRM_K;
   for (int e = threadIdx.x; e < P; e += blockDim.x) {
     tc[e] = twc[e];
     ts[e] = tws[e];
@@ -810,6 +878,13 @@ __global__ __launch_bounds__(256) void k_part_reduced_mfma(
       }
     }
   }
+}
+};
+__global__ __launch_bounds__(256) void k_part_reduced_mfma(
+    int nx, int P, const double *__restrict__ ends, const double *__restrict__ lv,
+    const double *__restrict__ twc, const double *__restrict__ tws,
+    double *__restrict__ bnd) {
+  PartReducedMfmaB::run(VB::here(), nx, P, ends, lv, twc, tws, bnd);
 }
 
 // The lane-loop reduced system (k_part_reduced) keeps (6 PART_KG + 2) P
@@ -1002,6 +1077,15 @@ static int ilog2_exact(int n) {
   return l;
 }
 
+// the register-FFT right-hand side of a power-of-two row length 2^LG
+template <int LG, bool P2, bool ADD>
+static void rhs_t(Ctx *c, int rk, size_t shmem, const double *rhs_add) {
+  go<RhsDct2TB<LG, P2, ADD>>(k_rhs_dct2_t<LG, P2, ADD>, dim3((c->ny + 1) / 2),
+                             dim3(c->nx >> dct_logr<LG>()), shmem, c->stream, c->L, c->ux,
+                             c->uy, c->dscal, c->gamma[rk], c->dx, c->dy, c->ctab, c->twid,
+                             c->qhat, rhs_add);
+}
+
 int launch_poisson(Ctx *c, int rk, const double *rhs_add) {
   const int nx = c->nx, ny = c->ny;
   const int logn = ilog2_exact(nx);
@@ -1013,24 +1097,20 @@ int launch_poisson(Ctx *c, int rk, const double *rhs_add) {
     ProfScope ps(c, "rhs_dct2");
 #define EIF_RHS_T(LG)                                                                    \
   case LG:                                                                               \
-    {                                                                                    \
-      auto kern = rhs_add ? (c->pow2_grid ? k_rhs_dct2_t<LG, true, true>                 \
-                                          : k_rhs_dct2_t<LG, false, true>)               \
-                          : (c->pow2_grid ? k_rhs_dct2_t<LG, true> : k_rhs_dct2_t<LG, false>); \
-      hipLaunchKernelGGL(kern, dim3((ny + 1) / 2), dim3(nx >> dct_logr<LG>()), shmem,    \
-                         c->stream,                                                      \
-                         c->L, c->ux,                                                    \
-                         c->uy, c->dscal, c->gamma[rk], c->dx, c->dy, c->ctab, c->twid,  \
-                         c->qhat, rhs_add);                                              \
-    }                                                                                    \
+    if (rhs_add)                                                                         \
+      c->pow2_grid ? rhs_t<LG, true, true>(c, rk, shmem, rhs_add)                        \
+                   : rhs_t<LG, false, true>(c, rk, shmem, rhs_add);                      \
+    else                                                                                 \
+      c->pow2_grid ? rhs_t<LG, true, false>(c, rk, shmem, rhs_add)                       \
+                   : rhs_t<LG, false, false>(c, rk, shmem, rhs_add);                     \
     break;
     switch (mode == 2 ? logn : -1) {
       EIF_RHS_T(6) EIF_RHS_T(7) EIF_RHS_T(8) EIF_RHS_T(9) EIF_RHS_T(10) EIF_RHS_T(11)
       EIF_RHS_T(12) EIF_RHS_T(13)
       default:
-        hipLaunchKernelGGL(k_rhs_dct2, dim3(ny), dim3(threads), shmem, c->stream, c->L,
-                           c->ux, c->uy, c->dscal, c->gamma[rk], c->dx, c->dy, c->ctab,
-                           c->twid, logn, mode, c->qhat, rhs_add);
+        go<RhsDct2B>(k_rhs_dct2, dim3(ny), dim3(threads), shmem, c->stream, c->L, c->ux, c->uy,
+                     c->dscal, c->gamma[rk], c->dx, c->dy, c->ctab, c->twid, logn, mode,
+                     c->qhat, rhs_add);
     }
 #undef EIF_RHS_T
     EIF_HIP(hipGetLastError());
@@ -1044,17 +1124,17 @@ int launch_poisson(Ctx *c, int rk, const double *rhs_add) {
     const int Lc = c->part_L, P = c->part_P;
     dim3 g1((nx + PART_LANES - 1) / PART_LANES, P);
     switch (Lc) {
-      case 64: hipLaunchKernelGGL(k_part_local<64>, g1, dim3(PART_LANES), 0, c->stream, nx, c->qhat, c->pt_fwd, c->pt_u, l, c->pt_ends); break;
-      case 32: hipLaunchKernelGGL(k_part_local<32>, g1, dim3(PART_LANES), 0, c->stream, nx, c->qhat, c->pt_fwd, c->pt_u, l, c->pt_ends); break;
-      case 16: hipLaunchKernelGGL(k_part_local<16>, g1, dim3(PART_LANES), 0, c->stream, nx, c->qhat, c->pt_fwd, c->pt_u, l, c->pt_ends); break;
-      default: hipLaunchKernelGGL(k_part_local<8>, g1, dim3(PART_LANES), 0, c->stream, nx, c->qhat, c->pt_fwd, c->pt_u, l, c->pt_ends); break;
+      case 64: go<PartLocalB<64>>(k_part_local<64>, g1, dim3(PART_LANES), 0, c->stream, nx, c->qhat, c->pt_fwd, c->pt_u, l, c->pt_ends); break;
+      case 32: go<PartLocalB<32>>(k_part_local<32>, g1, dim3(PART_LANES), 0, c->stream, nx, c->qhat, c->pt_fwd, c->pt_u, l, c->pt_ends); break;
+      case 16: go<PartLocalB<16>>(k_part_local<16>, g1, dim3(PART_LANES), 0, c->stream, nx, c->qhat, c->pt_fwd, c->pt_u, l, c->pt_ends); break;
+      default: go<PartLocalB<8>>(k_part_local<8>, g1, dim3(PART_LANES), 0, c->stream, nx, c->qhat, c->pt_fwd, c->pt_u, l, c->pt_ends); break;
     }
     if (P % 16 == 0 && P <= 64)  // the chunk DFTs on the matrix cores
-      hipLaunchKernelGGL(k_part_reduced_mfma, dim3((nx + PRM_K - 1) / PRM_K), dim3(4 * P),
+      go<PartReducedMfmaB>(k_part_reduced_mfma, dim3((nx + PRM_K - 1) / PRM_K), dim3(4 * P),
                          (size_t)(2 + 6 * PRM_K) * P * sizeof(double), c->stream, nx, P,
                          c->pt_ends, c->pt_lvl, c->pt_red, c->pt_w, c->pt_bnd);
     else
-      hipLaunchKernelGGL(k_part_reduced, dim3((nx + PART_KG - 1) / PART_KG), dim3(64 * PART_KG),
+      go<PartReducedB>(k_part_reduced, dim3((nx + PART_KG - 1) / PART_KG), dim3(64 * PART_KG),
                          (size_t)(PART_KG * 6 + 2) * P * sizeof(double), c->stream, nx, P,
                          c->pt_ends, c->pt_lvl, c->pt_red, c->pt_w, c->pt_bnd);
     EIF_HIP(hipGetLastError());
@@ -1065,7 +1145,7 @@ int launch_poisson(Ctx *c, int rk, const double *rhs_add) {
     ca.bnd = c->pt_bnd;
   } else {
     ProfScope ps(c, "thomas");
-    hipLaunchKernelGGL(k_thomas, dim3((nx + 63) / 64), dim3(64), 0, c->stream, nx, ny,
+    go<ThomasB>(k_thomas, dim3((nx + 63) / 64), dim3(64), 0, c->stream, nx, ny,
                        c->qhat, c->th_val, c->th_u, c->th_flag, c->th_den + nx,
                        c->th_den, l, c->th_last);
     EIF_HIP(hipGetLastError());
@@ -1079,7 +1159,7 @@ int launch_poisson(Ctx *c, int rk, const double *rhs_add) {
 #define EIF_DCT3_T(LG)                                                                  \
   case LG:                                                                              \
     halo_done = true;                                                                   \
-    hipLaunchKernelGGL(k_dct3_psi_t<LG>, dim3((ny + 1) / 2), dim3(nx >> dct_logr<LG>()),  \
+    go<Dct3PsiTB<LG>>(k_dct3_psi_t<LG>, dim3((ny + 1) / 2), dim3(nx >> dct_logr<LG>()),   \
                        shmem,                                                           \
                        c->stream, c->L,                                                 \
                        c->qhat, ca, c->ctab, c->twid, c->psi);                          \
@@ -1088,7 +1168,7 @@ int launch_poisson(Ctx *c, int rk, const double *rhs_add) {
       EIF_DCT3_T(6) EIF_DCT3_T(7) EIF_DCT3_T(8) EIF_DCT3_T(9) EIF_DCT3_T(10) EIF_DCT3_T(11)
       EIF_DCT3_T(12) EIF_DCT3_T(13)
       default:
-        hipLaunchKernelGGL(k_dct3_psi, dim3(ny), dim3(threads), shmem, c->stream, c->L,
+        go<Dct3PsiB>(k_dct3_psi, dim3(ny), dim3(threads), shmem, c->stream, c->L,
                            c->qhat, ca, c->ctab, c->twid, logn, mode, c->psi);
     }
 #undef EIF_DCT3_T

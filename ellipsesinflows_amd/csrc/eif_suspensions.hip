@@ -577,15 +577,21 @@ __device__ inline void write_head(const IbmArgs &a, int n, double px, double py,
 }
 
 // heads at the particles' positions, with the increments when with_dx
-__global__ void k_heads(IbmArgs a, int with_dx) {
+struct HeadsB {
+  static constexpr int T = 1024, M = 0;
+  static __device__ __forceinline__ void run(const VB &vb, IbmArgs a, int with_dx) {
   if (halted(a.halt)) return;
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = vb.bx * blockDim.x + threadIdx.x;
   if (n >= a.np) return;
   const double *q = a.prt + (long)n * R_N;
   if (with_dx)
     write_head(a, n, q[R_X] + q[R_DX], q[R_Y] + q[R_DY], q[R_AZ] + q[R_DAZ], true);
   else
     write_head(a, n, q[R_X], q[R_Y], q[R_AZ], true);
+}
+};
+__global__ void k_heads(IbmArgs a, int with_dx) {
+  HeadsB::run(VB::here(), a, with_dx);
 }
 
 // src/suspensions/inertia.c:9-70.  One wave per slice of IBM_SLICE box
@@ -681,9 +687,15 @@ __device__ __forceinline__ void inertia_slice(const IbmArgs &a, int cnstep, int 
   }
 }
 
-__global__ __launch_bounds__(64) void k_inertia(IbmArgs a, int cnstep) {
+struct InertiaB {
+  static constexpr int T = 64, M = 0;
+  static __device__ __forceinline__ void run(const VB &vb, IbmArgs a, int cnstep) {
   if (halted(a.halt)) return;
-  inertia_slice(a, cnstep, blockIdx.x, sub_open(a.res, a.sub));
+  inertia_slice(a, cnstep, vb.bx, sub_open(a.res, a.sub));
+}
+};
+__global__ __launch_bounds__(64) void k_inertia(IbmArgs a, int cnstep) {
+  InertiaB::run(VB::here(), a, cnstep);
 }
 
 // Per-particle totals of the sliced inertia / exchange kernels, by the 64
@@ -760,11 +772,17 @@ __device__ inline void finalize_x0_particle(const IbmArgs &a, int n) {
 }
 
 // one wave per particle, four per workgroup
-__global__ __launch_bounds__(256) void k_ibm_finalize(IbmArgs a, int which) {
+struct IbmFinalizeB {
+  static constexpr int T = 256, M = 0;
+  static __device__ __forceinline__ void run(const VB &vb, IbmArgs a, int which) {
   if (halted(a.halt)) return;
   if (!sub_open(a.res, a.sub)) return;
-  const int n = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+  const int n = vb.bx * 4 + (int)(threadIdx.x >> 6);
   if (n < a.np) finalize_particle(a, n, which);
+}
+};
+__global__ __launch_bounds__(256) void k_ibm_finalize(IbmArgs a, int which) {
+  IbmFinalizeB::run(VB::here(), a, which);
 }
 
 // Measurement only: the reference's FP64 iteration (find_normal_t,
@@ -838,10 +856,12 @@ __device__ inline void register_tile(const IbmArgs &a, int tile, int box_id) {
 // increments dx, dy, daz when with_dx) where the per-particle cache does not
 // already hold them; sliced like k_inertia.  Used before k_exchange when
 // the cache may be stale.
-__global__ __launch_bounds__(64) void k_ibm_dist(IbmArgs a) {
+struct IbmDistB {
+  static constexpr int T = 64, M = 0;
+  static __device__ __forceinline__ void run(const VB &vb, IbmArgs a) {
   if (halted(a.halt)) return;
   if (!sub_open(a.res, a.sub)) return;
-  const int slice = blockIdx.x;
+  const int slice = vb.bx;
   const int n = a.slice_owner[slice];
   const int s0 = a.slice_first[n], nsl = a.slice_first[n + 1] - s0;
   const SliceHead &hd = a.heads[n];
@@ -878,6 +898,10 @@ __global__ __launch_bounds__(64) void k_ibm_dist(IbmArgs a) {
     if (slice == s0) a.slice_info[2 * (long)n + 1] = cached ? 0 : total;
   }
 }
+};
+__global__ __launch_bounds__(64) void k_ibm_dist(IbmArgs a) {
+  IbmDistB::run(VB::here(), a);
+}
 
 // src/suspensions/exchange.c:13-88, sliced like k_inertia, on distances
 // already in the per-particle cache (the position is the one of the
@@ -886,9 +910,11 @@ __global__ __launch_bounds__(64) void k_ibm_dist(IbmArgs a) {
 // own_counts: no k_ibm_dist ran before, so this kernel reports no
 // distance work.
 // The heads hold the particles' positions without increments.
-__global__ __launch_bounds__(64) void k_exchange(IbmArgs a, int own_counts) {
+struct ExchangeB {
+  static constexpr int T = 64, M = 0;
+  static __device__ __forceinline__ void run(const VB &vb, IbmArgs a, int own_counts) {
   if (halted(a.halt)) return;
-  const int slice = blockIdx.x;
+  const int slice = vb.bx;
   const int n = a.slice_owner[slice];
   const int s0 = a.slice_first[n], nsl = a.slice_first[n + 1] - s0;
   const double *q = a.prt + (long)n * R_N;
@@ -1028,6 +1054,10 @@ __global__ __launch_bounds__(64) void k_exchange(IbmArgs a, int own_counts) {
     }
   }
 }
+};
+__global__ __launch_bounds__(64) void k_exchange(IbmArgs a, int own_counts) {
+  ExchangeB::run(VB::here(), a, own_counts);
+}
 
 // ---------------------------------------------------------------------
 // deposit + apply over active tiles (update.c:36-59)
@@ -1157,21 +1187,33 @@ __device__ __forceinline__ void tile_body(const TileArgs &a, int tile) {
   }
 }
 
-__global__ __launch_bounds__(TILE_THREADS) void k_tiles(TileArgs a) {
+struct TilesB {
+  static constexpr int T = TILE_THREADS, M = 0;
+  static __device__ __forceinline__ void run(const VB &vb, TileArgs a) {
   if (halted(a.halt)) return;
-  tile_body(a, blockIdx.x);
+  tile_body(a, vb.bx);
+}
+};
+__global__ __launch_bounds__(TILE_THREADS) void k_tiles(TileArgs a) {
+  TilesB::run(VB::here(), a);
 }
 
 // The stage start's deposit with the exchange's per-particle sums (and
 // inertia(0)'s) in the same launch: the first np workgroups finalise one
 // particle each, the rest are the tiles.  Neither reads what the other
 // writes.
-__global__ __launch_bounds__(TILE_THREADS) void k_tiles_x0(TileArgs t, IbmArgs a) {
+struct TilesX0B {
+  static constexpr int T = TILE_THREADS, M = 0;
+  static __device__ __forceinline__ void run(const VB &vb, TileArgs t, IbmArgs a) {
   if (halted(a.halt)) return;
-  if ((int)blockIdx.x < a.np)
-    finalize_x0_particle(a, blockIdx.x);
+  if ((int)vb.bx < a.np)
+    finalize_x0_particle(a, vb.bx);
   else
-    tile_body(t, blockIdx.x - a.np);
+    tile_body(t, vb.bx - a.np);
+}
+};
+__global__ __launch_bounds__(TILE_THREADS) void k_tiles_x0(TileArgs t, IbmArgs a) {
+  TilesX0B::run(VB::here(), t, a);
 }
 
 // ---------------------------------------------------------------------
@@ -1447,10 +1489,16 @@ __device__ void collide_particle(const CollArgs &a, int n) {
 }
 
 // one 64-lane workgroup per particle
-__global__ __launch_bounds__(64) void k_collide(CollArgs a) {
+struct CollideB {
+  static constexpr int T = 64, M = 0;
+  static __device__ __forceinline__ void run(const VB &vb, CollArgs a) {
   if (halted(a.halt)) return;
   if (!sub_open(a.res, a.sub)) return;
-  collide_particle(a, blockIdx.x);
+  collide_particle(a, vb.bx);
+}
+};
+__global__ __launch_bounds__(64) void k_collide(CollArgs a) {
+  CollideB::run(VB::here(), a);
 }
 
 // One launch for a sub-iteration's inertia(1) and collision(1), which read
@@ -1459,19 +1507,31 @@ __global__ __launch_bounds__(64) void k_collide(CollArgs a) {
 // slices.  Bounded to 4 waves per SIMD (128 VGPRs): the collision path
 // alone needs more and spills a little, which costs less than the
 // occupancy its registers would take from the slices.
-__global__ __launch_bounds__(64, 4) void k_inertia_collide(IbmArgs a, CollArgs ca, int cnstep) {
+struct InertiaCollideB {
+  static constexpr int T = 64, M = 4;
+  static __device__ __forceinline__ void run(const VB &vb, IbmArgs a, CollArgs ca, int cnstep) {
   if (halted(a.halt)) return;
   const bool open = sub_open(a.res, a.sub);
-  if ((int)blockIdx.x < ca.np) {
-    if (open) collide_particle(ca, blockIdx.x);
+  if ((int)vb.bx < ca.np) {
+    if (open) collide_particle(ca, vb.bx);
   } else {
-    inertia_slice(a, cnstep, blockIdx.x - ca.np, open);
+    inertia_slice(a, cnstep, vb.bx - ca.np, open);
   }
 }
+};
+__global__ __launch_bounds__(64, 4) void k_inertia_collide(IbmArgs a, CollArgs ca, int cnstep) {
+  InertiaCollideB::run(VB::here(), a, ca, cnstep);
+}
 
-__global__ void k_zero_int(int *p, int n) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+struct ZeroIntB {
+  static constexpr int T = 1024, M = 0;
+  static __device__ __forceinline__ void run(const VB &vb, int *p, int n) {
+  const int t = vb.bx * blockDim.x + threadIdx.x;
   if (t < n) p[t] = 0;
+}
+};
+__global__ void k_zero_int(int *p, int n) {
+  ZeroIntB::run(VB::here(), p, n);
 }
 
 
@@ -1483,21 +1543,29 @@ __device__ inline void atomic_max_nonneg(double *addr, double v) {
             (unsigned long long)__double_as_longlong(v));
 }
 
-__global__ void k_zero_residual(double *dscal) { dscal[S_RESIDUAL] = 0.; }
+struct ZeroResidualB {
+  static constexpr int T = 1024, M = 0;
+  static __device__ __forceinline__ void run(const VB &vb, double *dscal) { dscal[S_RESIDUAL] = 0.; }
+};
+__global__ void k_zero_residual(double *dscal) {
+  ZeroResidualB::run(VB::here(), dscal);
+}
 
 // src/suspensions/increment.c:10-64
 // fin: also finalises the preceding inertia(1) call (its slice sums and
 // cache records) for this particle first, saving a launch per sub-iteration
 // one wave per particle (its lanes finalise the inertia sums), four per
 // workgroup; lane 0 computes the increment
-__global__ __launch_bounds__(256) void k_increment(double *prt, int np, const double *dscal,
+struct IncrementB {
+  static constexpr int T = 256, M = 0;
+  static __device__ __forceinline__ void run(const VB &vb, double *prt, int np, const double *dscal,
                                                    double ga, double Fr, double *res_out,
                                                    IbmArgs ia, int fin) {
   if (halted(ia.halt)) return;
   const bool open = sub_open(ia.res, ia.sub);
   __shared__ double red[4];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int n = blockIdx.x * 4 + w;
+  const int n = vb.bx * 4 + w;
   // the slice range and lane 0's record inputs are loaded before the gate
   // is tested (all threads of the workgroup see the same gate), so that
   // their latencies overlap
@@ -1579,11 +1647,19 @@ __global__ __launch_bounds__(256) void k_increment(double *prt, int np, const do
   if (threadIdx.x == 0)
     atomic_max_nonneg(res_out, fmax(fmax(red[0], red[1]), fmax(red[2], red[3])));
 }
+};
+__global__ __launch_bounds__(256) void k_increment(double *prt, int np, const double *dscal,
+                                                   double ga, double Fr, double *res_out,
+                                                   IbmArgs ia, int fin) {
+  IncrementB::run(VB::here(), prt, np, dscal, ga, Fr, res_out, ia, fin);
+}
 
 // src/suspensions/update.c:8-34
-__global__ void k_update_particles(double *prt, int np, double ly, const int *halt) {
+struct UpdateParticlesB {
+  static constexpr int T = 1024, M = 0;
+  static __device__ __forceinline__ void run(const VB &vb, double *prt, int np, double ly, const int *halt) {
   if (halted(halt)) return;
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = vb.bx * blockDim.x + threadIdx.x;
   if (n >= np) return;
   double *p = prt + (long)n * R_N;
   p[R_UX] += p[R_DUX];
@@ -1597,15 +1673,21 @@ __global__ void k_update_particles(double *prt, int np, double ly, const int *ha
   if (p[R_AZ] < 0.) p[R_AZ] += 2. * EIF_PI;
   if (p[R_AZ] > 2. * EIF_PI) p[R_AZ] -= 2. * EIF_PI;
 }
+};
+__global__ void k_update_particles(double *prt, int np, double ly, const int *halt) {
+  UpdateParticlesB::run(VB::here(), prt, np, ly, halt);
+}
 
 // src/suspensions/reset.c:7-18
 // res (RES_SLOTS sub-iteration residuals), tile_cnt (ntiles) and
 // active[0]: cleared when given, for the stage's sub-iterations and the
 // next exchange's tile registrations
 // Also the heads of the stage-start pass (positions without increments).
-__global__ void k_reset(IbmArgs a, double *res, int *tile_cnt, int ntiles, int *active) {
+struct ResetB {
+  static constexpr int T = 1024, M = 0;
+  static __device__ __forceinline__ void run(const VB &vb, IbmArgs a, double *res, int *tile_cnt, int ntiles, int *active) {
   if (halted(a.halt)) return;
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = vb.bx * blockDim.x + threadIdx.x;
   if (res && n < RES_SLOTS) res[n] = 0.;
   if (tile_cnt) {
     if (n < ntiles) tile_cnt[n] = 0;
@@ -1623,6 +1705,10 @@ __global__ void k_reset(IbmArgs a, double *res, int *tile_cnt, int ntiles, int *
   p[R_DY] = 0.;
   p[R_DAZ] = 0.;
   write_head(a, n, p[R_X], p[R_Y], p[R_AZ], true);
+}
+};
+__global__ void k_reset(IbmArgs a, double *res, int *tile_cnt, int ntiles, int *active) {
+  ResetB::run(VB::here(), a, res, tile_cnt, ntiles, active);
 }
 
 // ---------------------------------------------------------------------
@@ -1757,7 +1843,7 @@ int launch_reset(Ctx *c, hipStream_t st, bool clear_tiles) {
   // also clears the sub-iteration residuals, so it runs without particles too
   const int ntiles = c->np > 0 && clear_tiles ? c->ntx * c->nty : 0;
   const int n = std::max(std::max(c->np, ntiles), RES_SLOTS);
-  hipLaunchKernelGGL(k_reset, dim3((n + 255) / 256), dim3(256), 0, st, ibm_args(c),
+  go<ResetB>(k_reset, dim3((n + 255) / 256), dim3(256), 0, st, ibm_args(c),
                      c->res_slots, ntiles > 0 ? c->tile_cnt : nullptr, ntiles, c->active);
   EIF_HIP(hipGetLastError());
   return EIF_OK;
@@ -1768,10 +1854,10 @@ int launch_inertia(Ctx *c, int cnstep, hipStream_t st, bool defer_finalize) {
   if (!st) st = c->stream;
   ProfScope ps(c, "ibm_inertia", st);
   IbmArgs a = ibm_args(c);
-  hipLaunchKernelGGL(k_heads, dim3((c->np + 255) / 256), dim3(256), 0, st, a, 1);
-  hipLaunchKernelGGL(k_inertia, dim3(c->nslices), dim3(64), 0, st, a, cnstep);
+  go<HeadsB>(k_heads, dim3((c->np + 255) / 256), dim3(256), 0, st, a, 1);
+  go<InertiaB>(k_inertia, dim3(c->nslices), dim3(64), 0, st, a, cnstep);
   if (!defer_finalize)
-    hipLaunchKernelGGL(k_ibm_finalize, dim3((c->np + 3) / 4), dim3(256), 0, st, a, cnstep);
+    go<IbmFinalizeB>(k_ibm_finalize, dim3((c->np + 3) / 4), dim3(256), 0, st, a, cnstep);
   EIF_HIP(hipGetLastError());
   return EIF_OK;
 }
@@ -1781,15 +1867,15 @@ int launch_exchange(Ctx *c, bool cached_known, bool tiles_clear, bool defer_fina
   ProfScope ps(c, "ibm_exchange");
   const size_t ntiles = (size_t)c->ntx * c->nty;
   if (!tiles_clear) {  // clear the previous call's tile registrations
-    hipLaunchKernelGGL(k_zero_int, dim3((unsigned)((ntiles + 255) / 256)), dim3(256), 0,
+    go<ZeroIntB>(k_zero_int, dim3((unsigned)((ntiles + 255) / 256)), dim3(256), 0,
                        c->stream, c->tile_cnt, (int)ntiles);
-    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, c->active, 1);
+    go<ZeroIntB>(k_zero_int, dim3(1), dim3(64), 0, c->stream, c->active, 1);
   }
   IbmArgs a = ibm_args(c);
-  hipLaunchKernelGGL(k_heads, dim3((c->np + 255) / 256), dim3(256), 0, c->stream, a, 0);
+  go<HeadsB>(k_heads, dim3((c->np + 255) / 256), dim3(256), 0, c->stream, a, 0);
   if (!cached_known)
-    hipLaunchKernelGGL(k_ibm_dist, dim3(c->nslices), dim3(64), 0, c->stream, a);
-  hipLaunchKernelGGL(k_exchange, dim3(c->nslices), dim3(64), 0, c->stream, a,
+    go<IbmDistB>(k_ibm_dist, dim3(c->nslices), dim3(64), 0, c->stream, a);
+  go<ExchangeB>(k_exchange, dim3(c->nslices), dim3(64), 0, c->stream, a,
                      cached_known ? 1 : 0);
   EIF_HIP(hipGetLastError());
   if (!defer_finalize) return launch_exchange_finalize(c, c->stream);
@@ -1800,7 +1886,7 @@ int launch_exchange(Ctx *c, bool cached_known, bool tiles_clear, bool defer_fina
 // positions without increments, into the per-cell cache
 int launch_ibm_dist(Ctx *c, hipStream_t st) {
   if (c->np == 0) return EIF_OK;
-  hipLaunchKernelGGL(k_ibm_dist, dim3(c->nslices), dim3(64), 0, st ? st : c->stream,
+  go<IbmDistB>(k_ibm_dist, dim3(c->nslices), dim3(64), 0, st ? st : c->stream,
                      ibm_args(c));
   EIF_HIP(hipGetLastError());
   return EIF_OK;
@@ -1816,7 +1902,7 @@ int launch_exchange_inertia0(Ctx *c, const double *us_x, const double *us_y) {
   IbmArgs a = ibm_args(c);
   a.us_x = us_x;
   a.us_y = us_y;
-  hipLaunchKernelGGL(k_exchange, dim3(c->nslices), dim3(64), 0, c->stream, a, 0);
+  go<ExchangeB>(k_exchange, dim3(c->nslices), dim3(64), 0, c->stream, a, 0);
   EIF_HIP(hipGetLastError());
   return EIF_OK;
 }
@@ -1825,7 +1911,7 @@ int launch_exchange_inertia0(Ctx *c, const double *us_x, const double *us_y) {
 // reads them, so this may run on another stream after the exchange
 int launch_exchange_finalize(Ctx *c, hipStream_t st) {
   if (c->np == 0) return EIF_OK;
-  hipLaunchKernelGGL(k_ibm_finalize, dim3((c->np + 3) / 4), dim3(256), 0, st, ibm_args(c), 2);
+  go<IbmFinalizeB>(k_ibm_finalize, dim3((c->np + 3) / 4), dim3(256), 0, st, ibm_args(c), 2);
   EIF_HIP(hipGetLastError());
   return EIF_OK;
 }
@@ -1860,7 +1946,7 @@ int launch_update_momentum(Ctx *c) {
   if (c->np == 0) return EIF_OK;
   ProfScope ps(c, "ibm_apply");
   TileArgs t = tile_args(c, 1);
-  hipLaunchKernelGGL(k_tiles, dim3(tile_grid(c)), dim3(TILE_THREADS), 0, c->stream, t);
+  go<TilesB>(k_tiles, dim3(tile_grid(c)), dim3(TILE_THREADS), 0, c->stream, t);
   EIF_HIP(hipGetLastError());
   return EIF_OK;
 }
@@ -1870,7 +1956,7 @@ int launch_update_momentum_x0(Ctx *c) {
   if (c->np == 0) return EIF_OK;
   ProfScope ps(c, "ibm_apply");
   TileArgs t = tile_args(c, 1);
-  hipLaunchKernelGGL(k_tiles_x0, dim3(c->np + tile_grid(c)), dim3(TILE_THREADS), 0, c->stream,
+  go<TilesX0B>(k_tiles_x0, dim3(c->np + tile_grid(c)), dim3(TILE_THREADS), 0, c->stream,
                      t, ibm_args(c));
   EIF_HIP(hipGetLastError());
   return EIF_OK;
@@ -1882,7 +1968,7 @@ int launch_materialise_du(Ctx *c) {
   EIF_HIP(hipMemsetAsync(c->duy, 0, bytes, c->stream));
   if (c->np > 0) {
     TileArgs t = tile_args(c, 0);
-    hipLaunchKernelGGL(k_tiles, dim3(tile_grid(c)), dim3(TILE_THREADS), 0, c->stream,
+    go<TilesB>(k_tiles, dim3(tile_grid(c)), dim3(TILE_THREADS), 0, c->stream,
                        t);
     EIF_HIP(hipGetLastError());
   }
@@ -1912,7 +1998,7 @@ int launch_collision(Ctx *c, int cnstep, hipStream_t st) {
   if (c->np == 0) return EIF_OK;
   if (!st) st = c->stream;
   ProfScope ps(c, "collision", st);
-  hipLaunchKernelGGL(k_collide, dim3(c->np), dim3(64), 0, st, coll_args(c, cnstep));
+  go<CollideB>(k_collide, dim3(c->np), dim3(64), 0, st, coll_args(c, cnstep));
   EIF_HIP(hipGetLastError());
   return EIF_OK;
 }
@@ -1927,7 +2013,7 @@ int launch_substep_forces(Ctx *c, int sub) {
     // after the reset): collision(1) would repeat collision(0) operand for
     // operand, so the increment copies collision(0)'s forces instead
     ProfScope ps(c, "ibm_inertia");
-    hipLaunchKernelGGL(k_inertia, dim3(c->nslices), dim3(64), 0, c->stream, a, 1);
+    go<InertiaB>(k_inertia, dim3(c->nslices), dim3(64), 0, c->stream, a, 1);
     EIF_HIP(hipGetLastError());
     return EIF_OK;
   }
@@ -1939,7 +2025,7 @@ int launch_substep_forces(Ctx *c, int sub) {
     hipExtLaunchKernelGGL(k_inertia_collide, dim3(c->np + c->nslices), dim3(64), 0, c->stream,
                           e0, e1, 0, a, ca, 1);
   else
-    hipLaunchKernelGGL(k_inertia_collide, dim3(c->np + c->nslices), dim3(64), 0, c->stream, a,
+    go<InertiaCollideB>(k_inertia_collide, dim3(c->np + c->nslices), dim3(64), 0, c->stream, a,
                        ca, 1);
   EIF_HIP(hipGetLastError());
   return EIF_OK;
@@ -1949,14 +2035,14 @@ int launch_increment(Ctx *c, int rk, int sub) {
   IbmArgs ia = ibm_args(c);
   double *res_out = c->dscal + S_RESIDUAL;
   if (sub < 0) {  // not gated: the residual starts from zero
-    hipLaunchKernelGGL(k_zero_residual, dim3(1), dim3(1), 0, c->stream, c->dscal);
+    go<ZeroResidualB>(k_zero_residual, dim3(1), dim3(1), 0, c->stream, c->dscal);
   } else {  // the residual slot was cleared at the stage start
     ia.res = c->res_slots;
     ia.sub = sub;
     res_out = c->res_slots + sub;
   }
   if (c->np > 0)
-    hipLaunchKernelGGL(k_increment, dim3((c->np + 3) / 4), dim3(256), 0, c->stream,
+    go<IncrementB>(k_increment, dim3((c->np + 3) / 4), dim3(256), 0, c->stream,
                        c->prt, c->np, c->dscal, c->gamma[rk], c->cfg.Fr, res_out, ia,
                        sub >= 0 ? 1 : 0);
   EIF_HIP(hipGetLastError());
@@ -1993,7 +2079,9 @@ int launch_status(Ctx *c, int queued) {
 // many of the first K ran and the last residual; if the loop would go on
 // (sub-iteration K is open), stops the rest of the step (halt = stage + 1)
 // for the host to resume.
-__global__ void k_stage_gate(const double *res, int K, int stage, int *counts, double *resid,
+struct StageGateB {
+  static constexpr int T = 1024, M = 0;
+  static __device__ __forceinline__ void run(const VB &vb, const double *res, int K, int stage, int *counts, double *resid,
                              int *halt) {
   if (*halt) return;
   int done = 0;
@@ -2006,9 +2094,14 @@ __global__ void k_stage_gate(const double *res, int K, int stage, int *counts, d
   resid[stage] = r;
   if (sub_open(res, K)) *halt = stage + 1;
 }
+};
+__global__ void k_stage_gate(const double *res, int K, int stage, int *counts, double *resid,
+                             int *halt) {
+  StageGateB::run(VB::here(), res, K, stage, counts, resid, halt);
+}
 
 int launch_stage_gate(Ctx *c, int stage, int K) {
-  hipLaunchKernelGGL(k_stage_gate, dim3(1), dim3(1), 0, c->stream, c->res_slots, K, stage,
+  go<StageGateB>(k_stage_gate, dim3(1), dim3(1), 0, c->stream, c->res_slots, K, stage,
                      c->g_counts, c->g_resid, c->halt);
   EIF_HIP(hipGetLastError());
   return EIF_OK;
@@ -2017,7 +2110,9 @@ int launch_stage_gate(Ctx *c, int stage, int K) {
 // Graph steps: the step's record for the host (two slots, by the parity of
 // the device's step counter), the record number stored last with
 // system-scope release.  Not gated: a stopped step reports its stop.
-__global__ void k_graph_status(const int *halt, const int *counts, const double *resid,
+struct GraphStatusB {
+  static constexpr int T = 1024, M = 0;
+  static __device__ __forceinline__ void run(const VB &vb, const int *halt, const int *counts, const double *resid,
                                const double *dscal, const int *flags, unsigned *seq,
                                GraphStatus *h) {
   const unsigned s = *seq + 1;
@@ -2032,9 +2127,15 @@ __global__ void k_graph_status(const int *halt, const int *counts, const double 
   o->flags = flags[0];
   __hip_atomic_store(&o->seq, s, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
+};
+__global__ void k_graph_status(const int *halt, const int *counts, const double *resid,
+                               const double *dscal, const int *flags, unsigned *seq,
+                               GraphStatus *h) {
+  GraphStatusB::run(VB::here(), halt, counts, resid, dscal, flags, seq, h);
+}
 
 int launch_graph_status(Ctx *c) {
-  hipLaunchKernelGGL(k_graph_status, dim3(1), dim3(1), 0, c->stream, c->halt, c->g_counts,
+  go<GraphStatusB>(k_graph_status, dim3(1), dim3(1), 0, c->stream, c->halt, c->g_counts,
                      c->g_resid, c->dscal, c->flags, c->g_seq, c->gstat);
   EIF_HIP(hipGetLastError());
   return EIF_OK;
@@ -2042,7 +2143,7 @@ int launch_graph_status(Ctx *c) {
 
 int launch_update_particles(Ctx *c, hipStream_t st) {
   if (c->np == 0) return EIF_OK;
-  hipLaunchKernelGGL(k_update_particles, dim3((c->np + 255) / 256), dim3(256), 0,
+  go<UpdateParticlesB>(k_update_particles, dim3((c->np + 255) / 256), dim3(256), 0,
                      st ? st : c->stream, c->prt, c->np, c->ly, c->halt);
   EIF_HIP(hipGetLastError());
   return EIF_OK;

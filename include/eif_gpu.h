@@ -228,8 +228,11 @@ int eif_gpu_autotune(eif_ctx *ctx, int flags, char *report, int report_len);
  * one captured graph on its own stream, all of a step's graphs launched
  * before the host waits for any (graph mode is switched on for them); 2:
  * as 1, with every context's step graph nested in one graph, so that an
- * ensemble step is a single launch.  Results equal each context's plain
- * run bit for bit.  Synchronising. */
+ * ensemble step is a single launch; 3: batched launches, each kernel of the
+ * step launched once for up to 64 contexts (per-context argument records in
+ * device memory), the whole ensemble step one captured graph.  A context
+ * appears at most once.  Results equal each context's plain run bit for
+ * bit.  Synchronising. */
 int eif_gpu_ensemble_run_steps(eif_ctx **ctxs, int ncases, int nsteps, int concurrent);
 
 /* Adjoint of one RK stage (no counterpart in the reference).  The stage

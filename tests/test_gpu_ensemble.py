@@ -1,8 +1,9 @@
 """Ensembles of independent cases (eif_gpu_ensemble_run_steps): the
-concurrent path (each case's steps as captured graphs on its own stream,
-every case's step launched before any is waited for) against each case run
-alone through the plain path, bit for bit, on cases of uneven grids and
-particle counts drawn like the benchmark's ensemble."""
+concurrent paths (each case's steps as captured graphs on its own stream;
+all cases' step graphs in one graph; batched launches that run each kernel
+of a step once for all cases from per-case argument records) against each
+case run alone through the plain path, bit for bit, on cases of uneven
+grids and particle counts drawn like the benchmark's ensemble."""
 import numpy as np
 import pytest
 
@@ -51,11 +52,11 @@ def test_concurrent_ensemble_equals_each_case_alone():
         assert np.array_equal(g.particles(), alone[k][1]), k
         launches, resumes, captures, kk = g.graph_stats()
         assert launches >= steps - 1
-    # the sequential form, and one merged graph launch per ensemble step,
-    # give the same again
+    # the sequential form, one merged graph launch per ensemble step, and
+    # batched launches (each kernel once for all cases) give the same again
     for g in solvers:
         g.close()
-    for mode in (0, 2):
+    for mode in (0, 2, 3):
         solvers = [_make(cfg, parts) for cfg, parts in sel]
         eif.ensemble_run_steps(solvers, steps, concurrent=mode)
         for k, g in enumerate(solvers):
@@ -64,3 +65,38 @@ def test_concurrent_ensemble_equals_each_case_alone():
                 assert np.array_equal(f[name], alone[k][0][name]), (mode, k, name)
             assert np.array_equal(g.particles(), alone[k][1]), (mode, k)
             g.close()
+
+
+def test_batched_ensemble_longer_run_and_rebuilds():
+    """Batched launches over more steps than a graph is kept for: the
+    sub-iteration bound of each stage follows the cases (a case whose loop
+    needs more stops at the gate and is resumed alone; bounds that fall
+    rebuild the batch), and a second call continues from the first."""
+    cases = initparticles.ensemble_cases(32, seed=5)
+    sel = [c for c in cases if c[0]["itot"] <= 512][:5]
+    print("ensemble of", [(c["itot"], len(p["a"])) for c, p in sel], flush=True)
+    alone = []
+    for cfg, parts in sel:
+        g = _make(cfg, parts)
+        g.run_steps(30)
+        alone.append((g.download(), g.particles()))
+        g.close()
+    solvers = [_make(cfg, parts) for cfg, parts in sel]
+    eif.ensemble_run_steps(solvers, 12, concurrent=3)
+    eif.ensemble_run_steps(solvers, 18, concurrent=3)
+    for k, g in enumerate(solvers):
+        f = g.download()
+        for name in ("ux", "uy", "p"):
+            assert np.array_equal(f[name], alone[k][0][name]), (k, name)
+        assert np.array_equal(g.particles(), alone[k][1]), k
+        g.close()
+
+
+def test_ensemble_rejects_repeated_context_and_bad_mode():
+    cases = initparticles.ensemble_cases(2, seed=1)
+    g = _make(*cases[0])
+    with pytest.raises(eif.EIFError):
+        eif.ensemble_run_steps([g, g], 1, concurrent=3)
+    with pytest.raises(eif.EIFError):
+        eif.ensemble_run_steps([g], 1, concurrent=4)
+    g.close()
