@@ -1380,84 +1380,143 @@ static bool batch_matches(const BatchGraph &b, const std::vector<Ctx *> &cs, con
   return true;
 }
 
+// One batched ensemble step launched: the cases of `run` in chunks of
+// BATCH_MAX, each chunk's graph (cached, or recorded and captured now) on
+// its own stream; every launched case's record number is taken.  `busy`:
+// graphs still running, which the cache keeps.
+struct StepLaunch {
+  std::vector<BatchGraph *> graphs;
+  std::vector<unsigned> seq;  // per case of each graph, in order
+};
+
+static int batch_launch_step(const std::vector<Ctx *> &run, const StepLaunch &busy,
+                             StepLaunch *out) {
+  out->graphs.clear();
+  out->seq.clear();
+  int K[3] = {0, 0, 0};
+  for (Ctx *c : run)
+    for (int r = 0; r < 3; r++) K[r] = std::max(K[r], c->graphs->K[r]);
+  for (size_t k0 = 0; k0 < run.size(); k0 += BATCH_MAX) {
+    const std::vector<Ctx *> cs(run.begin() + k0,
+                                run.begin() + std::min(run.size(), k0 + BATCH_MAX));
+    BatchGraph *b = nullptr;
+    for (auto &x : g_batches)
+      if (batch_matches(*x, cs, K)) b = x.get();
+    if (!b) {
+      // a cached step that is not running is idle
+      auto in_use = [&](BatchGraph *x) {
+        return std::find(busy.graphs.begin(), busy.graphs.end(), x) != busy.graphs.end() ||
+               std::find(out->graphs.begin(), out->graphs.end(), x) != out->graphs.end();
+      };
+      for (size_t j = 0; j < g_batches.size() && g_batches.size() >= BATCH_CACHE;) {
+        if (!in_use(g_batches[j].get())) {
+          batch_free(g_batches[j].get());
+          g_batches.erase(g_batches.begin() + j);
+        } else {
+          j++;
+        }
+      }
+      g_batches.emplace_back(new BatchGraph());
+      b = g_batches.back().get();
+      const int rc = batch_build(cs, K, b);
+      if (rc) {
+        batch_free(b);
+        g_batches.pop_back();
+        return rc;
+      }
+      for (Ctx *c : cs) c->graphs->captures++;
+    }
+    // the chunk's cases are idle on their own streams (host-synchronised)
+    EIF_HIP(hipGraphLaunch(b->exec, b->stream));
+    out->graphs.push_back(b);
+    for (Ctx *c : b->cases) {
+      c->graphs->launches++;
+      out->seq.push_back(++c->graphs->seq);
+    }
+  }
+  return EIF_OK;
+}
+
 // nsteps graph steps of every case, as batched ensemble steps of up to
-// BATCH_MAX cases each (chunks run on their own streams); *nodes_out:
-// batched launches per ensemble step of the last chunk
+// BATCH_MAX cases each (chunks on their own streams).  Steps are pipelined
+// like eif_gpu_run_steps: the next batched step is launched before the
+// records of the previous one are read.  A case that stopped at a gate
+// (its loop needed more sub-iterations than captured) did nothing in the
+// step behind it; after that step has drained, its record is consumed, the
+// case is resumed alone and its step is run again later.
 static int ensemble_batched(eif_ctx **ctxs, int ncases, int nsteps, int *nodes_out) {
-  std::vector<int> done(ncases, 0);
+  std::vector<int> queued(ncases, 0);  // steps launched (or run) per case
   for (int k = 0; k < ncases; k++) {
     bool ran = false;
     int sub[3];
     if (nsteps > 0) TRY(graph_prepare(ctxs[k], &ran, sub));
-    done[k] = ran ? 1 : 0;
+    queued[k] = ran ? 1 : 0;
   }
   // the batched launches run on their own streams, after each case's queued work
   for (int k = 0; k < ncases; k++) EIF_HIP(hipStreamSynchronize(ctxs[k]->stream));
+  auto index = [&](Ctx *c) { return (int)(std::find(ctxs, ctxs + ncases, c) - ctxs); };
   std::lock_guard<std::mutex> lk(g_batch_mu);
+  StepLaunch prev, cur;
+  std::vector<char> skip(ncases, 0);  // case's record in `prev` already consumed
   int rc = EIF_OK;
-  while (!rc) {
+  for (;;) {
     std::vector<Ctx *> run;
-    int K[3] = {0, 0, 0};
     for (int k = 0; k < ncases; k++)
-      if (done[k] < nsteps) {
-        run.push_back(ctxs[k]);
-        for (int r = 0; r < 3; r++) K[r] = std::max(K[r], ctxs[k]->graphs->K[r]);
-      }
-    if (run.empty()) break;
-    std::vector<BatchGraph *> step;
-    for (size_t k0 = 0; k0 < run.size() && !rc; k0 += BATCH_MAX) {
-      const std::vector<Ctx *> cs(run.begin() + k0,
-                                  run.begin() + std::min(run.size(), k0 + BATCH_MAX));
-      BatchGraph *b = nullptr;
-      for (auto &x : g_batches)
-        if (batch_matches(*x, cs, K)) b = x.get();
-      if (!b) {
-        // every cached step not launched in this step is idle (each step
-        // is waited for before the next)
-        for (size_t j = 0; j < g_batches.size() && g_batches.size() >= BATCH_CACHE;) {
-          if (std::find(step.begin(), step.end(), g_batches[j].get()) == step.end()) {
-            batch_free(g_batches[j].get());
-            g_batches.erase(g_batches.begin() + j);
-          } else {
-            j++;
-          }
-        }
-        g_batches.emplace_back(new BatchGraph());
-        b = g_batches.back().get();
-        rc = batch_build(cs, K, b);
-        if (rc) {
-          batch_free(b);
-          g_batches.pop_back();
-          break;
-        }
-        for (Ctx *c : cs) c->graphs->captures++;
-      }
-      if (hipGraphLaunch(b->exec, b->stream) != hipSuccess)
-        rc = hip_fail(hipErrorUnknown, "hipGraphLaunch", __FILE__, __LINE__);
-      else
-        step.push_back(b);
-      if (nodes_out) *nodes_out = b->nodes;
+      if (queued[k] < nsteps) run.push_back(ctxs[k]);
+    cur.graphs.clear();
+    cur.seq.clear();
+    if (!run.empty()) {
+      if ((rc = batch_launch_step(run, prev, &cur))) break;
+      for (Ctx *c : run) queued[index(c)]++;
+      if (nodes_out && !cur.graphs.empty()) *nodes_out = cur.graphs.back()->nodes;
     }
-    for (BatchGraph *b : step)
+    if (prev.graphs.empty() && cur.graphs.empty()) break;
+    // the records of the previous step
+    bool drained = false;  // cur has finished (before any resumption)
+    size_t q = 0;
+    for (BatchGraph *b : prev.graphs) {
       if (!rc && hipStreamSynchronize(b->stream) != hipSuccess)
         rc = hip_fail(hipErrorUnknown, "hipStreamSynchronize", __FILE__, __LINE__);
-    for (BatchGraph *b : step) {
-      for (size_t j = 0; j < b->cases.size() && !rc; j++) {
+      for (size_t j = 0; j < b->cases.size() && !rc; j++, q++) {
         Ctx *c = b->cases[j];
-        c->graphs->launches++;
-        const unsigned seq = ++c->graphs->seq;
+        const int k = index(c);
+        if (skip[k]) continue;
         GraphStatus st;
-        if ((rc = graph_wait(c, seq, &st))) break;
+        if ((rc = graph_wait(c, prev.seq[q], &st))) break;
+        if (st.halt) {
+          // the case's step in cur did nothing; once cur has drained, its
+          // record (the same stop) is consumed and the step queued again
+          if (!drained) {
+            for (BatchGraph *g : cur.graphs)
+              if (!rc && hipStreamSynchronize(g->stream) != hipSuccess)
+                rc = hip_fail(hipErrorUnknown, "hipStreamSynchronize", __FILE__, __LINE__);
+            drained = true;
+          }
+          size_t qc = 0;
+          for (BatchGraph *g : cur.graphs)
+            for (Ctx *x : g->cases) {
+              if (x == c && !rc) {
+                GraphStatus skipped;
+                rc = graph_wait(c, cur.seq[qc], &skipped);
+                c->graphs->launches--;
+                queued[k]--;
+                skip[k] = 2;  // consumed in cur
+              }
+              qc++;
+            }
+          if (rc) break;
+        }
         double dt;
         int sub[3];
         if ((rc = graph_finish(c, &b->entries[j], st, &dt, sub))) break;
-        // a resumption ran on the case's own stream, ahead of the next batch
         if (st.halt && hipStreamSynchronize(c->stream) != hipSuccess)
           rc = hip_fail(hipErrorUnknown, "hipStreamSynchronize", __FILE__, __LINE__);
-        const int k = (int)(std::find(ctxs, ctxs + ncases, c) - ctxs);
-        done[k]++;
       }
     }
+    if (rc) break;
+    // cases whose cur record was consumed are skipped when cur is read
+    for (int k = 0; k < ncases; k++) skip[k] = skip[k] == 2 ? 1 : 0;
+    prev = cur;
   }
   for (auto &x : g_batches)
     if (x->stream) (void)hipStreamSynchronize(x->stream);

@@ -83,7 +83,13 @@ def test_batched_ensemble_longer_run_and_rebuilds():
         g.close()
     solvers = [_make(cfg, parts) for cfg, parts in sel]
     eif.ensemble_run_steps(solvers, 12, concurrent=3)
+    # one sub-iteration per stage: every case whose loops need more stops at
+    # a gate (with the pipelined step behind it already launched) and is
+    # resumed alone until its bounds have grown again
+    for g in solvers:
+        g.graph_set_subiterations((1, 1, 1))
     eif.ensemble_run_steps(solvers, 18, concurrent=3)
+    assert sum(g.graph_stats()[1] for g in solvers) > 0
     for k, g in enumerate(solvers):
         f = g.download()
         for name in ("ux", "uy", "p"):
