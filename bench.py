@@ -135,9 +135,12 @@ def cpu_baseline(parts, xc, warm=1, timed=4):
 # algorithmic HBM bytes per launch of the fluid kernels, in units of
 # N = nx*ny values of 8 B (SURVEY-style compulsory traffic): the predictor
 # reads ux, uy, p and (stages 1-2) the kept previous velocity, and writes
-# the new velocity: 5N at stage 0, 7N after, 19/3 N on average
+# the new velocity: 5N at stage 0, 7N after, 19/3 N on average; the fused
+# DCT-III + projection reads the solved coefficients, ux, uy and p and
+# writes ux, uy and p (7N; psi is never stored)
 FLUID_BYTES_N = {"predictor": 19.0 / 3.0, "rhs_dct2": 3.0, "ysolve_partition": 2.0,
-                 "thomas": 2.0, "dct3_psi": 2.0, "correct_pressure": 7.0}
+                 "thomas": 2.0, "dct3_psi": 2.0, "correct_pressure": 7.0,
+                 "dct3_project": 7.0}
 
 
 ENSEMBLE_SEED = 3

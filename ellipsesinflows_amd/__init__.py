@@ -51,7 +51,7 @@ EXPORTED = (
     "suspensions_gpu_update_particles", "suspensions_gpu_colliding_pairs",
     "eif_gpu_integrate", "eif_gpu_step", "eif_gpu_synchronize", "eif_gpu_last_error",
     "eif_gpu_run_steps", "eif_gpu_graph_enable", "eif_gpu_graph_stats",
-    "eif_gpu_graph_set_subiterations", "eif_gpu_ensemble_run_steps",
+    "eif_gpu_graph_set_subiterations", "eif_gpu_ensemble_run_steps", "eif_gpu_set_projection",
     "eif_gpu_tape_begin", "eif_gpu_step_fixed_dt", "eif_gpu_stage_adjoint",
     "eif_gpu_adjoint_get", "eif_gpu_adjoint_add", "fluid_gpu_wall_shear",
     "eif_gpu_adjoint_add_wall_shear", "eif_gpu_autotune",
@@ -395,6 +395,11 @@ class Solver:
 
     def graph(self, enable=True):
         self._check(self.lib.eif_gpu_graph_enable(self.h, int(bool(enable))))
+
+    def set_projection(self, strip):
+        """0: DCT-III and projection as two kernels (psi stored); 2, 4, 8:
+        one fused kernel over strips of that many rows (psi not stored)."""
+        self._check(self.lib.eif_gpu_set_projection(self.h, int(strip)))
 
     def graph_set_subiterations(self, k):
         arr = (ctypes.c_int * 3)(*[int(v) for v in k])

@@ -29,7 +29,8 @@ int hip_fail(hipError_t e, const char *what, const char *file, int line) {
 // profiling: HIP events around each named launch on the context stream
 // ---------------------------------------------------------------------
 static const char *k_prof_names =
-    "bc,predictor,rhs_dct2,thomas,ysolve_partition,dct3_psi,correct_pressure,decide_dt,"
+    "bc,predictor,rhs_dct2,thomas,ysolve_partition,dct3_psi,correct_pressure,dct3_project,"
+    "decide_dt,"
     "ibm_inertia,ibm_substep,ibm_exchange,ibm_apply,collision";
 
 // Events that only time work: without the system-scope fence, recording one
@@ -876,10 +877,10 @@ static int stage_start(Ctx *c, int rk, bool pred_queued, bool side_start) {
   // the deposit, with the exchange's and inertia(0)'s particle sums in the
   // same launch
   TRY(launch_update_momentum_x0(c));
-  TRY(launch_poisson(c, rk));
-  // the last stage also reduces the advective dt limit of the final
-  // velocities for the next param_decide_dt
-  TRY(launch_correct_pressure(c, rk, true, true, rk == 2));
+  // the potential, the projection and the pressure update, fused where the
+  // row length allows; the last stage also reduces the advective dt limit
+  // of the final velocities for the next param_decide_dt
+  TRY(launch_poisson_project(c, rk, rk == 2));
   if (c->recording) TRY(rec_fields(c, 2, c->ux, c->uy));  // u^{k+1}
   // the sub-iterations read collision(0)
   EIF_HIP(hipStreamWaitEvent(c->stream, c->ev_c0, 0));
@@ -1719,6 +1720,17 @@ int eif_gpu_autotune(eif_ctx *c, int flags, char *report, int report_len) {
       c->cfg.poisson_mode = mode_was;
     }
   }
+  // strip height of the fused DCT-III + projection, for the chosen y-solve
+  if (!rc) {
+    double *pt = nullptr;
+    if (hipMalloc(&pt, c->L.p_len() * sizeof(double)) != hipSuccess) {
+      rc = hip_fail(hipErrorOutOfMemory, "hipMalloc", __FILE__, __LINE__);
+    } else {
+      rep += "; ";
+      rc = project_tune(c, c->ux_spare, c->uy_spare, pt, &rep);
+      (void)hipFree(pt);
+    }
+  }
   c->dt_adv_parts = parts_was;
   c->dt_adv_valid = dtv_was;
   c->prof->on = prof_was;
@@ -1727,6 +1739,11 @@ int eif_gpu_autotune(eif_ctx *c, int flags, char *report, int report_len) {
     report[report_len - 1] = 0;
   }
   return rc;
+}
+
+int eif_gpu_set_projection(eif_ctx *c, int strip) {
+  CHECK_CTX(c);
+  return set_projection(c, strip);
 }
 
 int eif_gpu_graph_enable(eif_ctx *c, int enable) {

@@ -148,6 +148,9 @@ struct Ctx {
   int part_L = 0, part_P = 0;
   // tuned launch shapes (eif_gpu_autotune); 0 is the built-in default
   int pred_rows = 0, cp_rows = 0;
+  // rows per strip of the fused DCT-III + projection (2, 4, 8), or 0: the
+  // unfused kernels (the default)
+  int proj_strip = 0;
   double *pt_fwd = nullptr, *pt_u = nullptr, *pt_v = nullptr, *pt_w = nullptr;
   double *pt_bnd = nullptr, *pt_red = nullptr, *pt_lvl = nullptr;
   double *pt_ends = nullptr, *pt_g = nullptr;
@@ -270,6 +273,15 @@ int poisson_init(Ctx *c);
 int partition_layout(int ny, int *chunk, int *nchunks);
 // rhs_add (cell field, P layout): solve for div u / (gamma dt) + rhs_add / (gamma dt)^2
 int launch_poisson(Ctx *c, int rk, const double *rhs_add = nullptr);
+// the stage's potential, projection and pressure update (launch_poisson +
+// launch_correct_pressure(correct, pressure)), in one fused kernel after the
+// solves where nx is a power of two in 64..8192; psi is then not stored
+int launch_poisson_project(Ctx *c, int rk, bool dt_fused);
+bool project_fusable(const Ctx *c);
+// the tuner's choice of unfused kernels or fused strip height, timed on the
+// given copies of the fields
+int project_tune(Ctx *c, double *ux, double *uy, double *p, std::string *rep);
+int set_projection(Ctx *c, int strip);
 int susp_init(Ctx *c);
 // defer_finalize: the slice sums are added by the next launch_increment
 // with fin_inertia (one launch fewer per sub-iteration)

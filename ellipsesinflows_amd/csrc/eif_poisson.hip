@@ -33,6 +33,12 @@
 
 namespace eif {
 
+#define TRY_P(x)          \
+  do {                    \
+    const int _r = (x);   \
+    if (_r) return _r;    \
+  } while (0)
+
 constexpr int DCT_THREADS = 256;
 
 __device__ inline int bitrev(int v, int logn) {
@@ -126,11 +132,11 @@ __device__ inline void phase_twiddles(int p, int t, const double *__restrict__ W
   }
 }
 
+// t: the calling thread's index in the workgroup
 template <int LOGN, int LOGR, int SIGN>
 __device__ inline void reg_fft_t(double (&xr)[1 << LOGR], double (&xi)[1 << LOGR], double *lre,
-                                 double *lim, const double *__restrict__ W) {
+                                 double *lim, const double *__restrict__ W, int t) {
   constexpr int R = 1 << LOGR;
-  const int t = threadIdx.x;
   constexpr int NP = (LOGN + LOGR - 1) / LOGR;
   // each phase's twiddles are loaded before the LDS exchange that precedes
   // the phase, so their latency overlaps the exchange
@@ -385,7 +391,7 @@ struct RhsDct2TB {
     xr[s] = re[sw(R * threadIdx.x + s)];
     xi[s] = im[sw(R * threadIdx.x + s)];
   }
-  reg_fft_t<LOGN, LOGR, -1>(xr, xi, re, im, W);
+  reg_fft_t<LOGN, LOGR, -1>(xr, xi, re, im, W, threadIdx.x);
   constexpr int NP = (LOGN + LOGR - 1) / LOGR;
   const RegMap ml = reg_phase_r<LOGR>(NP - 1, LOGN);
   __syncthreads();
@@ -412,6 +418,50 @@ __global__ __launch_bounds__((1 << LOGN) >> dct_logr<LOGN>()) void k_rhs_dct2_t(
     double dx, double dy, const double *__restrict__ C, const double *__restrict__ W,
     double *qhat, const double *add = nullptr) {
   RhsDct2TB<LOGN, P2, ADD>::run(VB::here(), L, ux, uy, dscal, ga, dx, dy, C, W, qhat, add);
+}
+
+// The DCT-III (compute_potential.c:118-137) of two rows packed as one
+// complex transform: re, im hold the rows' combined coefficients at sw(k)
+// (written and synchronised by the caller); on return they hold 2 nx psi
+// of the two rows in the Makhoul order, cell i at sw(i / 2) for even i and
+// at sw(nx - 1 - i / 2) for odd i (see psi_at).
+template <int LOGN>
+__device__ __forceinline__ void dct3_pair_lds(double *re, double *im,
+                                              const double *__restrict__ C,
+                                              const double *__restrict__ W, int t) {
+  constexpr int LOGR = dct_logr<LOGN>(), R = 1 << LOGR;
+  constexpr int nx = 1 << LOGN;
+  double xr[R], xi[R];
+#pragma unroll
+  for (int s = 0; s < R; s++) {
+    const int k = bitrev(R * t + s, LOGN);
+    const double ar = re[sw(k)], ai = k == 0 ? 0. : -re[sw(nx - k)];
+    const double br = im[sw(k)], bi = k == 0 ? 0. : -im[sw(nx - k)];
+    const double var = C[k] * ar - C[nx - k] * ai, vai = C[k] * ai + C[nx - k] * ar;
+    const double vbr = C[k] * br - C[nx - k] * bi, vbi = C[k] * bi + C[nx - k] * br;
+    xr[s] = var - vbi;
+    xi[s] = vai + vbr;
+  }
+  reg_fft_t<LOGN, LOGR, +1>(xr, xi, re, im, W, t);
+  constexpr int NP = (LOGN + LOGR - 1) / LOGR;
+  const RegMap ml = reg_phase_r<LOGR>(NP - 1, LOGN);
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < R; s++) {
+    const int q = sw(ml.pos(t, s));
+    re[q] = xr[s];
+    im[q] = xi[s];
+  }
+  __syncthreads();
+}
+
+// psi of cell i (0-based) of a row left by dct3_pair_lds
+template <int LOGN>
+__device__ __forceinline__ double psi_at(const double *src, int i) {
+  constexpr int nx = 1 << LOGN;
+  constexpr double inv_norm = 1. / (2. * nx);  // a power of two: exact product
+  const double v = (i & 1) ? src[sw(nx - 1 - (i >> 1))] : src[sw(i >> 1)];
+  return v * inv_norm;
 }
 
 template <int LOGN>
@@ -461,28 +511,7 @@ struct Dct3PsiTB {
     }
   }
   __syncthreads();
-  double xr[R], xi[R];
-#pragma unroll
-  for (int s = 0; s < R; s++) {
-    const int k = bitrev(R * threadIdx.x + s, LOGN);
-    const double ar = re[sw(k)], ai = k == 0 ? 0. : -re[sw(nx - k)];
-    const double br = im[sw(k)], bi = k == 0 ? 0. : -im[sw(nx - k)];
-    const double var = C[k] * ar - C[nx - k] * ai, vai = C[k] * ai + C[nx - k] * ar;
-    const double vbr = C[k] * br - C[nx - k] * bi, vbi = C[k] * bi + C[nx - k] * br;
-    xr[s] = var - vbi;
-    xi[s] = vai + vbr;
-  }
-  reg_fft_t<LOGN, LOGR, +1>(xr, xi, re, im, W);
-  constexpr int NP = (LOGN + LOGR - 1) / LOGR;
-  const RegMap ml = reg_phase_r<LOGR>(NP - 1, LOGN);
-  __syncthreads();
-#pragma unroll
-  for (int s = 0; s < R; s++) {
-    const int q = sw(ml.pos(threadIdx.x, s));
-    re[q] = xr[s];
-    im[q] = xi[s];
-  }
-  __syncthreads();
+  dct3_pair_lds<LOGN>(re, im, C, W, threadIdx.x);
   // psi rows and their boundary values (the Neumann walls, and the periodic
   // halo rows 0 and ny+1 copied from rows ny and 1: k_bc_p's values)
 #pragma unroll
@@ -513,6 +542,171 @@ __global__ __launch_bounds__((1 << LOGN) >> dct_logr<LOGN>()) void k_dct3_psi_t(
     Layout L, const double *qhat, CombineArgs ca, const double *__restrict__ C,
     const double *__restrict__ W, double *psi) {
   Dct3PsiTB<LOGN>::run(VB::here(), L, qhat, ca, C, W, psi);
+}
+
+// ---------------------------------------------------------------------
+// DCT-III fused with the projection and the pressure update
+// (compute_potential.c:118-137, correct_velocity.c:16-41,
+// update_pressure.c:14-23): one workgroup per strip of S rows (S even).
+// Rows are transformed in the pairs (2q, 2q+1) of k_dct3_psi_t's Thomas
+// mode, so psi has the same bits; each row's psi stays in LDS and
+// registers and is never stored.  A thread owns cells i = tid + s T of
+// every row: psi(i - 1, j) comes from LDS, psi(i, j - 1) from the row
+// before, kept in registers (for the strip's first row, the pair holding
+// row j - 1, periodic, is transformed first).  The arithmetic and the
+// boundary values written are those of k_correct_pressure; with DT the
+// advective time-step limit is left as one partial minimum per strip.
+// ---------------------------------------------------------------------
+template <int LOGN, bool P2, bool DT>
+struct ProjectTB {
+  // at least 4 waves per SIMD (at most 128 VGPRs), as the unfused DCT kernels
+  static constexpr int T = (1 << LOGN) >> dct_logr<LOGN>(), M = 4;
+
+  // the pair of rows (pb, pb + 1) into LDS
+  static __device__ __forceinline__ void pair(const CombineArgs &ca, const double *qhat,
+                                              int ny, int pb, double *re, double *im,
+                                              const double *__restrict__ C,
+                                              const double *__restrict__ W, int t) {
+    constexpr int LOGR = dct_logr<LOGN>(), R = 1 << LOGR;
+    constexpr int nx = 1 << LOGN;
+    constexpr int TT = nx / R;
+    const bool hasb = pb + 1 < ny;
+    __syncthreads();  // the previous pair's values are read
+#pragma unroll
+    for (int s = 0; s < R; s++) {
+      const int k = t + s * TT;
+      re[sw(k)] = combined(ca, qhat, nx, ny, pb, k);
+      im[sw(k)] = hasb ? combined(ca, qhat, nx, ny, pb + 1, k) : 0.;
+    }
+    __syncthreads();
+    dct3_pair_lds<LOGN>(re, im, C, W, t);
+  }
+
+  // the projection and pressure update of row j (k_correct_pressure's
+  // arithmetic and boundary values; ny >= 2, so that row j has at most one
+  // periodic halo copy: row ny to row 0, row 1 to row ny + 1)
+  static __device__ __forceinline__ void row(const Layout &L, const double *src, int t, int j,
+                                             double *ux, double *uy, double *p,
+                                             const double *dxc, double ga, double dt,
+                                             double dy, double (&prev)[1 << dct_logr<LOGN>()],
+                                             double &m) {
+    constexpr int LOGR = dct_logr<LOGN>(), R = 1 << LOGR;
+    constexpr int nx = 1 << LOGN;
+    constexpr int TT = nx / R;
+    const int ny = L.ny;
+    // the row's fields from uniform row bases (32-bit cell offsets):
+    // ux(i, j) = uxr[i], uy(i, j) = uyr[i], p(i, j) = pr[i]; *h the halo copy
+    const int jh = j == ny ? 0 : (j == 1 ? ny + 1 : -1);
+    double *uxr = ux + L.ux(0, j), *uyr = uy + L.p(0, j), *pr = p + L.p(0, j);
+    double *uxh = jh >= 0 ? ux + L.ux(0, jh) : nullptr;
+    double *uyh = jh >= 0 ? uy + L.p(0, jh) : nullptr;
+    double *ph = jh >= 0 ? p + L.p(0, jh) : nullptr;
+#pragma unroll
+    for (int s = 0; s < R; s++) {
+      const int i = t + s * TT + 1;  // 1..nx
+      const double ps_c = psi_at<LOGN>(src, i - 1);
+      if (i >= 2) {
+        const double ps_w = psi_at<LOGN>(src, i - 2);
+        const double dxi = 1. / dxc[i - 1];
+        const double v = uxr[i] - ga * dt * (ps_c - ps_w) * dxi;
+        uxr[i] = v;
+        if (uxh) uxh[i] = v;
+        if (DT) m = fmin(m, dxc[i - 1] / fmax(fabs(v), 1.e-8));
+      }
+      {
+        const double v = uyr[i] - qdiv<P2>(ga * dt * (ps_c - prev[s]), dy, 1. / dy);
+        uyr[i] = v;
+        if (uyh) uyh[i] = v;
+        if (DT) m = fmin(m, dy / fmax(fabs(v), 1.e-8));
+      }
+      {
+        const double v = pr[i] + ps_c;
+        pr[i] = v;
+        if (ph) ph[i] = v;
+        // the Neumann walls (with their halo corners)
+        if (i == 1) {
+          pr[0] = v;
+          if (ph) ph[0] = v;
+        }
+        if (i == nx) {
+          pr[nx + 1] = v;
+          if (ph) ph[nx + 1] = v;
+        }
+      }
+      prev[s] = ps_c;
+    }
+  }
+
+  static __device__ __forceinline__ void run(const VB &vb, Layout L, const double *qhat,
+                                             CombineArgs ca, const double *__restrict__ C,
+                                             const double *__restrict__ W, int S, double *ux,
+                                             double *uy, double *p, const double *dxc,
+                                             const double *dscal, double ga, double dy,
+                                             double *dt_part) {
+    if (halted(ca.halt)) return;
+    extern __shared__ double sh[];
+    constexpr int LOGR = dct_logr<LOGN>(), R = 1 << LOGR;
+    constexpr int nx = 1 << LOGN;
+    constexpr int TT = nx / R;
+    const int ny = L.ny;
+    double *re = sh, *im = sh + nx;
+    const double dt = dscal[S_DT];
+    const int r0 = vb.bx * S, r1 = min(r0 + S, ny);  // rows r0 .. r1-1 (0-based)
+    double m = 5.e-2;
+    double prev[R];
+    // the pair holding row r0 - 1 (periodic) first, for prev only; then the
+    // strip's pairs (one call site of the transform)
+    const int rp = r0 == 0 ? ny - 1 : r0 - 1;
+    int pb = rp & ~1;
+    bool first = true;
+    for (;;) {
+      // the tables' addresses and the thread index made opaque per pair:
+      // the transform's table loads and index arithmetic are redone per
+      // pair instead of hoisted out of the loop (every phase's twiddles and
+      // positions would be held in registers at once)
+      const double *Cl = C, *Wl = W;
+      int t = threadIdx.x;
+      asm volatile("" : "+s"(Cl), "+s"(Wl), "+v"(t));
+      pair(ca, qhat, ny, pb, re, im, Cl, Wl, t);
+      if (first) {
+        const double *src = (rp & 1) ? im : re;
+#pragma unroll
+        for (int s = 0; s < R; s++) prev[s] = psi_at<LOGN>(src, t + s * TT);
+        first = false;
+        pb = r0;
+        continue;
+      }
+#pragma unroll 1
+      for (int h = 0; h < 2; h++) {
+        const int r = pb + h;
+        if (r >= r1) break;
+        const double *src = h ? im : re;
+        const int j = r + 1;
+        row(L, src, t, j, ux, uy, p, dxc, ga, dt, dy, prev, m);
+      }
+      pb += 2;
+      if (pb >= r1) break;
+    }
+    if (DT) {
+      __syncthreads();  // sh is free
+      double *red = sh;
+      red[threadIdx.x] = m;
+      __syncthreads();
+      for (int st = TT / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + st]);
+        __syncthreads();
+      }
+      if (threadIdx.x == 0) dt_part[vb.bx] = red[0];
+    }
+  }
+};
+template <int LOGN, bool P2, bool DT>
+__global__ __launch_bounds__((1 << LOGN) >> dct_logr<LOGN>(), 4) void k_project_t(
+    Layout L, const double *qhat, CombineArgs ca, const double *__restrict__ C,
+    const double *__restrict__ W, int S, double *ux, double *uy, double *p, const double *dxc,
+    const double *dscal, double ga, double dy, double *dt_part) {
+  ProjectTB<LOGN, P2, DT>::run(VB::here(), L, qhat, ca, C, W, S, ux, uy, p, dxc, dscal, ga, dy,
+                               dt_part);
 }
 
 // ---------------------------------------------------------------------
@@ -1086,7 +1280,9 @@ static void rhs_t(Ctx *c, int rk, size_t shmem, const double *rhs_add) {
                              c->qhat, rhs_add);
 }
 
-int launch_poisson(Ctx *c, int rk, const double *rhs_add) {
+// the right-hand side and its DCT-II along x, then the solves along y;
+// *ca: how the DCT-III stage combines the solves' results
+static int poisson_lines(Ctx *c, int rk, const double *rhs_add, CombineArgs *out) {
   const int nx = c->nx, ny = c->ny;
   const int logn = ilog2_exact(nx);
   const size_t shmem = 2 * (size_t)nx * sizeof(double);
@@ -1153,6 +1349,18 @@ int launch_poisson(Ctx *c, int rk, const double *rhs_add) {
     ca.q1 = c->th_q1;
     ca.last = c->th_last;
   }
+  *out = ca;
+  return EIF_OK;
+}
+
+int launch_poisson(Ctx *c, int rk, const double *rhs_add) {
+  const int nx = c->nx, ny = c->ny;
+  const int logn = ilog2_exact(nx);
+  const size_t shmem = 2 * (size_t)nx * sizeof(double);
+  const int mode = logn < 0 ? 0 : ((nx >= 64 && nx <= 8192) ? 2 : 1);
+  const int threads = DCT_THREADS;
+  CombineArgs ca;
+  TRY_P(poisson_lines(c, rk, rhs_add, &ca));
   bool halo_done = false;  // the register-FFT kernel writes psi's boundary values
   {
     ProfScope ps(c, "dct3_psi");
@@ -1175,6 +1383,144 @@ int launch_poisson(Ctx *c, int rk, const double *rhs_add) {
     EIF_HIP(hipGetLastError());
   }
   return halo_done ? EIF_OK : launch_bc_p(c, c->psi);
+}
+
+// the fused DCT-III + projection + pressure update of 2^LG-point rows
+template <int LG, bool P2, bool DT>
+static void project_t(Ctx *c, int rk, const CombineArgs &ca, int S, double *ux, double *uy,
+                      double *p) {
+  go<ProjectTB<LG, P2, DT>>(k_project_t<LG, P2, DT>, dim3((c->ny + S - 1) / S),
+                            dim3(c->nx >> dct_logr<LG>()), 2 * (size_t)c->nx * sizeof(double),
+                            c->stream, c->L, c->qhat, ca, c->ctab, c->twid, S, ux, uy, p,
+                            c->dxc, c->dscal, c->gamma[rk], c->dy, c->dt_part);
+}
+
+template <int LG>
+static void project_lg(Ctx *c, int rk, const CombineArgs &ca, int S, bool dt, double *ux,
+                       double *uy, double *p) {
+  if (c->pow2_grid)
+    dt ? project_t<LG, true, true>(c, rk, ca, S, ux, uy, p)
+       : project_t<LG, true, false>(c, rk, ca, S, ux, uy, p);
+  else
+    dt ? project_t<LG, false, true>(c, rk, ca, S, ux, uy, p)
+       : project_t<LG, false, false>(c, rk, ca, S, ux, uy, p);
+}
+
+// rows per strip of the fused kernel: 2, 4 or 8 (0: the unfused kernels,
+// the default: on MI355X at 2048 x 4096 the fused kernel's extra transforms
+// and its serial transform/stream phases cost more than the psi round trip
+// it saves; the tuner times both)
+int project_strip(const Ctx *c) { return c->proj_strip; }
+
+bool project_fusable(const Ctx *c) {
+  const int logn = ilog2_exact(c->nx);
+  return logn >= 6 && logn <= 13 && c->ny >= 2;
+}
+
+int launch_project_on(Ctx *c, int rk, const CombineArgs &ca, bool dt_fused, double *ux,
+                      double *uy, double *p) {
+  ProfScope ps(c, "dct3_project");
+  const int S = project_strip(c);
+  const int parts = (c->ny + S - 1) / S;
+  const bool dt = dt_fused && parts <= DT_PARTS_CAP;
+  switch (ilog2_exact(c->nx)) {
+    case 6: project_lg<6>(c, rk, ca, S, dt, ux, uy, p); break;
+    case 7: project_lg<7>(c, rk, ca, S, dt, ux, uy, p); break;
+    case 8: project_lg<8>(c, rk, ca, S, dt, ux, uy, p); break;
+    case 9: project_lg<9>(c, rk, ca, S, dt, ux, uy, p); break;
+    case 10: project_lg<10>(c, rk, ca, S, dt, ux, uy, p); break;
+    case 11: project_lg<11>(c, rk, ca, S, dt, ux, uy, p); break;
+    case 12: project_lg<12>(c, rk, ca, S, dt, ux, uy, p); break;
+    case 13: project_lg<13>(c, rk, ca, S, dt, ux, uy, p); break;
+    default:
+      set_error("fused projection: nx must be a power of two in 64..8192");
+      return EIF_ERR_ARG;
+  }
+  EIF_HIP(hipGetLastError());
+  c->dt_adv_parts = dt ? parts : 0;
+  return EIF_OK;
+}
+
+// compute_potential + correct_velocity + update_pressure of stage rk (the
+// stage's whole projection), fused where the row length allows; psi is
+// then not stored
+int launch_poisson_project(Ctx *c, int rk, bool dt_fused) {
+  if (c->proj_strip <= 0 || !project_fusable(c)) {
+    TRY_P(launch_poisson(c, rk));
+    return launch_correct_pressure(c, rk, true, true, dt_fused);
+  }
+  CombineArgs ca;
+  TRY_P(poisson_lines(c, rk, nullptr, &ca));
+  return launch_project_on(c, rk, ca, dt_fused, c->ux, c->uy, c->p);
+}
+
+// The tuner's choice between the unfused DCT-III + projection kernels and
+// the fused kernel with strips of 2, 4 or 8 rows: each candidate's whole
+// potential + projection + pressure update, 10 timed repetitions on the
+// given copies of the fields (events without the system-scope fence).
+// Every candidate gives the same psi bits in Thomas mode; in partition mode
+// the fused kernel pairs consecutive rows (rounding only).  Appends
+// "projection: unfused=..ms 2=..ms 4=..ms 8=..ms -> S; " (S = 0: unfused).
+int project_tune(Ctx *c, double *ux, double *uy, double *p, std::string *rep) {
+  if (!project_fusable(c)) {
+    c->proj_strip = 0;
+    *rep += "projection: unfused only; ";
+    return EIF_OK;
+  }
+  hipEvent_t a, b;
+  EIF_HIP(hipEventCreateWithFlags(&a, hipEventDisableSystemFence));
+  EIF_HIP(hipEventCreateWithFlags(&b, hipEventDisableSystemFence));
+  double best = 1e30;
+  int pick = 0, rc = EIF_OK;
+  char buf[64];
+  *rep += "projection:";
+  auto once = [&](int S) -> int {
+    if (S == 0) {
+      TRY_P(launch_poisson(c, 1));
+      return launch_correct_pressure_on(c, 1, true, true, true, ux, uy, p);
+    }
+    CombineArgs ca;
+    TRY_P(poisson_lines(c, 1, nullptr, &ca));
+    return launch_project_on(c, 1, ca, true, ux, uy, p);
+  };
+  for (int S : {0, 2, 4, 8}) {
+    c->proj_strip = S;
+    rc = once(S);  // untimed first run
+    if (!rc) rc = hipEventRecord(a, c->stream) == hipSuccess ? EIF_OK : EIF_ERR_HIP;
+    for (int k = 0; k < 10 && !rc; k++) rc = once(S);
+    if (!rc) rc = hipEventRecord(b, c->stream) == hipSuccess ? EIF_OK : EIF_ERR_HIP;
+    float el = 0.f;
+    if (!rc && (hipEventSynchronize(b) != hipSuccess || hipEventElapsedTime(&el, a, b) != hipSuccess))
+      rc = hip_fail(hipErrorUnknown, "timing events", __FILE__, __LINE__);
+    if (rc) break;
+    const double ms = el / 10.;
+    snprintf(buf, sizeof(buf), " %s=%.4fms", S ? std::to_string(S).c_str() : "unfused", ms);
+    *rep += buf;
+    if (ms < best) {
+      best = ms;
+      pick = S;
+    }
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  c->proj_strip = pick;
+  snprintf(buf, sizeof(buf), " -> %s; ", pick ? std::to_string(pick).c_str() : "unfused");
+  *rep += buf;
+  return rc;
+}
+
+// eif_gpu_set_projection: 0 (unfused) or a strip height of 2, 4 or 8
+int set_projection(Ctx *c, int strip) {
+  if (strip != 0 && strip != 2 && strip != 4 && strip != 8) {
+    set_error("projection strip must be 0 (unfused), 2, 4 or 8");
+    return EIF_ERR_ARG;
+  }
+  if (strip && !project_fusable(c)) {
+    set_error("the fused projection needs nx a power of two in 64..8192 and ny >= 2");
+    return EIF_ERR_ARG;
+  }
+  c->proj_strip = strip;
+  return EIF_OK;
 }
 
 }  // namespace eif

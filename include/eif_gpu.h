@@ -220,6 +220,17 @@ int eif_gpu_graph_stats(eif_ctx *ctx, long out[4]);
  * choices.  Synchronising; leaves psi holding its last timing solve. */
 int eif_gpu_autotune(eif_ctx *ctx, int flags, char *report, int report_len);
 
+/* How eif_gpu_integrate (and graph steps) run each stage's potential,
+ * projection and pressure update (compute_potential.c:48-138,
+ * correct_velocity.c:6-42, update_pressure.c:7-24; no counterpart in the
+ * reference's interface): strip = 0, the DCT-III kernel storing psi and then
+ * the projection kernel (the default); 2, 4 or 8, one kernel that fuses the
+ * DCT-III with the projection and the pressure update over strips of that
+ * many rows and never stores psi (psi is then undefined after a step).
+ * Needs nx a power of two in 64..8192.  In Thomas mode both give the same
+ * bits; eif_gpu_autotune times both. */
+int eif_gpu_set_projection(eif_ctx *ctx, int strip);
+
 /* Ensembles of independent cases (no counterpart in the reference, which
  * runs one case per process): advances ncases contexts, of any grid sizes
  * and particle counts, by nsteps steps each.  concurrent = 0: one context

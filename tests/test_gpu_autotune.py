@@ -1,7 +1,8 @@
 """The autotuner (eif_gpu_autotune): the launch shapes it picks change no
 result (predictor and projection rows) or only the rounding of the
-partitioned y-solve (its chunk length), and a Thomas-mode context stays
-bit-identical to an untuned one."""
+partitioned y-solve (its chunk length, and the row pairing of the fused
+projection), and a Thomas-mode context stays bit-identical to an untuned
+one."""
 import re
 
 import numpy as np
@@ -40,10 +41,14 @@ def test_autotuned_context_matches_untuned(mode):
     print("autotune:", rep)
     assert re.search(r"predictor_rows:.* -> (8|16|32)", rep)
     assert re.search(r"projection_rows:.* -> (4|8|16)", rep)
+    assert re.search(r"projection:.* -> (unfused|2|4|8);", rep)
     for _ in range(3):
         assert ref.step() == pytest.approx(tuned.step(), rel=1e-13)
     a, b = ref.download(), tuned.download()
-    same_split = mode == eif.POISSON_THOMAS or "partition 64" in rep
+    # the fused projection pairs rows as the Thomas-mode transform does:
+    # in partition mode it changes rounding
+    same_split = mode == eif.POISSON_THOMAS or ("partition 64" in rep and
+                                                "-> unfused;" in rep)
     for k in ("ux", "uy"):
         if same_split:
             assert np.array_equal(a[k], b[k]), k
