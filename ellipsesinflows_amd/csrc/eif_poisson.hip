@@ -227,12 +227,92 @@ __device__ inline double rhs_value_add(const Layout &L, const double *ux, const 
   return fac * (div + fac * add[L.p(i + 1, j)]);
 }
 
+// How a one-row kernel transforms a row of n values: logn >= 0, the
+// radix-2 LDS FFT of n = 2^logn; otherwise the Stockham FFT with the stage
+// radices r[0..nr) (n's prime factors in increasing order)
+struct DctPlan {
+  int logn;
+  int nr;
+  int r[24];
+};
+
+// exp(sign 2 pi i e / n) from the table C[m] = cos(pi m / (2n)):
+// cos(2 pi e / n) = C(4e), sin(2 pi e / n) = C(|n - 4e|)
+__device__ inline void unit_root(const double *C, int n, long e, double sign, double &wr,
+                                 double &wi) {
+  const long d = (long)n - 4 * e;
+  wr = cos_lookup(C, n, 4 * e);
+  wi = sign * cos_lookup(C, n, d < 0 ? -d : d);
+}
+
+// Stockham autosort FFT of any length n in LDS (sign -1 forward, +1
+// backward, unnormalised), from (xr, xi) with (yr, yi) as the second
+// buffer; *outr, *outi: the pair that holds the result.  In the stage of
+// radix R after radices of product Ns, butterfly j (k = j mod Ns) takes
+// v_r = x[j + r n/R] w^(k r), w = e^(sign 2 pi i / (Ns R)), and stores
+// sum_r v_r e^(sign 2 pi i q r / R) at (j / Ns) Ns R + k + q Ns.  Factors
+// with exponent 0 are not multiplied; the oracle's fft_mixed performs the
+// same operations in the same order.
+__device__ void lds_fft_mixed(double *xr, double *xi, double *yr, double *yi, int n,
+                              const DctPlan &pl, const double *__restrict__ C, double sign,
+                              double **outr, double **outi) {
+  int Ns = 1;
+  for (int st = 0; st < pl.nr; st++) {
+    const int R = pl.r[st], nR = n / R, tw = n / (Ns * R);
+    for (int j = threadIdx.x; j < nR; j += blockDim.x) {
+      const int k = j % Ns;
+      const int o = (j / Ns) * Ns * R + k;
+      for (int q = 0; q < R; q++) {
+        double sr = 0., si = 0.;
+        for (int r = 0; r < R; r++) {
+          double vr = xr[j + r * nR], vi = xi[j + r * nR];
+          const long e = (long)k * r * tw;
+          if (e != 0) {
+            double wr, wi;
+            unit_root(C, n, e, sign, wr, wi);
+            const double tr = vr * wr - vi * wi, ti = vr * wi + vi * wr;
+            vr = tr;
+            vi = ti;
+          }
+          if (r == 0) {
+            sr = vr;
+            si = vi;
+            continue;
+          }
+          const long e2 = (long)((q * r) % R) * nR;
+          if (e2 == 0) {
+            sr = sr + vr;
+            si = si + vi;
+          } else {
+            double cr, ci;
+            unit_root(C, n, e2, sign, cr, ci);
+            sr = sr + (vr * cr - vi * ci);
+            si = si + (vr * ci + vi * cr);
+          }
+        }
+        yr[o + q * Ns] = sr;
+        yi[o + q * Ns] = si;
+      }
+    }
+    __syncthreads();
+    double *t = xr;
+    xr = yr;
+    yr = t;
+    t = xi;
+    xi = yi;
+    yi = t;
+    Ns *= R;
+  }
+  *outr = xr;
+  *outi = xi;
+}
+
 struct RhsDct2B {
   static constexpr int T = 1024, M = 0;
   static __device__ __forceinline__ void run(const VB &vb, 
     Layout L, const double *ux, const double *uy, const double *dscal, double ga,
     double dx, double dy, const double *__restrict__ C, const double *__restrict__ W,
-    int logn, int mode, double *qhat, const double *add) {
+    DctPlan pl, int mode, double *qhat, const double *add) {
   extern __shared__ double sh[];
   const int nx = L.nx;
   const int j = vb.bx + 1;
@@ -240,7 +320,19 @@ struct RhsDct2B {
   const double dt = dscal[S_DT];
   const double fac = 1. / (ga * dt);
   double *out = qhat + (long)(j - 1) * nx;
-  if (mode == 1) {
+  if (mode == 3) {  // Makhoul order, then the Stockham FFT (4 nx values of LDS)
+    for (int i = threadIdx.x; i < nx; i += blockDim.x) {
+      const int m = (i & 1) ? nx - 1 - (i >> 1) : (i >> 1);
+      re[m] = rhs_value_add(L, ux, uy, add, fac, dx, dy, i, j);
+      im[m] = 0.;
+    }
+    __syncthreads();
+    double *Xr, *Xi;
+    lds_fft_mixed(re, im, sh + 2 * nx, sh + 3 * nx, nx, pl, C, -1., &Xr, &Xi);
+    for (int k = threadIdx.x; k < nx; k += blockDim.x)
+      out[k] = 2. * (C[k] * Xr[k] + C[nx - k] * Xi[k]);
+  } else if (mode == 1) {
+    const int logn = pl.logn;
     for (int i = threadIdx.x; i < nx; i += blockDim.x) {
       const int m = (i & 1) ? nx - 1 - (i >> 1) : (i >> 1);
       const int pos = bitrev(m, logn);
@@ -267,8 +359,8 @@ struct RhsDct2B {
 __global__ __launch_bounds__(1024) void k_rhs_dct2(
     Layout L, const double *ux, const double *uy, const double *dscal, double ga,
     double dx, double dy, const double *__restrict__ C, const double *__restrict__ W,
-    int logn, int mode, double *qhat, const double *add) {
-  RhsDct2B::run(VB::here(), L, ux, uy, dscal, ga, dx, dy, C, W, logn, mode, qhat, add);
+    DctPlan pl, int mode, double *qhat, const double *add) {
+  RhsDct2B::run(VB::here(), L, ux, uy, dscal, ga, dx, dy, C, W, pl, mode, qhat, add);
 }
 
 // ---------------------------------------------------------------------
@@ -305,14 +397,29 @@ struct Dct3PsiB {
   static constexpr int T = 1024, M = 0;
   static __device__ __forceinline__ void run(const VB &vb, 
     Layout L, const double *qhat, CombineArgs ca, const double *__restrict__ C,
-    const double *__restrict__ W, int logn, int mode, double *psi) {
+    const double *__restrict__ W, DctPlan pl, int mode, double *psi) {
   if (halted(ca.halt)) return;
   extern __shared__ double sh[];
   const int nx = L.nx, ny = L.ny;
   const int j0 = vb.bx;
   double *re = sh, *im = sh + nx;
   const double norm = 2. * nx;
-  if (mode == 1) {
+  if (mode == 3) {  // pre-twiddled spectrum, Stockham FFT, Makhoul order back
+    for (int k = threadIdx.x; k < nx; k += blockDim.x) {
+      const double xr = combined(ca, qhat, nx, ny, j0, k);
+      const double xi = k == 0 ? 0. : -combined(ca, qhat, nx, ny, j0, nx - k);
+      re[k] = C[k] * xr - C[nx - k] * xi;
+      im[k] = C[k] * xi + C[nx - k] * xr;
+    }
+    __syncthreads();
+    double *Xr, *Xi;
+    lds_fft_mixed(re, im, sh + 2 * nx, sh + 3 * nx, nx, pl, C, +1., &Xr, &Xi);
+    for (int i = threadIdx.x; i < nx; i += blockDim.x) {
+      const double v = (i & 1) ? Xr[nx - 1 - (i >> 1)] : Xr[i >> 1];
+      psi[L.p(i + 1, j0 + 1)] = v / norm;
+    }
+  } else if (mode == 1) {
+    const int logn = pl.logn;
     for (int k = threadIdx.x; k < nx; k += blockDim.x) {
       const double xr = combined(ca, qhat, nx, ny, j0, k);
       const double xi = k == 0 ? 0. : -combined(ca, qhat, nx, ny, j0, nx - k);
@@ -341,8 +448,8 @@ struct Dct3PsiB {
 };
 __global__ __launch_bounds__(1024) void k_dct3_psi(
     Layout L, const double *qhat, CombineArgs ca, const double *__restrict__ C,
-    const double *__restrict__ W, int logn, int mode, double *psi) {
-  Dct3PsiB::run(VB::here(), L, qhat, ca, C, W, logn, mode, psi);
+    const double *__restrict__ W, DctPlan pl, int mode, double *psi) {
+  Dct3PsiB::run(VB::here(), L, qhat, ca, C, W, pl, mode, psi);
 }
 
 // Register-FFT kernels specialised for nx = 2^LOGN (nx / 2^LOGR threads),
@@ -1271,6 +1378,32 @@ static int ilog2_exact(int n) {
   return l;
 }
 
+// lengths that are not powers of two: the Stockham FFT up to this many
+// values per row (4 n doubles of LDS), the direct sum beyond
+constexpr int DCT_MIXED_MAX = 5120;
+
+// 2: register FFT; 1: radix-2 LDS FFT; 3: Stockham FFT; 0: direct sum
+static int dct_mode(int nx) {
+  const int logn = ilog2_exact(nx);
+  if (logn >= 0) return (nx >= 64 && nx <= 8192) ? 2 : 1;
+  return nx <= DCT_MIXED_MAX ? 3 : 0;
+}
+
+static DctPlan dct_plan(int n) {
+  DctPlan pl{};
+  pl.logn = ilog2_exact(n);
+  pl.nr = 0;
+  for (int p = 2; n > 1 && pl.nr < 24;) {
+    if (n % p == 0) {
+      pl.r[pl.nr++] = p;
+      n /= p;
+    } else {
+      p++;
+    }
+  }
+  return pl;
+}
+
 // the register-FFT right-hand side of a power-of-two row length 2^LG
 template <int LG, bool P2, bool ADD>
 static void rhs_t(Ctx *c, int rk, size_t shmem, const double *rhs_add) {
@@ -1285,9 +1418,12 @@ static void rhs_t(Ctx *c, int rk, size_t shmem, const double *rhs_add) {
 static int poisson_lines(Ctx *c, int rk, const double *rhs_add, CombineArgs *out) {
   const int nx = c->nx, ny = c->ny;
   const int logn = ilog2_exact(nx);
-  const size_t shmem = 2 * (size_t)nx * sizeof(double);
-  // register FFT for 64 <= nx <= 8192, LDS FFT for other powers of two
-  const int mode = logn < 0 ? 0 : ((nx >= 64 && nx <= 8192) ? 2 : 1);
+  // register FFT for 64 <= nx <= 8192, LDS FFT for other powers of two,
+  // the Stockham FFT for other lengths up to DCT_MIXED_MAX (its two buffer
+  // pairs in LDS), the direct sum beyond
+  const int mode = dct_mode(nx);
+  const DctPlan pl = dct_plan(nx);
+  const size_t shmem = (mode == 3 ? 4 : 2) * (size_t)nx * sizeof(double);
   const int threads = DCT_THREADS;  // of the one-row kernels (modes 0, 1)
   {
     ProfScope ps(c, "rhs_dct2");
@@ -1305,7 +1441,7 @@ static int poisson_lines(Ctx *c, int rk, const double *rhs_add, CombineArgs *out
       EIF_RHS_T(12) EIF_RHS_T(13)
       default:
         go<RhsDct2B>(k_rhs_dct2, dim3(ny), dim3(threads), shmem, c->stream, c->L, c->ux, c->uy,
-                     c->dscal, c->gamma[rk], c->dx, c->dy, c->ctab, c->twid, logn, mode,
+                     c->dscal, c->gamma[rk], c->dx, c->dy, c->ctab, c->twid, pl, mode,
                      c->qhat, rhs_add);
     }
 #undef EIF_RHS_T
@@ -1356,8 +1492,9 @@ static int poisson_lines(Ctx *c, int rk, const double *rhs_add, CombineArgs *out
 int launch_poisson(Ctx *c, int rk, const double *rhs_add) {
   const int nx = c->nx, ny = c->ny;
   const int logn = ilog2_exact(nx);
-  const size_t shmem = 2 * (size_t)nx * sizeof(double);
-  const int mode = logn < 0 ? 0 : ((nx >= 64 && nx <= 8192) ? 2 : 1);
+  const int mode = dct_mode(nx);
+  const DctPlan pl = dct_plan(nx);
+  const size_t shmem = (mode == 3 ? 4 : 2) * (size_t)nx * sizeof(double);
   const int threads = DCT_THREADS;
   CombineArgs ca;
   TRY_P(poisson_lines(c, rk, rhs_add, &ca));
@@ -1377,7 +1514,7 @@ int launch_poisson(Ctx *c, int rk, const double *rhs_add) {
       EIF_DCT3_T(12) EIF_DCT3_T(13)
       default:
         go<Dct3PsiB>(k_dct3_psi, dim3(ny), dim3(threads), shmem, c->stream, c->L,
-                           c->qhat, ca, c->ctab, c->twid, logn, mode, c->psi);
+                           c->qhat, ca, c->ctab, c->twid, pl, mode, c->psi);
     }
 #undef EIF_DCT3_T
     EIF_HIP(hipGetLastError());

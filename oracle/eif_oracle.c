@@ -423,7 +423,8 @@ int orc_update_velocity(orc_t *s, int rk) {
  *   REDFT10: Y_k = 2 sum_{j<n} X_j cos(pi (j+1/2) k / n)
  *   REDFT01: Y_k = X_0 + 2 sum_{1<=j<n} X_j cos(pi j (k+1/2) / n)
  * Powers of two use Makhoul's n-point complex-FFT algorithm (iterative
- * radix-2 decimation in time), other lengths the direct sum.  Every
+ * radix-2 decimation in time), other lengths up to DCT_MIXED_MAX the same
+ * with a Stockham mixed-radix FFT, longer ones the direct sum.  Every
  * trigonometric factor is read from one table C[m] = cos(pi m / (2n)),
  * m = 0..2n, using cos(pi m/2n) = C[4n-m] and sin(pi k/2n) = C[|n-k|].
  * ------------------------------------------------------------------- */
@@ -469,6 +470,80 @@ static void fft_pow2(int n, const double *C, double *re, double *im, int sign) {
 
 static int is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
 
+/* lengths that are not powers of two: the Stockham FFT up to this many
+ * values (the device's LDS bound), the direct sum beyond */
+#define DCT_MIXED_MAX 5120
+
+/* exp(sign 2 pi i e / n) from the table: cos(2 pi e/n) = C(4e),
+ * sin(2 pi e/n) = C(|n - 4e|) */
+static void unit_root(const double *C, int n, long e, double sign, double *wr, double *wi) {
+  const long d = (long)n - 4 * e;
+  *wr = cos_lookup(C, n, 4 * e);
+  *wi = sign * cos_lookup(C, n, d < 0 ? -d : d);
+}
+
+/* Stockham autosort FFT of any length n (sign -1 forward, +1 backward,
+ * unnormalised); stage radices are n's prime factors in increasing order.
+ * In the stage of radix R after radices of product Ns, butterfly j
+ * (k = j mod Ns) takes v_r = x[j + r n/R] w^(k r), w = e^(sign 2 pi i/(Ns R)),
+ * and stores sum_r v_r e^(sign 2 pi i q r / R) at (j/Ns) Ns R + k + q Ns.
+ * Factors with exponent 0 are not multiplied.  The result is left in
+ * (re, im); (tr, ti) is the second buffer. */
+static void fft_mixed(int n, const double *C, double *re, double *im, double *tr, double *ti,
+                      int sign) {
+  double *xr = re, *xi = im, *yr = tr, *yi = ti;
+  double *vr = xcalloc(n, sizeof(double)), *vi = xcalloc(n, sizeof(double));
+  int Ns = 1, m = n;
+  while (m > 1) {
+    int R = 2;
+    while (m % R) R++;
+    m /= R;
+    const int nR = n / R, tw = n / (Ns * R);
+    for (int j = 0; j < nR; j++) {
+      const int k = j % Ns;
+      const int o = (j / Ns) * Ns * R + k;
+      for (int r = 0; r < R; r++) {
+        vr[r] = xr[j + r * nR];
+        vi[r] = xi[j + r * nR];
+        const long e = (long)k * r * tw;
+        if (e != 0) {
+          double wr, wi;
+          unit_root(C, n, e, sign, &wr, &wi);
+          const double a = vr[r] * wr - vi[r] * wi, b = vr[r] * wi + vi[r] * wr;
+          vr[r] = a;
+          vi[r] = b;
+        }
+      }
+      for (int q = 0; q < R; q++) {
+        double sr = vr[0], si = vi[0];
+        for (int r = 1; r < R; r++) {
+          const long e2 = (long)((q * r) % R) * nR;
+          if (e2 == 0) {
+            sr = sr + vr[r];
+            si = si + vi[r];
+          } else {
+            double cr, ci;
+            unit_root(C, n, e2, sign, &cr, &ci);
+            sr = sr + (vr[r] * cr - vi[r] * ci);
+            si = si + (vr[r] * ci + vi[r] * cr);
+          }
+        }
+        yr[o + q * Ns] = sr;
+        yi[o + q * Ns] = si;
+      }
+    }
+    double *t = xr; xr = yr; yr = t;
+    t = xi; xi = yi; yi = t;
+    Ns *= R;
+  }
+  if (xr != re) {
+    memcpy(re, xr, sizeof(double) * n);
+    memcpy(im, xi, sizeof(double) * n);
+  }
+  free(vr);
+  free(vi);
+}
+
 static void dct_direct(int n, const double *C, double *x, int type3) {
   double *y = xcalloc(n, sizeof(double));
   for (int k = 0; k < n; k++) {
@@ -484,8 +559,21 @@ static void dct_direct(int n, const double *C, double *x, int type3) {
 }
 
 static void dct2_tab(int n, const double *C, double *x, double *re, double *im) {
-  if (!is_pow2(n)) {
+  if (!is_pow2(n) && n > DCT_MIXED_MAX) {
     dct_direct(n, C, x, 0);
+    return;
+  }
+  if (!is_pow2(n)) {  /* Makhoul order, Stockham FFT */
+    double *tr = xcalloc(n, sizeof(double)), *ti = xcalloc(n, sizeof(double));
+    for (int i = 0; i < n; i++) {
+      const int m = (i & 1) ? n - 1 - (i >> 1) : (i >> 1);
+      re[m] = x[i];
+      im[m] = 0.;
+    }
+    fft_mixed(n, C, re, im, tr, ti, -1);
+    for (int k = 0; k < n; k++) x[k] = 2. * (C[k] * re[k] + C[n - k] * im[k]);
+    free(tr);
+    free(ti);
     return;
   }
   for (int j = 0; j < n / 2; j++) {
@@ -498,8 +586,22 @@ static void dct2_tab(int n, const double *C, double *x, double *re, double *im) 
 }
 
 static void dct3_tab(int n, const double *C, double *x, double *re, double *im) {
-  if (!is_pow2(n)) {
+  if (!is_pow2(n) && n > DCT_MIXED_MAX) {
     dct_direct(n, C, x, 1);
+    return;
+  }
+  if (!is_pow2(n)) {  /* pre-twiddled spectrum, Stockham FFT, Makhoul order */
+    double *tr = xcalloc(n, sizeof(double)), *ti = xcalloc(n, sizeof(double));
+    for (int k = 0; k < n; k++) {
+      const double xr = x[k];
+      const double xi = k == 0 ? 0. : -x[n - k];
+      re[k] = C[k] * xr - C[n - k] * xi;
+      im[k] = C[k] * xi + C[n - k] * xr;
+    }
+    fft_mixed(n, C, re, im, tr, ti, +1);
+    for (int i = 0; i < n; i++) x[i] = (i & 1) ? re[n - 1 - (i >> 1)] : re[i >> 1];
+    free(tr);
+    free(ti);
     return;
   }
   for (int k = 0; k < n; k++) {

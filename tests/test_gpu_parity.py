@@ -40,9 +40,13 @@ def both(g, o, fn, *args):
 
 
 # ----------------------------------------------------------------- fluid
-# ny = 12 and 7: y-sweeps shorter than the Thomas kernel's 16-row load batch
+# ny = 12 and 7: y-sweeps shorter than the Thomas kernel's 16-row load batch;
+# nx = 48, 96, 100, 45, 97: rows that are not powers of two (the Stockham
+# mixed-radix transform: radices 2 and 3, 2 and 5, odd 3 and 5, one prime)
 @pytest.mark.parametrize("nx,ny,ly", [(64, 96, 1.0), (48, 40, 1.0), (256, 128, 2.0), (128, 36, 0.5),
-                                      (64, 33, 0.5), (32, 12, 0.5), (64, 7, 0.25)])
+                                      (64, 33, 0.5), (32, 12, 0.5), (64, 7, 0.25),
+                                      (96, 64, 1.0), (100, 36, 0.5), (45, 20, 0.5),
+                                      (97, 12, 0.25)])
 def test_fluid_functions_bit_exact_thomas(nx, ny, ly):
     g, o = pair(0, itot=nx, jtot=ny, ly=ly, Re=120.0, extfrcy=0.3, uy_wall_xm=-0.1, uy_wall_xp=0.2)
     ux, uy, p = cases.random_fields(nx, ny, nx + ny)

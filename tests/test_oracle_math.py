@@ -23,10 +23,14 @@ def _ptr(a):
 
 
 # ---------------------------------------------------------------------- DCT
-@pytest.mark.parametrize("n", [8, 16, 48, 64, 96, 128, 256, 2048])
+@pytest.mark.parametrize("n", [8, 16, 48, 64, 96, 128, 256, 2048,
+                               12, 45, 77, 97, 100, 360, 1000])
 def test_dct_definitions_against_long_double_sums(n):
     """REDFT10: Y_k = 2 sum_j X_j cos(pi (j+1/2) k / n);
-    REDFT01: Y_k = X_0 + 2 sum_{j>=1} X_j cos(pi j (k+1/2) / n)."""
+    REDFT01: Y_k = X_0 + 2 sum_{j>=1} X_j cos(pi j (k+1/2) / n).
+    Lengths that are not powers of two go through the Stockham mixed-radix
+    FFT: even and odd lengths, a prime (97, one radix-97 stage) and
+    radices 2, 3, 5, 7, 11."""
     lib = oracle_lib()
     rng = np.random.default_rng(n)
     x = rng.standard_normal(n)
