@@ -394,6 +394,12 @@ def main():
                 "algorithmic_flops_per_cell": FLOPS_CELL + FLOPS_ITER * mean_iters,
                 "cells_per_launch": cells, "compute_unit": "FP64 vector ALU",
                 "timing": "eager segment of %d steps after the timed steps" % seg}
+    # the FP64 work the kernel actually issues (rocprofv3 FP64 instruction
+    # counters, profiles/ibm_substep_valu_mix.md) over the same launch time
+    executed = traffic_from_profiles(DOMINANT + "_fp64_lane_flops")
+    if executed and avg_s > 0:
+        roofline["executed_fp64_tflops"] = executed / avg_s / 1e12
+        roofline["executed_frac"] = executed / avg_s / PEAK_FP64
     out = {
         "metric": METRIC, "value": steps_per_s, "unit": "timesteps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
