@@ -106,3 +106,50 @@ def test_ensemble_rejects_repeated_context_and_bad_mode():
     with pytest.raises(eif.EIFError):
         eif.ensemble_run_steps([g], 1, concurrent=4)
     g.close()
+
+
+def _small(nx, ny, n_parts, seed):
+    cfg = dict(itot=nx, jtot=ny, ly=2.0, Re=200.0, extfrcy=8.0 / 200.0)
+    parts = initparticles.random_ellipses(n_parts, 2.0, (0.03, 0.05), (0.5, 1.0), 0.02, seed) \
+        if n_parts else None
+    return cfg, parts
+
+
+def _make_any(cfg, parts):
+    g = eif.Solver(poisson_mode=eif.POISSON_PARTITION, **cfg)
+    nx, ny = cfg["itot"], cfg["jtot"]
+    xc = g.grid()["xc"]
+    uy = np.zeros((ny + 2, nx + 2))
+    uy[1:-1, 1:-1] = 4.0 * xc[None, 1:-1] * (1 - xc[None, 1:-1])
+    g.upload(uy=uy)
+    g.update_boundaries()
+    if parts is not None:
+        g.set_particles(**{k: parts[k] for k in KEYS})
+    return g
+
+
+def test_batched_ensemble_beyond_one_batch_with_fluid_only_cases():
+    """66 small cases: two batches per ensemble step (64 + 2 cases); every
+    fourth case has no particles, so its launch sequence has no IBM kernels
+    and the cases' sequences do not line up position by position.  Each
+    case still equals its own run bit for bit."""
+    sel = [_small(64, 128, 0 if k % 4 == 0 else 2 + k % 3, 100 + k) for k in range(66)]
+    print("66 cases, particle counts", [0 if p is None else len(p["a"]) for _, p in sel],
+          flush=True)
+    steps = 4
+    alone = []
+    for cfg, parts in sel:
+        g = _make_any(cfg, parts)
+        g.run_steps(steps)
+        alone.append((g.download(), g.particles() if parts is not None else None))
+        g.close()
+    solvers = [_make_any(cfg, parts) for cfg, parts in sel]
+    eif.ensemble_run_steps(solvers, steps, concurrent=3)
+    for k, g in enumerate(solvers):
+        f = g.download()
+        for name in ("ux", "uy", "p"):
+            assert np.array_equal(f[name], alone[k][0][name]), (k, name)
+        if alone[k][1] is not None:
+            assert np.array_equal(g.particles(), alone[k][1]), k
+    for g in solvers:
+        g.close()
