@@ -146,8 +146,11 @@ FLUID_BYTES_N = {"predictor": 19.0 / 3.0, "rhs_dct2": 3.0, "ysolve_partition": 2
 ENSEMBLE_SEED = 3
 
 
-def _ensemble_solvers(cases):
+def _ensemble_solvers(cases, tune=True):
+    """One context per case; with tune, one case per grid shape is
+    autotuned and its shapes applied to the others of that grid."""
     out = []
+    tuned = {}
     for cfg, parts in cases:
         g = eif.Solver(poisson_mode=eif.POISSON_PARTITION, **cfg)
         nx, ny = cfg["itot"], cfg["jtot"]
@@ -158,6 +161,13 @@ def _ensemble_solvers(cases):
         g.update_boundaries()
         g.set_particles(**{k: parts[k] for k in ("den", "a", "b", "x", "y", "az", "ux", "uy",
                                                  "vz")})
+        if tune:
+            key = (nx, ny)
+            if key not in tuned:
+                tuned[key] = g
+                g.autotune()
+            else:
+                g.tune_copy(tuned[key])
         out.append(g)
     return out
 
@@ -189,7 +199,9 @@ def ensemble_bench(nsteps, warm=3):
         key = "%dx%d" % (cfg["itot"], cfg["jtot"])
         grids[key] = grids.get(key, 0) + 1
     return {"workload": "32 independent cases, seed %d: grids %s, 8-256 ellipses each, "
-                        "Re~logU[100,1000]" % (ENSEMBLE_SEED, grids),
+                        "Re~logU[100,1000]; launch shapes autotuned on one case per grid "
+                        "and applied to the others, the same in every mode"
+                        % (ENSEMBLE_SEED, grids),
             "steps_timed_per_case": nsteps,
             "case_steps_per_s_sequential": res["sequential"],
             "case_steps_per_s_graphs_on_streams": res["concurrent"],

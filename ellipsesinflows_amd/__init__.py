@@ -52,6 +52,7 @@ EXPORTED = (
     "eif_gpu_integrate", "eif_gpu_step", "eif_gpu_synchronize", "eif_gpu_last_error",
     "eif_gpu_run_steps", "eif_gpu_graph_enable", "eif_gpu_graph_stats",
     "eif_gpu_graph_set_subiterations", "eif_gpu_ensemble_run_steps", "eif_gpu_set_projection",
+    "eif_gpu_tune_copy",
     "eif_gpu_tape_begin", "eif_gpu_step_fixed_dt", "eif_gpu_stage_adjoint",
     "eif_gpu_adjoint_get", "eif_gpu_adjoint_add", "fluid_gpu_wall_shear",
     "eif_gpu_adjoint_add_wall_shear", "eif_gpu_autotune",
@@ -395,6 +396,11 @@ class Solver:
 
     def graph(self, enable=True):
         self._check(self.lib.eif_gpu_graph_enable(self.h, int(bool(enable))))
+
+    def tune_copy(self, src):
+        """Apply the launch shapes autotune() chose for src, a Solver of the
+        same grid, to this one."""
+        self._check(self.lib.eif_gpu_tune_copy(self.h, src.h))
 
     def set_projection(self, strip):
         """0: DCT-III and projection as two kernels (psi stored); 2, 4, 8:

@@ -6,8 +6,10 @@ save/restart, logging) into ``libeif_host.so`` and the C driver
 ``eif_gpu_main`` (a drop-in for the reference's ``a.out``) against
 ``libeif_gpu.so``.  Everything builds in-tree.
 """
+import concurrent.futures
 import os
 import subprocess
+import tempfile
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
@@ -39,10 +41,20 @@ def _run(cmd, cwd):
     return proc.stdout
 
 
-def build_gpu(hipcc="hipcc"):
-    cmd = [hipcc] + HIPCC_FLAGS + ["-shared", "-I", os.path.join(ROOT, "include"),
-                                   "-o", LIB + ".tmp"] + HIP_SOURCES
-    _run(cmd, CSRC)
+def build_gpu(hipcc="hipcc", jobs=None):
+    """Each HIP source to an object in parallel (in a temporary directory),
+    then one shared library."""
+    jobs = jobs or min(len(HIP_SOURCES), os.cpu_count() or 1)
+    with tempfile.TemporaryDirectory(prefix="eif_build_") as tmp:
+        def compile_one(src):
+            obj = os.path.join(tmp, os.path.splitext(src)[0] + ".o")
+            _run([hipcc] + HIPCC_FLAGS + ["-I", os.path.join(ROOT, "include"), "-c", src,
+                                          "-o", obj], CSRC)
+            return obj
+        with concurrent.futures.ThreadPoolExecutor(max_workers=jobs) as pool:
+            objs = list(pool.map(compile_one, HIP_SOURCES))
+        _run([hipcc, "--offload-arch=gfx950", "-fPIC", "-shared", "-o", LIB + ".tmp"] + objs,
+             CSRC)
     os.replace(LIB + ".tmp", LIB)
     return LIB
 

@@ -1720,6 +1720,27 @@ int eif_gpu_autotune(eif_ctx *c, int flags, char *report, int report_len) {
       c->cfg.poisson_mode = mode_was;
     }
   }
+  // values per thread of the register FFT (same butterflies, same bits)
+  if (!rc && fft_tunable(c)) {
+    double best = 1e30;
+    int pick = 0;
+    rep += "; fft_values_per_thread:";
+    for (int lr : {3, 2, 4}) {
+      if (!fft_logr_valid(c, lr)) continue;
+      c->fft_logr = lr == 3 ? 0 : lr;
+      double ms = 0.;
+      if ((rc = time_launches(c, [&] { return launch_poisson(c, 1); }, 10, &ms))) break;
+      snprintf(buf, sizeof(buf), " %d=%.4fms", 1 << lr, ms);
+      rep += buf;
+      if (ms < best) {
+        best = ms;
+        pick = c->fft_logr;
+      }
+    }
+    c->fft_logr = pick;
+    snprintf(buf, sizeof(buf), " -> %d", 1 << (pick ? pick : 3));
+    rep += buf;
+  }
   // strip height of the fused DCT-III + projection, for the chosen y-solve
   if (!rc) {
     double *pt = nullptr;
@@ -1739,6 +1760,28 @@ int eif_gpu_autotune(eif_ctx *c, int flags, char *report, int report_len) {
     report[report_len - 1] = 0;
   }
   return rc;
+}
+
+int eif_gpu_tune_copy(eif_ctx *dst, const eif_ctx *src) {
+  CHECK_CTX(dst);
+  CHECK_CTX(const_cast<eif_ctx *>(src));
+  if (dst->nx != src->nx || dst->ny != src->ny) {
+    set_error("tuned shapes apply to a context of the same grid");
+    return EIF_ERR_ARG;
+  }
+  EIF_HIP(hipStreamSynchronize(dst->stream));
+  dst->pred_rows = src->pred_rows;
+  dst->cp_rows = src->cp_rows;
+  dst->fft_logr = src->fft_logr;
+  dst->proj_strip = src->proj_strip;
+  if (src->cfg.poisson_mode != dst->cfg.poisson_mode || src->part_L != dst->part_L) {
+    dst->cfg.poisson_mode = src->cfg.poisson_mode;
+    if (dst->cfg.poisson_mode == EIF_POISSON_PARTITION && src->part_L)
+      TRY(partition_setup(dst, src->part_L));
+  }
+  // graphs captured with the old shapes are dropped
+  if (dst->graph_on) TRY(eif_gpu_graph_enable(dst, 1));
+  return EIF_OK;
 }
 
 int eif_gpu_set_projection(eif_ctx *c, int strip) {

@@ -151,6 +151,8 @@ struct Ctx {
   // rows per strip of the fused DCT-III + projection (2, 4, 8), or 0: the
   // unfused kernels (the default)
   int proj_strip = 0;
+  // log2 of the register FFT's values per thread (2, 4; 0: the default 3)
+  int fft_logr = 0;
   double *pt_fwd = nullptr, *pt_u = nullptr, *pt_v = nullptr, *pt_w = nullptr;
   double *pt_bnd = nullptr, *pt_red = nullptr, *pt_lvl = nullptr;
   double *pt_ends = nullptr, *pt_g = nullptr;
@@ -282,6 +284,10 @@ bool project_fusable(const Ctx *c);
 // given copies of the fields
 int project_tune(Ctx *c, double *ux, double *uy, double *p, std::string *rep);
 int set_projection(Ctx *c, int strip);
+// the register FFT's values per thread: tunable for this context's rows,
+// and 2^lr available there
+bool fft_tunable(const Ctx *c);
+bool fft_logr_valid(const Ctx *c, int lr);
 int susp_init(Ctx *c);
 // defer_finalize: the slice sums are added by the next launch_increment
 // with fin_inertia (one launch fewer per sub-iteration)

@@ -189,12 +189,13 @@ __device__ inline void reg_fft_t(double (&xr)[1 << LOGR], double (&xi)[1 << LOGR
   }
 }
 
-// log2 of the values per thread of the register FFT for n = 2^LOGN: 8
-// (16 per thread measured slower at n = 2048: fewer waves hide the LDS
-// round trips less well than the saved phase gains)
+// log2 of the values per thread of the register FFT: 8 by default (the
+// tuner may take 4 or 16 per context, eif_gpu_autotune; every choice
+// performs the same butterflies)
+constexpr int FFT_LOGR_DEFAULT = 3;
 template <int LOGN>
 constexpr int dct_logr() {
-  return 3;
+  return FFT_LOGR_DEFAULT;
 }
 
 __device__ inline double cos_lookup(const double *C, int n, long m) {
@@ -459,15 +460,15 @@ __global__ __launch_bounds__(1024) void k_dct3_psi(
 // (Z[k] - conj Z[kk]) / 2i (the halves cancel against REDFT10's factor 2);
 // the inverse takes the two pre-twiddled spectra as Va + i Vb.  An odd last
 // row pairs with a zero row.
-template <int LOGN, bool P2, bool ADD = false>
+template <int LOGN, bool P2, bool ADD = false, int LOGR_ = dct_logr<LOGN>()>
 struct RhsDct2TB {
-  static constexpr int T = (1 << LOGN) >> dct_logr<LOGN>(), M = 0;
+  static constexpr int T = (1 << LOGN) >> LOGR_, M = 0;
   static __device__ __forceinline__ void run(const VB &vb, 
     Layout L, const double *ux, const double *uy, const double *dscal, double ga,
     double dx, double dy, const double *__restrict__ C, const double *__restrict__ W,
     double *qhat, const double *add = nullptr) {
   extern __shared__ double sh[];
-  constexpr int LOGR = dct_logr<LOGN>(), R = 1 << LOGR;
+  constexpr int LOGR = LOGR_, R = 1 << LOGR;
   constexpr int nx = 1 << LOGN;
   constexpr int T = nx / R;
   const int ja = 2 * vb.bx + 1, jb = ja + 1;
@@ -519,12 +520,13 @@ struct RhsDct2TB {
   }
 }
 };
-template <int LOGN, bool P2, bool ADD = false>
-__global__ __launch_bounds__((1 << LOGN) >> dct_logr<LOGN>()) void k_rhs_dct2_t(
+template <int LOGN, bool P2, bool ADD = false, int LOGR_ = dct_logr<LOGN>()>
+__global__ __launch_bounds__((1 << LOGN) >> LOGR_) void k_rhs_dct2_t(
     Layout L, const double *ux, const double *uy, const double *dscal, double ga,
     double dx, double dy, const double *__restrict__ C, const double *__restrict__ W,
     double *qhat, const double *add = nullptr) {
-  RhsDct2TB<LOGN, P2, ADD>::run(VB::here(), L, ux, uy, dscal, ga, dx, dy, C, W, qhat, add);
+  RhsDct2TB<LOGN, P2, ADD, LOGR_>::run(VB::here(), L, ux, uy, dscal, ga, dx, dy, C, W, qhat,
+                                       add);
 }
 
 // The DCT-III (compute_potential.c:118-137) of two rows packed as one
@@ -532,11 +534,11 @@ __global__ __launch_bounds__((1 << LOGN) >> dct_logr<LOGN>()) void k_rhs_dct2_t(
 // (written and synchronised by the caller); on return they hold 2 nx psi
 // of the two rows in the Makhoul order, cell i at sw(i / 2) for even i and
 // at sw(nx - 1 - i / 2) for odd i (see psi_at).
-template <int LOGN>
+template <int LOGN, int LOGR_ = dct_logr<LOGN>()>
 __device__ __forceinline__ void dct3_pair_lds(double *re, double *im,
                                               const double *__restrict__ C,
                                               const double *__restrict__ W, int t) {
-  constexpr int LOGR = dct_logr<LOGN>(), R = 1 << LOGR;
+  constexpr int LOGR = LOGR_, R = 1 << LOGR;
   constexpr int nx = 1 << LOGN;
   double xr[R], xi[R];
 #pragma unroll
@@ -571,15 +573,15 @@ __device__ __forceinline__ double psi_at(const double *src, int i) {
   return v * inv_norm;
 }
 
-template <int LOGN>
+template <int LOGN, int LOGR_ = dct_logr<LOGN>()>
 struct Dct3PsiTB {
-  static constexpr int T = (1 << LOGN) >> dct_logr<LOGN>(), M = 0;
+  static constexpr int T = (1 << LOGN) >> LOGR_, M = 0;
   static __device__ __forceinline__ void run(const VB &vb, 
     Layout L, const double *qhat, CombineArgs ca, const double *__restrict__ C,
     const double *__restrict__ W, double *psi) {
   if (halted(ca.halt)) return;
   extern __shared__ double sh[];
-  constexpr int LOGR = dct_logr<LOGN>(), R = 1 << LOGR;
+  constexpr int LOGR = LOGR_, R = 1 << LOGR;
   constexpr int nx = 1 << LOGN;
   constexpr int T = nx / R;
   const int ny = L.ny;
@@ -618,7 +620,7 @@ struct Dct3PsiTB {
     }
   }
   __syncthreads();
-  dct3_pair_lds<LOGN>(re, im, C, W, threadIdx.x);
+  dct3_pair_lds<LOGN, LOGR>(re, im, C, W, threadIdx.x);
   // psi rows and their boundary values (the Neumann walls, and the periodic
   // halo rows 0 and ny+1 copied from rows ny and 1: k_bc_p's values)
 #pragma unroll
@@ -644,11 +646,11 @@ struct Dct3PsiTB {
   }
 }
 };
-template <int LOGN>
-__global__ __launch_bounds__((1 << LOGN) >> dct_logr<LOGN>()) void k_dct3_psi_t(
+template <int LOGN, int LOGR_ = dct_logr<LOGN>()>
+__global__ __launch_bounds__((1 << LOGN) >> LOGR_) void k_dct3_psi_t(
     Layout L, const double *qhat, CombineArgs ca, const double *__restrict__ C,
     const double *__restrict__ W, double *psi) {
-  Dct3PsiTB<LOGN>::run(VB::here(), L, qhat, ca, C, W, psi);
+  Dct3PsiTB<LOGN, LOGR_>::run(VB::here(), L, qhat, ca, C, W, psi);
 }
 
 // ---------------------------------------------------------------------
@@ -1404,13 +1406,55 @@ static DctPlan dct_plan(int n) {
   return pl;
 }
 
+// values per thread (2^LOGR) of the register FFT of 2^LG-point rows: the
+// default 8, or the tuner's 4 or 16 where the workgroup keeps 32..1024
+// threads (every choice performs the same butterflies: same bits)
+constexpr bool fft_logr_ok(int lg, int lr) {
+  return lr == FFT_LOGR_DEFAULT || (lg - lr >= 5 && lg - lr <= 10);
+}
+// whether the context's rows use the register FFT, and whether 2^lr values
+// per thread are available there (host, for the tuner)
+bool fft_tunable(const Ctx *c) { return dct_mode(c->nx) == 2; }
+bool fft_logr_valid(const Ctx *c, int lr) {
+  return fft_tunable(c) && fft_logr_ok(ilog2_exact(c->nx), lr);
+}
+static int fft_logr(const Ctx *c, int lg) {
+  const int lr = c->fft_logr;
+  return (lr == 2 || lr == 4) && fft_logr_ok(lg, lr) ? lr : FFT_LOGR_DEFAULT;
+}
+
 // the register-FFT right-hand side of a power-of-two row length 2^LG
+template <int LG, bool P2, bool ADD, int LR>
+static void rhs_tr(Ctx *c, int rk, size_t shmem, const double *rhs_add) {
+  if constexpr (fft_logr_ok(LG, LR))
+    go<RhsDct2TB<LG, P2, ADD, LR>>(k_rhs_dct2_t<LG, P2, ADD, LR>, dim3((c->ny + 1) / 2),
+                                   dim3(c->nx >> LR), shmem, c->stream, c->L, c->ux, c->uy,
+                                   c->dscal, c->gamma[rk], c->dx, c->dy, c->ctab, c->twid,
+                                   c->qhat, rhs_add);
+}
 template <int LG, bool P2, bool ADD>
 static void rhs_t(Ctx *c, int rk, size_t shmem, const double *rhs_add) {
-  go<RhsDct2TB<LG, P2, ADD>>(k_rhs_dct2_t<LG, P2, ADD>, dim3((c->ny + 1) / 2),
-                             dim3(c->nx >> dct_logr<LG>()), shmem, c->stream, c->L, c->ux,
-                             c->uy, c->dscal, c->gamma[rk], c->dx, c->dy, c->ctab, c->twid,
-                             c->qhat, rhs_add);
+  switch (fft_logr(c, LG)) {
+    case 2: rhs_tr<LG, P2, ADD, 2>(c, rk, shmem, rhs_add); break;
+    case 4: rhs_tr<LG, P2, ADD, 4>(c, rk, shmem, rhs_add); break;
+    default: rhs_tr<LG, P2, ADD, dct_logr<LG>()>(c, rk, shmem, rhs_add); break;
+  }
+}
+
+// the register-FFT DCT-III of 2^LG-point rows into psi
+template <int LG, int LR>
+static void dct3_tr(Ctx *c, size_t shmem, const CombineArgs &ca) {
+  if constexpr (fft_logr_ok(LG, LR))
+    go<Dct3PsiTB<LG, LR>>(k_dct3_psi_t<LG, LR>, dim3((c->ny + 1) / 2), dim3(c->nx >> LR),
+                          shmem, c->stream, c->L, c->qhat, ca, c->ctab, c->twid, c->psi);
+}
+template <int LG>
+static void dct3_t(Ctx *c, size_t shmem, const CombineArgs &ca) {
+  switch (fft_logr(c, LG)) {
+    case 2: dct3_tr<LG, 2>(c, shmem, ca); break;
+    case 4: dct3_tr<LG, 4>(c, shmem, ca); break;
+    default: dct3_tr<LG, dct_logr<LG>()>(c, shmem, ca); break;
+  }
 }
 
 // the right-hand side and its DCT-II along x, then the solves along y;
@@ -1504,10 +1548,7 @@ int launch_poisson(Ctx *c, int rk, const double *rhs_add) {
 #define EIF_DCT3_T(LG)                                                                  \
   case LG:                                                                              \
     halo_done = true;                                                                   \
-    go<Dct3PsiTB<LG>>(k_dct3_psi_t<LG>, dim3((ny + 1) / 2), dim3(nx >> dct_logr<LG>()),   \
-                       shmem,                                                           \
-                       c->stream, c->L,                                                 \
-                       c->qhat, ca, c->ctab, c->twid, c->psi);                          \
+    dct3_t<LG>(c, shmem, ca);                                                           \
     break;
     switch (mode == 2 ? logn : -1) {
       EIF_DCT3_T(6) EIF_DCT3_T(7) EIF_DCT3_T(8) EIF_DCT3_T(9) EIF_DCT3_T(10) EIF_DCT3_T(11)

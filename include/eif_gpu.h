@@ -211,14 +211,23 @@ int eif_gpu_graph_set_subiterations(eif_ctx *ctx, const int K[3]);
 int eif_gpu_graph_stats(eif_ctx *ctx, long out[4]);
 
 /* Autotuner (no counterpart in the reference): times the candidate launch
- * shapes on this context's grid and keeps the fastest: rows per workgroup
- * of the predictor (8, 16, 32) and of the projection (4, 8, 16), which
- * change no result, and the chunk length of the partitioned y-solve (16,
- * 32, 64 rows where they divide jtot), which changes rounding only;
- * flags & 1 also lets it switch between the Thomas sweep and the
- * partitioned solve.  report (may be NULL) receives the timings and the
- * choices.  Synchronising; leaves psi holding its last timing solve. */
+ * shapes on this context's grid and keeps the fastest.  Rows per workgroup
+ * of the predictor (8, 16, 32) and of the projection (4, 8, 16) and the
+ * register FFT's values per thread (4, 8, 16: the same butterflies) change
+ * no result.  The chunk length of the partitioned y-solve (16, 32, 64 rows
+ * where they divide jtot), and in partition mode the fused projection (see
+ * eif_gpu_set_projection), change rounding only; flags & 1 also lets it
+ * switch between the Thomas sweep and the partitioned solve.  report (may
+ * be NULL) receives the timings and the choices.  Synchronising; leaves psi
+ * holding its last timing solve. */
 int eif_gpu_autotune(eif_ctx *ctx, int flags, char *report, int report_len);
+
+/* The launch shapes eif_gpu_autotune chose for src (rows per workgroup,
+ * y-solve and its chunk length, FFT values per thread, projection form)
+ * applied to dst, a context of the same grid: an ensemble tunes one case
+ * per grid shape.  No counterpart in the reference.  Drops dst's captured
+ * graphs. */
+int eif_gpu_tune_copy(eif_ctx *dst, const eif_ctx *src);
 
 /* How eif_gpu_integrate (and graph steps) run each stage's potential,
  * projection and pressure update (compute_potential.c:48-138,

@@ -42,6 +42,7 @@ def test_autotuned_context_matches_untuned(mode):
     assert re.search(r"predictor_rows:.* -> (8|16|32)", rep)
     assert re.search(r"projection_rows:.* -> (4|8|16)", rep)
     assert re.search(r"projection:.* -> (unfused|2|4|8);", rep)
+    assert re.search(r"fft_values_per_thread:.* -> (4|8|16)", rep)
     for _ in range(3):
         assert ref.step() == pytest.approx(tuned.step(), rel=1e-13)
     a, b = ref.download(), tuned.download()
@@ -60,3 +61,25 @@ def test_autotuned_context_matches_untuned(mode):
         assert np.max(np.abs(ref.particles()[:, 3:6] - tuned.particles()[:, 3:6])) < 1e-12
     ref.close()
     tuned.close()
+
+
+def test_tuned_shapes_copied_to_a_case_of_the_same_grid():
+    """eif_gpu_tune_copy: a context given another's tuned shapes runs
+    exactly as the tuned one; a different grid is refused."""
+    parts = cases.place_ellipses(16, 2.0, (0.03, 0.05), (0.5, 0.5), 0.01, 2)
+    src = _make(eif.POISSON_PARTITION, parts)
+    dst = _make(eif.POISSON_PARTITION, parts)
+    rep = src.autotune()
+    print("autotune:", rep)
+    dst.tune_copy(src)
+    for _ in range(3):
+        assert src.step() == dst.step()
+    a, b = src.download(), dst.download()
+    for k in ("ux", "uy", "p"):
+        assert np.array_equal(a[k], b[k]), k
+    assert np.array_equal(src.particles(), dst.particles())
+    other = eif.Solver(itot=256, jtot=512, ly=2.0, Re=500.0)
+    with pytest.raises(eif.EIFError):
+        other.tune_copy(src)
+    for g in (src, dst, other):
+        g.close()
