@@ -573,9 +573,11 @@ __device__ __forceinline__ double psi_at(const double *src, int i) {
   return v * inv_norm;
 }
 
+// up to 8 values per thread bounded to 4 waves per SIMD (128 VGPRs, no
+// spills; 3 waves otherwise); 16 values per thread need more registers
 template <int LOGN, int LOGR_ = dct_logr<LOGN>()>
 struct Dct3PsiTB {
-  static constexpr int T = (1 << LOGN) >> LOGR_, M = 0;
+  static constexpr int T = (1 << LOGN) >> LOGR_, M = LOGR_ <= 3 ? 4 : 0;
   static __device__ __forceinline__ void run(const VB &vb, 
     Layout L, const double *qhat, CombineArgs ca, const double *__restrict__ C,
     const double *__restrict__ W, double *psi) {
@@ -647,7 +649,7 @@ struct Dct3PsiTB {
 }
 };
 template <int LOGN, int LOGR_ = dct_logr<LOGN>()>
-__global__ __launch_bounds__((1 << LOGN) >> LOGR_) void k_dct3_psi_t(
+__global__ __launch_bounds__((1 << LOGN) >> LOGR_, LOGR_ <= 3 ? 4 : 1) void k_dct3_psi_t(
     Layout L, const double *qhat, CombineArgs ca, const double *__restrict__ C,
     const double *__restrict__ W, double *psi) {
   Dct3PsiTB<LOGN, LOGR_>::run(VB::here(), L, qhat, ca, C, W, psi);
