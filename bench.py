@@ -133,7 +133,7 @@ def cpu_baseline(parts, xc, warm=1, timed=4):
 
 
 # algorithmic HBM bytes per launch of the fluid kernels, in units of
-# N = nx*ny values of 8 B (SURVEY-style compulsory traffic): the predictor
+# N = nx*ny values of 8 B (compulsory traffic, each field once): the predictor
 # reads ux, uy, p and (stages 1-2) the kept previous velocity, and writes
 # the new velocity: 5N at stage 0, 7N after, 19/3 N on average; the fused
 # DCT-III + projection reads the solved coefficients, ux, uy and p and

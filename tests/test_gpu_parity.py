@@ -1,7 +1,7 @@
 """Parity of the gfx950 path (through the C-ABI, libeif_gpu.so) with the
 oracle on the same seeded inputs.
 
-Bars (north_star): per-step relative L2 of ux, uy <= 1e-12 and of p (less
+Bars: per-step relative L2 of ux, uy <= 1e-12 and of p (less
 its spatial mean: the reference's pressure gauge is round-off determined,
 src/tdm.c:100-102) <= 1e-12; particle x, y, az <= 1e-9 relative after 1000
 steps of c0; colliding-pair sets identical.  The fluid functions are

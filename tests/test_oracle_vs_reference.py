@@ -1,8 +1,8 @@
 """Pins the oracle against the reference's OWN compiled sources
 (oracle/_ref, built from NaokiHori/EllipsesInFlows src/ellipse.c,
 src/suspensions/{auxiliary,collision,reset,init}.c, src/param/init.c,
-src/fileio.c, src/simple_npyio.c).  Skipped where the reference is not
-available (the GPU box); the restatement is compared bit for bit."""
+src/fileio.c, src/simple_npyio.c).  Skipped where the reference's sources
+are not available; the restatement is compared bit for bit."""
 import ctypes
 import os
 import tempfile
