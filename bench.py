@@ -289,6 +289,64 @@ def adjoint_bench(nsteps=200, every=20):
             "objective": J, "grad_norm_positions": float(np.linalg.norm(pb[:, :2]))}
 
 
+# the collision broad phase's sweep: particle counts at ~17-30 % area
+# fraction, each with a grid that resolves its ellipses by 2.5-8 cells
+COLLISION_SWEEP = (
+    ("1024 ellipses, a .013-.019", 512, 1024, 1024, (0.013, 0.019), -0.004),
+    ("4096 ellipses, a .006-.009", 1024, 2048, 4096, (0.006, 0.009), -0.001),
+    ("16384 ellipses, a .003-.0045", 1024, 2048, 16384, (0.003, 0.0045), -0.0005),
+)
+
+
+def collision_sweep(nsteps=3):
+    """Time per step, and the average collision launch (with the cell-list
+    build) and sub-iteration launch, of eager steps with the broad
+    phase testing all pairs and with the cell list, on the same cases; the
+    two runs' particle records must be identical."""
+    out = []
+    for name, nx, ny, npart, a_range, gap in COLLISION_SWEEP:
+        parts = initparticles.random_ellipses(npart, 2.0, a_range, (0.5, 1.0), gap, npart)
+        res = {}
+        for search in (eif.COLLISION_ALL_PAIRS, eif.COLLISION_CELLS):
+            g = eif.Solver(itot=nx, jtot=ny, ly=2.0, Re=500.0, extfrcy=8.0 / 500.0,
+                           poisson_mode=eif.POISSON_PARTITION)
+            xc = g.grid()["xc"]
+            uy = np.zeros((ny + 2, nx + 2))
+            uy[1:-1, 1:-1] = 4.0 * xc[None, 1:-1] * (1 - xc[None, 1:-1])
+            g.upload(uy=uy)
+            g.update_boundaries()
+            g.set_particles(**{k: parts[k] for k in ("den", "a", "b", "x", "y", "az", "ux", "uy", "vz")})
+            g.set_collision_search(search)
+            g.step()
+            # timed eager steps, then eager steps with each launch between
+            # events
+            g.synchronize()
+            t0 = time.perf_counter()
+            g.run_steps(nsteps)
+            g.synchronize()
+            r = {"ms_per_step": 1e3 * (time.perf_counter() - t0) / nsteps}
+            g.profile_select(None)
+            g.profile_sample(1)
+            g.profile(True)
+            g.profile_reset()
+            g.run_steps(nsteps)
+            g.synchronize()
+            g.profile(False)
+            for k in ("collision", "ibm_substep"):
+                cnt, tms = g.profile_query(k)
+                r[k + "_avg_launch_ms"] = tms / max(cnt, 1)
+            res[search] = (r, g.particles())
+            g.close()
+        a, c = res[eif.COLLISION_ALL_PAIRS], res[eif.COLLISION_CELLS]
+        if not np.array_equal(a[1], c[1]):
+            raise RuntimeError("cell list and all pairs differ: " + name)
+        row = {"case": name, "grid": [nx, ny], "particles": npart, "all_pairs": a[0],
+               "cell_list": c[0], "speedup": a[0]["ms_per_step"] / c[0]["ms_per_step"]}
+        log("collision sweep:", json.dumps(row))
+        out.append(row)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -307,11 +365,17 @@ def main():
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
                     help="before the workload, copy a buffer of this size once on the GPU "
                          "(a kernel of known traffic for scaling FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--collision-sweep", action="store_true",
+                    help="only compare the collision broad phases (all pairs, cell list) "
+                         "at 1024, 4096 and 16384 particles")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.collision_sweep:
+        print(json.dumps({"collision_sweep": collision_sweep()}))
+        return
 
     if args.calibration_copy_mib > 0:
         import torch

@@ -52,6 +52,7 @@ EXPORTED = (
     "eif_gpu_integrate", "eif_gpu_step", "eif_gpu_synchronize", "eif_gpu_last_error",
     "eif_gpu_run_steps", "eif_gpu_graph_enable", "eif_gpu_graph_stats",
     "eif_gpu_graph_set_subiterations", "eif_gpu_ensemble_run_steps", "eif_gpu_set_projection",
+    "eif_gpu_set_collision_search",
     "eif_gpu_tune_copy",
     "eif_gpu_tape_begin", "eif_gpu_step_fixed_dt", "eif_gpu_stage_adjoint",
     "eif_gpu_adjoint_get", "eif_gpu_adjoint_add", "fluid_gpu_wall_shear",
@@ -78,6 +79,7 @@ def partition_layout(jtot):
 
 
 ENSEMBLE_SEQUENTIAL, ENSEMBLE_STREAMS, ENSEMBLE_ONE_GRAPH, ENSEMBLE_BATCHED = 0, 1, 2, 3
+COLLISION_AUTO, COLLISION_ALL_PAIRS, COLLISION_CELLS = 0, 1, 2
 
 
 def ensemble_run_steps(solvers, nsteps, concurrent=True):
@@ -406,6 +408,11 @@ class Solver:
         """0: DCT-III and projection as two kernels (psi stored); 2, 4, 8:
         one fused kernel over strips of that many rows (psi not stored)."""
         self._check(self.lib.eif_gpu_set_projection(self.h, int(strip)))
+
+    def set_collision_search(self, mode):
+        """Collision broad phase: COLLISION_AUTO (cell list from 4096
+        particles), COLLISION_ALL_PAIRS or COLLISION_CELLS; the same results."""
+        self._check(self.lib.eif_gpu_set_collision_search(self.h, int(mode)))
 
     def graph_set_subiterations(self, k):
         arr = (ctypes.c_int * 3)(*[int(v) for v in k])

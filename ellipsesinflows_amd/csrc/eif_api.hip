@@ -191,7 +191,7 @@ static void free_ctx(Ctx *c) {
   void *others[] = {c->th_flag, c->boxes, c->box_off, c->tile_cnt, c->tile_list,
                     c->active, c->flags, c->rows, c->row_cnt, c->wall_hit,
                     c->ibm_stats, c->slice_first, c->slice_info, c->ibm_pp, c->slice_owner,
-                    c->sc_key, c->heads};
+                    c->sc_key, c->heads, c->cl_start, c->cl_items, c->cl_cell, c->cl_slot, c->cl_count};
   for (auto o : others)
     if (o) (void)hipFree(o);
   if (c->hscal) (void)hipHostFree(c->hscal);
@@ -600,6 +600,9 @@ int statistics_gpu_download(eif_ctx *c, double *ux1, double *ux2, double *uy1,
   return EIF_OK;
 }
 
+// drops captured graphs (defined with the graph steps)
+static int graphs_recapture(eif_ctx *c);
+
 int suspensions_gpu_create(eif_ctx *c, int n, const double *den, const double *a,
                            const double *b, const double *x, const double *y,
                            const double *az, const double *ux, const double *uy,
@@ -650,6 +653,13 @@ int suspensions_gpu_upload(eif_ctx *c, const double *records) {
   EIF_HIP(hipMemsetAsync(c->sc_pos, 0, 4 * (size_t)c->np * sizeof(double), c->stream));
   EIF_HIP(hipMemsetAsync(c->sc_key, 0, 8 * (size_t)c->np * sizeof(int), c->stream));
   EIF_HIP(hipStreamSynchronize(c->stream));
+  // new sizes may change the cells of the collision broad phase, which
+  // captured graphs hold
+  const int ncx = c->cl_ncx, ncy = c->cl_ncy;
+  const double hx = c->cl_hx, hy = c->cl_hy;
+  TRY(coll_cells_setup(c, records));
+  if (ncx != c->cl_ncx || ncy != c->cl_ncy || hx != c->cl_hx || hy != c->cl_hy)
+    TRY(graphs_recapture(c));
   return EIF_OK;
 }
 
@@ -1794,13 +1804,19 @@ int eif_gpu_tune_copy(eif_ctx *dst, const eif_ctx *src) {
       TRY(partition_setup(dst, src->part_L));
   }
   // graphs captured with the old shapes are dropped
-  if (dst->graph_on) TRY(eif_gpu_graph_enable(dst, 1));
-  return EIF_OK;
+  return graphs_recapture(dst);
 }
 
 int eif_gpu_set_projection(eif_ctx *c, int strip) {
   CHECK_CTX(c);
-  return set_projection(c, strip);
+  TRY(set_projection(c, strip));
+  return graphs_recapture(c);
+}
+
+int eif_gpu_set_collision_search(eif_ctx *c, int mode) {
+  CHECK_CTX(c);
+  TRY(set_collision_search(c, mode));
+  return graphs_recapture(c);
 }
 
 int eif_gpu_graph_enable(eif_ctx *c, int enable) {
@@ -1812,6 +1828,17 @@ int eif_gpu_graph_enable(eif_ctx *c, int enable) {
     c->graphs = new GraphCache();
     EIF_HIP(hipMemcpy(&c->graphs->seq, c->g_seq, sizeof(unsigned), hipMemcpyDeviceToHost));
   }
+  return EIF_OK;
+}
+
+// graphs captured with the kernels or arguments a setting changes are
+// dropped; the sub-iterations per stage are kept
+static int graphs_recapture(eif_ctx *c) {
+  if (!c->graph_on || !c->graphs) return EIF_OK;
+  int K[3];
+  for (int r = 0; r < 3; r++) K[r] = c->graphs->K[r];
+  TRY(eif_gpu_graph_enable(c, 1));
+  for (int r = 0; r < 3; r++) c->graphs->K[r] = K[r];
   return EIF_OK;
 }
 

@@ -55,6 +55,8 @@ constexpr int TILE_X = 64;
 constexpr int TILE_Y = 4;
 constexpr int TILE_CAP = 48;     // boxes registered per tile
 constexpr int CONTACT_CAP = 64;  // colliding partners per particle
+constexpr int COLL_CELLS_MIN = 4096;  // particles from which the cell list is chosen
+constexpr int COLL_CELLS_CAP = 14336; // cells at most: their counts and a scan in 60 KiB of LDS
 constexpr int EQC_ITER_CAP = 100000;  // guard of the uncapped fixed point
 constexpr int DT_PARTS_CAP = 16384;   // workgroups of the projection with fused dt
 constexpr int RES_SLOTS = 128;        // sub-iterations of one stage queued at most
@@ -205,6 +207,18 @@ struct Ctx {
   unsigned hstat_seq = 0;       // number of the last status record requested
   int substep_guess = 3; // sub-iterations queued before the first check
   // collisions
+  // broad phase: 0 chooses (cell list from COLL_CELLS_MIN particles, where
+  // the grid of cells allows it), 1 all pairs, 2 cell list where allowed
+  int coll_mode = 0;
+  // the cell list: ncx x ncy cells of cl_hx x cl_hy, each at least the
+  // widest pair reach 2 max(a, b) (0 cells: not allowed for these sizes)
+  int cl_ncx = 0, cl_ncy = 0;
+  double cl_hx = 0., cl_hy = 0.;
+  int *cl_start = nullptr;  // ncells + 1 offsets into cl_items
+  int *cl_items = nullptr;  // np particle indices, by cell
+  int *cl_cell = nullptr;   // np: each particle's cell
+  int *cl_slot = nullptr;   // np: its place within the cell
+  int *cl_count = nullptr;  // COLL_CELLS_CAP per-cell counts, zero between builds
   int *rows = nullptr;      // np x CONTACT_CAP colliding partners, ascending
   int *row_cnt = nullptr;   // np
   int *wall_hit = nullptr;  // np x 2
@@ -284,6 +298,10 @@ bool project_fusable(const Ctx *c);
 // given copies of the fields
 int project_tune(Ctx *c, double *ux, double *uy, double *p, std::string *rep);
 int set_projection(Ctx *c, int strip);
+// the collision broad phase's cells for the particles' sizes (records on
+// the host, np x R_N); called whenever the records are set
+int coll_cells_setup(Ctx *c, const double *records);
+int set_collision_search(Ctx *c, int mode);
 // the register FFT's values per thread: tunable for this context's rows,
 // and 2^lr available there
 bool fft_tunable(const Ctx *c);

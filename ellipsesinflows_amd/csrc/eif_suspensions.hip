@@ -36,6 +36,12 @@
 
 namespace eif {
 
+#define TRY_S(x)          \
+  do {                    \
+    const int _r = (x);   \
+    if (_r) return _r;    \
+  } while (0)
+
 #define NEXTRA 3
 #define BETA 2.
 
@@ -1278,6 +1284,12 @@ struct CollArgs {
   int *wall_hit;   // np x 2
   int *flags;
   const int *halt;
+  // the cell list (use_cells): partners are sought in a particle's cell and
+  // its eight neighbours (periodic in y) instead of among all particles
+  int use_cells;
+  int ncx, ncy;
+  double hx, hy;
+  int *cl_start, *cl_items, *cl_cell, *cl_slot;
 };
 
 __device__ inline double harmonic(double v0, double v1) {
@@ -1421,23 +1433,71 @@ __device__ bool contact_force(const CollArgs &a, int n0, int n1, int wall, int s
 // index, then wall x = 0, then wall x = lx.
 constexpr int COLL_CAND_CAP = 256;  // candidate partners per particle
 
-__device__ void collide_particle(const CollArgs &a, int n) {
-  __shared__ int s_cand[COLL_CAND_CAP];
+// the candidates of particle n among the particles of cell c (in the cell
+// list's order), appended to s_cand
+__device__ inline void cell_candidates(const CollArgs &a, int n, int c, int *s_cand, int &cnt) {
   const int lane = threadIdx.x;
-  int cnt = 0;
-  for (int base = 0; base < a.np; base += 64) {
-    const int m = base + lane;
-    const bool c = m < a.np && m != n && pair_candidate(a, min(n, m), max(n, m));
-    const unsigned long long b = __ballot(c);
-    if (c) {
+  const int s0 = a.cl_start[c], s1 = a.cl_start[c + 1];
+  for (int base = s0; base < s1; base += 64) {
+    const int k = base + lane;
+    const int m = k < s1 ? a.cl_items[k] : n;
+    const bool hit = m != n && pair_candidate(a, min(n, m), max(n, m));
+    const unsigned long long b = __ballot(hit);
+    if (hit) {
       const int pos = cnt + __popcll(b & ((1ull << lane) - 1ull));
       if (pos < COLL_CAND_CAP) s_cand[pos] = m;
     }
     cnt += __popcll(b);
   }
-  if (cnt > COLL_CAND_CAP) {
-    if (lane == 0) atomicOr(&a.flags[0], 4);
-    cnt = COLL_CAND_CAP;
+}
+
+__device__ void collide_particle(const CollArgs &a, int n) {
+  __shared__ int s_cand[COLL_CAND_CAP];
+  __shared__ int s_sorted[COLL_CAND_CAP];
+  const int lane = threadIdx.x;
+  int cnt = 0;
+  const int *cand = s_cand;
+  if (a.use_cells) {
+    // the cell and its neighbours (ncy >= 3: the three rows are distinct);
+    // two particles within reach are in the same or adjacent cells
+    const int c0 = a.cl_cell[n], cx0 = c0 % a.ncx, cy0 = c0 / a.ncx;
+    for (int dyc = -1; dyc <= 1; dyc++) {
+      const int cy = (cy0 + dyc + a.ncy) % a.ncy;
+      for (int dxc = -1; dxc <= 1; dxc++) {
+        const int cx = cx0 + dxc;
+        if (cx < 0 || cx >= a.ncx) continue;
+        cell_candidates(a, n, cy * a.ncx + cx, s_cand, cnt);
+      }
+    }
+    if (cnt > COLL_CAND_CAP) {
+      if (lane == 0) atomicOr(&a.flags[0], 4);
+      cnt = COLL_CAND_CAP;
+    }
+    __syncthreads();
+    // ascending partner order, as the scan over all particles finds them:
+    // each candidate's rank is the number of smaller ones (all distinct)
+    for (int k = lane; k < cnt; k += 64) {
+      const int v = s_cand[k];
+      int r = 0;
+      for (int j = 0; j < cnt; j++) r += s_cand[j] < v;
+      s_sorted[r] = v;
+    }
+    cand = s_sorted;
+  } else {
+    for (int base = 0; base < a.np; base += 64) {
+      const int m = base + lane;
+      const bool c = m < a.np && m != n && pair_candidate(a, min(n, m), max(n, m));
+      const unsigned long long b = __ballot(c);
+      if (c) {
+        const int pos = cnt + __popcll(b & ((1ull << lane) - 1ull));
+        if (pos < COLL_CAND_CAP) s_cand[pos] = m;
+      }
+      cnt += __popcll(b);
+    }
+    if (cnt > COLL_CAND_CAP) {
+      if (lane == 0) atomicOr(&a.flags[0], 4);
+      cnt = COLL_CAND_CAP;
+    }
   }
   __syncthreads();
   double fx = 0., fy = 0., tz = 0.;
@@ -1451,7 +1511,7 @@ __device__ void collide_particle(const CollArgs &a, int n) {
     double w0 = 0., w1 = 0., w2 = 0.;
     if (k < njobs) {
       const int wall = k < cnt ? -1 : k - cnt;
-      if (wall < 0) m = s_cand[k];
+      if (wall < 0) m = cand[k];
       double v[6] = {0., 0., 0., 0., 0., 0.};
       hit = contact_force(a, wall < 0 ? min(n, m) : n, wall < 0 ? max(n, m) : n, wall, side, v);
       const bool first = wall >= 0 || n < m;  // n's own contributions
@@ -1499,6 +1559,111 @@ struct CollideB {
 };
 __global__ __launch_bounds__(64) void k_collide(CollArgs a) {
   CollideB::run(VB::here(), a);
+}
+
+// The cell list of the collision broad phase at the particles' current
+// positions (x + dx, y + dy), in three launches:
+//  - count: each particle's cell, and its place in the cell from a device
+//    atomic on the cell's count (the order within a cell follows the
+//    atomics; the broad phase sorts each particle's candidates);
+//  - scan: one workgroup turns the counts into offsets and clears them, so
+//    that every build starts from zero counts;
+//  - place: each particle's index stored at its cell's offset plus its place.
+// Count and place are gated like the collisions they serve; the scan only
+// by the sub-iteration (not by the halt flag, which another stream may
+// raise between the launches), so it runs whenever the count ran.
+
+// the cell of a particle at (x, y): reciprocals may move a cell boundary by
+// rounding, which the 1e-6 margin of the cell size covers
+__device__ inline int cell_of(const CollArgs &a, double x, double y) {
+  const double yw = y - floor(y * (1. / a.ly)) * a.ly;  // into [0, ly), for the cell only
+  const int cx = min(max((int)floor(x * (1. / a.hx)), 0), a.ncx - 1);
+  const int cy = min(max((int)floor(yw * (1. / a.hy)), 0), a.ncy - 1);
+  return cy * a.ncx + cx;
+}
+
+struct CellCountB {
+  static constexpr int T = 256, M = 0;
+  static __device__ __forceinline__ void run(const VB &vb, CollArgs a, int *counts) {
+    if (halted(a.halt)) return;
+    if (!sub_open(a.res, a.sub)) return;
+    const int n = vb.bx * blockDim.x + threadIdx.x;
+    if (n >= a.np) return;
+    const double *q = a.prt + (long)n * R_N;
+    const int c = cell_of(a, q[R_X] + q[R_DX], q[R_Y] + q[R_DY]);
+    a.cl_cell[n] = c;
+    a.cl_slot[n] = atomicAdd(&counts[c], 1);
+  }
+};
+__global__ __launch_bounds__(256) void k_cell_count(CollArgs a, int *counts) {
+  CellCountB::run(VB::here(), a, counts);
+}
+
+struct CellScanB {
+  static constexpr int T = 1024, M = 0;
+  static __device__ __forceinline__ void run(const VB &vb, CollArgs a, int *counts) {
+    if (!sub_open(a.res, a.sub)) return;
+    extern __shared__ int cl_lds[];
+    const int nc = a.ncx * a.ncy;
+    int *cnt = cl_lds;          // nc counts
+    int *part = cl_lds + nc;    // blockDim.x partial sums
+    for (int k = threadIdx.x; k < nc; k += blockDim.x) {
+      cnt[k] = counts[k];
+      counts[k] = 0;
+    }
+    __syncthreads();
+    // exclusive scan: each thread sums a contiguous run; the runs' sums
+    // are scanned within each wave by lane shuffles, the waves' totals by
+    // the first wave; each run is offset in LDS, then stored coalesced
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int per = (nc + blockDim.x - 1) / blockDim.x;
+    const int k0 = min(t * per, nc), k1 = min(k0 + per, nc);
+    int sum = 0;
+    for (int k = k0; k < k1; k++) sum += cnt[k];
+    int inc = sum;  // inclusive scan of the wave's run sums
+    for (int d = 1; d < 64; d <<= 1) {
+      const int v = __shfl_up(inc, d, 64);
+      if (lane >= d) inc += v;
+    }
+    if (lane == 63) part[w] = inc;
+    __syncthreads();
+    if (w == 0) {
+      const int nw = blockDim.x >> 6;
+      int tot = lane < nw ? part[lane] : 0;
+      for (int d = 1; d < 64; d <<= 1) {
+        const int v = __shfl_up(tot, d, 64);
+        if (lane >= d) tot += v;
+      }
+      if (lane < nw) part[32 + lane] = tot;  // inclusive totals of waves
+    }
+    __syncthreads();
+    int run = (w > 0 ? part[32 + w - 1] : 0) + inc - sum;  // exclusive offset of this run
+    for (int k = k0; k < k1; k++) {
+      const int v = cnt[k];
+      cnt[k] = run;
+      run += v;
+    }
+    __syncthreads();
+    for (int k = t; k < nc; k += blockDim.x) a.cl_start[k] = cnt[k];
+    if (t == blockDim.x - 1) a.cl_start[nc] = part[32 + (blockDim.x >> 6) - 1];
+  }
+};
+__global__ __launch_bounds__(1024) void k_cell_scan(CollArgs a, int *counts) {
+  CellScanB::run(VB::here(), a, counts);
+}
+
+struct CellPlaceB {
+  static constexpr int T = 256, M = 0;
+  static __device__ __forceinline__ void run(const VB &vb, CollArgs a) {
+    if (halted(a.halt)) return;
+    if (!sub_open(a.res, a.sub)) return;
+    const int n = vb.bx * blockDim.x + threadIdx.x;
+    if (n >= a.np) return;
+    a.cl_items[a.cl_start[a.cl_cell[n]] + a.cl_slot[n]] = n;
+  }
+};
+__global__ __launch_bounds__(256) void k_cell_place(CollArgs a) {
+  CellPlaceB::run(VB::here(), a);
 }
 
 // One launch for a sub-iteration's inertia(1) and collision(1), which read
@@ -1789,6 +1954,7 @@ int susp_init(Ctx *c) {
   }
   c->scratch_cells = off[c->np];
   c->nslices = first[c->np];
+  TRY_S(coll_cells_setup(c, rec.data()));
   EIF_HIP(hipMalloc(&c->slice_first, (c->np + 1) * sizeof(int)));
   EIF_HIP(hipMemcpy(c->slice_first, first.data(), (c->np + 1) * sizeof(int), hipMemcpyHostToDevice));
   {
@@ -1977,6 +2143,63 @@ int launch_materialise_du(Ctx *c) {
   return launch_bc_p(c, c->duy);
 }
 
+// whether the collision broad phase uses the cell list
+static bool coll_use_cells(const Ctx *c) {
+  if (c->cl_ncy < 3 || c->coll_mode == 1) return false;
+  return c->coll_mode == 2 || c->np >= COLL_CELLS_MIN;
+}
+
+int coll_cells_setup(Ctx *c, const double *rec) {
+  double rmax = 0.;
+  for (int n = 0; n < c->np; n++)
+    rmax = fmax(rmax, fmax(rec[(size_t)n * R_N + R_A], rec[(size_t)n * R_N + R_B]));
+  int ncx = 0, ncy = 0;
+  if (rmax > 0.) {
+    // cells at least the widest reach (with a margin above rounding), and
+    // at most COLL_CELLS_CAP of them
+    double h = fmax(2. * rmax * (1. + 1e-6), sqrt(c->lx * c->ly / COLL_CELLS_CAP));
+    for (;;) {
+      const double fx = fmax(floor(c->lx / h), 1.), fy = fmax(floor(c->ly / h), 1.);
+      if (fx * fy <= COLL_CELLS_CAP) {
+        ncx = (int)fx;
+        ncy = (int)fy;
+        break;
+      }
+      h *= 1.05;
+    }
+  }
+  // sized for the most cells once: graphs keep the addresses.  The counts
+  // start at zero and every build leaves them at zero; cells and places
+  // start at zero, so that a build whose count was halted still places
+  // within bounds
+  if (!c->cl_start) {
+    EIF_HIP(hipMalloc(&c->cl_start, (COLL_CELLS_CAP + 1) * sizeof(int)));
+    EIF_HIP(hipMalloc(&c->cl_count, COLL_CELLS_CAP * sizeof(int)));
+    const size_t n = (size_t)(c->np > 0 ? c->np : 1) * sizeof(int);
+    EIF_HIP(hipMalloc(&c->cl_items, n));
+    EIF_HIP(hipMalloc(&c->cl_cell, n));
+    EIF_HIP(hipMalloc(&c->cl_slot, n));
+    EIF_HIP(hipMemset(c->cl_start, 0, (COLL_CELLS_CAP + 1) * sizeof(int)));
+    EIF_HIP(hipMemset(c->cl_count, 0, COLL_CELLS_CAP * sizeof(int)));
+    EIF_HIP(hipMemset(c->cl_cell, 0, n));
+    EIF_HIP(hipMemset(c->cl_slot, 0, n));
+  }
+  c->cl_ncx = ncx;
+  c->cl_ncy = ncy;
+  c->cl_hx = ncx > 0 ? c->lx / ncx : 0.;
+  c->cl_hy = ncy > 0 ? c->ly / ncy : 0.;
+  return EIF_OK;
+}
+
+int set_collision_search(Ctx *c, int mode) {
+  if (mode < 0 || mode > 2) {
+    set_error("collision search must be 0 (by particle count), 1 (all pairs) or 2 (cell list)");
+    return EIF_ERR_ARG;
+  }
+  c->coll_mode = mode;
+  return EIF_OK;
+}
+
 static CollArgs coll_args(Ctx *c, int cnstep) {
   CollArgs a;
   a.res = nullptr;
@@ -1991,14 +2214,35 @@ static CollArgs coll_args(Ctx *c, int cnstep) {
   a.wall_hit = c->wall_hit;
   a.flags = c->flags;
   a.halt = c->halt;
+  a.use_cells = coll_use_cells(c) ? 1 : 0;
+  a.ncx = c->cl_ncx;
+  a.ncy = c->cl_ncy;
+  a.hx = c->cl_hx;
+  a.hy = c->cl_hy;
+  a.cl_start = c->cl_start;
+  a.cl_items = c->cl_items;
+  a.cl_cell = c->cl_cell;
+  a.cl_slot = c->cl_slot;
   return a;
+}
+
+// the cell list for a collision launch with these (gated) arguments
+static void launch_cell_list(Ctx *c, const CollArgs &a, hipStream_t st) {
+  if (!a.use_cells) return;
+  const size_t lds = ((size_t)c->cl_ncx * c->cl_ncy + 1024) * sizeof(int);
+  const dim3 g((c->np + 255) / 256);
+  go<CellCountB>(k_cell_count, g, dim3(256), 0, st, a, c->cl_count);
+  go<CellScanB>(k_cell_scan, dim3(1), dim3(1024), lds, st, a, c->cl_count);
+  go<CellPlaceB>(k_cell_place, g, dim3(256), 0, st, a);
 }
 
 int launch_collision(Ctx *c, int cnstep, hipStream_t st) {
   if (c->np == 0) return EIF_OK;
   if (!st) st = c->stream;
   ProfScope ps(c, "collision", st);
-  go<CollideB>(k_collide, dim3(c->np), dim3(64), 0, st, coll_args(c, cnstep));
+  const CollArgs a = coll_args(c, cnstep);
+  launch_cell_list(c, a, st);
+  go<CollideB>(k_collide, dim3(c->np), dim3(64), 0, st, a);
   EIF_HIP(hipGetLastError());
   return EIF_OK;
 }
@@ -2020,6 +2264,7 @@ int launch_substep_forces(Ctx *c, int sub) {
   CollArgs ca = coll_args(c, 1);
   ca.res = a.res;
   ca.sub = sub;
+  launch_cell_list(c, ca, c->stream);
   hipEvent_t e0, e1;
   if (prof_kernel_events(c, "ibm_substep", &e0, &e1))  // a sampled launch stamps its events
     hipExtLaunchKernelGGL(k_inertia_collide, dim3(c->np + c->nslices), dim3(64), 0, c->stream,

@@ -240,6 +240,20 @@ int eif_gpu_tune_copy(eif_ctx *dst, const eif_ctx *src);
  * bits; eif_gpu_autotune times both. */
 int eif_gpu_set_projection(eif_ctx *ctx, int strip);
 
+/* The broad phase of the collisions, which finds the pairs whose bounding
+ * circles overlap before the contacts are solved (it replaces the pair-matrix
+ * loop of collision.c:320-332 with its early exit, collision.c:172-189; no
+ * counterpart in the reference's interface): mode 1 tests all pairs; mode 2
+ * a cell list rebuilt before each collision launch, cells at least the
+ * widest pair reach (2 max(a, b) over the particles) and at most 14336 of
+ * them, each particle tested against the particles of its cell and the eight
+ * neighbours (periodic in y); mode 0 (the default) takes the cell list from
+ * 4096 particles on.  Both find the same partners in the same order: results
+ * are bit-identical.  The cell list needs at least three cells across y
+ * (ly >= 6 max(a, b)); otherwise all pairs are tested.  Drops captured
+ * graphs. */
+int eif_gpu_set_collision_search(eif_ctx *ctx, int mode);
+
 /* Ensembles of independent cases (no counterpart in the reference, which
  * runs one case per process): advances ncases contexts, of any grid sizes
  * and particle counts, by nsteps steps each.  concurrent = 0: one context
