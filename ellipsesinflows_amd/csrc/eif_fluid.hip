@@ -118,11 +118,17 @@ int launch_bc_p(Ctx *c, double *p) {
 //               values are the same bits); srcA is neither read nor
 //               written, which saves 2N values per stage.
 // PRED_SRC only evaluates A of (ux, uy) into srcA: srcuxa/srcuya on demand
-// after PRED_PREV stages.
+// after PRED_PREV stages.  PRED_NOB is PRED_PREV of a stage without a beta
+// term (the first RK stage): no previous-velocity window, so fewer
+// registers and more waves per SIMD.
 // ---------------------------------------------------------------------
 constexpr int PRED_BX = 256;
 constexpr int PRED_ROWS = 16;  // default rows per workgroup (the tuner may pick 8 or 32)
-enum { PRED_STORE = 0, PRED_PREV = 1, PRED_SRC = 2 };
+enum { PRED_STORE = 0, PRED_PREV = 1, PRED_SRC = 2, PRED_NOB = 3 };
+// bit MODE set: that mode loads a row ahead.  Measured on the 2048x4096
+// grid: the first stage's kernel (PRED_NOB) 116 -> 108 us; PRED_PREV,
+// whose two windows leave 3 waves per SIMD either way, 135 -> 138 us
+constexpr int PRED_LOOKAHEAD = 1 << PRED_NOB;
 
 struct PredArgs {
   Layout L;
@@ -165,6 +171,29 @@ __device__ inline void win_row(const Layout &L, const double *ux, const double *
   w.uyp_c = uy[L.p(i, j + 1)];
   w.uyp_e = uy[L.p(i + 1, j + 1)];
 }
+// one row's stencil values (ux and uy at columns i-1, i, i+1) loaded ahead
+// of the row that reads them
+struct PredRow {
+  double uxw, uxc, uxe, uyw, uyc, uye;
+};
+__device__ inline void row_load(const Layout &L, const double *ux, const double *uy, int i,
+                                int jr, bool hx, PredRow &r) {
+  r.uxw = hx ? ux[L.ux(i - 1, jr)] : 0.;
+  r.uxc = ux[L.ux(i, jr)];
+  r.uxe = ux[L.ux(i + 1, jr)];
+  r.uyw = uy[L.p(i - 1, jr)];
+  r.uyc = uy[L.p(i, jr)];
+  r.uye = uy[L.p(i + 1, jr)];
+}
+__device__ inline void row_into(const PredRow &r, PredWin &w) {
+  w.uxp_w = r.uxw;
+  w.uxp_c = r.uxc;
+  w.uxp_e = r.uxe;
+  w.uyp_w = r.uyw;
+  w.uyp_c = r.uyc;
+  w.uyp_e = r.uye;
+}
+
 __device__ inline void win_slide(PredWin &w) {
   w.uxm_c = w.uxc_c; w.uxm_e = w.uxc_e;
   w.uxc_w = w.uxp_w; w.uxc_c = w.uxp_c; w.uxc_e = w.uxp_e;
@@ -245,17 +274,53 @@ struct PredictorB {
   g.idy = 1. / g.dy;
   const bool prev = MODE == PRED_PREV && a.use_b;
   const double *p = a.p;
+  // LA: the next row's loads are issued a row ahead, so that each wave
+  // keeps two rows of loads in flight (the march waits on its loads more
+  // than on its arithmetic)
+  constexpr bool LA = (PRED_LOOKAHEAD >> MODE) & 1;
   PredWin w, wo;
   win_start(L, a.ux, a.uy, i, j0, hx, w);
   if (prev) win_start(L, a.uxo, a.uyo, i, j0, hx, wo);
   double pm_c = MODE == PRED_SRC ? 0. : p[L.p(i, j0 - 1)];
-  for (int j = j0; j <= j1; j++) {
-    win_row(L, a.ux, a.uy, i, j, hx, w);
-    if (prev) win_row(L, a.uxo, a.uyo, i, j, hx, wo);
-    double pc_w = 0., pc_c = 0.;
+  PredRow rn, rno;
+  double pn_w = 0., pn_c = 0.;
+  if (LA) {
+    row_load(L, a.ux, a.uy, i, j0 + 1, hx, rn);
+    if (prev) row_load(L, a.uxo, a.uyo, i, j0 + 1, hx, rno);
     if (MODE != PRED_SRC) {
-      pc_w = p[L.p(i - 1, j)];
-      pc_c = p[L.p(i, j)];
+      pn_w = p[L.p(i - 1, j0)];
+      pn_c = p[L.p(i, j0)];
+    }
+  }
+  for (int j = j0; j <= j1; j++) {
+    double pc_w = 0., pc_c = 0.;
+    if (LA) {
+      // rows j + 2 (velocity) and j + 1 (pressure), clamped to the last
+      // halo row: the values loaded past the march are not used
+      const int jv = min(j + 2, ny + 1), jp = min(j + 1, ny + 1);
+      PredRow rnn, rnno;
+      row_load(L, a.ux, a.uy, i, jv, hx, rnn);
+      if (prev) row_load(L, a.uxo, a.uyo, i, jv, hx, rnno);
+      double pnn_w = 0., pnn_c = 0.;
+      if (MODE != PRED_SRC) {
+        pnn_w = p[L.p(i - 1, jp)];
+        pnn_c = p[L.p(i, jp)];
+      }
+      row_into(rn, w);
+      if (prev) row_into(rno, wo);
+      pc_w = pn_w;
+      pc_c = pn_c;
+      rn = rnn;
+      if (prev) rno = rnno;
+      pn_w = pnn_w;
+      pn_c = pnn_c;
+    } else {
+      win_row(L, a.ux, a.uy, i, j, hx, w);
+      if (prev) win_row(L, a.uxo, a.uyo, i, j, hx, wo);
+      if (MODE != PRED_SRC) {
+        pc_w = p[L.p(i - 1, j)];
+        pc_c = p[L.p(i, j)];
+      }
     }
     // ---- x-momentum at face (i, j)
     if (hx) {
@@ -333,6 +398,12 @@ __global__ __launch_bounds__(PRED_BX) void k_predictor(PredArgs a) {
 
 template <int MODE, int ROWS>
 static void pred_launch_rows(Ctx *c, const PredArgs &a, hipStream_t st) {
+  if constexpr (MODE == PRED_PREV) {
+    if (!a.use_b) {  // the same operations without the unused window
+      pred_launch_rows<PRED_NOB, ROWS>(c, a, st);
+      return;
+    }
+  }
   dim3 grid((c->nx + PRED_BX - 1) / PRED_BX, (c->ny + ROWS - 1) / ROWS);
   if (c->pow2_grid)
     go<PredictorB<true, MODE, ROWS>>(k_predictor<true, MODE, ROWS>, grid, dim3(PRED_BX), 0, st, a);
