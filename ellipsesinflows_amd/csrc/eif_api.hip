@@ -1400,30 +1400,55 @@ struct StepLaunch {
   std::vector<unsigned> seq;  // per case of each graph, in order
 };
 
-// batched steps launched side by side (measured on the 32-case ensemble:
-// 1 batch 2399, 2 batches 2636, 4 batches 2489 case-steps/s)
+// Batches of one step launched side by side.  Measured on the 32-case
+// ensemble (14 cases of 1024x2048, 11 of 512x1024, 7 of 256x512), in
+// case-steps/s: cases dealt round robin into 1 batch 2399, 2 batches
+// 2594-2667, 4 batches 2489; one batch per grid shape 2740-2749 (the
+// 14-case batch split in two 2722-2750, every shape of more than 8 cases
+// split 2652-2655).  Cases of one shape are batched by shape when the
+// ensemble has two shapes or more; otherwise they are dealt round robin
+// into BATCH_CONCURRENT batches.
 constexpr size_t BATCH_CONCURRENT = 2;
+constexpr bool BATCH_BY_SHAPE = true;
 
 static int batch_launch_step(const std::vector<Ctx *> &run, const StepLaunch &busy,
                              StepLaunch *out) {
   out->graphs.clear();
   out->seq.clear();
-  int K[3] = {0, 0, 0};
-  for (Ctx *c : run)
-    for (int r = 0; r < 3; r++) K[r] = std::max(K[r], c->graphs->K[r]);
-  // at least BATCH_CONCURRENT batches, each on its own stream: the batches'
-  // kernels interleave on the GPU, the VALU-bound IBM launches of one beside
-  // the memory-bound fluid launches of the other.  Cases are dealt round
-  // robin in order of decreasing grid size, so the batches carry similar work.
   std::vector<Ctx *> order(run);
   std::stable_sort(order.begin(), order.end(), [](const Ctx *a, const Ctx *b) {
     return (long)a->nx * a->ny > (long)b->nx * b->ny;
   });
-  const size_t nbat = std::min(order.size(), std::max<size_t>(
-      (order.size() + BATCH_MAX - 1) / BATCH_MAX, BATCH_CONCURRENT));
-  for (size_t b0 = 0; b0 < nbat; b0++) {
-    std::vector<Ctx *> cs;
-    for (size_t k = b0; k < order.size(); k += nbat) cs.push_back(order[k]);
+  // the batches, each on its own stream so that their kernels interleave on
+  // the GPU (the VALU-bound IBM launches of one beside the memory-bound
+  // fluid launches of another)
+  std::vector<std::vector<Ctx *>> batches;
+  if (BATCH_BY_SHAPE) {
+    // one batch per grid shape (chunks of at most BATCH_MAX): kernels
+    // specialised on the grid (its DCT length, power-of-two spacing) are
+    // then one launch per position of the step, not one per shape in turn
+    for (size_t k = 0; k < order.size();) {
+      size_t e = k;
+      while (e < order.size() && order[e]->nx == order[k]->nx && order[e]->ny == order[k]->ny) e++;
+      for (size_t c0 = k; c0 < e; c0 += BATCH_MAX)
+        batches.emplace_back(order.begin() + c0, order.begin() + std::min(e, c0 + BATCH_MAX));
+      k = e;
+    }
+  }
+  if (batches.size() < 2) {
+    // at least BATCH_CONCURRENT batches: cases dealt round robin in order
+    // of decreasing grid size, so the batches carry similar work
+    batches.clear();
+    const size_t nbat = std::min(order.size(), std::max<size_t>(
+        (order.size() + BATCH_MAX - 1) / BATCH_MAX, BATCH_CONCURRENT));
+    batches.resize(nbat);
+    for (size_t k = 0; k < order.size(); k++) batches[k % nbat].push_back(order[k]);
+  }
+  for (const std::vector<Ctx *> &cs : batches) {
+    // each batch captures as many sub-iterations per stage as its cases last needed
+    int K[3] = {0, 0, 0};
+    for (Ctx *c : cs)
+      for (int r = 0; r < 3; r++) K[r] = std::max(K[r], c->graphs->K[r]);
     BatchGraph *b = nullptr;
     for (auto &x : g_batches)
       if (batch_matches(*x, cs, K)) b = x.get();
