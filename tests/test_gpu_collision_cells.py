@@ -113,6 +113,29 @@ def test_cells_in_graph_steps_and_batched_ensemble():
         g.close()
 
 
+def test_records_of_other_sizes_resize_the_cells_in_graph_steps():
+    """Particle records uploaded with 5 % larger semi-axes change the cells
+    (their size follows the widest reach) and drop the captured graphs:
+    graph steps with the cell list then equal eager steps testing all
+    pairs."""
+    kw = dict(itot=512, jtot=1024, ly=2.0, Re=1000.0, extfrcy=8.0 / 1000.0)
+    parts = _dense(5)
+    out = []
+    for search, graph in ((eif.COLLISION_ALL_PAIRS, False), (eif.COLLISION_CELLS, True)):
+        g = _make(kw, parts, search, graph=graph)
+        g.run_steps(2)
+        rec = g.particles()
+        rec[:, 1:3] *= 1.05  # a, b
+        g.set_particle_records(rec)
+        g.run_steps(2)
+        out.append(g)
+    print("resized records: pairs", len(out[0].colliding_pairs()), flush=True)
+    assert out[0].colliding_pairs() == out[1].colliding_pairs()
+    _same(out[0], out[1], "resized")
+    for g in out:
+        g.close()
+
+
 def test_default_takes_cells_for_many_particles():
     """6000 small ellipses: the default (cell list from 4096 particles)
     against all pairs, two steps, with each mode's time per step."""

@@ -78,6 +78,36 @@ def test_fused_projection_with_ellipses_graph_and_eager():
     assert np.max(np.abs(f[1][:, 3:6] - u[1][:, 3:6])) < 1e-12
 
 
+def test_set_projection_after_graph_capture_takes_effect():
+    """Graph steps captured with the unfused kernels, then the fused
+    projection set: the next graph steps use it (captured graphs are
+    dropped), so they equal eager steps that switch at the same step.  In
+    partition mode the two forms round differently, so steps that kept the
+    old graph would differ."""
+    parts = cases.place_ellipses(6, 2.0, (0.03, 0.05), (0.5, 0.8), 0.03, 9)
+    cfg = dict(itot=256, jtot=512, ly=2.0, Re=300.0, extfrcy=8.0 / 300.0)
+    runs = []
+    for graph in (False, True):
+        g = eif.Solver(poisson_mode=eif.POISSON_PARTITION, **cfg)
+        xc = g.grid()["xc"]
+        uy = np.zeros((512 + 2, 256 + 2))
+        uy[1:-1, 1:-1] = 4.0 * xc[None, 1:-1] * (1 - xc[None, 1:-1])
+        g.upload(uy=uy)
+        g.update_boundaries()
+        g.set_particles(**parts)
+        if graph:
+            g.graph(True)
+        g.run_steps(3)
+        g.set_projection(4)
+        g.run_steps(3)
+        runs.append((g.download(), g.particles()))
+        g.close()
+    (a, pa), (b, pb) = runs
+    for k in ("ux", "uy", "p"):
+        assert np.array_equal(a[k], b[k]), k
+    assert np.array_equal(pa, pb)
+
+
 def test_set_projection_rejects_bad_arguments():
     g = eif.Solver(itot=48, jtot=16, ly=1.0, Re=100.0)
     with pytest.raises(eif.EIFError):
