@@ -125,9 +125,14 @@ int launch_bc_p(Ctx *c, double *p) {
 constexpr int PRED_BX = 256;
 constexpr int PRED_ROWS = 16;  // default rows per workgroup (the tuner may pick 8 or 32)
 enum { PRED_STORE = 0, PRED_PREV = 1, PRED_SRC = 2, PRED_NOB = 3 };
-// bit MODE set: that mode loads a row ahead.  Measured on the 2048x4096
-// grid: the first stage's kernel (PRED_NOB) 116 -> 108 us; PRED_PREV,
-// whose two windows leave 3 waves per SIMD either way, 135 -> 138 us
+// bit MODE set: that mode loads a row ahead.  The row loop is unrolled
+// over ROWS, so that the compiler's waits on the loads are per row, not
+// for every load in flight at the loop's back edge.  Measured on the
+// 2048x4096 grid (graph steps 498 -> 508 /s): the first stage's kernel
+// (PRED_NOB, look-ahead, 112 VGPRs) 108 -> 89 us; PRED_PREV unrolled
+// without look-ahead 135 -> 129 us (140 VGPRs, 3 waves per SIMD); with the
+// look-ahead of both windows 135 us (178 VGPRs, 2 waves), of the current
+// window alone 156 us
 constexpr int PRED_LOOKAHEAD = 1 << PRED_NOB;
 
 struct PredArgs {
@@ -282,46 +287,8 @@ struct PredictorB {
   win_start(L, a.ux, a.uy, i, j0, hx, w);
   if (prev) win_start(L, a.uxo, a.uyo, i, j0, hx, wo);
   double pm_c = MODE == PRED_SRC ? 0. : p[L.p(i, j0 - 1)];
-  PredRow rn, rno;
-  double pn_w = 0., pn_c = 0.;
-  if (LA) {
-    row_load(L, a.ux, a.uy, i, j0 + 1, hx, rn);
-    if (prev) row_load(L, a.uxo, a.uyo, i, j0 + 1, hx, rno);
-    if (MODE != PRED_SRC) {
-      pn_w = p[L.p(i - 1, j0)];
-      pn_c = p[L.p(i, j0)];
-    }
-  }
-  for (int j = j0; j <= j1; j++) {
-    double pc_w = 0., pc_c = 0.;
-    if (LA) {
-      // rows j + 2 (velocity) and j + 1 (pressure), clamped to the last
-      // halo row: the values loaded past the march are not used
-      const int jv = min(j + 2, ny + 1), jp = min(j + 1, ny + 1);
-      PredRow rnn, rnno;
-      row_load(L, a.ux, a.uy, i, jv, hx, rnn);
-      if (prev) row_load(L, a.uxo, a.uyo, i, jv, hx, rnno);
-      double pnn_w = 0., pnn_c = 0.;
-      if (MODE != PRED_SRC) {
-        pnn_w = p[L.p(i - 1, jp)];
-        pnn_c = p[L.p(i, jp)];
-      }
-      row_into(rn, w);
-      if (prev) row_into(rno, wo);
-      pc_w = pn_w;
-      pc_c = pn_c;
-      rn = rnn;
-      if (prev) rno = rnno;
-      pn_w = pnn_w;
-      pn_c = pnn_c;
-    } else {
-      win_row(L, a.ux, a.uy, i, j, hx, w);
-      if (prev) win_row(L, a.uxo, a.uyo, i, j, hx, wo);
-      if (MODE != PRED_SRC) {
-        pc_w = p[L.p(i - 1, j)];
-        pc_c = p[L.p(i, j)];
-      }
-    }
+  // one row's update from the window (rows j-1, j, j+1) and p(i-1, j), p(i, j)
+  auto row = [&](int j, double pc_w, double pc_c) {
     // ---- x-momentum at face (i, j)
     if (hx) {
       const double A = src_x<P2>(w, g);
@@ -388,6 +355,53 @@ struct PredictorB {
     win_slide(w);
     if (prev) win_slide(wo);
     pm_c = pc_c;
+  };
+  if (!LA) {
+#pragma unroll
+    for (int jj = 0; jj < ROWS; jj++) {
+      const int j = j0 + jj;
+      if (j > j1) break;
+      win_row(L, a.ux, a.uy, i, j, hx, w);
+      if (prev) win_row(L, a.uxo, a.uyo, i, j, hx, wo);
+      double pc_w = 0., pc_c = 0.;
+      if (MODE != PRED_SRC) {
+        pc_w = p[L.p(i - 1, j)];
+        pc_c = p[L.p(i, j)];
+      }
+      row(j, pc_w, pc_c);
+    }
+    return;
+  }
+  // LA: rows in pairs with two register sets (A, B) taking turns.  Each
+  // half issues the loads of the row after next into the idle set (rows
+  // j + 2 of the velocity and j + 1 of p, clamped to the last halo row:
+  // values loaded past the march are not used), then moves its own set,
+  // loaded a row earlier, into the window.  No set is copied into the
+  // other, so a row's loads are first waited for a row after they issue.
+  PredRow ra, rao, rb, rbo;
+  double pa_w = 0., pa_c = 0., pb_w = 0., pb_c = 0.;
+  auto fetch = [&](int jv, int jp, PredRow &r, PredRow &ro, double &pw, double &pc) {
+    row_load(L, a.ux, a.uy, i, min(jv, ny + 1), hx, r);
+    if (prev) row_load(L, a.uxo, a.uyo, i, min(jv, ny + 1), hx, ro);
+    if (MODE != PRED_SRC) {
+      pw = p[L.p(i - 1, min(jp, ny + 1))];
+      pc = p[L.p(i, min(jp, ny + 1))];
+    }
+  };
+  fetch(j0 + 1, j0, ra, rao, pa_w, pa_c);
+#pragma unroll
+  for (int jj = 0; jj < ROWS; jj += 2) {
+    const int j = j0 + jj;
+    if (j > j1) break;
+    fetch(j + 2, j + 1, rb, rbo, pb_w, pb_c);
+    row_into(ra, w);
+    if (prev) row_into(rao, wo);
+    row(j, pa_w, pa_c);
+    if (j + 1 > j1) break;
+    fetch(j + 3, j + 2, ra, rao, pa_w, pa_c);
+    row_into(rb, w);
+    if (prev) row_into(rbo, wo);
+    row(j + 1, pb_w, pb_c);
   }
 }
 };
