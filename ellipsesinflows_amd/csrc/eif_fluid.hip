@@ -529,8 +529,11 @@ constexpr int CP_ROWS = 8;   // default rows per workgroup of k_correct_pressure
 template <bool P2, bool DT, int ROWS = CP_ROWS>
 struct CorrectPressureB {
   static constexpr int T = 256, M = 0;
+  // ux, uy, p and psi are distinct arrays (restrict): the loads of a row
+  // need not wait for the stores of the field before them
   static __device__ __forceinline__ void run(const VB &vb, 
-    Layout L, double *ux, double *uy, double *p, const double *psi, const double *dxc,
+    Layout L, double *__restrict__ ux, double *__restrict__ uy, double *__restrict__ p,
+    const double *__restrict__ psi, const double *__restrict__ dxc,
     const double *dscal, double ga, double dy, int do_correct, int do_pressure,
     double *dt_part, const int *halt) {
   if (halted(halt)) return;
@@ -540,43 +543,67 @@ struct CorrectPressureB {
   // with DT: the advective time-step limit of decide_dt.c:26-48 over the
   // corrected velocities, as one partial minimum per workgroup
   double m = 5.e-2;
-  for (int j = vb.by * ROWS + 1; j <= ny && j <= (int)(vb.by + 1) * ROWS; j++) {
-    if (i > nx) break;
-    const double ps_c = psi[L.p(i, j)];
-    if (do_correct) {
-      if (i >= 2) {
-        const double dxi = 1. / dxc[i - 1];
-        const long k = L.ux(i, j);
-        const double v = ux[k] - ga * dt * (ps_c - psi[L.p(i - 1, j)]) * dxi;
-        ux[k] = v;
-        if (j == ny) ux[L.ux(i, 0)] = v;
-        if (j == 1) ux[L.ux(i, ny + 1)] = v;
-        if (DT) m = fmin(m, dxc[i - 1] / fmax(fabs(v), 1.e-8));
+  // rows in chunks of up to 4: the chunk's loads first (threads past nx
+  // and rows past ny load clamped addresses inside the arrays, and do not
+  // use them), then its updates and stores
+  constexpr int CH = ROWS < 4 ? ROWS : 4;
+  const int ic = min(i, nx);
+  const double dxc_i = dxc[ic - 1], dxi = 1. / dxc_i;
+#pragma unroll
+  for (int r0 = 0; r0 < ROWS; r0 += CH) {
+    double psc[CH], psw[CH], pss[CH], uxv[CH], uyv[CH], pv[CH];
+#pragma unroll
+    for (int q = 0; q < CH; q++) {
+      const int jc = min((int)vb.by * ROWS + 1 + r0 + q, ny);
+      psc[q] = psi[L.p(ic, jc)];
+      psw[q] = pss[q] = uxv[q] = uyv[q] = pv[q] = 0.;
+      if (do_correct) {
+        psw[q] = psi[L.p(ic - 1, jc)];
+        pss[q] = psi[L.p(ic, jc - 1)];
+        uxv[q] = ux[L.ux(ic, jc)];
+        uyv[q] = uy[L.p(ic, jc)];
       }
-      {
-        const long k = L.p(i, j);
-        const double v = uy[k] - qdiv<P2>(ga * dt * (ps_c - psi[L.p(i, j - 1)]), dy, 1. / dy);
-        uy[k] = v;
-        if (j == ny) uy[L.p(i, 0)] = v;
-        if (j == 1) uy[L.p(i, ny + 1)] = v;
-        if (DT) m = fmin(m, dy / fmax(fabs(v), 1.e-8));
-      }
+      if (do_pressure) pv[q] = p[L.p(ic, jc)];
     }
-    if (do_pressure) {
-      const long k = L.p(i, j);
-      const double v = p[k] + ps_c;
-      p[k] = v;
-      if (j == ny) p[L.p(i, 0)] = v;
-      if (j == 1) p[L.p(i, ny + 1)] = v;
-      if (i == 1) {
-        p[L.p(0, j)] = v;
-        if (j == ny) p[L.p(0, 0)] = v;
-        if (j == 1) p[L.p(0, ny + 1)] = v;
+#pragma unroll
+    for (int q = 0; q < CH; q++) {
+      const int j = vb.by * ROWS + 1 + r0 + q;
+      if (j > ny || i > nx) break;
+      const double ps_c = psc[q];
+      if (do_correct) {
+        if (i >= 2) {
+          const long k = L.ux(i, j);
+          const double v = uxv[q] - ga * dt * (ps_c - psw[q]) * dxi;
+          ux[k] = v;
+          if (j == ny) ux[L.ux(i, 0)] = v;
+          if (j == 1) ux[L.ux(i, ny + 1)] = v;
+          if (DT) m = fmin(m, dxc_i / fmax(fabs(v), 1.e-8));
+        }
+        {
+          const long k = L.p(i, j);
+          const double v = uyv[q] - qdiv<P2>(ga * dt * (ps_c - pss[q]), dy, 1. / dy);
+          uy[k] = v;
+          if (j == ny) uy[L.p(i, 0)] = v;
+          if (j == 1) uy[L.p(i, ny + 1)] = v;
+          if (DT) m = fmin(m, dy / fmax(fabs(v), 1.e-8));
+        }
       }
-      if (i == nx) {
-        p[L.p(nx + 1, j)] = v;
-        if (j == ny) p[L.p(nx + 1, 0)] = v;
-        if (j == 1) p[L.p(nx + 1, ny + 1)] = v;
+      if (do_pressure) {
+        const long k = L.p(i, j);
+        const double v = pv[q] + ps_c;
+        p[k] = v;
+        if (j == ny) p[L.p(i, 0)] = v;
+        if (j == 1) p[L.p(i, ny + 1)] = v;
+        if (i == 1) {
+          p[L.p(0, j)] = v;
+          if (j == ny) p[L.p(0, 0)] = v;
+          if (j == 1) p[L.p(0, ny + 1)] = v;
+        }
+        if (i == nx) {
+          p[L.p(nx + 1, j)] = v;
+          if (j == ny) p[L.p(nx + 1, 0)] = v;
+          if (j == 1) p[L.p(nx + 1, ny + 1)] = v;
+        }
       }
     }
   }
