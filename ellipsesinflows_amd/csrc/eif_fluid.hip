@@ -742,7 +742,15 @@ struct DtFinalB {
   __shared__ double adv;
   if (nparts > 0) {
     double m = 5.e-2;
-    for (int b = threadIdx.x; b < nparts; b += blockDim.x) m = fmin(m, part[b]);
+    // eight loads in flight per round; an index past the end reads the
+    // last partial again, which leaves the minimum unchanged
+    for (int b0 = threadIdx.x; b0 < nparts; b0 += 8 * blockDim.x) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) v[u] = part[min(b0 + u * (int)blockDim.x, nparts - 1)];
+#pragma unroll
+      for (int u = 0; u < 8; u++) m = fmin(m, v[u]);
+    }
     red[threadIdx.x] = m;
     __syncthreads();
     for (int s = blockDim.x / 2; s > 0; s >>= 1) {
