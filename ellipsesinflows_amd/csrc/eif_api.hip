@@ -1405,9 +1405,9 @@ struct StepLaunch {
 // case-steps/s: cases dealt round robin into 1 batch 2399, 2 batches
 // 2594-2667, 4 batches 2489; one batch per grid shape 2740-2749 (the
 // 14-case batch split in two 2722-2750, every shape of more than 8 cases
-// split 2652-2655).  Cases of one shape are batched by shape when the
-// ensemble has two shapes or more; otherwise they are dealt round robin
-// into BATCH_CONCURRENT batches.
+// split 2652-2655).  An ensemble of two grid shapes or more is batched
+// per shape; an ensemble of one shape is dealt round robin into
+// BATCH_CONCURRENT batches.
 constexpr size_t BATCH_CONCURRENT = 2;
 constexpr bool BATCH_BY_SHAPE = true;
 
