@@ -189,7 +189,7 @@ static void free_ctx(Ctx *c) {
   for (auto b : bufs)
     if (*b) (void)hipFree(*b);
   void *others[] = {c->th_flag, c->boxes, c->box_off, c->tile_cnt, c->tile_list,
-                    c->active, c->flags, c->rows, c->row_cnt, c->wall_hit,
+                    c->flags, c->rows, c->row_cnt, c->wall_hit,
                     c->ibm_stats, c->slice_first, c->slice_info, c->ibm_pp, c->slice_owner,
                     c->sc_key, c->heads, c->cl_start, c->cl_items, c->cl_cell, c->cl_slot, c->cl_count};
   for (auto o : others)

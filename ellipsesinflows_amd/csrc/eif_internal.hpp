@@ -198,7 +198,6 @@ struct Ctx {
   int nslices = 0;
   int *tile_cnt = nullptr, *tile_list = nullptr;
   int ntx = 0, nty = 0;
-  int *active = nullptr;   // [0] count, then tile ids
   int *flags = nullptr;    // device error flags
   unsigned long long *ibm_stats = nullptr;  // [8] measurement counters
   unsigned long long *ibm_pp = nullptr;     // np x 4 IBM work counters, then calls

@@ -21,7 +21,7 @@
 //     first inertia(1), src/main.c:21-38), the cache serves the repeats with
 //     the same bits;
 //   * the exchange writes each box cell's forcing to box-local scratch and
-//     registers the box with the 64x4 tiles it touches; a gather over active
+//     registers the box with the 64x4 tiles it touches; a gather over the
 //     tiles then adds the contributions cell by cell in the reference's
 //     particle/image order and applies them to ux/uy (no atomics, no
 //     full-field pass);
@@ -381,7 +381,7 @@ struct IbmArgs {
   double *sc_pos;
   double *sc_t;   // per box cell: foot parameter of the last evaluation
   int *sc_key;    // per particle: boxes of that evaluation (8 ints)
-  int *tile_cnt, *tile_list, *active;
+  int *tile_cnt, *tile_list;
   int ntx, nty;
   int *flags;
   unsigned long long *stats;  // per particle: cells, cells evaluated, FP64 and FP32 iterations
@@ -848,10 +848,6 @@ __device__ inline void box_tile_ranges(const IbmArgs &a, int imin, int imax, int
 
 __device__ inline void register_tile(const IbmArgs &a, int tile, int box_id) {
   const int slot = atomicAdd(&a.tile_cnt[tile], 1);
-  if (slot == 0) {
-    const int pos = atomicAdd(&a.active[0], 1);
-    a.active[1 + pos] = tile;
-  }
   if (slot < TILE_CAP)
     a.tile_list[(long)tile * TILE_CAP + slot] = box_id;
   else
@@ -952,9 +948,15 @@ struct ExchangeB {
         off += m.cnt;
       }
     }
-    // register the boxes with their tiles, one tile per lane
+    // register the boxes with their tiles, one tile per lane: the first
+    // tile of each of the six ranges is claimed by atomics issued together,
+    // then the list entries stored (the registrations of one range do not
+    // wait for those of the range before); a range of more than 64 tiles
+    // registers the rest one by one
+    int tile_[6], slot_[6];
 #pragma unroll
     for (int r = 0; r < 6; r++) {
+      tile_[r] = -1;
       const Img m = pick(bs, r / 2);
       if (m.cnt <= 0) continue;
       TileRange tr[2];
@@ -962,8 +964,19 @@ struct ExchangeB {
       const TileRange &rg = tr[r & 1];
       const int cnt = rg.count();
       const int w = rg.tx1 - rg.tx0 + 1;
-      for (int e_ = threadIdx.x; e_ < cnt; e_ += 64)
+      if ((int)threadIdx.x < cnt) tile_[r] = (rg.ty0 + (int)threadIdx.x / w) * a.ntx + rg.tx0 + (int)threadIdx.x % w;
+      for (int e_ = threadIdx.x + 64; e_ < cnt; e_ += 64)
         register_tile(a, (rg.ty0 + e_ / w) * a.ntx + rg.tx0 + e_ % w, n * 3 + r / 2);
+    }
+#pragma unroll
+    for (int r = 0; r < 6; r++) slot_[r] = tile_[r] >= 0 ? atomicAdd(&a.tile_cnt[tile_[r]], 1) : -1;
+#pragma unroll
+    for (int r = 0; r < 6; r++) {
+      if (tile_[r] < 0) continue;
+      if (slot_[r] < TILE_CAP)
+        a.tile_list[(long)tile_[r] * TILE_CAP + slot_[r]] = n * 3 + r / 2;
+      else
+        atomicOr(&a.flags[0], 2);
     }
   }
   const double *dist = a.sc_dist + hd.off;
@@ -1073,7 +1086,7 @@ struct TileArgs {
   int nx, ny, ntx;
   const Box *boxes;
   const double *sc_fx, *sc_fy;
-  const int *tile_cnt, *tile_list, *active;
+  const int *tile_cnt, *tile_list;
   double *ux, *uy;
   double *dux, *duy;  // materialisation target (or null)
   int apply;
@@ -1844,20 +1857,17 @@ __global__ void k_update_particles(double *prt, int np, double ly, const int *ha
 }
 
 // src/suspensions/reset.c:7-18
-// res (RES_SLOTS sub-iteration residuals), tile_cnt (ntiles) and
-// active[0]: cleared when given, for the stage's sub-iterations and the
-// next exchange's tile registrations
+// res (RES_SLOTS sub-iteration residuals) and tile_cnt (ntiles): cleared
+// when given, for the stage's sub-iterations and the next exchange's tile
+// registrations
 // Also the heads of the stage-start pass (positions without increments).
 struct ResetB {
   static constexpr int T = 1024, M = 0;
-  static __device__ __forceinline__ void run(const VB &vb, IbmArgs a, double *res, int *tile_cnt, int ntiles, int *active) {
+  static __device__ __forceinline__ void run(const VB &vb, IbmArgs a, double *res, int *tile_cnt, int ntiles) {
   if (halted(a.halt)) return;
   const int n = vb.bx * blockDim.x + threadIdx.x;
   if (res && n < RES_SLOTS) res[n] = 0.;
-  if (tile_cnt) {
-    if (n < ntiles) tile_cnt[n] = 0;
-    if (n == 0) active[0] = 0;
-  }
+  if (tile_cnt && n < ntiles) tile_cnt[n] = 0;
   if (n >= a.np) return;
   // foot-point warm starts reach back no further than the stage start, so
   // that a run restarted from a save repeats the same operations
@@ -1872,8 +1882,8 @@ struct ResetB {
   write_head(a, n, p[R_X], p[R_Y], p[R_AZ], true);
 }
 };
-__global__ void k_reset(IbmArgs a, double *res, int *tile_cnt, int ntiles, int *active) {
-  ResetB::run(VB::here(), a, res, tile_cnt, ntiles, active);
+__global__ void k_reset(IbmArgs a, double *res, int *tile_cnt, int ntiles) {
+  ResetB::run(VB::here(), a, res, tile_cnt, ntiles);
 }
 
 // ---------------------------------------------------------------------
@@ -1907,7 +1917,6 @@ static IbmArgs ibm_args(Ctx *c) {
   a.sc_key = c->sc_key;
   a.tile_cnt = c->tile_cnt;
   a.tile_list = c->tile_list;
-  a.active = c->active;
   a.ntx = c->ntx;
   a.nty = c->nty;
   a.flags = c->flags;
@@ -1990,8 +1999,6 @@ int susp_init(Ctx *c) {
   EIF_HIP(hipMalloc(&c->tile_cnt, ntiles * sizeof(int)));
   EIF_HIP(hipMemset(c->tile_cnt, 0, ntiles * sizeof(int)));
   EIF_HIP(hipMalloc(&c->tile_list, ntiles * TILE_CAP * sizeof(int)));
-  EIF_HIP(hipMalloc(&c->active, (ntiles + 1) * sizeof(int)));
-  EIF_HIP(hipMemset(c->active, 0, (ntiles + 1) * sizeof(int)));
   // collisions
   {
     const size_t npp = (size_t)(c->np > 0 ? c->np : 1);
@@ -2010,7 +2017,7 @@ int launch_reset(Ctx *c, hipStream_t st, bool clear_tiles) {
   const int ntiles = c->np > 0 && clear_tiles ? c->ntx * c->nty : 0;
   const int n = std::max(std::max(c->np, ntiles), RES_SLOTS);
   go<ResetB>(k_reset, dim3((n + 255) / 256), dim3(256), 0, st, ibm_args(c),
-                     c->res_slots, ntiles > 0 ? c->tile_cnt : nullptr, ntiles, c->active);
+                     c->res_slots, ntiles > 0 ? c->tile_cnt : nullptr, ntiles);
   EIF_HIP(hipGetLastError());
   return EIF_OK;
 }
@@ -2035,7 +2042,6 @@ int launch_exchange(Ctx *c, bool cached_known, bool tiles_clear, bool defer_fina
   if (!tiles_clear) {  // clear the previous call's tile registrations
     go<ZeroIntB>(k_zero_int, dim3((unsigned)((ntiles + 255) / 256)), dim3(256), 0,
                        c->stream, c->tile_cnt, (int)ntiles);
-    go<ZeroIntB>(k_zero_int, dim3(1), dim3(64), 0, c->stream, c->active, 1);
   }
   IbmArgs a = ibm_args(c);
   go<HeadsB>(k_heads, dim3((c->np + 255) / 256), dim3(256), 0, c->stream, a, 0);
@@ -2093,7 +2099,6 @@ static TileArgs tile_args(Ctx *c, int apply) {
   t.sc_fy = c->sc_fy;
   t.tile_cnt = c->tile_cnt;
   t.tile_list = c->tile_list;
-  t.active = c->active;
   t.ux = c->ux;
   t.uy = c->uy;
   t.dux = c->dux;
