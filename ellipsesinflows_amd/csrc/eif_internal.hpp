@@ -139,6 +139,7 @@ struct Ctx {
   // Poisson
   double *qhat = nullptr;      // spectral field, x-contiguous rows [ny][nx]
   double *ctab = nullptr;      // C[m] = cos(pi m/(2 nx)), m=0..2nx
+  double *ctab3 = nullptr;     // DCT-III pre-processing factors in thread order (3 x 2nx)
   double *twid = nullptr;      // per-stage radix-2 twiddles (entries of C)
   double *th_val = nullptr;    // Thomas tables [ny-1][nx] (forward factors)
   double *th_u = nullptr;      // [ny-1][nx] modified upper diagonal

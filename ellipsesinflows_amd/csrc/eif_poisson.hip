@@ -534,20 +534,27 @@ __global__ __launch_bounds__((1 << LOGN) >> LOGR_) void k_rhs_dct2_t(
 // (written and synchronised by the caller); on return they hold 2 nx psi
 // of the two rows in the Makhoul order, cell i at sw(i / 2) for even i and
 // at sw(nx - 1 - i / 2) for odd i (see psi_at).
-template <int LOGN, int LOGR_ = dct_logr<LOGN>()>
+// ORD: the factors C[k], C[nx - k] of the pre-processing come from D, the
+// same values stored in the order the threads read them (D[s T + t] and
+// D[nx + s T + t] for k = bitrev(R t + s)), so that a wave's loads are
+// contiguous instead of bit-reversed across the table
+template <int LOGN, int LOGR_ = dct_logr<LOGN>(), bool ORD = false>
 __device__ __forceinline__ void dct3_pair_lds(double *re, double *im,
                                               const double *__restrict__ C,
-                                              const double *__restrict__ W, int t) {
+                                              const double *__restrict__ W, int t,
+                                              const double *__restrict__ D = nullptr) {
   constexpr int LOGR = LOGR_, R = 1 << LOGR;
   constexpr int nx = 1 << LOGN;
+  constexpr int T = nx / R;
   double xr[R], xi[R];
 #pragma unroll
   for (int s = 0; s < R; s++) {
     const int k = bitrev(R * t + s, LOGN);
+    const double ck = ORD ? D[s * T + t] : C[k], cnk = ORD ? D[nx + s * T + t] : C[nx - k];
     const double ar = re[sw(k)], ai = k == 0 ? 0. : -re[sw(nx - k)];
     const double br = im[sw(k)], bi = k == 0 ? 0. : -im[sw(nx - k)];
-    const double var = C[k] * ar - C[nx - k] * ai, vai = C[k] * ai + C[nx - k] * ar;
-    const double vbr = C[k] * br - C[nx - k] * bi, vbi = C[k] * bi + C[nx - k] * br;
+    const double var = ck * ar - cnk * ai, vai = ck * ai + cnk * ar;
+    const double vbr = ck * br - cnk * bi, vbi = ck * bi + cnk * br;
     xr[s] = var - vbi;
     xi[s] = vai + vbr;
   }
@@ -580,7 +587,7 @@ struct Dct3PsiTB {
   static constexpr int T = (1 << LOGN) >> LOGR_, M = LOGR_ <= 3 ? 4 : 0;
   static __device__ __forceinline__ void run(const VB &vb, 
     Layout L, const double *qhat, CombineArgs ca, const double *__restrict__ C,
-    const double *__restrict__ W, double *psi) {
+    const double *__restrict__ W, const double *__restrict__ D, double *psi) {
   if (halted(ca.halt)) return;
   extern __shared__ double sh[];
   constexpr int LOGR = LOGR_, R = 1 << LOGR;
@@ -622,7 +629,7 @@ struct Dct3PsiTB {
     }
   }
   __syncthreads();
-  dct3_pair_lds<LOGN, LOGR>(re, im, C, W, threadIdx.x);
+  dct3_pair_lds<LOGN, LOGR, true>(re, im, C, W, threadIdx.x, D);
   // psi rows and their boundary values (the Neumann walls, and the periodic
   // halo rows 0 and ny+1 copied from rows ny and 1: k_bc_p's values)
 #pragma unroll
@@ -651,8 +658,8 @@ struct Dct3PsiTB {
 template <int LOGN, int LOGR_ = dct_logr<LOGN>()>
 __global__ __launch_bounds__((1 << LOGN) >> LOGR_, LOGR_ <= 3 ? 4 : 1) void k_dct3_psi_t(
     Layout L, const double *qhat, CombineArgs ca, const double *__restrict__ C,
-    const double *__restrict__ W, double *psi) {
-  Dct3PsiTB<LOGN, LOGR_>::run(VB::here(), L, qhat, ca, C, W, psi);
+    const double *__restrict__ W, const double *__restrict__ D, double *psi) {
+  Dct3PsiTB<LOGN, LOGR_>::run(VB::here(), L, qhat, ca, C, W, D, psi);
 }
 
 // ---------------------------------------------------------------------
@@ -1301,6 +1308,28 @@ int poisson_init(Ctx *c) {
   for (int m = 0; m <= 2 * nx; m++) C[m] = cos(EIF_PI * m / (2. * nx));
   EIF_HIP(hipMalloc(&c->ctab, C.size() * sizeof(double)));
   EIF_HIP(hipMemcpy(c->ctab, C.data(), C.size() * sizeof(double), hipMemcpyHostToDevice));
+  // the DCT-III pre-processing factors in thread order, one block of 2 nx
+  // per register-FFT width R = 4, 8, 16: D[s T + t] = C[k] and
+  // D[nx + s T + t] = C[nx - k] for k = bitrev(R t + s), T = nx / R
+  if ((nx & (nx - 1)) == 0 && nx >= 16) {
+    int logn = 0;
+    while ((1 << logn) < nx) logn++;
+    std::vector<double> D(3 * 2 * (size_t)nx);
+    for (int lr = 2; lr <= 4; lr++) {
+      const int R = 1 << lr, T = nx / R;
+      double *blk = D.data() + (size_t)(lr - 2) * 2 * nx;
+      for (int t = 0; t < T; t++)
+        for (int s = 0; s < R; s++) {
+          const unsigned v = (unsigned)(R * t + s);
+          unsigned k = 0;
+          for (int b = 0; b < logn; b++) k |= ((v >> b) & 1u) << (logn - 1 - b);
+          blk[(size_t)s * T + t] = C[k];
+          blk[(size_t)nx + (size_t)s * T + t] = C[nx - k];
+        }
+    }
+    EIF_HIP(hipMalloc(&c->ctab3, D.size() * sizeof(double)));
+    EIF_HIP(hipMemcpy(c->ctab3, D.data(), D.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
   // per-stage twiddles of the radix-2 transform (copies of C entries)
   {
     std::vector<double> W(2 * (size_t)(nx > 1 ? nx - 1 : 1), 0.);
@@ -1443,12 +1472,18 @@ static void rhs_t(Ctx *c, int rk, size_t shmem, const double *rhs_add) {
   }
 }
 
+// the block of ctab3 for register-FFT width 2^lr
+static inline size_t dct3_table_offset(const Ctx *c, int lr) {
+  return (size_t)(lr - 2) * 2 * c->nx;
+}
+
 // the register-FFT DCT-III of 2^LG-point rows into psi
 template <int LG, int LR>
 static void dct3_tr(Ctx *c, size_t shmem, const CombineArgs &ca) {
   if constexpr (fft_logr_ok(LG, LR))
     go<Dct3PsiTB<LG, LR>>(k_dct3_psi_t<LG, LR>, dim3((c->ny + 1) / 2), dim3(c->nx >> LR),
-                          shmem, c->stream, c->L, c->qhat, ca, c->ctab, c->twid, c->psi);
+                          shmem, c->stream, c->L, c->qhat, ca, c->ctab, c->twid,
+                          c->ctab3 + dct3_table_offset(c, LR), c->psi);
 }
 template <int LG>
 static void dct3_t(Ctx *c, size_t shmem, const CombineArgs &ca) {
