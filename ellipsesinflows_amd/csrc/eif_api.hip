@@ -180,7 +180,7 @@ const char *eif_gpu_profile_names(void) { return k_prof_names; }
 static void free_ctx(Ctx *c) {
   double **bufs[] = {&c->xf, &c->xc, &c->dxf, &c->dxc, &c->yf, &c->yc, &c->ux, &c->uy,
                      &c->p, &c->psi, &c->ux_alt, &c->uy_alt, &c->ux_spare, &c->uy_spare, &c->srcuxa, &c->srcuya,
-                     &c->dux, &c->duy, &c->qhat, &c->ctab, &c->ctab3, &c->twid, &c->th_val, &c->th_u,
+                     &c->dux, &c->duy, &c->qhat, &c->ctab, &c->ctab3, &c->twid, &c->cpos, &c->th_val, &c->th_u,
                      &c->th_q1, &c->th_den, &c->th_last, &c->pt_fwd, &c->pt_u,
                      &c->pt_v, &c->pt_w, &c->pt_bnd, &c->pt_red, &c->pt_lvl, &c->pt_ends, &c->pt_g, &c->dscal, &c->dred,
                      &c->prt, &c->pconst, &c->dt_part, &c->sc_fx, &c->sc_fy, &c->sc_dist, &c->sc_pos,

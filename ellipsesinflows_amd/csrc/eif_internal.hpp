@@ -219,6 +219,9 @@ struct Ctx {
   int *cl_cell = nullptr;   // np: each particle's cell
   int *cl_slot = nullptr;   // np: its place within the cell
   int *cl_count = nullptr;  // COLL_CELLS_CAP per-cell counts, zero between builds
+  // np x 4: x + dx, y + dy, max(a, b) of each particle (and a pad), as the
+  // broad phase compares them, packed so that a wave's reads are contiguous
+  double *cpos = nullptr;
   int *rows = nullptr;      // np x CONTACT_CAP colliding partners, ascending
   int *row_cnt = nullptr;   // np
   int *wall_hit = nullptr;  // np x 2

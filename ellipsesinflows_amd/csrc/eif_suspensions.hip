@@ -385,6 +385,7 @@ struct IbmArgs {
   int ntx, nty;
   int *flags;
   unsigned long long *stats;  // per particle: cells, cells evaluated, FP64 and FP32 iterations
+  double *cpos;               // np x 4: the broad phase's packed positions and radii
   int np;
   const int *slice_first;  // np+1: first slice of each particle
   const int *slice_owner;  // particle of each slice
@@ -1303,6 +1304,7 @@ struct CollArgs {
   int ncx, ncy;
   double hx, hy;
   int *cl_start, *cl_items, *cl_cell, *cl_slot;
+  double *cpos;  // np x 4: x + dx, y + dy, max(a, b) (written by k_coll_pack and k_increment)
 };
 
 __device__ inline double harmonic(double v0, double v1) {
@@ -1324,13 +1326,35 @@ __device__ inline double min_image_offset(double p0y, double p1y, double ly) {
 
 // The early exit of collision.c:156-189 for the pair n0 < n1: the bounding
 // circles (radius max(a, b)) of the minimum image overlap
+// (the positions x + dx, y + dy and radii max(a, b) from the packed cpos,
+// the same values as the records give)
 __device__ inline bool pair_candidate(const CollArgs &a, int n0, int n1) {
-  const double *p0 = a.prt + (long)n0 * R_N, *p1 = a.prt + (long)n1 * R_N;
-  const double x0 = p0[R_X] + p0[R_DX], y0 = p0[R_Y] + p0[R_DY];
-  const double x1 = p1[R_X] + p1[R_DX], y1r = p1[R_Y] + p1[R_DY];
+  const double *p0 = a.cpos + 4 * (long)n0, *p1 = a.cpos + 4 * (long)n1;
+  const double x0 = p0[0], y0 = p0[1];
+  const double x1 = p1[0], y1r = p1[1];
   const double y1 = y1r + min_image_offset(y0, y1r, a.ly);
   const double norm = fmax(hypot(x1 - x0, y1 - y0), DBL_EPSILON);
-  return !(fmax(p0[R_A], p0[R_B]) + fmax(p1[R_A], p1[R_B]) - norm < 0.);
+  return !(p0[2] + p1[2] - norm < 0.);
+}
+
+// the packed positions and radii of the broad phase, from the records
+struct CollPackB {
+  static constexpr int T = 256, M = 0;
+  static __device__ __forceinline__ void run(const VB &vb, CollArgs a) {
+    if (halted(a.halt)) return;
+    if (!sub_open(a.res, a.sub)) return;
+    const int n = vb.bx * blockDim.x + threadIdx.x;
+    if (n >= a.np) return;
+    const double *q = a.prt + (long)n * R_N;
+    double *o = a.cpos + 4 * (long)n;
+    o[0] = q[R_X] + q[R_DX];
+    o[1] = q[R_Y] + q[R_DY];
+    o[2] = fmax(q[R_A], q[R_B]);
+    o[3] = 0.;
+  }
+};
+__global__ __launch_bounds__(256) void k_coll_pack(CollArgs a) {
+  CollPackB::run(VB::here(), a);
 }
 
 // One contact of collision.c:156-304, by the two lanes l and l^1.
@@ -1817,6 +1841,11 @@ struct IncrementB {
     p[R_DX] = dxn;
     p[R_DY] = dyn;
     p[R_DAZ] = dazn;
+    // the broad phase's packed position for the next sub-iteration's collisions
+    double *cp = ia.cpos + 4 * (long)n;
+    cp[0] = x0 + dxn;
+    cp[1] = y0 + dyn;
+    cp[2] = rad;
     // the next sub-iteration's inertia pass
     write_head(ia, n, x0 + dxn, y0 + dyn, az0 + dazn, false, fin ? &hd : nullptr, rad);
   }
@@ -1921,6 +1950,7 @@ static IbmArgs ibm_args(Ctx *c) {
   a.nty = c->nty;
   a.flags = c->flags;
   a.stats = c->ibm_pp;
+  a.cpos = c->cpos;
   a.np = c->np;
   a.slice_first = c->slice_first;
   a.slice_owner = c->slice_owner;
@@ -2003,6 +2033,8 @@ int susp_init(Ctx *c) {
   {
     const size_t npp = (size_t)(c->np > 0 ? c->np : 1);
     EIF_HIP(hipMalloc(&c->rows, npp * CONTACT_CAP * sizeof(int)));
+    EIF_HIP(hipMalloc(&c->cpos, 4 * npp * sizeof(double)));
+    EIF_HIP(hipMemset(c->cpos, 0, 4 * npp * sizeof(double)));
     EIF_HIP(hipMalloc(&c->row_cnt, npp * sizeof(int)));
     EIF_HIP(hipMemset(c->row_cnt, 0, npp * sizeof(int)));
   }
@@ -2228,6 +2260,7 @@ static CollArgs coll_args(Ctx *c, int cnstep) {
   a.cl_items = c->cl_items;
   a.cl_cell = c->cl_cell;
   a.cl_slot = c->cl_slot;
+  a.cpos = c->cpos;
   return a;
 }
 
@@ -2246,6 +2279,7 @@ int launch_collision(Ctx *c, int cnstep, hipStream_t st) {
   if (!st) st = c->stream;
   ProfScope ps(c, "collision", st);
   const CollArgs a = coll_args(c, cnstep);
+  go<CollPackB>(k_coll_pack, dim3((c->np + 255) / 256), dim3(256), 0, st, a);
   launch_cell_list(c, a, st);
   go<CollideB>(k_collide, dim3(c->np), dim3(64), 0, st, a);
   EIF_HIP(hipGetLastError());
